@@ -1,0 +1,279 @@
+"""Python host binding of libamrlbm.so (the MI355X two-phase AMR-LBM step).
+
+The product is the C-ABI library (include/amrlbm.h); this module is a thin
+ctypes layer over it for tests, benchmarks and scripting. It mirrors the
+reference's subroutine surface: Mesh.fill_ghost_cells_c2() is
+Fill_Ghost_Cells_C2, Mesh.refine_derefine_mesh() is Refine_Derefine_Mesh, and
+so on. There is no CPU fallback: if the library is missing or no GPU is
+present, constructing a Mesh raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .workloads import WORKLOADS, Workload  # noqa: F401
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "lib", "libamrlbm.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "amrlbm.h")
+
+NX = NY = 16
+ERRORS = {1: "HIP", 2: "ARG", 3: "CAPACITY", 4: "TOPOLOGY", 5: "NONFINITE", 6: "STATE"}
+
+
+class AmrLbmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"amrlbm error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("nx", "ny", "L0", "nroots", "min_levels", "max_levels", "max_blocks")] + \
+               [(n, ctypes.c_double) for n in
+                ("M0", "W", "sigma", "beta", "kappa", "taul", "tauh", "rhol", "rhoh", "gy")]
+
+
+def params_from_workload(w):
+    p = Params()
+    p.nx, p.ny = w.nx, w.ny
+    p.L0, p.nroots, p.min_levels, p.max_levels = w.L0, w.nroots, w.min_levels, w.max_levels
+    p.max_blocks = w.max_blocks
+    p.M0, p.W, p.sigma, p.beta, p.kappa = w.M0, w.W, w.sigma, w.beta, w.kappa
+    p.taul, p.tauh, p.rhol, p.rhoh, p.gy = w.taul, w.tauh, w.rhol, w.rhoh, w.gy
+    return p
+
+
+_lib = None
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+
+
+def exported_functions():
+    """Names of the functions include/amrlbm.h declares."""
+    import re
+    src = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(amrlbm_\w+)\s*\(", src, re.M)))
+
+
+def lib():
+    """Load libamrlbm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built; run amr-lbm-openmp-2d_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        PP = ctypes.POINTER(Params)
+        sig = {
+            "amrlbm_create": [PP, ctypes.POINTER(vp)],
+            "amrlbm_destroy": [vp],
+            "amrlbm_setup_rising_bubble": [vp, dp, dp, dp],
+            "amrlbm_get_counts": [vp, _IP],
+            "amrlbm_get_topology": [vp, _IP, _IP],
+            "amrlbm_set_topology": [vp, ip, _IP, ip, ip, _IP, ip],
+            "amrlbm_put_block": [vp, ip, _DP, _DP, _DP, _DP, _DP, _DP],
+            "amrlbm_get_block": [vp, ip, _DP, _DP, _DP, _DP, _DP, _DP],
+            "amrlbm_determine_min_max_levels": [vp, _IP, _IP],
+            "amrlbm_fill_ghost_cells_c2": [vp],
+            "amrlbm_collide": [vp],
+            "amrlbm_fill_ghost_cells_hg": [vp],
+            "amrlbm_stream": [vp],
+            "amrlbm_phase_field_lbm_h_g": [vp, _DP],
+            "amrlbm_macroscopic_values_h": [vp],
+            "amrlbm_fill_ghost_cells_c": [vp],
+            "amrlbm_macroscopic_values_g": [vp],
+            "amrlbm_refine_derefine_mesh": [vp, dp, dp],
+            "amrlbm_step": [vp, ip, ip],
+            "amrlbm_profile_steps": [vp, ip, ip, _DP],
+            "amrlbm_output": [vp, _DP],
+            "amrlbm_synchronize": [vp],
+            "amrlbm_last_step_ms": [vp, _DP],
+            "amrlbm_collision_block": [PP, ip, _DP, _DP, _DP, _DP, _DP, _DP],
+            "amrlbm_macro_g_block": [PP, ip, _DP, _DP, _DP, _DP, _DP],
+            "amrlbm_stream_block": [PP, ip, _DP],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.amrlbm_last_error.restype = ctypes.c_char_p
+        L.amrlbm_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise AmrLbmError(rc, lib().amrlbm_last_error().decode())
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_DP)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_IP)
+
+
+def pop_array():
+    """Empty array in the reference layout h(9, 0:nx+1, 0:ny+1)."""
+    return np.zeros((9, NX + 2, NY + 2), order="F")
+
+
+class Mesh:
+    """One AMR mesh resident on the GPU."""
+
+    def __init__(self, workload):
+        self.w = workload
+        self.params = params_from_workload(workload)
+        h = ctypes.c_void_p()
+        _check(lib().amrlbm_create(ctypes.byref(self.params), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().amrlbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- set-up and state exchange -------------------------------------
+    def setup_rising_bubble(self, radius=None, x0=None, y0=None):
+        w = self.w
+        _check(lib().amrlbm_setup_rising_bubble(
+            self.h, w.radius if radius is None else radius, w.x0 if x0 is None else x0,
+            w.y0 if y0 is None else y0))
+
+    def counts(self):
+        c = np.zeros(7, dtype=np.int32)
+        _check(lib().amrlbm_get_counts(self.h, _i(c)))
+        keys = ("nblks", "nleaf", "levelmin", "levelmax", "kk", "nfree", "T")
+        return dict(zip(keys, (int(v) for v in c)))
+
+    def topology(self):
+        """(blocks[nblks,21], free list) in the reference's terms."""
+        c = self.counts()
+        t = np.zeros((max(c["nblks"], 1), 21), dtype=np.int32)
+        f = np.zeros(max(c["nfree"], 1), dtype=np.int32)
+        _check(lib().amrlbm_get_topology(self.h, _i(t), _i(f)))
+        return t[:c["nblks"]], list(f[:c["nfree"]])
+
+    def set_topology(self, blocks, kk, free, T=0):
+        blocks = np.ascontiguousarray(blocks, dtype=np.int32)
+        free = np.ascontiguousarray(np.asarray(free, dtype=np.int32).reshape(-1))
+        _check(lib().amrlbm_set_topology(self.h, blocks.shape[0], _i(blocks), int(kk), len(free),
+                                         _i(free) if len(free) else None, int(T)))
+
+    def put_block(self, bid, h=None, g=None, C=None, P=None, Ux=None, Uy=None):
+        conv = [None if a is None else np.asfortranarray(a, dtype=np.float64) for a in (h, g, C, P, Ux, Uy)]
+        _check(lib().amrlbm_put_block(self.h, int(bid), *[_d(a) for a in conv]))
+
+    def get_block(self, bid):
+        h, g = pop_array(), pop_array()
+        C = np.zeros((NX + 4, NY + 4), order="F")
+        P, Ux, Uy = (np.zeros((NX, NY), order="F") for _ in range(3))
+        _check(lib().amrlbm_get_block(self.h, int(bid), _d(h), _d(g), _d(C), _d(P), _d(Ux), _d(Uy)))
+        return {"h": h, "g": g, "C": C, "P": P, "Ux": Ux, "Uy": Uy}
+
+    def leaves(self):
+        t, _ = self.topology()
+        return [n + 1 for n in range(t.shape[0]) if t[n, 5]]
+
+    # ---- the reference's per-step subroutines ----------------------------
+    def determine_min_max_levels(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(lib().amrlbm_determine_min_max_levels(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def fill_ghost_cells_c2(self):
+        _check(lib().amrlbm_fill_ghost_cells_c2(self.h))
+
+    def collide(self):
+        _check(lib().amrlbm_collide(self.h))
+
+    def fill_ghost_cells_hg(self):
+        _check(lib().amrlbm_fill_ghost_cells_hg(self.h))
+
+    def stream(self):
+        _check(lib().amrlbm_stream(self.h))
+
+    def phase_field_lbm_h_g(self):
+        ms = (ctypes.c_double * 3)()
+        _check(lib().amrlbm_phase_field_lbm_h_g(self.h, ms))
+        return list(ms)
+
+    def macroscopic_values_h(self):
+        _check(lib().amrlbm_macroscopic_values_h(self.h))
+
+    def fill_ghost_cells_c(self):
+        _check(lib().amrlbm_fill_ghost_cells_c(self.h))
+
+    def macroscopic_values_g(self):
+        _check(lib().amrlbm_macroscopic_values_g(self.h))
+
+    def refine_derefine_mesh(self, refin_tol=0.002, deref_tol=0.001):
+        _check(lib().amrlbm_refine_derefine_mesh(self.h, refin_tol, deref_tol))
+
+    def step(self, nsteps=1, regrid_every=1):
+        _check(lib().amrlbm_step(self.h, int(nsteps), int(regrid_every)))
+
+    def profile_steps(self, nsteps, regrid_every=1):
+        """Per-kernel-class device times (HIP events) over nsteps steps."""
+        st = (ctypes.c_double * 12)()
+        _check(lib().amrlbm_profile_steps(self.h, int(nsteps), int(regrid_every), st))
+        names = ("c2_collide", "hg_stream", "c_macro_g")
+        out = {n: {"ms": st[3 * k], "launches": int(st[3 * k + 1]), "cells": int(st[3 * k + 2])}
+               for k, n in enumerate(names)}
+        out["regrid"] = {"ms": st[9], "count": int(st[10])}
+        return out
+
+    def last_step_ms(self):
+        v = ctypes.c_double()
+        _check(lib().amrlbm_last_step_ms(self.h, ctypes.byref(v)))
+        return v.value
+
+    def output(self, allow_nonfinite=False):
+        """rho_min, rho_max, max|Uy|, mass, bubble area, centroid x, y, rise velocity, non-finite."""
+        out = (ctypes.c_double * 9)()
+        rc = lib().amrlbm_output(self.h, out)
+        if rc != 0 and not (allow_nonfinite and rc == 5):
+            _check(rc)
+        return np.array(out[:])
+
+    def synchronize(self):
+        _check(lib().amrlbm_synchronize(self.h))
+
+    def phase(self, name):
+        {"c2": self.fill_ghost_cells_c2, "collide": self.collide, "hg": self.fill_ghost_cells_hg,
+         "stream": self.stream, "macro_h": self.macroscopic_values_h, "c": self.fill_ghost_cells_c,
+         "macro_g": self.macroscopic_values_g, "levels": self.determine_min_max_levels,
+         "regrid": self.refine_derefine_mesh}[name]()
+
+
+# ---- per-block kernels on host arrays (reference layouts) ---------------
+def collision_block(workload, dx, C, P, Ux, Uy, h, g):
+    """Collision_h_g_AMR on one block; h, g are updated in place (Fortran order)."""
+    p = params_from_workload(workload)
+    _check(lib().amrlbm_collision_block(ctypes.byref(p), int(dx), _d(C), _d(P), _d(Ux), _d(Uy), _d(h), _d(g)))
+
+
+def macro_g_block(workload, dx, g, C, P, Ux, Uy):
+    p = params_from_workload(workload)
+    _check(lib().amrlbm_macro_g_block(ctypes.byref(p), int(dx), _d(g), _d(C), _d(P), _d(Ux), _d(Uy)))
+
+
+def stream_block(workload, dx, f):
+    p = params_from_workload(workload)
+    _check(lib().amrlbm_stream_block(ctypes.byref(p), int(dx), _d(f)))

@@ -1,0 +1,63 @@
+// Internal declarations shared by the kernels and the host runtime of
+// libamrlbm.so. Not part of the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "amrlbm.h"
+
+namespace amrlbm {
+
+// Block geometry of this build: 16 x 16 cells (the workloads' blocks).
+constexpr int NX = 16;
+constexpr int NY = 16;
+constexpr int FW = NX + 2;            // population row width with the 1-cell ghost ring
+constexpr int FS = FW * (NY + 2);     // 324 cells per population per block
+constexpr int POPB = 2 * 9 * FS;      // doubles per block per population buffer: h(9) then g(9)
+constexpr int CW = NX + 4;            // C row width with 2 ghost rings
+constexpr int CS = CW * (NY + 4);     // 400
+constexpr int PS = NX * NY;           // 256
+constexpr int PUB = 3 * PS;           // P, Ux, Uy
+
+// Physics constants of D2Q9_Mod.f90 plus wc (D2Q9_Mod.f90:11).
+struct Phys {
+    double M0, wc, W, sigma, beta, kappa, taul, tauh, rhol, rhoh, gy;
+};
+
+// Per-slot block descriptor on the device (slot = reference id - 1).
+struct BlockDev {
+    int level;    // -1 for free slots
+    int dx;       // cell size in finest cells
+    int I, J;     // block coordinates at its level
+    int isleaf;
+    int child0;   // slot of the first child when refined, else -1
+    int pad0, pad1;
+};
+
+// Device view of a mesh, passed by value to the kernels.
+struct DevMesh {
+    double *pop[2];          // [slot][set 0:h 1:g][q][FS], ping-pong buffers
+    double *C;               // [slot][CS], C(-1:nx+2,-1:ny+2)
+    double *PU;              // [slot][3][PS]
+    const BlockDev *blk;     // [slot]
+    const int *nb;           // [slot][8] resolved nghbr(1:8)%n: reference id (>0) or -k (wall)
+    int *err_flags;          // [0]: bad level jump seen by a ghost fill
+};
+
+// kernel launchers (amrlbm_kernels.hip); all enqueue on stream s
+enum FillOp { OP_NONE = 0, OP_COLLIDE = 1, OP_MACRO_G = 2 };
+void launch_fill_c(const DevMesh &m, int cur, const int *list, int n, int rings, int op, const Phys &ph,
+                   bool do_fill, hipStream_t s);
+void launch_fill_hg(const DevMesh &m, int cur, const int *list, int n, bool stream_after, bool macro_h,
+                    bool do_fill, hipStream_t s, bool copy_ghosts = false);
+void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream_t s);
+void launch_err(const DevMesh &m, const int *list, int n, double *err, hipStream_t s);
+void launch_restrict(const DevMesh &m, int cur, const int *parents, int n, hipStream_t s);
+void launch_prolong(const DevMesh &m, int cur, const int *pairs, int n, const Phys &ph, hipStream_t s);
+void launch_init_fields(const DevMesh &m, int cur, const int *list, int n, const Phys &ph, double R,
+                        hipStream_t s);
+void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, double *partial, double *out,
+                 hipStream_t s);
+
+}  // namespace amrlbm

@@ -1,0 +1,765 @@
+// gfx950 kernels of the two-phase AMR-LBM step, FP64.
+//
+// Layout (one slot per reference block id, slot = id - 1):
+//   pop[b][slot][set][q][y][x]   x,y in 0..17 (1-cell ghost ring), set 0 = h, 1 = g
+//   C[slot][y+1][x+1]            x,y in -1..18 (2 ghost rings)
+//   PU[slot][k][j-1][i-1]        k = P, Ux, Uy
+// One workgroup of 256 threads owns one 16 x 16 block: thread t <-> cell
+// (t % 16 + 1, t / 16 + 1), so each population row is one 128-byte run.
+//
+// Arithmetic follows the reference expression by expression (left-to-right
+// sums, integer weights converted before the multiply, divisions kept as
+// divisions) and the library is built with -ffp-contract=off, so results
+// match the reference build to the last bit wherever the GPU's IEEE division
+// and square root round as the host's do.
+#include "amrlbm_internal.h"
+
+namespace amrlbm {
+
+__constant__ int c_ex[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+__constant__ int c_ey[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+// opposite population (BC_Cell.f90:15, J = (i-2) + 4 MOD(i/2,2)), 0-based
+__constant__ int c_opp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+__constant__ int c_dsx[8] = {-1, 1, 0, 0, -1, -1, 1, 1};
+__constant__ int c_dsy[8] = {0, 0, -1, 1, -1, 1, -1, 1};
+
+// lattice weights Wa = [16,4,4,4,4,1,1,1,1]/36 (D2Q9_Mod.f90:8)
+__device__ __forceinline__ double wa(int q) { return q == 0 ? 16.0 / 36.0 : (q < 5 ? 4.0 / 36.0 : 1.0 / 36.0); }
+
+__device__ __forceinline__ int fidx(int set, int q, int x, int y) { return (set * 9 + q) * FS + y * FW + x; }
+__device__ __forceinline__ int cidx(int x, int y) { return (y + 1) * CW + (x + 1); }
+__device__ __forceinline__ int pidx(int i, int j) { return (i - 1) + NX * (j - 1); }
+
+// ---------------------------------------------------------------------------
+// Cell-level numerics (LBM_Subroutines.f90)
+// ---------------------------------------------------------------------------
+
+// Laplacian_AMR (LBM_Subroutines.f90:316-333) + Chemical_Potential_AMR (:94-112)
+__device__ __forceinline__ double chem_pot(const double *sC, int i, int j, int dx, const Phys &ph)
+{
+    double d2c = (sC[cidx(i - 1, j - 1)] + sC[cidx(i + 1, j - 1)] + sC[cidx(i - 1, j + 1)] + sC[cidx(i + 1, j + 1)] +
+                  4.0 * (sC[cidx(i, j - 1)] + sC[cidx(i - 1, j)] + sC[cidx(i + 1, j)] + sC[cidx(i, j + 1)]) -
+                  20.0 * sC[cidx(i, j)]) /
+                 6.0;
+    double c = sC[cidx(i, j)];
+    return 4.0 * ph.beta * c * (c - 1.0) * (c - 0.5) - ph.kappa * d2c / (double)(dx * dx);
+}
+
+// Gradient_ED_AMR (LBM_Subroutines.f90:281-289)
+__device__ __forceinline__ void gradient_ed(const double e[9], double &gx, double &gy)
+{
+    gx = (e[1] - e[3]) / 3.0 + (e[5] - e[6] - e[7] + e[8]) / 12.0;
+    gy = (e[2] - e[4]) / 3.0 + (e[5] + e[6] - e[7] - e[8]) / 12.0;
+}
+
+// ED_C_AMR (LBM_Subroutines.f90:244-258)
+__device__ __forceinline__ void ed_c(const double *sC, int i, int j, int dx, double e[9])
+{
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+        e[k] = (sC[cidx(i + c_ex[k], j + c_ey[k])] - sC[cidx(i - c_ex[k], j - c_ey[k])]) * 0.5 / (double)dx;
+}
+
+// ED_M_AMR (LBM_Subroutines.f90:262-277)
+__device__ __forceinline__ void ed_m(const double *sC, int i, int j, int dx, double e[9])
+{
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+        e[k] = (-sC[cidx(i - c_ex[k], j - c_ey[k])] - 3.0 * sC[cidx(i, j)] + 5.0 * sC[cidx(i + c_ex[k], j + c_ey[k])] -
+                sC[cidx(i + 2 * c_ex[k], j + 2 * c_ey[k])]) *
+               0.25 / (double)dx;
+}
+
+// Equilibrium_h_g_AMR (LBM_Subroutines.f90:2-21)
+__device__ __forceinline__ void equilibrium(double C, double U, double V, double ni, double nj, const Phys &ph,
+                                            double Gamma[9], double heq[9])
+{
+    double tmp = ((12.0 * (1.0 - C)) / ph.W) * ph.M0;
+    double U2 = U * U + V * V;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        double eU = (double)c_ex[i] * U + (double)c_ey[i] * V;
+        double eN = (double)c_ex[i] * ni + (double)c_ey[i] * nj;
+        Gamma[i] = wa(i) * ((1.0 + eU * (3.0 + 4.5 * eU)) - 1.5 * U2);
+        heq[i] = C * (Gamma[i] + tmp * wa(i) * eN);
+    }
+}
+
+// hlp of LBM_Subroutines.f90:154-155 and :164-165
+__device__ __forceinline__ void forcing(const double Gamma[9], double dRho3, double mu, const double e[9], double Ux,
+                                        double Uy, double gx, double gy, double Fb, double hlp[9])
+{
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+        hlp[i] = ((Gamma[i] - wa(i)) * dRho3 + Gamma[i] * mu) * (e[i] - (Ux * gx + Uy * gy)) +
+                 ((double)c_ey[i] - Uy) * Gamma[i] * Fb;
+}
+
+// MRT_Collision_AMR (LBM_Subroutines.f90:348-361) with Convert_D2Q9_standard
+// (:365-380) and Reconvt_D2Q9_standard (:384-401)
+__device__ __forceinline__ void mrt(double s9, const double geq[9], double g[9])
+{
+    double f[9], o[9], t[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) f[i] = g[i] - geq[i];
+    o[0] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+    o[1] = -4.0 * f[0] - f[1] - f[2] - f[3] - f[4] + 2.0 * (f[5] + f[6] + f[7] + f[8]);
+    o[2] = 4.0 * f[0] - 2.0 * (f[1] + f[2] + f[3] + f[4]) + f[5] + f[6] + f[7] + f[8];
+    o[3] = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
+    o[4] = -2.0 * (f[1] - f[3]) + f[5] - f[6] - f[7] + f[8];
+    o[5] = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
+    o[6] = -2.0 * (f[2] - f[4]) + f[5] + f[6] - f[7] - f[8];
+    o[7] = f[1] - f[2] + f[3] - f[4];
+    o[8] = f[5] - f[6] + f[7] - f[8];
+#pragma unroll
+    for (int i = 0; i < 7; i++) t[i] = o[i] * 1.0;
+    t[7] = o[7] * s9;
+    t[8] = o[8] * s9;
+    o[0] = 4.0 * (t[0] - t[1] + t[2]);
+    o[1] = 4.0 * t[0] - t[1] - 2.0 * t[2] + 6.0 * (t[3] - t[4]) + 9.0 * t[7];
+    o[2] = 4.0 * t[0] - t[1] - 2.0 * t[2] + 6.0 * (t[5] - t[6]) - 9.0 * t[7];
+    o[3] = 4.0 * t[0] - t[1] - 2.0 * t[2] - 6.0 * (t[3] - t[4]) + 9.0 * t[7];
+    o[4] = 4.0 * t[0] - t[1] - 2.0 * t[2] - 6.0 * (t[5] - t[6]) - 9.0 * t[7];
+    o[5] = 4.0 * t[0] + 2.0 * t[1] + t[2] + 6.0 * t[3] + 3.0 * t[4] + 6.0 * t[5] + 3.0 * t[6] + 9.0 * t[8];
+    o[6] = 4.0 * t[0] + 2.0 * t[1] + t[2] - 6.0 * t[3] - 3.0 * t[4] + 6.0 * t[5] + 3.0 * t[6] - 9.0 * t[8];
+    o[7] = 4.0 * t[0] + 2.0 * t[1] + t[2] - 6.0 * t[3] - 3.0 * t[4] - 6.0 * t[5] - 3.0 * t[6] + 9.0 * t[8];
+    o[8] = 4.0 * t[0] + 2.0 * t[1] + t[2] + 6.0 * t[3] + 3.0 * t[4] - 6.0 * t[5] - 3.0 * t[6] - 9.0 * t[8];
+#pragma unroll
+    for (int i = 0; i < 9; i++) g[i] = g[i] - o[i] / 36.0;
+}
+
+// Collision_h_g_AMR (LBM_Subroutines.f90:116-189) for cell (i,j); h, g in registers
+__device__ __forceinline__ void collide_cell(const double *sC, int i, int j, int dx, double P, double Ux, double Uy,
+                                             const Phys &ph, double h[9], double g[9])
+{
+    const double dRho3 = (ph.rhoh - ph.rhol) / 3.0;
+    double e[9], hlp[9], Gamma[9], heq[9], geq[9], gx, gy, ni, nj;
+    double mu = chem_pot(sC, i, j, dx, ph);
+    double c = sC[cidx(i, j)];
+    ed_c(sC, i, j, dx, e);
+    gradient_ed(e, gx, gy);
+    double tmp = sqrt(gx * gx + gy * gy + 1.0e-32);   // normal_AMR (:25-37)
+    ni = gx / tmp;
+    nj = gy / tmp;
+    equilibrium(c, Ux, Uy, ni, nj, ph, Gamma, heq);
+#pragma unroll
+    for (int k = 0; k < 9; k++) h[k] = h[k] * (1.0 - ph.wc) + heq[k] * ph.wc;
+    double Rho = ph.rhol + c * (ph.rhoh - ph.rhol);
+    double Fb = (ph.rhoh - Rho) * ph.gy;
+    forcing(Gamma, dRho3, mu, e, Ux, Uy, gx, gy, Fb, hlp);
+#pragma unroll
+    for (int k = 0; k < 9; k++) geq[k] = P * wa(k) + (Gamma[k] - wa(k)) * Rho / 3.0 - 0.5 * hlp[k];
+    ed_m(sC, i, j, dx, e);
+    gradient_ed(e, gx, gy);
+    forcing(Gamma, dRho3, mu, e, Ux, Uy, gx, gy, Fb, hlp);
+    double tau;
+    if (c < 0.0)
+        tau = ph.taul + 0.5;
+    else if (c > 1.0)
+        tau = ph.tauh + 0.5;
+    else {
+        tau = c * (1.0 / ph.tauh - 1.0 / ph.taul) + 1.0 / ph.taul;
+        tau = 1.0 / tau + 0.5;
+    }
+    mrt(1.0 / tau, geq, g);
+#pragma unroll
+    for (int k = 0; k < 9; k++) g[k] = g[k] + hlp[k];
+}
+
+// Macroscopic_Values_g_AMR (LBM_Subroutines.f90:205-240) with Gradient_AMR (:293-312)
+__device__ __forceinline__ void macro_g_cell(const double *sC, int i, int j, int dx, const double g[9], const Phys &ph,
+                                             double &P, double &Ux, double &Uy)
+{
+    const double dRho6 = (ph.rhoh - ph.rhol) / 6.0;
+    double gx = ((sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)]) / 3.0 +
+                 (sC[cidx(i + 1, j + 1)] + sC[cidx(i + 1, j - 1)] - sC[cidx(i - 1, j + 1)] - sC[cidx(i - 1, j - 1)]) /
+                     12.0) /
+                (double)dx;
+    double gy = ((sC[cidx(i, j + 1)] - sC[cidx(i, j - 1)]) / 3.0 +
+                 (sC[cidx(i + 1, j + 1)] + sC[cidx(i - 1, j + 1)] - sC[cidx(i + 1, j - 1)] - sC[cidx(i - 1, j - 1)]) /
+                     12.0) /
+                (double)dx;
+    double mu = chem_pot(sC, i, j, dx, ph);
+    double Rho = ph.rhol + sC[cidx(i, j)] * (ph.rhoh - ph.rhol);
+    double Fb = (ph.rhoh - Rho) * ph.gy;
+    double Fx = mu * gx;
+    double Fy = mu * gy + Fb;
+    double sx = 0.0, sy = 0.0, s = 0.0;
+#pragma unroll
+    for (int k = 1; k < 9; k++) sx += g[k] * (double)c_ex[k];
+#pragma unroll
+    for (int k = 1; k < 9; k++) sy += g[k] * (double)c_ey[k];
+#pragma unroll
+    for (int k = 0; k < 9; k++) s += g[k];
+    Ux = (3.0 * sx + 0.5 * Fx) / Rho;
+    Uy = (3.0 * sy + 0.5 * Fy) / Rho;
+    P = s + dRho6 * (Ux * gx + Uy * gy);
+}
+
+// ---------------------------------------------------------------------------
+// Ghost fills (Fill_Ghost_Cells_{C,C2,hg}.f90, Interpolations.f90:2-138)
+// ---------------------------------------------------------------------------
+// Ghost cell t of a ring-R block: sides L,R,B,T (16R cells each, layer-major),
+// then the corners LB,LT,RB,RT (R x R each). d is the neighbour direction
+// 0..7 = nghbr(1..8): L R B T LB LT RB RT (AMR_Rising_2D_OMP.f90:97-101).
+template <int R>
+__device__ __forceinline__ void ghost_cell(int t, int &d, int &x, int &y)
+{
+    constexpr int SIDE = 16 * R;
+    if (t < 4 * SIDE) {
+        d = t / SIDE;
+        int k = t - d * SIDE, l = k / 16, pos = k % 16 + 1;
+        x = d == 0 ? -l : (d == 1 ? 17 + l : pos);
+        y = d == 2 ? -l : (d == 3 ? 17 + l : pos);
+    } else {
+        int k = t - 4 * SIDE, c = k / (R * R), r = k % (R * R), lx = r % R, ly = r / R;
+        d = 4 + c;
+        x = c < 2 ? -lx : 17 + lx;
+        y = (c == 0 || c == 2) ? -ly : 17 + ly;
+    }
+}
+
+// One axis of Biquadratic_from_Fine as the directional fills use it: three
+// source cells in the 32-wide fine frame of the refined neighbour, listed from
+// the interface outward (normal axis) or upward (tangential axis), with the
+// weights (3,6,-1) or (-1,6,3) by the parity of the coarse index
+// (Interpolations.f90:13-31; Fill_Left_hg, Fill_Ghost_Cells_hg.f90:74-85).
+__device__ __forceinline__ void fine_axis(int s, int t, int f[3], int w[3])
+{
+    int k = s == 0 ? t : (s < 0 ? 1 - t : t - NX);
+    int m0;
+    if (k & 1) {
+        m0 = 2 * k - 1;
+        w[0] = 3; w[1] = 6; w[2] = -1;
+    } else {
+        m0 = 2 * k - 2;
+        w[0] = -1; w[1] = 6; w[2] = 3;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) f[a] = s < 0 ? 2 * NX + 1 - (m0 + a) : m0 + a;
+}
+
+// One axis of Biquadratic_from_Coarse: three coarse cells of the coarser
+// neighbour around the fine cell, the first lying toward the block being
+// filled on a normal axis (Fill_Left_hg:91-96, Fill_Left_Bottom_hg:311-315),
+// weights (5,30,-3) on the side the fine cell leans to (Interpolations.f90:85-101).
+__device__ __forceinline__ void coarse_axis(int s, int t, int pos, int c[3], int w[3])
+{
+    int fp = pos * NX + t;
+    int cp = (fp + 1) >> 1;
+    int sn = cp < 1 ? -1 : (cp > NX ? 1 : 0);
+    int cc = cp - sn * NX;
+    bool upper = (fp & 1) == 0;
+    if (s < 0) {
+        c[0] = cc + 1; c[1] = cc; c[2] = cc - 1;
+        if (upper) { w[0] = 5; w[1] = 30; w[2] = -3; } else { w[0] = -3; w[1] = 30; w[2] = 5; }
+    } else {
+        c[0] = cc - 1; c[1] = cc; c[2] = cc + 1;
+        if (upper) { w[0] = -3; w[1] = 30; w[2] = 5; } else { w[0] = 5; w[1] = 30; w[2] = -3; }
+    }
+}
+
+// value of ghost cell (x,y) in direction d from neighbour id N1; get(slot,x,y)
+// reads the field. Returns false for an unsupported level jump.
+template <class Get>
+__device__ __forceinline__ bool ghost_value(const DevMesh &m, const BlockDev &me, int d, int N1, int x, int y, Get get,
+                                            double &v)
+{
+    const int sx = c_dsx[d], sy = c_dsy[d];
+    const int ns = N1 - 1;
+    const BlockDev nbk = m.blk[ns];
+    if (nbk.level == me.level) {
+        if (nbk.isleaf) {   // case (i): copy (Fill_Left_hg:66-70)
+            v = get(ns, x - sx * NX, y - sy * NY);
+            return true;
+        }
+        // case (ii): biquadratic from the neighbour's children (Fill_Left_hg:72-85)
+        int fx[3], wx[3], fy[3], wy[3];
+        fine_axis(sx, x, fx, wx);
+        fine_axis(sy, y, fy, wy);
+        double t[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            int ix = fx[a] > NX, lx = fx[a] - ix * NX;
+            double u[3];
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                int iy = fy[b] > NY, ly = fy[b] - iy * NY;
+                u[b] = get(nbk.child0 + ix + 2 * iy, lx, ly);
+            }
+            t[a] = (double)wy[0] * u[0] + (double)wy[1] * u[1] + (double)wy[2] * u[2];
+        }
+        v = ((double)wx[0] * t[0] + (double)wx[1] * t[1] + (double)wx[2] * t[2]) / 64.0;
+        return true;
+    }
+    if (me.level == nbk.level + 1) {
+        // case (iii): biquadratic from the coarser neighbour, its ghosts included (Fill_Left_hg:89-99)
+        int cx[3], wx[3], cy[3], wy[3];
+        coarse_axis(sx, x, me.I & 1, cx, wx);
+        coarse_axis(sy, y, me.J & 1, cy, wy);
+        double t[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            t[a] = (double)wy[0] * get(ns, cx[a], cy[0]) + (double)wy[1] * get(ns, cx[a], cy[1]) +
+                   (double)wy[2] * get(ns, cx[a], cy[2]);
+        v = ((double)wx[0] * t[0] + (double)wx[1] * t[1] + (double)wx[2] * t[2]) / 1024.0;
+        return true;
+    }
+    return false;
+}
+
+// Fill_Ghost_Cells_C (R=1) / Fill_Ghost_Cells_C2 (R=2) for the leaves in
+// list (one level), then optionally the block's collision (after C2) or
+// Macroscopic_Values_g_AMR (after C): both read only this block's C, which
+// is final once its ghosts are filled, and no later fill pass reads what
+// they write.
+template <int R, int OP>
+__global__ __launch_bounds__(256) void k_fill_c(DevMesh m, int cur, const int *__restrict__ list, Phys ph,
+                                                int do_fill)
+{
+    const int slot = list[blockIdx.x];
+    const int tid = threadIdx.x;
+    const BlockDev me = m.blk[slot];
+    double *C = m.C + (size_t)slot * CS;
+    __shared__ int nbv[8];
+    __shared__ double sC[CS];
+    if (tid < 8) nbv[tid] = m.nb[slot * 8 + tid];
+    __syncthreads();
+    if (do_fill) {
+        constexpr int NG = 64 * R + 4 * R * R;
+        auto getC = [&](int s, int x, int y) { return m.C[(size_t)s * CS + cidx(x, y)]; };
+        for (int t = tid; t < NG; t += 256) {
+            int d, x, y;
+            ghost_cell<R>(t, d, x, y);
+            int N1 = nbv[d];
+            if (N1 <= 0) continue;
+            double v;
+            if (ghost_value(m, me, d, N1, x, y, getC, v))
+                C[cidx(x, y)] = v;
+            else
+                atomicOr(m.err_flags, 1);
+        }
+        __syncthreads();
+        // wall mirror: Symmetric_C_3/4 (BC_Cell.f90:46-72), Symmetric_C2_3/4 (:76-104)
+        if (tid < 64) {
+            int top = tid >= 32, X = (tid & 31) - (R - 1);
+            if (nbv[2 + top] < 0 && X <= 17 + (R - 1)) {
+                if (!top) {
+                    if (R == 2) C[cidx(X, -1)] = C[cidx(X, 2)];
+                    C[cidx(X, 0)] = C[cidx(X, 1)];
+                } else {
+                    C[cidx(X, NY + 1)] = C[cidx(X, NY)];
+                    if (R == 2) C[cidx(X, NY + 2)] = C[cidx(X, NY - 1)];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (OP == OP_NONE) return;
+    for (int t = tid; t < CS; t += 256) sC[t] = C[t];
+    __syncthreads();
+    const int i = tid % NX + 1, j = tid / NX + 1;
+    double *F = m.pop[cur] + (size_t)slot * POPB;
+    double *PU = m.PU + (size_t)slot * PUB;
+    if (OP == OP_COLLIDE) {
+        double h[9], g[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            h[q] = F[fidx(0, q, i, j)];
+            g[q] = F[fidx(1, q, i, j)];
+        }
+        collide_cell(sC, i, j, me.dx, PU[pidx(i, j)], PU[PS + pidx(i, j)], PU[2 * PS + pidx(i, j)], ph, h, g);
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            F[fidx(0, q, i, j)] = h[q];
+            F[fidx(1, q, i, j)] = g[q];
+        }
+    } else {
+        double g[9], P, Ux, Uy;
+#pragma unroll
+        for (int q = 0; q < 9; q++) g[q] = F[fidx(1, q, i, j)];
+        macro_g_cell(sC, i, j, me.dx, g, ph, P, Ux, Uy);
+        PU[pidx(i, j)] = P;
+        PU[PS + pidx(i, j)] = Ux;
+        PU[2 * PS + pidx(i, j)] = Uy;
+    }
+}
+
+// Fill_Ghost_Cells_hg for the leaves in list (one level), bounce-back walls
+// (BC_Cell.f90:2-42), then optionally the stream of the same blocks
+// (Perfect_Shift_AMR / Streaming_LaxWendroff_AMR, LBM_Subroutines.f90:41-90)
+// into the other population buffer and C = sum(h) (:193-201). The stream
+// writes only the other buffer, so finer passes still read the
+// post-collision populations and ghosts of this level.
+__global__ __launch_bounds__(256) void k_fill_hg(DevMesh m, int cur, const int *__restrict__ list, int flags)
+{
+    const int slot = list[blockIdx.x];
+    const int tid = threadIdx.x;
+    const BlockDev me = m.blk[slot];
+    const double *Fc = m.pop[cur];
+    double *F = m.pop[cur] + (size_t)slot * POPB;
+    __shared__ int nbv[8];
+    if (tid < 8) nbv[tid] = m.nb[slot * 8 + tid];
+    __syncthreads();
+    if (flags & 1) {
+        for (int t = tid; t < 68 * 18; t += 256) {
+            int comp = t / 68, cell = t - comp * 68;
+            int set = comp / 9, q = comp - set * 9;
+            int d, x, y;
+            ghost_cell<1>(cell, d, x, y);
+            int N1 = nbv[d];
+            if (N1 <= 0) continue;
+            auto getF = [&](int s, int xx, int yy) { return Fc[(size_t)s * POPB + fidx(set, q, xx, yy)]; };
+            double v;
+            if (ghost_value(m, me, d, N1, x, y, getF, v))
+                F[fidx(set, q, x, y)] = v;
+            else
+                atomicOr(m.err_flags, 1);
+        }
+        __syncthreads();
+        // Bounce_Back_3 (bottom, row 0) and Bounce_Back_4 (top, row ny+1)
+        for (int t = tid; t < 2 * 2 * 9 * FW; t += 256) {
+            int top = t / (2 * 9 * FW), r = t - top * (2 * 9 * FW);
+            int set = r / (9 * FW), r2 = r - set * (9 * FW), q = r2 / FW, X = r2 - q * FW;
+            if (nbv[2 + top] >= 0) continue;
+            int Y = top ? NY + 1 : 0, Yi = top ? NY : 1;
+            if (q == 0) {
+                F[fidx(set, 0, X, Y)] = F[fidx(set, 0, X, Yi)];
+            } else {
+                int e = c_ex[q] * c_ey[q];
+                int Xn = top ? X - e : X + e;
+                Xn = Xn < 0 ? 0 : (Xn > NX + 1 ? NX + 1 : Xn);
+                F[fidx(set, q, X, Y)] = F[fidx(set, c_opp[q], Xn, Yi)];
+            }
+        }
+        __syncthreads();
+    }
+    if (!(flags & 2)) return;
+    const int i = tid % NX + 1, j = tid / NX + 1;
+    double *G = m.pop[cur ^ 1] + (size_t)slot * POPB;
+    if (flags & 8) {   // carry the ghost ring along, as the reference's in-place stream leaves it
+        for (int t = tid; t < 68 * 18; t += 256) {
+            int comp = t / 68, cell = t - comp * 68, set = comp / 9, q = comp - set * 9, d, x, y;
+            ghost_cell<1>(cell, d, x, y);
+            G[fidx(set, q, x, y)] = F[fidx(set, q, x, y)];
+        }
+    }
+    const int dx = me.dx;
+    const double cfl = 1.0 / (double)dx;
+    const double a = cfl / 2.0 * (1.0 + cfl), b = 1.0 - cfl * cfl, c = cfl / 2.0 * (1.0 - cfl);
+    double csum = 0.0;
+#pragma unroll
+    for (int set = 0; set < 2; set++) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            double v;
+            if (q == 0)
+                v = F[fidx(set, 0, i, j)];
+            else if (cfl > 0.9)
+                v = F[fidx(set, q, i - c_ex[q], j - c_ey[q])];
+            else
+                v = a * F[fidx(set, q, i - c_ex[q], j - c_ey[q])] + b * F[fidx(set, q, i, j)] -
+                    c * F[fidx(set, q, i + c_ex[q], j + c_ey[q])];
+            G[fidx(set, q, i, j)] = v;
+            if (set == 0) csum += v;
+        }
+    }
+    if (flags & 4) m.C[(size_t)slot * CS + cidx(i, j)] = csum;
+}
+
+// Macroscopic_Values_h_AMR (LBM_Subroutines.f90:193-201)
+__global__ __launch_bounds__(256) void k_macro_h(DevMesh m, int cur, const int *__restrict__ list)
+{
+    const int slot = list[blockIdx.x];
+    const int i = threadIdx.x % NX + 1, j = threadIdx.x / NX + 1;
+    const double *F = m.pop[cur] + (size_t)slot * POPB;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) s += F[fidx(0, q, i, j)];
+    m.C[(size_t)slot * CS + cidx(i, j)] = s;
+}
+
+// Err_Interface (Refine_Derefine.f90:327-352): sqrt(max |C(i+1)-C(i-1)|^2 ...)/(2dx)
+__global__ __launch_bounds__(256) void k_err(DevMesh m, const int *__restrict__ list, double *err)
+{
+    const int slot = list[blockIdx.x];
+    const int i = threadIdx.x % NX + 1, j = threadIdx.x / NX + 1;
+    const double *C = m.C + (size_t)slot * CS;
+    double a = C[cidx(i + 1, j)] - C[cidx(i - 1, j)];
+    double b = C[cidx(i, j + 1)] - C[cidx(i, j - 1)];
+    double dc = a * a + b * b;
+    __shared__ double red[256];
+    red[threadIdx.x] = dc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            double o = red[threadIdx.x + s];
+            if (o > red[threadIdx.x]) red[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) err[blockIdx.x] = sqrt(red[0]) / (2.0 * (double)m.blk[slot].dx);
+}
+
+// Restrict_Bilinear (Restrict.f90:14-65) with Bilinear_from_Fine(_f)
+// (Interpolations.f90:142-194): parent interior = 2x2 mean of the children,
+// parent ghosts zeroed. pairs[2k] = parent slot, pairs[2k+1] = first child slot.
+__global__ __launch_bounds__(256) void k_restrict(DevMesh m, int cur, const int *__restrict__ pairs)
+{
+    const int ps = pairs[2 * blockIdx.x], c0 = pairs[2 * blockIdx.x + 1];
+    const int tid = threadIdx.x;
+    double *F = m.pop[cur] + (size_t)ps * POPB;
+    double *C = m.C + (size_t)ps * CS;
+    double *PU = m.PU + (size_t)ps * PUB;
+    for (int t = tid; t < POPB; t += 256) F[t] = 0.0;
+    for (int t = tid; t < CS; t += 256) C[t] = 0.0;
+    for (int t = tid; t < PUB; t += 256) PU[t] = 0.0;
+    __syncthreads();
+    const int X = tid % NX + 1, Y = tid / NX + 1;
+    const int cs = c0 + (X > NX / 2) + 2 * (Y > NY / 2);
+    const int lx = 2 * X - 1 - NX * (X > NX / 2), ly = 2 * Y - 1 - NY * (Y > NY / 2);   // (2X-1, 2Y-1) in the child
+    auto mean4 = [](double a, double b, double c, double d) {   // in(i,2Y-1), in(i,2Y), in(i+1,2Y-1), in(i+1,2Y)
+        return 0.5 * (0.5 * (a + b) + 0.5 * (c + d));
+    };
+    const double *Fch = m.pop[cur] + (size_t)cs * POPB;
+    for (int comp = 0; comp < 18; comp++) {
+        int set = comp / 9, q = comp % 9;
+        F[fidx(set, q, X, Y)] = mean4(Fch[fidx(set, q, lx, ly)], Fch[fidx(set, q, lx, ly + 1)],
+                                      Fch[fidx(set, q, lx + 1, ly)], Fch[fidx(set, q, lx + 1, ly + 1)]);
+    }
+    const double *Cch = m.C + (size_t)cs * CS;
+    C[cidx(X, Y)] = mean4(Cch[cidx(lx, ly)], Cch[cidx(lx, ly + 1)], Cch[cidx(lx + 1, ly)], Cch[cidx(lx + 1, ly + 1)]);
+    const double *Pch = m.PU + (size_t)cs * PUB;
+    for (int k = 0; k < 3; k++)
+        PU[k * PS + pidx(X, Y)] = mean4(Pch[k * PS + pidx(lx, ly)], Pch[k * PS + pidx(lx, ly + 1)],
+                                        Pch[k * PS + pidx(lx + 1, ly)], Pch[k * PS + pidx(lx + 1, ly + 1)]);
+}
+
+// Prolong_Bilinear (Prolong.f90:12-39): children of a refined block from
+// Bilinear_Interpolate_Extrapolate of h, g (:43-70), Bilinear_Interpolation of
+// C with one ghost ring (:74-101), then U, P by Macroscopic_Values_g_AMR
+// (Fill_Children_Data, :105-130). New children start zeroed
+// (AMR_Rising_2D_OMP.f90:572-577). pairs as in k_restrict.
+__global__ __launch_bounds__(256) void k_prolong(DevMesh m, int cur, const int *__restrict__ pairs, Phys ph)
+{
+    const int ps = pairs[2 * blockIdx.x], c0 = pairs[2 * blockIdx.x + 1];
+    const int tid = threadIdx.x;
+    const double *Fp = m.pop[cur] + (size_t)ps * POPB;
+    const double *Cp = m.C + (size_t)ps * CS;
+    __shared__ double sC[CS];
+    for (int k = 0; k < 4; k++) {
+        double *F = m.pop[cur] + (size_t)(c0 + k) * POPB;
+        for (int t = tid; t < POPB; t += 256) F[t] = 0.0;
+        double *C = m.C + (size_t)(c0 + k) * CS;
+        for (int t = tid; t < CS; t += 256) C[t] = 0.0;
+        double *PU = m.PU + (size_t)(c0 + k) * PUB;
+        for (int t = tid; t < PUB; t += 256) PU[t] = 0.0;
+    }
+    __syncthreads();
+    const int I = tid % NX + 1, J = tid / NX + 1;
+    for (int k = 0; k < 4; k++) {
+        const int i = I + NX * (k & 1), j = J + NY * (k >> 1);   // fine index 1..2n
+        int Xi = i / 2; Xi = Xi > NX - 1 ? NX - 1 : (Xi < 1 ? 1 : Xi);
+        int Yj = j / 2; Yj = Yj > NY - 1 ? NY - 1 : (Yj < 1 ? 1 : Yj);
+        const double ai = (double)(2 * i - 4 * Xi + 1), bi = 4.0 - ai;
+        const double aj = (double)(2 * j - 4 * Yj + 1), bj = 4.0 - aj;
+        double *F = m.pop[cur] + (size_t)(c0 + k) * POPB;
+        for (int comp = 0; comp < 18; comp++) {
+            int set = comp / 9, q = comp % 9;
+            double t0 = bi * Fp[fidx(set, q, Xi, Yj)] + ai * Fp[fidx(set, q, Xi + 1, Yj)];
+            double t1 = bi * Fp[fidx(set, q, Xi, Yj + 1)] + ai * Fp[fidx(set, q, Xi + 1, Yj + 1)];
+            F[fidx(set, q, I, J)] = (bj * t0 + aj * t1) / 16.0;
+        }
+    }
+    // C on (0:nx+1)^2 of each child from the parent's (0:nx+1)^2
+    for (int t = tid; t < 4 * FS; t += 256) {
+        int k = t / FS, r = t - k * FS, Ic = r % FW, Jc = r / FW;
+        int i = Ic + NX * (k & 1), j = Jc + NY * (k >> 1);      // 0..2n+1
+        int Xi = i / 2, Yj = j / 2;
+        double ai = (double)(2 * i - 4 * Xi + 1), bi = 4.0 - ai;
+        double aj = (double)(2 * j - 4 * Yj + 1), bj = 4.0 - aj;
+        double t0 = bi * Cp[cidx(Xi, Yj)] + ai * Cp[cidx(Xi + 1, Yj)];
+        double t1 = bi * Cp[cidx(Xi, Yj + 1)] + ai * Cp[cidx(Xi + 1, Yj + 1)];
+        m.C[(size_t)(c0 + k) * CS + cidx(Ic, Jc)] = (bj * t0 + aj * t1) / 16.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < 4; k++) {
+        const double *Cc = m.C + (size_t)(c0 + k) * CS;
+        for (int t = tid; t < CS; t += 256) sC[t] = Cc[t];
+        __syncthreads();
+        const double *F = m.pop[cur] + (size_t)(c0 + k) * POPB;
+        double g[9], P, Ux, Uy;
+#pragma unroll
+        for (int q = 0; q < 9; q++) g[q] = F[fidx(1, q, I, J)];
+        macro_g_cell(sC, I, J, m.blk[ps].dx / 2, g, ph, P, Ux, Uy);
+        double *PU = m.PU + (size_t)(c0 + k) * PUB;
+        PU[pidx(I, J)] = P;
+        PU[PS + pidx(I, J)] = Ux;
+        PU[2 * PS + pidx(I, J)] = Uy;
+        __syncthreads();
+    }
+}
+
+// Initialize_AMR_Rising_Bubble (AMR_Rising_2D_OMP.f90:186-225): with C set
+// (both rings), U = 0, P = -sigma/R C, h = heq, g = geq with Fb = 0.
+__global__ __launch_bounds__(256) void k_init_fields(DevMesh m, int cur, const int *__restrict__ list, Phys ph,
+                                                     double R)
+{
+    const int slot = list[blockIdx.x];
+    const int tid = threadIdx.x;
+    __shared__ double sC[CS];
+    const double *C = m.C + (size_t)slot * CS;
+    for (int t = tid; t < CS; t += 256) sC[t] = C[t];
+    __syncthreads();
+    const int i = tid % NX + 1, j = tid / NX + 1, dx = m.blk[slot].dx;
+    const double dRho3 = (ph.rhoh - ph.rhol) / 3.0;
+    double c = sC[cidx(i, j)];
+    double P = -(ph.sigma / R * c), Ux = 0.0, Uy = 0.0;
+    double *PU = m.PU + (size_t)slot * PUB;
+    PU[pidx(i, j)] = P;
+    PU[PS + pidx(i, j)] = Ux;
+    PU[2 * PS + pidx(i, j)] = Uy;
+    double mu = chem_pot(sC, i, j, dx, ph);
+    double e[9], hlp[9], Gamma[9], heq[9], gx, gy;
+    ed_c(sC, i, j, dx, e);
+    gradient_ed(e, gx, gy);
+    double tmp = sqrt(gx * gx + gy * gy + 1.0e-32);
+    double ni = gx / tmp, nj = gy / tmp;
+    double Rho = ph.rhol + c * (ph.rhoh - ph.rhol);
+    equilibrium(c, Ux, Uy, ni, nj, ph, Gamma, heq);
+    forcing(Gamma, dRho3, mu, e, Ux, Uy, gx, gy, 0.0, hlp);
+    double *F = m.pop[cur] + (size_t)slot * POPB;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        F[fidx(0, q, i, j)] = heq[q];
+        F[fidx(1, q, i, j)] = P * wa(q) + (Gamma[q] - wa(q)) * Rho / 3.0 - 0.5 * hlp[q];
+    }
+}
+
+// FP64 diagnostics, stage 1: per-leaf partials in a fixed reduction order
+// (min rho, max rho, max|Uy|, mass, area, x-moment, y-moment, Uy-moment, non-finite)
+__global__ __launch_bounds__(256) void k_diag_partial(DevMesh m, const int *__restrict__ list, Phys ph, double *part)
+{
+    const int slot = list[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int i = tid % NX + 1, j = tid / NX + 1;
+    const BlockDev b = m.blk[slot];
+    double c = m.C[(size_t)slot * CS + cidx(i, j)];
+    double uy = m.PU[(size_t)slot * PUB + 2 * PS + pidx(i, j)];
+    double rho = ph.rhol + c * (ph.rhoh - ph.rhol);
+    double dx2 = (double)(b.dx * b.dx);
+    double w = (1.0 - c) * dx2;
+    double xc = (double)(b.I * b.dx * NX) + (double)b.dx * ((double)i - 0.5);
+    double yc = (double)(b.J * b.dx * NY) + (double)b.dx * ((double)j - 0.5);
+    __shared__ double red[9][256];
+    red[0][tid] = rho;
+    red[1][tid] = rho;
+    red[2][tid] = fabs(uy);
+    red[3][tid] = rho * dx2;
+    red[4][tid] = w;
+    red[5][tid] = w * xc;
+    red[6][tid] = w * yc;
+    red[7][tid] = w * uy;
+    red[8][tid] = (fabs(c) <= 1.79e308) ? 0.0 : 1.0;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+            red[0][tid] = fmin(red[0][tid], red[0][tid + s]);
+            red[1][tid] = fmax(red[1][tid], red[1][tid + s]);
+            red[2][tid] = fmax(red[2][tid], red[2][tid + s]);
+#pragma unroll
+            for (int k = 3; k < 9; k++) red[k][tid] = red[k][tid] + red[k][tid + s];
+        }
+        __syncthreads();
+    }
+    if (tid < 9) part[(size_t)blockIdx.x * 9 + tid] = red[tid][0];
+}
+
+// stage 2: one workgroup folds the per-leaf partials in leaf-list order
+__global__ __launch_bounds__(256) void k_diag_final(const double *part, int n, double *out)
+{
+    const int tid = threadIdx.x;
+    __shared__ double red[9][256];
+    double a[9] = {10.0, -10.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = tid; k < n; k += 256) {
+        const double *p = part + (size_t)k * 9;
+        a[0] = fmin(a[0], p[0]);
+        a[1] = fmax(a[1], p[1]);
+        a[2] = fmax(a[2], p[2]);
+        for (int f = 3; f < 9; f++) a[f] += p[f];
+    }
+    for (int f = 0; f < 9; f++) red[f][tid] = a[f];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+            red[0][tid] = fmin(red[0][tid], red[0][tid + s]);
+            red[1][tid] = fmax(red[1][tid], red[1][tid + s]);
+            red[2][tid] = fmax(red[2][tid], red[2][tid + s]);
+            for (int f = 3; f < 9; f++) red[f][tid] = red[f][tid] + red[f][tid + s];
+        }
+        __syncthreads();
+    }
+    if (tid < 9) out[tid] = red[tid][0];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_fill_c(const DevMesh &m, int cur, const int *list, int n, int rings, int op, const Phys &ph, bool do_fill,
+                   hipStream_t s)
+{
+    if (n <= 0) return;
+    dim3 g(n), b(256);
+    int f = do_fill ? 1 : 0;
+    if (rings == 2) {
+        if (op == OP_COLLIDE) hipLaunchKernelGGL((k_fill_c<2, OP_COLLIDE>), g, b, 0, s, m, cur, list, ph, f);
+        else if (op == OP_MACRO_G) hipLaunchKernelGGL((k_fill_c<2, OP_MACRO_G>), g, b, 0, s, m, cur, list, ph, f);
+        else hipLaunchKernelGGL((k_fill_c<2, OP_NONE>), g, b, 0, s, m, cur, list, ph, f);
+    } else {
+        if (op == OP_COLLIDE) hipLaunchKernelGGL((k_fill_c<1, OP_COLLIDE>), g, b, 0, s, m, cur, list, ph, f);
+        else if (op == OP_MACRO_G) hipLaunchKernelGGL((k_fill_c<1, OP_MACRO_G>), g, b, 0, s, m, cur, list, ph, f);
+        else hipLaunchKernelGGL((k_fill_c<1, OP_NONE>), g, b, 0, s, m, cur, list, ph, f);
+    }
+}
+
+void launch_fill_hg(const DevMesh &m, int cur, const int *list, int n, bool stream_after, bool macro_h, bool do_fill,
+                    hipStream_t s, bool copy_ghosts)
+{
+    if (n <= 0) return;
+    int flags = (do_fill ? 1 : 0) | (stream_after ? 2 : 0) | (macro_h ? 4 : 0) | (copy_ghosts ? 8 : 0);
+    hipLaunchKernelGGL(k_fill_hg, dim3(n), dim3(256), 0, s, m, cur, list, flags);
+}
+
+void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_macro_h, dim3(n), dim3(256), 0, s, m, cur, list);
+}
+
+void launch_err(const DevMesh &m, const int *list, int n, double *err, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_err, dim3(n), dim3(256), 0, s, m, list, err);
+}
+
+void launch_restrict(const DevMesh &m, int cur, const int *pairs, int n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_restrict, dim3(n), dim3(256), 0, s, m, cur, pairs);
+}
+
+void launch_prolong(const DevMesh &m, int cur, const int *pairs, int n, const Phys &ph, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_prolong, dim3(n), dim3(256), 0, s, m, cur, pairs, ph);
+}
+
+void launch_init_fields(const DevMesh &m, int cur, const int *list, int n, const Phys &ph, double R, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_init_fields, dim3(n), dim3(256), 0, s, m, cur, list, ph, R);
+}
+
+void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, double *partial, double *out, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_diag_partial, dim3(n), dim3(256), 0, s, m, list, ph, partial);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, partial, n, out);
+}
+
+}  // namespace amrlbm

@@ -1,0 +1,1176 @@
+// Host runtime of libamrlbm.so: mesh topology, regrid bookkeeping, device
+// tables and the C-ABI declared in include/amrlbm.h.
+//
+// Topology model. The reference stores each neighbour as a Fortran pointer
+// into Blk(q)%child(c) of the parent's same-level neighbour q
+// (Neighbors.f90:19-37), so it reads as q's child c once q is refined and as q
+// itself while q is a leaf. Under 2:1 balance that is exactly "the block at
+// the adjacent position on my level if it exists, else the block covering it
+// one level up", which this runtime evaluates from a per-level position map.
+// Wall directions read -k (AMR_Rising_2D_OMP.f90:105, Neighbors.f90:47-50).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "amrlbm_internal.h"
+
+using namespace amrlbm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fail(AMRLBM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+struct Block {
+    int level = -1, dx = 0, I = 0, J = 0, parent = 0;
+    int child[4] = {0, 0, 0, 0};
+    bool has_child = false, isleaf = false, hasleaf = false, refin = false, deref = false;
+};
+
+}  // namespace
+
+struct amrlbm_mesh {
+    amrlbm_params prm{};
+    Phys ph{};
+    std::vector<Block> blk;          // 1-based, size max_blocks + 1
+    int nblks = 0, kk = 0, T = 0, levelmin = 0, levelmax = 0;
+    std::deque<int> freeq;
+    std::vector<int> leaf;           // Array_of_Leaf_Blocks order (ascending id)
+    std::vector<std::vector<int>> at;  // per level: position -> id (0 = none)
+    bool ready = false;
+
+    // device state
+    hipStream_t stream = nullptr;
+    DevMesh dm{};
+    BlockDev *d_blk = nullptr;
+    int *d_nb = nullptr;
+    int *d_err_flags = nullptr;
+    int *d_lists = nullptr;          // leaves grouped by level, then all leaves (ascending)
+    int lists_cap = 0;
+    int *d_pairs = nullptr;          // restrict / prolong work lists
+    int pairs_cap = 0;
+    double *d_err = nullptr;         // Err_Interface per leaf
+    double *d_part = nullptr;        // diagnostics partials
+    double *d_diag = nullptr;
+    double *h_pinned = nullptr;      // staging (err, diagnostics)
+    int cur = 0;
+    std::vector<int> lvl_off;        // offsets into d_lists per level (levels 0..max_levels), + all-leaves
+    std::vector<int> lvl_cnt;
+    int all_off = 0;
+    bool tables_dirty = true;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_step_ms = 0.0;
+
+    // graph of one whole step per population-buffer parity
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_topo = -1;
+    int topo_version = 0;
+
+    int nlev() const { return prm.max_levels + 1; }
+    int width(int L) const { return 1 << L; }
+    int height(int L) const { return prm.nroots << L; }
+    int &cell_at(int L, int I, int J) { return at[L][(size_t)J * width(L) + I]; }
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// topology
+// ---------------------------------------------------------------------------
+int nb_value(amrlbm_mesh *m, int n, int k)   // k = 1..8
+{
+    static const int dsx[8] = {-1, 1, 0, 0, -1, -1, 1, 1};
+    static const int dsy[8] = {0, 0, -1, 1, -1, 1, -1, 1};
+    const Block &b = m->blk[n];
+    int L = b.level;
+    int tJ = b.J + dsy[k - 1];
+    if (tJ < 0 || tJ >= m->height(L)) return -k;
+    int W = m->width(L);
+    int tI = ((b.I + dsx[k - 1]) % W + W) % W;
+    int id = m->cell_at(L, tI, tJ);
+    if (id > 0) return id;
+    if (L == 0) return 0;
+    return m->cell_at(L - 1, tI >> 1, tJ >> 1);   // 0 if absent: 2:1 violated
+}
+
+void place(amrlbm_mesh *m, int n)
+{
+    Block &b = m->blk[n];
+    m->cell_at(b.level, b.I, b.J) = n;
+}
+
+void unplace(amrlbm_mesh *m, int n)
+{
+    Block &b = m->blk[n];
+    if (b.level >= 0 && m->cell_at(b.level, b.I, b.J) == n) m->cell_at(b.level, b.I, b.J) = 0;
+}
+
+// Array_of_Leaf_Blocks (AMR_Rising_2D_OMP.f90:588-610)
+void array_of_leaf_blocks(amrlbm_mesh *m)
+{
+    m->leaf.clear();
+    for (int n = 1; n <= m->nblks; n++)
+        if (m->blk[n].isleaf) m->leaf.push_back(n);
+    m->tables_dirty = true;
+    m->topo_version++;
+}
+
+// Determine_Min_Max_Levels (AMR_Rising_2D_OMP.f90:434-452)
+void determine_min_max_levels(amrlbm_mesh *m)
+{
+    int lmin = 100, lmax = -100;
+    for (int n : m->leaf) {
+        lmin = std::min(lmin, m->blk[n].level);
+        lmax = std::max(lmax, m->blk[n].level);
+    }
+    m->levelmin = lmin;
+    m->levelmax = lmax;
+}
+
+// Add_Child_Blocks (AMR_Rising_2D_OMP.f90:517-584); the neighbour pointers of
+// Find_Neighbors are implied by the position map
+int add_child_blocks(amrlbm_mesh *m, int par, int freeblk)
+{
+    int N = freeblk - 1;
+    if (N + 4 > m->prm.max_blocks)
+        return fail(AMRLBM_ERR_CAPACITY, "number of blocks exceeded MaxBlks (" + std::to_string(m->prm.max_blocks) + ")");
+    Block &P = m->blk[par];
+    for (int i = 0; i < 4; i++) P.child[i] = N + 1 + i;
+    P.isleaf = false;
+    P.hasleaf = true;
+    if (P.parent > 0) m->blk[P.parent].hasleaf = false;
+    for (int J = 0; J <= 1; J++)
+        for (int I = 0; I <= 1; I++) {
+            N++;
+            Block &b = m->blk[N];
+            b.level = P.level + 1;
+            b.dx = P.dx / 2;
+            b.I = 2 * P.I + I;
+            b.J = 2 * P.J + J;
+            b.parent = par;
+            b.has_child = true;
+            for (int k = 0; k < 4; k++) b.child[k] = N;
+            b.isleaf = true;
+            b.hasleaf = false;
+            b.refin = false;
+            b.deref = false;
+            place(m, N);
+        }
+    m->nblks = std::max(m->nblks, N);
+    return AMRLBM_OK;
+}
+
+// Remove_Children (Refine_Derefine.f90:215-258)
+void remove_children(amrlbm_mesh *m, int par)
+{
+    int c1 = m->blk[par].child[0];
+    for (int c = c1; c <= c1 + 3; c++) {
+        unplace(m, c);
+        m->blk[c].level = -1;
+        m->blk[c].isleaf = false;
+        m->blk[c].has_child = false;
+    }
+    Block &P = m->blk[par];
+    for (int k = 0; k < 4; k++) P.child[k] = par;
+    P.isleaf = true;
+    P.hasleaf = false;
+    int gp = P.parent;
+    if (gp > 0) {
+        bool all = true;
+        for (int s = m->blk[gp].child[0]; s <= m->blk[gp].child[3]; s++) all = all && m->blk[s].isleaf;
+        if (all) m->blk[gp].hasleaf = true;
+    }
+    m->freeq.push_back(c1);
+    m->kk += 4;
+}
+
+// ---------------------------------------------------------------------------
+// device tables
+// ---------------------------------------------------------------------------
+int ensure_int_buffer(int **buf, int *cap, int need)
+{
+    if (need <= *cap) return AMRLBM_OK;
+    if (*buf) HIPCHK(hipFree(*buf));
+    int c = std::max(need, 1024);
+    HIPCHK(hipMalloc(buf, sizeof(int) * c));
+    *cap = c;
+    return AMRLBM_OK;
+}
+
+int upload_tables(amrlbm_mesh *m)
+{
+    if (!m->tables_dirty) return AMRLBM_OK;
+    const int nb = m->nblks;
+    std::vector<BlockDev> bd(nb);
+    std::vector<int> nbt((size_t)nb * 8, 0);
+    for (int n = 1; n <= nb; n++) {
+        const Block &b = m->blk[n];
+        BlockDev &d = bd[n - 1];
+        d.level = b.level;
+        d.dx = b.dx;
+        d.I = b.I;
+        d.J = b.J;
+        d.isleaf = b.isleaf ? 1 : 0;
+        d.child0 = (b.level >= 0 && !b.isleaf && b.has_child) ? b.child[0] - 1 : -1;
+        d.pad0 = d.pad1 = 0;
+        if (b.level >= 0)
+            for (int k = 1; k <= 8; k++) {
+                int v = nb_value(m, n, k);
+                if (v == 0) return fail(AMRLBM_ERR_TOPOLOGY, "block " + std::to_string(n) + " has no neighbour " +
+                                                                 std::to_string(k) + " (2:1 balance violated)");
+                nbt[(size_t)(n - 1) * 8 + k - 1] = v;
+            }
+    }
+    // leaves grouped by level, each group ascending; then all leaves
+    const int NL = m->nlev();
+    m->lvl_off.assign(NL + 1, 0);
+    m->lvl_cnt.assign(NL, 0);
+    for (int n : m->leaf) m->lvl_cnt[m->blk[n].level]++;
+    for (int L = 0; L < NL; L++) m->lvl_off[L + 1] = m->lvl_off[L] + m->lvl_cnt[L];
+    std::vector<int> lists(m->lvl_off[NL] + m->leaf.size());
+    std::vector<int> fillpos(m->lvl_off.begin(), m->lvl_off.end() - 1);
+    for (int n : m->leaf) lists[fillpos[m->blk[n].level]++] = n - 1;
+    m->all_off = m->lvl_off[NL];
+    for (size_t k = 0; k < m->leaf.size(); k++) lists[m->all_off + k] = m->leaf[k] - 1;
+    int rc = ensure_int_buffer(&m->d_lists, &m->lists_cap, (int)lists.size());
+    if (rc) return rc;
+    if (nb > 0) {
+        HIPCHK(hipMemcpyAsync(m->d_blk, bd.data(), sizeof(BlockDev) * nb, hipMemcpyHostToDevice, m->stream));
+        HIPCHK(hipMemcpyAsync(m->d_nb, nbt.data(), sizeof(int) * nbt.size(), hipMemcpyHostToDevice, m->stream));
+    }
+    if (!lists.empty())
+        HIPCHK(hipMemcpyAsync(m->d_lists, lists.data(), sizeof(int) * lists.size(), hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));   // host vectors go out of scope
+    m->dm.blk = m->d_blk;
+    m->dm.nb = m->d_nb;
+    m->tables_dirty = false;
+    return AMRLBM_OK;
+}
+
+const int *level_list(amrlbm_mesh *m, int L, int *n)
+{
+    if (L < 0 || L >= m->nlev()) {
+        *n = 0;
+        return nullptr;
+    }
+    *n = m->lvl_cnt[L];
+    return m->d_lists + m->lvl_off[L];
+}
+
+int check_err_flags(amrlbm_mesh *m)
+{
+    int f = 0;
+    HIPCHK(hipMemcpyAsync(&f, m->d_err_flags, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    if (f) {
+        HIPCHK(hipMemsetAsync(m->d_err_flags, 0, sizeof(int), m->stream));
+        return fail(AMRLBM_ERR_TOPOLOGY, "ghost fill met a neighbour more than one level apart");
+    }
+    return AMRLBM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// phases
+// ---------------------------------------------------------------------------
+int phase_fill_c(amrlbm_mesh *m, int rings, int op, bool do_fill)
+{
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    for (int L = m->levelmin; L <= m->levelmax; L++) {
+        int n;
+        const int *lst = level_list(m, L, &n);
+        launch_fill_c(m->dm, m->cur, lst, n, rings, op, m->ph, do_fill, m->stream);
+    }
+    HIPCHK(hipGetLastError());
+    return AMRLBM_OK;
+}
+
+int phase_collide(amrlbm_mesh *m)
+{
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    launch_fill_c(m->dm, m->cur, m->d_lists + m->all_off, (int)m->leaf.size(), 2, OP_COLLIDE, m->ph, false, m->stream);
+    HIPCHK(hipGetLastError());
+    return AMRLBM_OK;
+}
+
+int phase_fill_hg(amrlbm_mesh *m, bool stream_after, bool macro_h)
+{
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    for (int L = m->levelmin; L <= m->levelmax; L++) {
+        int n;
+        const int *lst = level_list(m, L, &n);
+        launch_fill_hg(m->dm, m->cur, lst, n, stream_after, macro_h, true, m->stream);
+    }
+    HIPCHK(hipGetLastError());
+    if (stream_after) m->cur ^= 1;
+    return AMRLBM_OK;
+}
+
+// stream only; the ghost ring travels with the populations so that the state
+// matches the reference's in-place stream cell for cell
+int phase_stream(amrlbm_mesh *m, bool macro_h)
+{
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    launch_fill_hg(m->dm, m->cur, m->d_lists + m->all_off, (int)m->leaf.size(), true, macro_h, false, m->stream, true);
+    HIPCHK(hipGetLastError());
+    m->cur ^= 1;
+    return AMRLBM_OK;
+}
+
+int phase_macro_h(amrlbm_mesh *m)
+{
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    launch_macro_h(m->dm, m->cur, m->d_lists + m->all_off, (int)m->leaf.size(), m->stream);
+    HIPCHK(hipGetLastError());
+    return AMRLBM_OK;
+}
+
+int phase_macro_g(amrlbm_mesh *m)
+{
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    launch_fill_c(m->dm, m->cur, m->d_lists + m->all_off, (int)m->leaf.size(), 1, OP_MACRO_G, m->ph, false, m->stream);
+    HIPCHK(hipGetLastError());
+    return AMRLBM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// regrid: Refine_Derefine_Mesh (Refine_Derefine.f90:2-14)
+// ---------------------------------------------------------------------------
+int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol)
+{
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    const int nleaf = (int)m->leaf.size();
+    // Err_Interface of every leaf on the device (Refine_Derefine.f90:36, :122)
+    launch_err(m->dm, m->d_lists + m->all_off, nleaf, m->d_err, m->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(m->h_pinned, m->d_err, sizeof(double) * nleaf, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    std::vector<double> err(m->nblks + 1, 0.0);
+    for (int k = 0; k < nleaf; k++) err[m->leaf[k]] = m->h_pinned[k];
+
+    const amrlbm_params &p = m->prm;
+    // Flag_for_refinement (Refine_Derefine.f90:18-54)
+    for (int n = 1; n <= m->nblks; n++) {
+        Block &b = m->blk[n];
+        b.refin = false;
+        if (b.isleaf) {
+            if (b.level < p.min_levels) b.refin = true;
+            else if (b.level < p.max_levels && err[n] > refin_tol) b.refin = true;
+        }
+    }
+    // Enforce_1_Level_Jump / Check_Level_Jump (:58-93): least fixed point
+    std::vector<int> work;
+    for (int n = 1; n <= m->nblks; n++)
+        if (m->blk[n].refin) work.push_back(n);
+    while (!work.empty()) {
+        int n = work.back();
+        work.pop_back();
+        for (int k = 1; k <= 8; k++) {
+            int b = nb_value(m, n, k);
+            if (b > 0 && !m->blk[b].refin && m->blk[b].level == m->blk[n].level - 1 && m->blk[b].isleaf) {
+                m->blk[b].refin = true;
+                work.push_back(b);
+            }
+        }
+    }
+    // Flag_for_derefinement (:97-145)
+    for (int n = 1; n <= m->nblks; n++) {
+        Block &b = m->blk[n];
+        b.deref = false;
+        if (!b.hasleaf || b.level < p.min_levels) continue;
+        int c1 = b.child[0];
+        bool any = false;
+        for (int c = c1; c <= c1 + 3; c++) any = any || m->blk[c].refin;
+        if (any) continue;
+        b.deref = true;
+        for (int c = c1; c <= c1 + 3; c++)
+            if (err[c] > deref_tol) {
+                b.deref = false;
+                break;
+            }
+    }
+    // enforce_1_Level_jump_deref_2D / Check_Level_Jump_Deref_2D (:149-194)
+    static const int pair_c1[8] = {2, 1, 3, 1, 4, 2, 3, 1};
+    static const int pair_c2[8] = {4, 3, 4, 2, 4, 2, 3, 1};
+    for (int L = m->levelmax - 1; L >= m->levelmin - 1; L--)
+        for (int n = 1; n <= m->nblks; n++) {
+            if (m->blk[n].level != L || !m->blk[n].deref) continue;
+            for (int k = 1; k <= 8 && m->blk[n].deref; k++) {
+                int b = nb_value(m, n, k);
+                if (b > 0 && !m->blk[b].isleaf) {
+                    int cc[2] = {m->blk[b].child[pair_c1[k - 1] - 1], m->blk[b].child[pair_c2[k - 1] - 1]};
+                    for (int c : cc)
+                        if (m->blk[c].refin || (!m->blk[c].isleaf && !m->blk[c].deref)) {
+                            m->blk[n].deref = false;
+                            break;
+                        }
+                }
+            }
+        }
+    // Derefine (:198-211): restrict on the device, then free the children
+    std::vector<int> rpairs;
+    for (int n = 1; n <= m->nblks; n++)
+        if (m->blk[n].deref) {
+            rpairs.push_back(n - 1);
+            rpairs.push_back(m->blk[n].child[0] - 1);
+            remove_children(m, n);
+        }
+    // Refine (:262-294): ids from the free-list head or Nblks+1
+    std::vector<int> ppairs;
+    for (int L = m->levelmin; L <= m->levelmax; L++) {
+        int nend = m->nblks;
+        for (int n = 1; n <= nend; n++) {
+            if (m->blk[n].level != L || !m->blk[n].refin) continue;
+            int c1;
+            if (!m->freeq.empty()) {
+                c1 = m->freeq.front();
+                rc = add_child_blocks(m, n, c1);
+                if (rc) return rc;
+                m->kk -= 4;
+                m->freeq.pop_front();
+            } else {
+                c1 = m->nblks + 1;
+                rc = add_child_blocks(m, n, c1);
+                if (rc) return rc;
+            }
+            ppairs.push_back(n - 1);
+            ppairs.push_back(c1 - 1);
+        }
+    }
+    // device data movement: restrict first (freed slots may be reused by refine)
+    std::vector<int> pairs(rpairs);
+    pairs.insert(pairs.end(), ppairs.begin(), ppairs.end());
+    if (!pairs.empty()) {
+        rc = ensure_int_buffer(&m->d_pairs, &m->pairs_cap, (int)pairs.size());
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(m->d_pairs, pairs.data(), sizeof(int) * pairs.size(), hipMemcpyHostToDevice, m->stream));
+        // the kernels read the pre-regrid block table (parent dx, children);
+        // it is replaced only after them
+        launch_restrict(m->dm, m->cur, m->d_pairs, (int)rpairs.size() / 2, m->stream);
+        launch_prolong(m->dm, m->cur, m->d_pairs + rpairs.size(), (int)ppairs.size() / 2, m->ph, m->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(m->stream));
+    }
+    array_of_leaf_blocks(m);
+    return upload_tables(m);
+}
+
+// ---------------------------------------------------------------------------
+// layout conversion between the reference's h(9,0:nx+1,0:ny+1) and the
+// device's [set][q][y][x]
+// ---------------------------------------------------------------------------
+void ref_to_dev_pop(const double *ref, double *dev)   // one set
+{
+    for (int y = 0; y < NY + 2; y++)
+        for (int x = 0; x < NX + 2; x++)
+            for (int q = 0; q < 9; q++) dev[q * FS + y * FW + x] = ref[q + 9 * (x + FW * y)];
+}
+
+void dev_to_ref_pop(const double *dev, double *ref)
+{
+    for (int y = 0; y < NY + 2; y++)
+        for (int x = 0; x < NX + 2; x++)
+            for (int q = 0; q < 9; q++) ref[q + 9 * (x + FW * y)] = dev[q * FS + y * FW + x];
+}
+
+int alloc_device(amrlbm_mesh *m)
+{
+    const size_t S = (size_t)m->prm.max_blocks;
+    HIPCHK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    HIPCHK(hipMalloc(&m->dm.pop[0], sizeof(double) * POPB * S));
+    HIPCHK(hipMalloc(&m->dm.pop[1], sizeof(double) * POPB * S));
+    HIPCHK(hipMalloc(&m->dm.C, sizeof(double) * CS * S));
+    HIPCHK(hipMalloc(&m->dm.PU, sizeof(double) * PUB * S));
+    HIPCHK(hipMemsetAsync(m->dm.pop[0], 0, sizeof(double) * POPB * S, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.pop[1], 0, sizeof(double) * POPB * S, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.C, 0, sizeof(double) * CS * S, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.PU, 0, sizeof(double) * PUB * S, m->stream));
+    HIPCHK(hipMalloc(&m->d_blk, sizeof(BlockDev) * S));
+    HIPCHK(hipMalloc(&m->d_nb, sizeof(int) * 8 * S));
+    HIPCHK(hipMalloc(&m->d_err_flags, sizeof(int)));
+    HIPCHK(hipMemsetAsync(m->d_err_flags, 0, sizeof(int), m->stream));
+    HIPCHK(hipMalloc(&m->d_err, sizeof(double) * S));
+    HIPCHK(hipMalloc(&m->d_part, sizeof(double) * 9 * S));
+    HIPCHK(hipMalloc(&m->d_diag, sizeof(double) * 16));
+    HIPCHK(hipHostMalloc(&m->h_pinned, sizeof(double) * std::max<size_t>(S, 16), hipHostMallocDefault));
+    HIPCHK(hipEventCreate(&m->ev0));
+    HIPCHK(hipEventCreate(&m->ev1));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    m->dm.err_flags = m->d_err_flags;
+    m->dm.blk = m->d_blk;
+    m->dm.nb = m->d_nb;
+    return AMRLBM_OK;
+}
+
+void reset_topology(amrlbm_mesh *m)
+{
+    m->blk.assign((size_t)m->prm.max_blocks + 1, Block());
+    m->at.assign(m->nlev(), std::vector<int>());
+    for (int L = 0; L < m->nlev(); L++) m->at[L].assign((size_t)m->width(L) * m->height(L), 0);
+    m->nblks = 0;
+    m->kk = 0;
+    m->T = 0;
+    m->freeq.clear();
+    m->leaf.clear();
+    m->tables_dirty = true;
+    m->topo_version++;
+}
+
+// analytic phase field of Initialize_AMR_Rising_Bubble (AMR_Rising_2D_OMP.f90:174-185)
+void bubble_c(const amrlbm_mesh *m, const Block &b, double R, double X0, double Y0, double *C)
+{
+    const int dx = b.dx;
+    for (int j = -1; j <= NY + 2; j++)
+        for (int i = -1; i <= NX + 2; i++) {
+            double Xc = (double)(b.I * dx * NX) + (double)dx * ((double)i - 0.5);
+            double Yc = (double)(b.J * dx * NY) + (double)dx * ((double)j - 0.5);
+            double Ri = std::sqrt((Xc - X0) * (Xc - X0) + (Yc - Y0) * (Yc - Y0));
+            C[(j + 1) * CW + (i + 1)] = 0.5 + 0.5 * std::tanh(2.0 * (Ri - R) / m->ph.W);
+        }
+}
+
+int init_bubble(amrlbm_mesh *m, double R, double X0, double Y0)
+{
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    const size_t nl = m->leaf.size();
+    std::vector<double> cbuf(nl * CS);
+    for (size_t k = 0; k < nl; k++) bubble_c(m, m->blk[m->leaf[k]], R, X0, Y0, &cbuf[k * CS]);
+    for (size_t k = 0; k < nl; k++)
+        HIPCHK(hipMemcpyAsync(m->dm.C + (size_t)(m->leaf[k] - 1) * CS, &cbuf[k * CS], sizeof(double) * CS,
+                              hipMemcpyHostToDevice, m->stream));
+    launch_init_fields(m->dm, m->cur, m->d_lists + m->all_off, (int)nl, m->ph, R, m->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return AMRLBM_OK;
+}
+
+// one whole step with the fused per-level kernels
+void enqueue_step(amrlbm_mesh *m, int cur)
+{
+    for (int L = m->levelmin; L <= m->levelmax; L++) {
+        int n;
+        const int *lst = level_list(m, L, &n);
+        launch_fill_c(m->dm, cur, lst, n, 2, OP_COLLIDE, m->ph, true, m->stream);
+    }
+    for (int L = m->levelmin; L <= m->levelmax; L++) {
+        int n;
+        const int *lst = level_list(m, L, &n);
+        launch_fill_hg(m->dm, cur, lst, n, true, true, true, m->stream);
+    }
+    for (int L = m->levelmin; L <= m->levelmax; L++) {
+        int n;
+        const int *lst = level_list(m, L, &n);
+        launch_fill_c(m->dm, cur ^ 1, lst, n, 1, OP_MACRO_G, m->ph, true, m->stream);
+    }
+}
+
+int build_graphs(amrlbm_mesh *m)
+{
+    for (int p = 0; p < 2; p++) {
+        if (m->graph[p]) HIPCHK(hipGraphExecDestroy(m->graph[p]));
+        m->graph[p] = nullptr;
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_step(m, p);
+        HIPCHK(hipStreamEndCapture(m->stream, &g));
+        HIPCHK(hipGraphInstantiate(&m->graph[p], g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+    }
+    m->graph_topo = m->topo_version;
+    return AMRLBM_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char *amrlbm_last_error(void) { return g_last_error.c_str(); }
+
+int amrlbm_create(const amrlbm_params *p, amrlbm_mesh **out)
+{
+    if (!p || !out) return fail(AMRLBM_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (p->nx != NX || p->ny != NY) return fail(AMRLBM_ERR_ARG, "this build supports 16 x 16 blocks only");
+    if (p->nroots < 1 || p->min_levels < 0 || p->max_levels < p->min_levels || p->max_levels > 14 ||
+        p->max_blocks < p->nroots || p->L0 != (NX << p->max_levels))
+        return fail(AMRLBM_ERR_ARG, "inconsistent mesh parameters (L0 must be nx * 2^max_levels)");
+    amrlbm_mesh *m = new amrlbm_mesh();
+    m->prm = *p;
+    m->ph.M0 = p->M0;
+    m->ph.wc = 1.0 / (0.5 + p->M0 * 3.0);
+    m->ph.W = p->W;
+    m->ph.sigma = p->sigma;
+    m->ph.beta = p->beta;
+    m->ph.kappa = p->kappa;
+    m->ph.taul = p->taul;
+    m->ph.tauh = p->tauh;
+    m->ph.rhol = p->rhol;
+    m->ph.rhoh = p->rhoh;
+    m->ph.gy = p->gy;
+    int rc = alloc_device(m);
+    if (rc) {
+        amrlbm_destroy(m);
+        return rc;
+    }
+    reset_topology(m);
+    *out = m;
+    return AMRLBM_OK;
+}
+
+int amrlbm_destroy(amrlbm_mesh *m)
+{
+    if (!m) return AMRLBM_OK;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (int p = 0; p < 2; p++)
+        if (m->graph[p]) (void)hipGraphExecDestroy(m->graph[p]);
+    (void)hipFree(m->dm.pop[0]);
+    (void)hipFree(m->dm.pop[1]);
+    (void)hipFree(m->dm.C);
+    (void)hipFree(m->dm.PU);
+    (void)hipFree(m->d_blk);
+    (void)hipFree(m->d_nb);
+    (void)hipFree(m->d_err_flags);
+    (void)hipFree(m->d_lists);
+    (void)hipFree(m->d_pairs);
+    (void)hipFree(m->d_err);
+    (void)hipFree(m->d_part);
+    (void)hipFree(m->d_diag);
+    if (m->h_pinned) (void)hipHostFree(m->h_pinned);
+    if (m->ev0) (void)hipEventDestroy(m->ev0);
+    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return AMRLBM_OK;
+}
+
+int amrlbm_setup_rising_bubble(amrlbm_mesh *m, double R, double x0, double y0)
+{
+    if (!m) return fail(AMRLBM_ERR_ARG, "null mesh");
+    reset_topology(m);
+    m->cur = 0;
+    const amrlbm_params &p = m->prm;
+    m->levelmin = m->levelmax = p.min_levels;
+    // Initialize_Base_Block (AMR_Rising_2D_OMP.f90:79-115), roots stacked in y
+    for (int N = 1; N <= p.nroots; N++) {
+        Block &b = m->blk[N];
+        b.level = 0;
+        b.dx = p.L0 / p.nx;
+        b.I = 0;
+        b.J = N - 1;
+        b.has_child = true;
+        for (int k = 0; k < 4; k++) b.child[k] = -1;
+        b.parent = -1;
+        b.isleaf = true;
+        m->nblks = N;
+        place(m, N);
+    }
+    // Initial_Mesh (AMR_Rising_2D_OMP.f90:493-513); device slots start zeroed
+    HIPCHK(hipMemsetAsync(m->dm.pop[0], 0, sizeof(double) * POPB * p.max_blocks, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.C, 0, sizeof(double) * CS * p.max_blocks, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.PU, 0, sizeof(double) * PUB * p.max_blocks, m->stream));
+    for (int L = 1; L <= m->levelmin; L++) {
+        int nend = m->nblks;
+        for (int n = 1; n <= nend; n++)
+            if (m->blk[n].level == L - 1) {
+                int rc = add_child_blocks(m, n, m->nblks + 1);
+                if (rc) return rc;
+            }
+    }
+    array_of_leaf_blocks(m);
+    int rc = init_bubble(m, R, x0, y0);
+    if (rc) return rc;
+    while (m->levelmax < p.max_levels) {   // AMR_Rising_2D_OMP.f90:63-67
+        rc = regrid(m, 0.002, 0.001);
+        if (rc) return rc;
+        rc = init_bubble(m, R, x0, y0);
+        if (rc) return rc;
+        m->levelmax++;
+    }
+    m->T = 0;
+    determine_min_max_levels(m);
+    m->ready = true;
+    return check_err_flags(m);
+}
+
+int amrlbm_get_counts(amrlbm_mesh *m, int counts[7])
+{
+    if (!m || !counts) return fail(AMRLBM_ERR_ARG, "null argument");
+    counts[0] = m->nblks;
+    counts[1] = (int)m->leaf.size();
+    counts[2] = m->levelmin;
+    counts[3] = m->levelmax;
+    counts[4] = m->kk;
+    counts[5] = (int)m->freeq.size();
+    counts[6] = m->T;
+    return AMRLBM_OK;
+}
+
+int amrlbm_get_topology(amrlbm_mesh *m, int *blocks, int *freeq)
+{
+    if (!m || !blocks) return fail(AMRLBM_ERR_ARG, "null argument");
+    for (int n = 1; n <= m->nblks; n++) {
+        const Block &b = m->blk[n];
+        int *r = blocks + (size_t)(n - 1) * 21;
+        r[0] = b.level; r[1] = b.dx; r[2] = b.I; r[3] = b.J; r[4] = b.parent;
+        r[5] = b.isleaf; r[6] = b.hasleaf; r[7] = b.refin; r[8] = b.deref;
+        for (int k = 0; k < 4; k++) r[9 + k] = (b.level >= 0 && b.has_child) ? b.child[k] : 0;
+        for (int k = 1; k <= 8; k++) r[12 + k] = b.level >= 0 ? nb_value(m, n, k) : 0;
+    }
+    if (freeq) {
+        int k = 0;
+        for (int v : m->freeq) freeq[k++] = v;
+    }
+    return AMRLBM_OK;
+}
+
+int amrlbm_set_topology(amrlbm_mesh *m, int nblks, const int *blocks, int kk, int nfree, const int *freeq, int T)
+{
+    if (!m || !blocks || nblks < 1 || nblks > m->prm.max_blocks || nfree < 0 || (nfree > 0 && !freeq))
+        return fail(AMRLBM_ERR_ARG, "bad topology arguments");
+    reset_topology(m);
+    m->nblks = nblks;
+    for (int n = 1; n <= nblks; n++) {
+        const int *r = blocks + (size_t)(n - 1) * 21;
+        Block &b = m->blk[n];
+        b.level = r[0]; b.dx = r[1]; b.I = r[2]; b.J = r[3]; b.parent = r[4];
+        b.isleaf = r[5]; b.hasleaf = r[6]; b.refin = r[7]; b.deref = r[8];
+        b.has_child = b.level >= 0;
+        for (int k = 0; k < 4; k++) b.child[k] = r[9 + k];
+        if (b.level > m->prm.max_levels) return fail(AMRLBM_ERR_ARG, "level above max_levels");
+        if (b.level >= 0) {
+            if (b.I < 0 || b.I >= m->width(b.level) || b.J < 0 || b.J >= m->height(b.level))
+                return fail(AMRLBM_ERR_ARG, "block position outside the domain");
+            place(m, n);
+        }
+    }
+    for (int n = 1; n <= nblks; n++) {
+        if (m->blk[n].level < 0) continue;
+        for (int k = 1; k <= 8; k++)
+            if (nb_value(m, n, k) != blocks[(size_t)(n - 1) * 21 + 12 + k])
+                return fail(AMRLBM_ERR_TOPOLOGY, "neighbour " + std::to_string(k) + " of block " + std::to_string(n) +
+                                                     " disagrees with the quadtree");
+    }
+    for (int k = 0; k < nfree; k++) m->freeq.push_back(freeq[k]);
+    m->kk = kk;
+    m->T = T;
+    array_of_leaf_blocks(m);
+    determine_min_max_levels(m);
+    m->ready = true;
+    return upload_tables(m);
+}
+
+int amrlbm_put_block(amrlbm_mesh *m, int id, const double *h, const double *g, const double *C, const double *P,
+                     const double *Ux, const double *Uy)
+{
+    if (!m || id < 1 || id > m->prm.max_blocks) return fail(AMRLBM_ERR_ARG, "bad block id");
+    const size_t s = (size_t)(id - 1);
+    std::vector<double> buf(POPB);
+    double *F = m->dm.pop[m->cur] + s * POPB;
+    if (h) {
+        ref_to_dev_pop(h, buf.data());
+        HIPCHK(hipMemcpyAsync(F, buf.data(), sizeof(double) * 9 * FS, hipMemcpyHostToDevice, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+    }
+    if (g) {
+        ref_to_dev_pop(g, buf.data());
+        HIPCHK(hipMemcpyAsync(F + 9 * FS, buf.data(), sizeof(double) * 9 * FS, hipMemcpyHostToDevice, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+    }
+    if (C) HIPCHK(hipMemcpyAsync(m->dm.C + s * CS, C, sizeof(double) * CS, hipMemcpyHostToDevice, m->stream));
+    if (P) HIPCHK(hipMemcpyAsync(m->dm.PU + s * PUB, P, sizeof(double) * PS, hipMemcpyHostToDevice, m->stream));
+    if (Ux) HIPCHK(hipMemcpyAsync(m->dm.PU + s * PUB + PS, Ux, sizeof(double) * PS, hipMemcpyHostToDevice, m->stream));
+    if (Uy) HIPCHK(hipMemcpyAsync(m->dm.PU + s * PUB + 2 * PS, Uy, sizeof(double) * PS, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return AMRLBM_OK;
+}
+
+int amrlbm_get_block(amrlbm_mesh *m, int id, double *h, double *g, double *C, double *P, double *Ux, double *Uy)
+{
+    if (!m || id < 1 || id > m->prm.max_blocks) return fail(AMRLBM_ERR_ARG, "bad block id");
+    const size_t s = (size_t)(id - 1);
+    std::vector<double> buf(POPB);
+    HIPCHK(hipMemcpyAsync(buf.data(), m->dm.pop[m->cur] + s * POPB, sizeof(double) * POPB, hipMemcpyDeviceToHost,
+                          m->stream));
+    if (C) HIPCHK(hipMemcpyAsync(C, m->dm.C + s * CS, sizeof(double) * CS, hipMemcpyDeviceToHost, m->stream));
+    if (P) HIPCHK(hipMemcpyAsync(P, m->dm.PU + s * PUB, sizeof(double) * PS, hipMemcpyDeviceToHost, m->stream));
+    if (Ux) HIPCHK(hipMemcpyAsync(Ux, m->dm.PU + s * PUB + PS, sizeof(double) * PS, hipMemcpyDeviceToHost, m->stream));
+    if (Uy) HIPCHK(hipMemcpyAsync(Uy, m->dm.PU + s * PUB + 2 * PS, sizeof(double) * PS, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    if (h) dev_to_ref_pop(buf.data(), h);
+    if (g) dev_to_ref_pop(buf.data() + 9 * FS, g);
+    return AMRLBM_OK;
+}
+
+int amrlbm_determine_min_max_levels(amrlbm_mesh *m, int *levelmin, int *levelmax)
+{
+    if (!m) return fail(AMRLBM_ERR_ARG, "null mesh");
+    determine_min_max_levels(m);
+    if (levelmin) *levelmin = m->levelmin;
+    if (levelmax) *levelmax = m->levelmax;
+    return AMRLBM_OK;
+}
+
+#define NEED_READY(m)                                                       \
+    do {                                                                    \
+        if (!(m)) return fail(AMRLBM_ERR_ARG, "null mesh");                 \
+        if (!(m)->ready) return fail(AMRLBM_ERR_STATE, "mesh not set up"); \
+    } while (0)
+
+int amrlbm_fill_ghost_cells_c2(amrlbm_mesh *m)
+{
+    NEED_READY(m);
+    int rc = phase_fill_c(m, 2, OP_NONE, true);
+    return rc ? rc : check_err_flags(m);
+}
+
+int amrlbm_collide(amrlbm_mesh *m)
+{
+    NEED_READY(m);
+    return phase_collide(m);
+}
+
+int amrlbm_fill_ghost_cells_hg(amrlbm_mesh *m)
+{
+    NEED_READY(m);
+    int rc = phase_fill_hg(m, false, false);
+    return rc ? rc : check_err_flags(m);
+}
+
+int amrlbm_stream(amrlbm_mesh *m)
+{
+    NEED_READY(m);
+    return phase_stream(m, false);
+}
+
+int amrlbm_phase_field_lbm_h_g(amrlbm_mesh *m, double ms[3])
+{
+    NEED_READY(m);
+    float t[3] = {0, 0, 0};
+    hipEvent_t e[4];
+    for (int k = 0; k < 4; k++) HIPCHK(hipEventCreate(&e[k]));
+    HIPCHK(hipEventRecord(e[0], m->stream));
+    int rc = phase_collide(m);
+    if (!rc) {
+        HIPCHK(hipEventRecord(e[1], m->stream));
+        rc = phase_fill_hg(m, false, false);
+    }
+    if (!rc) {
+        HIPCHK(hipEventRecord(e[2], m->stream));
+        rc = phase_stream(m, false);
+    }
+    if (!rc) {
+        HIPCHK(hipEventRecord(e[3], m->stream));
+        HIPCHK(hipEventSynchronize(e[3]));
+        for (int k = 0; k < 3; k++) HIPCHK(hipEventElapsedTime(&t[k], e[k], e[k + 1]));
+        if (ms)
+            for (int k = 0; k < 3; k++) ms[k] = t[k];
+    }
+    for (int k = 0; k < 4; k++) (void)hipEventDestroy(e[k]);
+    return rc ? rc : check_err_flags(m);
+}
+
+int amrlbm_macroscopic_values_h(amrlbm_mesh *m)
+{
+    NEED_READY(m);
+    return phase_macro_h(m);
+}
+
+int amrlbm_fill_ghost_cells_c(amrlbm_mesh *m)
+{
+    NEED_READY(m);
+    int rc = phase_fill_c(m, 1, OP_NONE, true);
+    return rc ? rc : check_err_flags(m);
+}
+
+int amrlbm_macroscopic_values_g(amrlbm_mesh *m)
+{
+    NEED_READY(m);
+    return phase_macro_g(m);
+}
+
+int amrlbm_refine_derefine_mesh(amrlbm_mesh *m, double refin_tol, double deref_tol)
+{
+    NEED_READY(m);
+    return regrid(m, refin_tol, deref_tol);
+}
+
+int amrlbm_step(amrlbm_mesh *m, int nsteps, int regrid_every)
+{
+    NEED_READY(m);
+    if (nsteps < 0 || regrid_every < 1) return fail(AMRLBM_ERR_ARG, "bad step arguments");
+    int rc = AMRLBM_OK;
+    HIPCHK(hipEventRecord(m->ev0, m->stream));
+    for (int s = 0; s < nsteps; s++) {
+        determine_min_max_levels(m);
+        rc = upload_tables(m);
+        if (rc) return rc;
+        if (m->graph_topo != m->topo_version) {
+            rc = build_graphs(m);
+            if (rc) return rc;
+        }
+        HIPCHK(hipGraphLaunch(m->graph[m->cur], m->stream));
+        m->cur ^= 1;
+        if ((m->T + 1) % regrid_every == 0) {
+            rc = regrid(m, 0.002, 0.001);
+            if (rc) return rc;
+        }
+        m->T++;
+    }
+    determine_min_max_levels(m);
+    HIPCHK(hipEventRecord(m->ev1, m->stream));
+    HIPCHK(hipEventSynchronize(m->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    m->last_step_ms = ms;
+    return check_err_flags(m);
+}
+
+int amrlbm_profile_steps(amrlbm_mesh *m, int nsteps, int regrid_every, double stats[12])
+{
+    NEED_READY(m);
+    if (nsteps < 0 || regrid_every < 1 || !stats) return fail(AMRLBM_ERR_ARG, "bad profile arguments");
+    for (int k = 0; k < 12; k++) stats[k] = 0.0;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;
+    std::vector<int> cells;
+    auto mark = [&](int kd, int nc) -> int {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(e, m->stream));
+        ev.push_back(e);
+        kind.push_back(kd);
+        cells.push_back(nc);
+        return AMRLBM_OK;
+    };
+    int rc = AMRLBM_OK;
+    for (int s = 0; s < nsteps && !rc; s++) {
+        determine_min_max_levels(m);
+        if ((rc = upload_tables(m))) break;
+        ev.clear(); kind.clear(); cells.clear();
+        if ((rc = mark(-1, 0))) break;
+        for (int pass = 0; pass < 3 && !rc; pass++)
+            for (int L = m->levelmin; L <= m->levelmax && !rc; L++) {
+                int n;
+                const int *lst = level_list(m, L, &n);
+                if (pass == 0) launch_fill_c(m->dm, m->cur, lst, n, 2, OP_COLLIDE, m->ph, true, m->stream);
+                else if (pass == 1) launch_fill_hg(m->dm, m->cur, lst, n, true, true, true, m->stream);
+                else launch_fill_c(m->dm, m->cur ^ 1, lst, n, 1, OP_MACRO_G, m->ph, true, m->stream);
+                rc = mark(pass, n * PS);
+            }
+        if (rc) break;
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(m->stream));
+        for (size_t k = 1; k < ev.size(); k++) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ev[k - 1], ev[k]));
+            if (cells[k] == 0) continue;
+            stats[3 * kind[k]] += ms;
+            stats[3 * kind[k] + 1] += 1;
+            stats[3 * kind[k] + 2] += cells[k];
+        }
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        m->cur ^= 1;
+        if ((m->T + 1) % regrid_every == 0) {
+            hipEvent_t a, b;
+            HIPCHK(hipEventCreate(&a));
+            HIPCHK(hipEventCreate(&b));
+            HIPCHK(hipEventRecord(a, m->stream));
+            rc = regrid(m, 0.002, 0.001);
+            HIPCHK(hipEventRecord(b, m->stream));
+            HIPCHK(hipEventSynchronize(b));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, a, b));
+            stats[9] += ms;
+            stats[10] += 1;
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+        m->T++;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    determine_min_max_levels(m);
+    return check_err_flags(m);
+}
+
+int amrlbm_output(amrlbm_mesh *m, double out[9])
+{
+    NEED_READY(m);
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    launch_diag(m->dm, m->d_lists + m->all_off, (int)m->leaf.size(), m->ph, m->d_part, m->d_diag, m->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(m->h_pinned, m->d_diag, sizeof(double) * 9, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    double *d = m->h_pinned;
+    for (int k = 0; k < 9; k++) out[k] = d[k];
+    out[5] = d[5] / d[4];
+    out[6] = d[6] / d[4];
+    out[7] = d[7] / d[4];
+    if (d[8] != 0.0) return fail(AMRLBM_ERR_NONFINITE, "non-finite phase field in " + std::to_string((long)d[8]) + " cells");
+    return AMRLBM_OK;
+}
+
+int amrlbm_synchronize(amrlbm_mesh *m)
+{
+    if (!m) return fail(AMRLBM_ERR_ARG, "null mesh");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return AMRLBM_OK;
+}
+
+int amrlbm_last_step_ms(amrlbm_mesh *m, double *ms)
+{
+    if (!m || !ms) return fail(AMRLBM_ERR_ARG, "null argument");
+    *ms = m->last_step_ms;
+    return AMRLBM_OK;
+}
+
+// ---- per-block kernels on host arrays ------------------------------------
+namespace {
+struct OneBlock {
+    DevMesh dm{};
+    BlockDev *blk = nullptr;
+    int *nb = nullptr, *list = nullptr, *flags = nullptr;
+    hipStream_t s = nullptr;
+    Phys ph{};
+    ~OneBlock()
+    {
+        if (s) (void)hipStreamSynchronize(s);
+        (void)hipFree(dm.pop[0]);
+        (void)hipFree(dm.pop[1]);
+        (void)hipFree(dm.C);
+        (void)hipFree(dm.PU);
+        (void)hipFree(blk);
+        (void)hipFree(nb);
+        (void)hipFree(list);
+        (void)hipFree(flags);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    int init(const amrlbm_params *p, int dx)
+    {
+        if (!p || p->nx != NX || p->ny != NY || dx < 1) return fail(AMRLBM_ERR_ARG, "bad per-block arguments");
+        ph.M0 = p->M0; ph.wc = 1.0 / (0.5 + p->M0 * 3.0); ph.W = p->W; ph.sigma = p->sigma; ph.beta = p->beta;
+        ph.kappa = p->kappa; ph.taul = p->taul; ph.tauh = p->tauh; ph.rhol = p->rhol; ph.rhoh = p->rhoh;
+        ph.gy = p->gy;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        HIPCHK(hipMalloc(&dm.pop[0], sizeof(double) * POPB));
+        HIPCHK(hipMalloc(&dm.pop[1], sizeof(double) * POPB));
+        HIPCHK(hipMalloc(&dm.C, sizeof(double) * CS));
+        HIPCHK(hipMalloc(&dm.PU, sizeof(double) * PUB));
+        HIPCHK(hipMalloc(&blk, sizeof(BlockDev)));
+        HIPCHK(hipMalloc(&nb, sizeof(int) * 8));
+        HIPCHK(hipMalloc(&list, sizeof(int)));
+        HIPCHK(hipMalloc(&flags, sizeof(int)));
+        HIPCHK(hipMemsetAsync(dm.pop[0], 0, sizeof(double) * POPB, s));
+        HIPCHK(hipMemsetAsync(dm.pop[1], 0, sizeof(double) * POPB, s));
+        HIPCHK(hipMemsetAsync(dm.C, 0, sizeof(double) * CS, s));
+        HIPCHK(hipMemsetAsync(dm.PU, 0, sizeof(double) * PUB, s));
+        HIPCHK(hipMemsetAsync(list, 0, sizeof(int), s));
+        HIPCHK(hipMemsetAsync(flags, 0, sizeof(int), s));
+        BlockDev b{0, dx, 0, 0, 1, -1, 0, 0};
+        int nbh[8] = {-1, -2, -3, -4, -5, -6, -7, -8};
+        HIPCHK(hipMemcpyAsync(blk, &b, sizeof(b), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(nb, nbh, sizeof(nbh), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        dm.blk = blk;
+        dm.nb = nb;
+        dm.err_flags = flags;
+        return AMRLBM_OK;
+    }
+    int put_pop(int set, const double *ref)
+    {
+        std::vector<double> buf(9 * FS);
+        ref_to_dev_pop(ref, buf.data());
+        HIPCHK(hipMemcpyAsync(dm.pop[0] + set * 9 * FS, buf.data(), sizeof(double) * 9 * FS, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return AMRLBM_OK;
+    }
+    int get_pop(int b, int set, double *ref)
+    {
+        std::vector<double> buf(9 * FS);
+        HIPCHK(hipMemcpyAsync(buf.data(), dm.pop[b] + set * 9 * FS, sizeof(double) * 9 * FS, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        dev_to_ref_pop(buf.data(), ref);
+        return AMRLBM_OK;
+    }
+};
+}  // namespace
+
+int amrlbm_collision_block(const amrlbm_params *p, int dx, const double *C, const double *P, const double *Ux,
+                           const double *Uy, double *h, double *g)
+{
+    if (!C || !P || !Ux || !Uy || !h || !g) return fail(AMRLBM_ERR_ARG, "null argument");
+    OneBlock o;
+    int rc = o.init(p, dx);
+    if (rc) return rc;
+    if ((rc = o.put_pop(0, h)) || (rc = o.put_pop(1, g))) return rc;
+    HIPCHK(hipMemcpyAsync(o.dm.C, C, sizeof(double) * CS, hipMemcpyHostToDevice, o.s));
+    HIPCHK(hipMemcpyAsync(o.dm.PU, P, sizeof(double) * PS, hipMemcpyHostToDevice, o.s));
+    HIPCHK(hipMemcpyAsync(o.dm.PU + PS, Ux, sizeof(double) * PS, hipMemcpyHostToDevice, o.s));
+    HIPCHK(hipMemcpyAsync(o.dm.PU + 2 * PS, Uy, sizeof(double) * PS, hipMemcpyHostToDevice, o.s));
+    launch_fill_c(o.dm, 0, o.list, 1, 2, OP_COLLIDE, o.ph, false, o.s);
+    HIPCHK(hipGetLastError());
+    if ((rc = o.get_pop(0, 0, h)) || (rc = o.get_pop(0, 1, g))) return rc;
+    return AMRLBM_OK;
+}
+
+int amrlbm_macro_g_block(const amrlbm_params *p, int dx, const double *g, const double *C, double *P, double *Ux,
+                         double *Uy)
+{
+    if (!g || !C || !P || !Ux || !Uy) return fail(AMRLBM_ERR_ARG, "null argument");
+    OneBlock o;
+    int rc = o.init(p, dx);
+    if (rc) return rc;
+    if ((rc = o.put_pop(1, g))) return rc;
+    HIPCHK(hipMemcpyAsync(o.dm.C, C, sizeof(double) * CS, hipMemcpyHostToDevice, o.s));
+    launch_fill_c(o.dm, 0, o.list, 1, 1, OP_MACRO_G, o.ph, false, o.s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(P, o.dm.PU, sizeof(double) * PS, hipMemcpyDeviceToHost, o.s));
+    HIPCHK(hipMemcpyAsync(Ux, o.dm.PU + PS, sizeof(double) * PS, hipMemcpyDeviceToHost, o.s));
+    HIPCHK(hipMemcpyAsync(Uy, o.dm.PU + 2 * PS, sizeof(double) * PS, hipMemcpyDeviceToHost, o.s));
+    HIPCHK(hipStreamSynchronize(o.s));
+    return AMRLBM_OK;
+}
+
+int amrlbm_stream_block(const amrlbm_params *p, int dx, double *f)
+{
+    if (!f) return fail(AMRLBM_ERR_ARG, "null argument");
+    OneBlock o;
+    int rc = o.init(p, dx);
+    if (rc) return rc;
+    if ((rc = o.put_pop(0, f))) return rc;
+    launch_fill_hg(o.dm, 0, o.list, 1, true, false, false, o.s, true);
+    HIPCHK(hipGetLastError());
+    return o.get_pop(1, 0, f);
+}
+
+}  // extern "C"

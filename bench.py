@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Throughput of the two-phase AMR-LBM step on one MI355X.
+
+Workload: the 2D rising bubble on a 1024 x 2048 finest-equivalent lattice,
+16 x 16-cell blocks, levels 3..6 (dx = 8, 4, 2, 1), D = 128 at (512, 512),
+x periodic, bounce-back walls in y, regrid every 10 steps. A step is one
+Main_Loop iteration (C2 fill, collision, hg fill, stream, C = sum h, C fill,
+U and P, plus the regrid every 10th step), all on the GPU through the C-ABI.
+
+value = finest-equivalent MLUPS = 1024 * 2048 * steps / wall seconds / 1e6,
+timed over exactly --steps steps with the device synchronised on both sides.
+The state starts resident in HBM (set-up is untimed).
+
+The roofline entry is for the dominant kernel class, timed with HIP events
+around every launch over a second run of the same number of steps; its
+algorithmic bytes per leaf cell are counted from the values the kernel must
+read and write once (FP64):
+  C2 fill + collision : read h,g (18) + C,P,Ux,Uy (4), write h,g (18) = 320 B
+  hg fill + stream    : read h,g (18), write h,g (18) + C (1)          = 296 B
+  C fill + U,P        : read g (9) + C (1), write P,Ux,Uy (3)          = 104 B
+Ghost-cell, neighbour-table and regrid traffic is overhead, not counted.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "amr-lbm-openmp-2d_amd"))
+
+METRIC = "finest-equiv MLUPS + % HBM peak: plain FP64 vs FP32+learned regrid, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+ALG_BYTES = {"c2_collide": 320, "hg_stream": 296, "c_macro_g": 104}
+KERNEL_NAMES = {"c2_collide": "k_fill_c<2,OP_COLLIDE>", "hg_stream": "k_fill_hg",
+                "c_macro_g": "k_fill_c<1,OP_MACRO_G>"}
+
+
+def cpu_baseline(workload, steps, regrid_every):
+    """The reference Fortran/OpenMP build (oracle/_ref) on the host cores for a
+    bounded sample of the same workload; falls back to the oracle's C
+    restatement (one thread) when the reference build is absent."""
+    nthreads = min(os.cpu_count() or 1, 16)
+    exe = os.path.join(ROOT, "oracle", "_ref", workload.name, "ref_harness")
+    fine = workload.fine_cells
+    if os.path.exists(exe):
+        with tempfile.TemporaryDirectory() as d:
+            subprocess.run([exe, d, str(steps), str(regrid_every), "0", str(nthreads), "-1"],
+                           check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            t = {}
+            for line in open(os.path.join(d, "timing.txt")):
+                k, v = line.split()
+                t[k] = float(v)
+        step_s = t["step_total"]
+        return {"value": fine * steps / step_s / 1e6, "unit": "MLUPS (finest-equivalent)",
+                "cores": nthreads, "kind": "reference",
+                "sample": f"{steps} steps of the same workload, reference Fortran/OpenMP build "
+                          f"(amdflang -O3 -fopenmp), {nthreads} threads, step time = sum of the "
+                          f"reference's phase timers"}
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    o = oracle.Oracle(workload)
+    o.setup()
+    n = max(steps // 10, 10)
+    t0 = time.perf_counter()
+    o.run(n, regrid_every)
+    dt = time.perf_counter() - t0
+    return {"value": fine * n / dt / 1e6, "unit": "MLUPS (finest-equivalent)", "cores": 1,
+            "kind": "port", "sample": f"{n} steps of the same workload, C restatement, 1 thread"}
+
+
+def pmc_traffic(kernel_key, workload):
+    """HBM bytes per launch from committed rocprofv3 --pmc summaries, if present."""
+    path = os.path.join(ROOT, "profiles", f"{kernel_key}_{workload}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--regrid-every", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=600)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import amrlbm
+    w = amrlbm.WORKLOADS[args.workload]
+    k = args.regrid_every
+    m = amrlbm.Mesh(w)
+    m.setup_rising_bubble()
+    m.step(args.warmup, k)
+    m.synchronize()
+    c0 = m.counts()
+    t0 = time.perf_counter()
+    m.step(args.steps, k)
+    m.synchronize()
+    wall = time.perf_counter() - t0
+    c1 = m.counts()
+    value = w.fine_cells * args.steps / wall / 1e6
+    nleaf_mean = 0.5 * (c0["nleaf"] + c1["nleaf"])
+
+    prof = m.profile_steps(args.steps, k)
+    diag = m.output()
+    m.close()
+
+    classes = {kk: v for kk, v in prof.items() if kk in ALG_BYTES}
+    dom = max(classes, key=lambda kk: classes[kk]["ms"])
+    d = classes[dom]
+    avg_ms = d["ms"] / max(d["launches"], 1)
+    bytes_per_launch = ALG_BYTES[dom] * d["cells"] / max(d["launches"], 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dom, w.name),
+                "kernel": KERNEL_NAMES[dom], "avg_launch_us": round(avg_ms * 1e3, 3),
+                "algorithmic_bytes_per_leaf_cell": ALG_BYTES[dom],
+                "bytes_per_launch": round(bytes_per_launch)}
+    total_kernel_ms = sum(v["ms"] for v in classes.values())
+    phase_share = {kk: round(v["ms"] / total_kernel_ms, 3) for kk, v in classes.items()}
+    phase_share["regrid_ms_per_call"] = round(prof["regrid"]["ms"] / max(prof["regrid"]["count"], 1), 3)
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "MLUPS (finest-equivalent)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: analytic tanh rising-bubble initial state, deterministic",
+        "config": {"workload": f"rising bubble {w.fine_shape[0]}x{w.fine_shape[1]} finest-equivalent, "
+                               f"16x16 blocks, levels {w.min_levels}..{w.max_levels}, D={int(w.D)}, "
+                               f"regrid every {k}, plain FP64 path",
+                   "leaf_blocks": [c0["nleaf"], c1["nleaf"]],
+                   "leaf_cell_mlups": round(nleaf_mean * 256 * args.steps / wall / 1e6, 1)},
+        "roofline": roofline,
+        "phase_share": phase_share,
+        "diagnostics": {"mass": diag[3], "rise_velocity": diag[7], "centroid_y": diag[6]},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, k)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

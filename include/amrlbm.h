@@ -1,0 +1,122 @@
+/*
+ * amrlbm.h -- C-ABI of the MI355X (gfx950) two-phase AMR-LBM step.
+ *
+ * The library owns the mesh on one GPU: block table, neighbour table and the
+ * per-block arrays of the reference (h, g, C, P, Ux, Uy), laid out block-major
+ * structure-of-arrays in HBM. Each entry point below replaces one subroutine of
+ * AbbasHub/AMR-LBM-OpenMP-2D (cited beside it) with the same meaning; the
+ * Fortran shim in amr-lbm-openmp-2d_amd/fortran binds them with ISO_C_BINDING
+ * under the reference's own subroutine names.
+ *
+ * Conventions
+ *  - every function returns 0 (AMRLBM_OK) or an AMRLBM_ERR_* code; the text of
+ *    the last error of the calling thread is amrlbm_last_error();
+ *  - block ids are the reference's 1-based Blk(n) ids (Parameters.f90:39);
+ *  - host arrays use the reference's layouts (column-major, population index
+ *    fastest): h,g (9,0:nx+1,0:ny+1); C (-1:nx+2,-1:ny+2); P,Ux,Uy (nx,ny);
+ *  - a mesh is not re-entrant: one thread at a time per mesh. All device work
+ *    of a mesh is ordered on one HIP stream of default priority; entry points
+ *    that return data synchronise that stream.
+ */
+#ifndef AMRLBM_H
+#define AMRLBM_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMRLBM_OK 0
+#define AMRLBM_ERR_HIP 1        /* a HIP runtime call failed */
+#define AMRLBM_ERR_ARG 2        /* invalid argument */
+#define AMRLBM_ERR_CAPACITY 3   /* MaxBlks exceeded (AMR_Rising_2D_OMP.f90:527-531 stops the run) */
+#define AMRLBM_ERR_TOPOLOGY 4   /* 2:1 balance violated / unresolvable neighbour (Fill_Left_hg:101-103 prints) */
+#define AMRLBM_ERR_NONFINITE 5  /* non-finite phase field found by amrlbm_output */
+#define AMRLBM_ERR_STATE 6      /* call out of order (e.g. no mesh set up) */
+
+typedef struct amrlbm_mesh amrlbm_mesh;
+
+/* Compile-time PARAMETERs and module variables of the reference
+ * (Parameters.f90:12-19, D2Q9_Mod.f90:10-16), passed at run time. */
+typedef struct {
+    int nx, ny;          /* cells per block side; this build supports 16 x 16 */
+    int L0;              /* root block extent in finest cells */
+    int nroots;          /* root blocks stacked in y (x is periodic) */
+    int min_levels, max_levels;
+    int max_blocks;      /* MaxBlks */
+    double M0, W, sigma, beta, kappa, taul, tauh, rhol, rhoh, gy;
+} amrlbm_params;
+
+/* ---- life cycle ---------------------------------------------------------- */
+int amrlbm_create(const amrlbm_params *p, amrlbm_mesh **out);   /* module parameters (Parameters.f90) */
+int amrlbm_destroy(amrlbm_mesh *m);
+const char *amrlbm_last_error(void);
+
+/* AMR_LBM_Rising_Bubble set-up (AMR_Rising_2D_OMP.f90:51-67): roots,
+ * Initial_Mesh (:493-513), Initialize_AMR_Rising_Bubble (:142-231) and the
+ * init-regrid loop, with bubble radius R centred at (x0,y0). */
+int amrlbm_setup_rising_bubble(amrlbm_mesh *m, double R, double x0, double y0);
+
+/* ---- state exchange with the caller's Blk(:) mirror ----------------------- */
+/* counts: Nblks, nLeaf, LevelMin, LevelMax, KK, free-list length, step T */
+int amrlbm_get_counts(amrlbm_mesh *m, int counts[7]);
+/* topology of blocks 1..Nblks, 21 ints each: Level, dx, I, J, Parent, IsLeaf,
+ * hasLeaf, Refin, Deref, child(1:4), nghbr(1:8)%n (Parameters.f90:26-35);
+ * freeq receives the free-list ids head first (Parameters.f90:41-45) */
+int amrlbm_get_topology(amrlbm_mesh *m, int *blocks, int *freeq);
+/* replace the mesh by a caller-built one (same 21-int rows); neighbour values
+ * are recomputed from the quadtree and checked against the given ones */
+int amrlbm_set_topology(amrlbm_mesh *m, int nblks, const int *blocks, int kk, int nfree,
+                        const int *freeq, int T);
+/* arrays of one live block (NULL pointers skip a field) */
+int amrlbm_put_block(amrlbm_mesh *m, int id, const double *h, const double *g, const double *C,
+                     const double *P, const double *Ux, const double *Uy);
+int amrlbm_get_block(amrlbm_mesh *m, int id, double *h, double *g, double *C, double *P,
+                     double *Ux, double *Uy);
+
+/* ---- whole-mesh phases of one step (Main_Loop, AMR_Rising_2D_OMP.f90:302-364) */
+int amrlbm_determine_min_max_levels(amrlbm_mesh *m, int *levelmin, int *levelmax); /* :434-452 */
+int amrlbm_fill_ghost_cells_c2(amrlbm_mesh *m);       /* Fill_Ghost_Cells_C2.f90:2-40 */
+int amrlbm_collide(amrlbm_mesh *m);                   /* :391-399 -> LBM_Subroutines.f90:116-189 */
+int amrlbm_fill_ghost_cells_hg(amrlbm_mesh *m);       /* Fill_Ghost_Cells_hg.f90:2-49 */
+int amrlbm_stream(amrlbm_mesh *m);                    /* :410-425 -> LBM_Subroutines.f90:41-90 */
+/* Phase_Field_LBM_h_g_AMR (:381-430): collide, hg fill, stream; phase times in ms */
+int amrlbm_phase_field_lbm_h_g(amrlbm_mesh *m, double ms[3]);
+int amrlbm_macroscopic_values_h(amrlbm_mesh *m);      /* :328-335 -> LBM_Subroutines.f90:193-201 */
+int amrlbm_fill_ghost_cells_c(amrlbm_mesh *m);        /* Fill_Ghost_Cells_C.f90:2-40 */
+int amrlbm_macroscopic_values_g(amrlbm_mesh *m);      /* :346-354 -> LBM_Subroutines.f90:205-240 */
+int amrlbm_refine_derefine_mesh(amrlbm_mesh *m, double refin_tol, double deref_tol); /* Refine_Derefine.f90:2-14 */
+
+/* nsteps whole steps (fused kernels, one HIP graph per mesh topology); the
+ * regrid runs after step T when (T+1) % regrid_every == 0, with the
+ * reference's tolerances 0.002 / 0.001 (AMR_Rising_2D_OMP.f90:276-277) */
+int amrlbm_step(amrlbm_mesh *m, int nsteps, int regrid_every);
+
+/* the same steps as amrlbm_step, launched one kernel at a time with a HIP
+ * event after each launch on the mesh's stream; stats[3k..3k+2] = total device
+ * ms, launches and leaf cells processed of kernel class k (0: C2 fill +
+ * collision, 1: hg fill + stream + C = sum h, 2: C fill + U,P); stats[9..10] =
+ * regrid ms and count. No counterpart in the reference: this is the
+ * measurement hook for the roofline report. */
+int amrlbm_profile_steps(amrlbm_mesh *m, int nsteps, int regrid_every, double stats[12]);
+
+/* FP64 diagnostics over the leaf interiors (Output, AMR_Rising_2D_OMP.f90:456-489):
+ * out[0..8] = rho_min, rho_max, max|Uy|, mass (sum rho dx^2), bubble area
+ * (sum (1-C) dx^2), bubble centroid x, y, bubble rise velocity, non-finite count.
+ * Returns AMRLBM_ERR_NONFINITE when the count is non-zero (outputs still set). */
+int amrlbm_output(amrlbm_mesh *m, double out[9]);
+
+/* block until all queued device work of the mesh is done */
+int amrlbm_synchronize(amrlbm_mesh *m);
+/* device time (ms) of the last amrlbm_step call, measured with HIP events */
+int amrlbm_last_step_ms(amrlbm_mesh *m, double *ms);
+
+/* ---- per-block kernels on caller (host) arrays, reference layouts --------- */
+int amrlbm_collision_block(const amrlbm_params *p, int dx, const double *C, const double *P,
+                           const double *Ux, const double *Uy, double *h, double *g);    /* LBM_Subroutines.f90:116-189 */
+int amrlbm_macro_g_block(const amrlbm_params *p, int dx, const double *g, const double *C,
+                         double *P, double *Ux, double *Uy);                             /* LBM_Subroutines.f90:205-240 */
+int amrlbm_stream_block(const amrlbm_params *p, int dx, double *f);                      /* LBM_Subroutines.f90:41-90 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif
