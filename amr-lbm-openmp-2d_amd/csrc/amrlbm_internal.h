@@ -35,18 +35,27 @@ struct BlockDev {
     int pad0, pad1;
 };
 
+// Neighbour descriptor: the resolved nghbr(k)%n plus what a ghost fill needs
+// to know about that block, so one load serves the whole recipe.
+struct NbDesc {
+    int id;       // reference id (>0) or -k for a wall
+    int level;
+    int isleaf;
+    int child0;   // slot of its first child when refined, else -1
+};
+
 // Device view of a mesh, passed by value to the kernels.
 struct DevMesh {
     double *pop[2];          // [slot][set 0:h 1:g][q][FS], ping-pong buffers
     double *C;               // [slot][CS], C(-1:nx+2,-1:ny+2)
     double *PU;              // [slot][3][PS]
     const BlockDev *blk;     // [slot]
-    const int *nb;           // [slot][8] resolved nghbr(1:8)%n: reference id (>0) or -k (wall)
+    const NbDesc *nbd;       // [slot][8] neighbours in nghbr(1:8) order
     int *err_flags;          // [0]: bad level jump seen by a ghost fill
 };
 
 // kernel launchers (amrlbm_kernels.hip); all enqueue on stream s
-enum FillOp { OP_NONE = 0, OP_COLLIDE = 1, OP_MACRO_G = 2 };
+enum FillOp { OP_NONE = 0, OP_COLLIDE = 1, OP_MACRO_G = 2, OP_MACRO_G_COLLIDE = 3 };
 void launch_fill_c(const DevMesh &m, int cur, const int *list, int n, int rings, int op, const Phys &ph,
                    bool do_fill, hipStream_t s);
 void launch_fill_hg(const DevMesh &m, int cur, const int *list, int n, bool stream_after, bool macro_h,

@@ -259,60 +259,80 @@ __device__ __forceinline__ void coarse_axis(int s, int t, int pos, int c[3], int
     }
 }
 
-// value of ghost cell (x,y) in direction d from neighbour id N1; get(slot,x,y)
-// reads the field. Returns false for an unsupported level jump.
-template <class Get>
-__device__ __forceinline__ bool ghost_value(const DevMesh &m, const BlockDev &me, int d, int N1, int x, int y, Get get,
-                                            double &v)
+// Interpolation recipe of one ghost cell: mode 0 copy (case (i)), 1 from the
+// finer children (case (ii), /64), 2 from the coarser neighbour (case (iii),
+// /1024); off[a][b] = element offsets of the source cells for x-point a and
+// y-point b, valid for component 0 of a field laid out with cell index
+// y*W + x inside a slot of SLOT elements.
+struct Recipe {
+    int mode;
+    int off[3][3];
+    int wx[3], wy[3];
+};
+
+template <int SLOT, int W, int ORG>
+__device__ __forceinline__ bool make_recipe(const NbDesc &nb, const BlockDev &me, int d, int x, int y, Recipe &r)
 {
     const int sx = c_dsx[d], sy = c_dsy[d];
-    const int ns = N1 - 1;
-    const BlockDev nbk = m.blk[ns];
-    if (nbk.level == me.level) {
-        if (nbk.isleaf) {   // case (i): copy (Fill_Left_hg:66-70)
-            v = get(ns, x - sx * NX, y - sy * NY);
+    const int ns = nb.id - 1;
+    if (nb.level == me.level) {
+        if (nb.isleaf) {
+            r.mode = 0;
+            r.off[0][0] = ns * SLOT + (y - sy * NY + ORG) * W + (x - sx * NX + ORG);
             return true;
         }
-        // case (ii): biquadratic from the neighbour's children (Fill_Left_hg:72-85)
-        int fx[3], wx[3], fy[3], wy[3];
-        fine_axis(sx, x, fx, wx);
-        fine_axis(sy, y, fy, wy);
-        double t[3];
+        r.mode = 1;
+        int fx[3], fy[3];
+        fine_axis(sx, x, fx, r.wx);
+        fine_axis(sy, y, fy, r.wy);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             int ix = fx[a] > NX, lx = fx[a] - ix * NX;
-            double u[3];
 #pragma unroll
             for (int b = 0; b < 3; b++) {
                 int iy = fy[b] > NY, ly = fy[b] - iy * NY;
-                u[b] = get(nbk.child0 + ix + 2 * iy, lx, ly);
+                r.off[a][b] = (nb.child0 + ix + 2 * iy) * SLOT + (ly + ORG) * W + (lx + ORG);
             }
-            t[a] = (double)wy[0] * u[0] + (double)wy[1] * u[1] + (double)wy[2] * u[2];
         }
-        v = ((double)wx[0] * t[0] + (double)wx[1] * t[1] + (double)wx[2] * t[2]) / 64.0;
         return true;
     }
-    if (me.level == nbk.level + 1) {
-        // case (iii): biquadratic from the coarser neighbour, its ghosts included (Fill_Left_hg:89-99)
-        int cx[3], wx[3], cy[3], wy[3];
-        coarse_axis(sx, x, me.I & 1, cx, wx);
-        coarse_axis(sy, y, me.J & 1, cy, wy);
-        double t[3];
+    if (me.level == nb.level + 1) {
+        r.mode = 2;
+        int cx[3], cy[3];
+        coarse_axis(sx, x, me.I & 1, cx, r.wx);
+        coarse_axis(sy, y, me.J & 1, cy, r.wy);
 #pragma unroll
         for (int a = 0; a < 3; a++)
-            t[a] = (double)wy[0] * get(ns, cx[a], cy[0]) + (double)wy[1] * get(ns, cx[a], cy[1]) +
-                   (double)wy[2] * get(ns, cx[a], cy[2]);
-        v = ((double)wx[0] * t[0] + (double)wx[1] * t[1] + (double)wx[2] * t[2]) / 1024.0;
+#pragma unroll
+            for (int b = 0; b < 3; b++) r.off[a][b] = ns * SLOT + (cy[b] + ORG) * W + (cx[a] + ORG);
         return true;
     }
     return false;
+}
+
+// apply a recipe to the field base + shift (tensor product: y pass, then x)
+__device__ __forceinline__ double apply_recipe(const Recipe &r, const double *__restrict__ base, int shift)
+{
+    if (r.mode == 0) return base[r.off[0][0] + shift];
+    double t[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        t[a] = (double)r.wy[0] * base[r.off[a][0] + shift] + (double)r.wy[1] * base[r.off[a][1] + shift] +
+               (double)r.wy[2] * base[r.off[a][2] + shift];
+    double o = (double)r.wx[0] * t[0] + (double)r.wx[1] * t[1] + (double)r.wx[2] * t[2];
+    return r.mode == 1 ? o / 64.0 : o / 1024.0;
 }
 
 // Fill_Ghost_Cells_C (R=1) / Fill_Ghost_Cells_C2 (R=2) for the leaves in
 // list (one level), then optionally the block's collision (after C2) or
 // Macroscopic_Values_g_AMR (after C): both read only this block's C, which
 // is final once its ghosts are filled, and no later fill pass reads what
-// they write.
+// they write. OP_MACRO_G_COLLIDE runs the previous step's
+// Macroscopic_Values_g_AMR and then this step's collision after a C2 fill:
+// between two steps with no regrid, the C fill that ends a step and the C2
+// fill that starts the next compute the same first ghost ring from the same
+// C, so U and P come out identical. The block's C is staged in LDS; its own
+// loads are issued before the neighbour gathers so the two overlap.
 template <int R, int OP>
 __global__ __launch_bounds__(256) void k_fill_c(DevMesh m, int cur, const int *__restrict__ list, Phys ph,
                                                 int do_fill)
@@ -321,63 +341,91 @@ __global__ __launch_bounds__(256) void k_fill_c(DevMesh m, int cur, const int *_
     const int tid = threadIdx.x;
     const BlockDev me = m.blk[slot];
     double *C = m.C + (size_t)slot * CS;
-    __shared__ int nbv[8];
+    __shared__ NbDesc nbd[8];
     __shared__ double sC[CS];
-    if (tid < 8) nbv[tid] = m.nb[slot * 8 + tid];
+    if (tid < 8) nbd[tid] = m.nbd[slot * 8 + tid];
+    const int i = tid % NX + 1, j = tid / NX + 1;
+    double *F = m.pop[cur] + (size_t)slot * POPB;
+    double *PU = m.PU + (size_t)slot * PUB;
+    double h[9], g[9], P = 0.0, Ux = 0.0, Uy = 0.0;
+    if (OP == OP_COLLIDE || OP == OP_MACRO_G_COLLIDE) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) h[q] = F[fidx(0, q, i, j)];
+    }
+    if (OP == OP_COLLIDE) {
+        P = PU[pidx(i, j)];
+        Ux = PU[PS + pidx(i, j)];
+        Uy = PU[2 * PS + pidx(i, j)];
+    }
+    if (OP != OP_NONE) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) g[q] = F[fidx(1, q, i, j)];
+    }
+    if (do_fill) {
+        sC[cidx(i, j)] = C[cidx(i, j)];
+    } else {
+        for (int t = tid; t < CS; t += 256) sC[t] = C[t];
+    }
     __syncthreads();
     if (do_fill) {
         constexpr int NG = 64 * R + 4 * R * R;
-        auto getC = [&](int s, int x, int y) { return m.C[(size_t)s * CS + cidx(x, y)]; };
-        for (int t = tid; t < NG; t += 256) {
+        if (tid < NG) {
             int d, x, y;
-            ghost_cell<R>(t, d, x, y);
-            int N1 = nbv[d];
-            if (N1 <= 0) continue;
-            double v;
-            if (ghost_value(m, me, d, N1, x, y, getC, v))
-                C[cidx(x, y)] = v;
-            else
-                atomicOr(m.err_flags, 1);
+            ghost_cell<R>(tid, d, x, y);
+            const NbDesc nb = nbd[d];
+            if (nb.id > 0) {
+                Recipe rc;
+                if (make_recipe<CS, CW, 1>(nb, me, d, x, y, rc)) {
+                    double v = apply_recipe(rc, m.C, 0);
+                    sC[cidx(x, y)] = v;
+                    C[cidx(x, y)] = v;
+                } else {
+                    atomicOr(m.err_flags, 1);
+                }
+            }
         }
         __syncthreads();
         // wall mirror: Symmetric_C_3/4 (BC_Cell.f90:46-72), Symmetric_C2_3/4 (:76-104)
         if (tid < 64) {
             int top = tid >= 32, X = (tid & 31) - (R - 1);
-            if (nbv[2 + top] < 0 && X <= 17 + (R - 1)) {
+            if (nbd[2 + top].id < 0 && X <= 17 + (R - 1)) {
                 if (!top) {
-                    if (R == 2) C[cidx(X, -1)] = C[cidx(X, 2)];
-                    C[cidx(X, 0)] = C[cidx(X, 1)];
+                    if (R == 2) {
+                        double v = sC[cidx(X, 2)];
+                        sC[cidx(X, -1)] = v;
+                        C[cidx(X, -1)] = v;
+                    }
+                    double v = sC[cidx(X, 1)];
+                    sC[cidx(X, 0)] = v;
+                    C[cidx(X, 0)] = v;
                 } else {
-                    C[cidx(X, NY + 1)] = C[cidx(X, NY)];
-                    if (R == 2) C[cidx(X, NY + 2)] = C[cidx(X, NY - 1)];
+                    double v = sC[cidx(X, NY)];
+                    sC[cidx(X, NY + 1)] = v;
+                    C[cidx(X, NY + 1)] = v;
+                    if (R == 2) {
+                        v = sC[cidx(X, NY - 1)];
+                        sC[cidx(X, NY + 2)] = v;
+                        C[cidx(X, NY + 2)] = v;
+                    }
                 }
             }
         }
         __syncthreads();
     }
-    if (OP == OP_NONE) return;
-    for (int t = tid; t < CS; t += 256) sC[t] = C[t];
-    __syncthreads();
-    const int i = tid % NX + 1, j = tid / NX + 1;
-    double *F = m.pop[cur] + (size_t)slot * POPB;
-    double *PU = m.PU + (size_t)slot * PUB;
-    if (OP == OP_COLLIDE) {
-        double h[9], g[9];
-#pragma unroll
-        for (int q = 0; q < 9; q++) {
-            h[q] = F[fidx(0, q, i, j)];
-            g[q] = F[fidx(1, q, i, j)];
-        }
-        collide_cell(sC, i, j, me.dx, PU[pidx(i, j)], PU[PS + pidx(i, j)], PU[2 * PS + pidx(i, j)], ph, h, g);
+    if (OP == OP_MACRO_G_COLLIDE) {
+        macro_g_cell(sC, i, j, me.dx, g, ph, P, Ux, Uy);
+        PU[pidx(i, j)] = P;
+        PU[PS + pidx(i, j)] = Ux;
+        PU[2 * PS + pidx(i, j)] = Uy;
+    }
+    if (OP == OP_COLLIDE || OP == OP_MACRO_G_COLLIDE) {
+        collide_cell(sC, i, j, me.dx, P, Ux, Uy, ph, h, g);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             F[fidx(0, q, i, j)] = h[q];
             F[fidx(1, q, i, j)] = g[q];
         }
-    } else {
-        double g[9], P, Ux, Uy;
-#pragma unroll
-        for (int q = 0; q < 9; q++) g[q] = F[fidx(1, q, i, j)];
+    } else if (OP == OP_MACRO_G) {
         macro_g_cell(sC, i, j, me.dx, g, ph, P, Ux, Uy);
         PU[pidx(i, j)] = P;
         PU[PS + pidx(i, j)] = Ux;
@@ -385,86 +433,119 @@ __global__ __launch_bounds__(256) void k_fill_c(DevMesh m, int cur, const int *_
     }
 }
 
-// Fill_Ghost_Cells_hg for the leaves in list (one level), bounce-back walls
-// (BC_Cell.f90:2-42), then optionally the stream of the same blocks
-// (Perfect_Shift_AMR / Streaming_LaxWendroff_AMR, LBM_Subroutines.f90:41-90)
-// into the other population buffer and C = sum(h) (:193-201). The stream
-// writes only the other buffer, so finer passes still read the
-// post-collision populations and ghosts of this level.
-__global__ __launch_bounds__(256) void k_fill_hg(DevMesh m, int cur, const int *__restrict__ list, int flags)
+// Perfect_Shift_AMR / Streaming_LaxWendroff_AMR (LBM_Subroutines.f90:41-90)
+// of one cell from the LDS copy of the block into G, and C = sum(h) (:193-201)
+__device__ __forceinline__ void stream_cell(const double *__restrict__ sF, double *__restrict__ G, int i, int j,
+                                            int dx, double *csum)
 {
-    const int slot = list[blockIdx.x];
-    const int tid = threadIdx.x;
-    const BlockDev me = m.blk[slot];
-    const double *Fc = m.pop[cur];
-    double *F = m.pop[cur] + (size_t)slot * POPB;
-    __shared__ int nbv[8];
-    if (tid < 8) nbv[tid] = m.nb[slot * 8 + tid];
-    __syncthreads();
-    if (flags & 1) {
-        for (int t = tid; t < 68 * 18; t += 256) {
-            int comp = t / 68, cell = t - comp * 68;
-            int set = comp / 9, q = comp - set * 9;
-            int d, x, y;
-            ghost_cell<1>(cell, d, x, y);
-            int N1 = nbv[d];
-            if (N1 <= 0) continue;
-            auto getF = [&](int s, int xx, int yy) { return Fc[(size_t)s * POPB + fidx(set, q, xx, yy)]; };
-            double v;
-            if (ghost_value(m, me, d, N1, x, y, getF, v))
-                F[fidx(set, q, x, y)] = v;
-            else
-                atomicOr(m.err_flags, 1);
-        }
-        __syncthreads();
-        // Bounce_Back_3 (bottom, row 0) and Bounce_Back_4 (top, row ny+1)
-        for (int t = tid; t < 2 * 2 * 9 * FW; t += 256) {
-            int top = t / (2 * 9 * FW), r = t - top * (2 * 9 * FW);
-            int set = r / (9 * FW), r2 = r - set * (9 * FW), q = r2 / FW, X = r2 - q * FW;
-            if (nbv[2 + top] >= 0) continue;
-            int Y = top ? NY + 1 : 0, Yi = top ? NY : 1;
-            if (q == 0) {
-                F[fidx(set, 0, X, Y)] = F[fidx(set, 0, X, Yi)];
-            } else {
-                int e = c_ex[q] * c_ey[q];
-                int Xn = top ? X - e : X + e;
-                Xn = Xn < 0 ? 0 : (Xn > NX + 1 ? NX + 1 : Xn);
-                F[fidx(set, q, X, Y)] = F[fidx(set, c_opp[q], Xn, Yi)];
-            }
-        }
-        __syncthreads();
-    }
-    if (!(flags & 2)) return;
-    const int i = tid % NX + 1, j = tid / NX + 1;
-    double *G = m.pop[cur ^ 1] + (size_t)slot * POPB;
-    if (flags & 8) {   // carry the ghost ring along, as the reference's in-place stream leaves it
-        for (int t = tid; t < 68 * 18; t += 256) {
-            int comp = t / 68, cell = t - comp * 68, set = comp / 9, q = comp - set * 9, d, x, y;
-            ghost_cell<1>(cell, d, x, y);
-            G[fidx(set, q, x, y)] = F[fidx(set, q, x, y)];
-        }
-    }
-    const int dx = me.dx;
     const double cfl = 1.0 / (double)dx;
     const double a = cfl / 2.0 * (1.0 + cfl), b = 1.0 - cfl * cfl, c = cfl / 2.0 * (1.0 - cfl);
-    double csum = 0.0;
+    double s = 0.0;
 #pragma unroll
     for (int set = 0; set < 2; set++) {
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             double v;
             if (q == 0)
-                v = F[fidx(set, 0, i, j)];
+                v = sF[fidx(set, 0, i, j)];
             else if (cfl > 0.9)
-                v = F[fidx(set, q, i - c_ex[q], j - c_ey[q])];
+                v = sF[fidx(set, q, i - c_ex[q], j - c_ey[q])];
             else
-                v = a * F[fidx(set, q, i - c_ex[q], j - c_ey[q])] + b * F[fidx(set, q, i, j)] -
-                    c * F[fidx(set, q, i + c_ex[q], j + c_ey[q])];
+                v = a * sF[fidx(set, q, i - c_ex[q], j - c_ey[q])] + b * sF[fidx(set, q, i, j)] -
+                    c * sF[fidx(set, q, i + c_ex[q], j + c_ey[q])];
             G[fidx(set, q, i, j)] = v;
-            if (set == 0) csum += v;
+            if (set == 0) s += v;
         }
     }
+    *csum = s;
+}
+
+// Fill_Ghost_Cells_hg for the leaves in list (one level), bounce-back walls
+// (BC_Cell.f90:2-42), then optionally the stream of the same blocks into the
+// other population buffer and C = sum(h). The stream writes only the other
+// buffer, so finer passes still read the post-collision populations and
+// ghosts of this level. The block (18 populations with ghosts) is staged in
+// LDS; each of 136 threads fills one ghost cell of one set for all 9
+// populations from one interpolation recipe.
+// flags: 1 fill, 2 stream, 4 C = sum h, 8 carry the ghost ring into the new buffer
+__global__ __launch_bounds__(256) void k_fill_hg(DevMesh m, int cur, const int *__restrict__ list, int flags)
+{
+    const int slot = list[blockIdx.x];
+    const int tid = threadIdx.x;
+    const BlockDev me = m.blk[slot];
+    const double *__restrict__ Fc = m.pop[cur];
+    double *F = m.pop[cur] + (size_t)slot * POPB;
+    __shared__ NbDesc nbd[8];
+    __shared__ double sF[POPB];
+    if (tid < 8) nbd[tid] = m.nbd[slot * 8 + tid];
+    const int i = tid % NX + 1, j = tid / NX + 1;
+#pragma unroll
+    for (int c = 0; c < 18; c++) sF[c * FS + j * FW + i] = F[c * FS + j * FW + i];
+    if (!(flags & 1)) {
+        for (int t = tid; t < 68 * 18; t += 256) {
+            int comp = t / 68, cell = t - comp * 68, d, x, y;
+            ghost_cell<1>(cell, d, x, y);
+            sF[comp * FS + y * FW + x] = F[comp * FS + y * FW + x];
+        }
+    }
+    __syncthreads();
+    if (flags & 1) {
+        if (tid < 136) {
+            int set = tid / 68, cell = tid - set * 68, d, x, y;
+            ghost_cell<1>(cell, d, x, y);
+            const NbDesc nb = nbd[d];
+            if (nb.id > 0) {
+                Recipe rc;
+                if (make_recipe<POPB, FW, 0>(nb, me, d, x, y, rc)) {
+                    double v[9];
+#pragma unroll
+                    for (int q = 0; q < 9; q++) v[q] = apply_recipe(rc, Fc, (set * 9 + q) * FS);
+#pragma unroll
+                    for (int q = 0; q < 9; q++) {
+                        sF[fidx(set, q, x, y)] = v[q];
+                        F[fidx(set, q, x, y)] = v[q];
+                    }
+                } else {
+                    atomicOr(m.err_flags, 1);
+                }
+            }
+        }
+        __syncthreads();
+        // Bounce_Back_3 (bottom, row 0) and Bounce_Back_4 (top, row ny+1)
+        if (tid < 2 * 2 * FW) {
+            int top = tid / (2 * FW), r = tid - top * (2 * FW), set = r / FW, X = r - set * FW;
+            if (nbd[2 + top].id < 0) {
+                int Y = top ? NY + 1 : 0, Yi = top ? NY : 1;
+                double v[9];
+                v[0] = sF[fidx(set, 0, X, Yi)];
+#pragma unroll
+                for (int q = 1; q < 9; q++) {
+                    int e = c_ex[q] * c_ey[q];
+                    int Xn = top ? X - e : X + e;
+                    Xn = Xn < 0 ? 0 : (Xn > NX + 1 ? NX + 1 : Xn);
+                    v[q] = sF[fidx(set, c_opp[q], Xn, Yi)];
+                }
+#pragma unroll
+                for (int q = 0; q < 9; q++) {
+                    sF[fidx(set, q, X, Y)] = v[q];
+                    F[fidx(set, q, X, Y)] = v[q];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!(flags & 2)) return;
+    double *G = m.pop[cur ^ 1] + (size_t)slot * POPB;
+    double csum;
+    stream_cell(sF, G, i, j, me.dx, &csum);
     if (flags & 4) m.C[(size_t)slot * CS + cidx(i, j)] = csum;
+    if (flags & 8) {
+        for (int t = tid; t < 68 * 18; t += 256) {
+            int comp = t / 68, cell = t - comp * 68, d, x, y;
+            ghost_cell<1>(cell, d, x, y);
+            G[comp * FS + y * FW + x] = sF[comp * FS + y * FW + x];
+        }
+    }
 }
 
 // Macroscopic_Values_h_AMR (LBM_Subroutines.f90:193-201)
@@ -712,7 +793,9 @@ void launch_fill_c(const DevMesh &m, int cur, const int *list, int n, int rings,
     dim3 g(n), b(256);
     int f = do_fill ? 1 : 0;
     if (rings == 2) {
-        if (op == OP_COLLIDE) hipLaunchKernelGGL((k_fill_c<2, OP_COLLIDE>), g, b, 0, s, m, cur, list, ph, f);
+        if (op == OP_MACRO_G_COLLIDE)
+            hipLaunchKernelGGL((k_fill_c<2, OP_MACRO_G_COLLIDE>), g, b, 0, s, m, cur, list, ph, f);
+        else if (op == OP_COLLIDE) hipLaunchKernelGGL((k_fill_c<2, OP_COLLIDE>), g, b, 0, s, m, cur, list, ph, f);
         else if (op == OP_MACRO_G) hipLaunchKernelGGL((k_fill_c<2, OP_MACRO_G>), g, b, 0, s, m, cur, list, ph, f);
         else hipLaunchKernelGGL((k_fill_c<2, OP_NONE>), g, b, 0, s, m, cur, list, ph, f);
     } else {

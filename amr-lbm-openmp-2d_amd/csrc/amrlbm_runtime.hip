@@ -61,7 +61,7 @@ struct amrlbm_mesh {
     hipStream_t stream = nullptr;
     DevMesh dm{};
     BlockDev *d_blk = nullptr;
-    int *d_nb = nullptr;
+    NbDesc *d_nbd = nullptr;
     int *d_err_flags = nullptr;
     int *d_lists = nullptr;          // leaves grouped by level, then all leaves (ascending)
     int lists_cap = 0;
@@ -79,8 +79,13 @@ struct amrlbm_mesh {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_step_ms = 0.0;
 
-    // graph of one whole step per population-buffer parity
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // one HIP graph per (segment length, population-buffer parity) for the
+    // current topology; a segment is the run of steps between two regrids
+    struct GraphEntry {
+        int len, parity;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
     int graph_topo = -1;
     int topo_version = 0;
 
@@ -220,7 +225,7 @@ int upload_tables(amrlbm_mesh *m)
     if (!m->tables_dirty) return AMRLBM_OK;
     const int nb = m->nblks;
     std::vector<BlockDev> bd(nb);
-    std::vector<int> nbt((size_t)nb * 8, 0);
+    std::vector<NbDesc> nbt((size_t)nb * 8, NbDesc{0, -1, 0, -1});
     for (int n = 1; n <= nb; n++) {
         const Block &b = m->blk[n];
         BlockDev &d = bd[n - 1];
@@ -236,7 +241,14 @@ int upload_tables(amrlbm_mesh *m)
                 int v = nb_value(m, n, k);
                 if (v == 0) return fail(AMRLBM_ERR_TOPOLOGY, "block " + std::to_string(n) + " has no neighbour " +
                                                                  std::to_string(k) + " (2:1 balance violated)");
-                nbt[(size_t)(n - 1) * 8 + k - 1] = v;
+                NbDesc &e = nbt[(size_t)(n - 1) * 8 + k - 1];
+                e.id = v;
+                if (v > 0) {
+                    const Block &q = m->blk[v];
+                    e.level = q.level;
+                    e.isleaf = q.isleaf ? 1 : 0;
+                    e.child0 = (!q.isleaf && q.has_child) ? q.child[0] - 1 : -1;
+                }
             }
     }
     // leaves grouped by level, each group ascending; then all leaves
@@ -254,13 +266,13 @@ int upload_tables(amrlbm_mesh *m)
     if (rc) return rc;
     if (nb > 0) {
         HIPCHK(hipMemcpyAsync(m->d_blk, bd.data(), sizeof(BlockDev) * nb, hipMemcpyHostToDevice, m->stream));
-        HIPCHK(hipMemcpyAsync(m->d_nb, nbt.data(), sizeof(int) * nbt.size(), hipMemcpyHostToDevice, m->stream));
+        HIPCHK(hipMemcpyAsync(m->d_nbd, nbt.data(), sizeof(NbDesc) * nbt.size(), hipMemcpyHostToDevice, m->stream));
     }
     if (!lists.empty())
         HIPCHK(hipMemcpyAsync(m->d_lists, lists.data(), sizeof(int) * lists.size(), hipMemcpyHostToDevice, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));   // host vectors go out of scope
     m->dm.blk = m->d_blk;
-    m->dm.nb = m->d_nb;
+    m->dm.nbd = m->d_nbd;
     m->tables_dirty = false;
     return AMRLBM_OK;
 }
@@ -475,6 +487,7 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol)
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(m->stream));
     }
+    if (rpairs.empty() && ppairs.empty()) return AMRLBM_OK;   // flags only: tables and graphs stay valid
     array_of_leaf_blocks(m);
     return upload_tables(m);
 }
@@ -510,7 +523,7 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipMemsetAsync(m->dm.C, 0, sizeof(double) * CS * S, m->stream));
     HIPCHK(hipMemsetAsync(m->dm.PU, 0, sizeof(double) * PUB * S, m->stream));
     HIPCHK(hipMalloc(&m->d_blk, sizeof(BlockDev) * S));
-    HIPCHK(hipMalloc(&m->d_nb, sizeof(int) * 8 * S));
+    HIPCHK(hipMalloc(&m->d_nbd, sizeof(NbDesc) * 8 * S));
     HIPCHK(hipMalloc(&m->d_err_flags, sizeof(int)));
     HIPCHK(hipMemsetAsync(m->d_err_flags, 0, sizeof(int), m->stream));
     HIPCHK(hipMalloc(&m->d_err, sizeof(double) * S));
@@ -522,7 +535,7 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipStreamSynchronize(m->stream));
     m->dm.err_flags = m->d_err_flags;
     m->dm.blk = m->d_blk;
-    m->dm.nb = m->d_nb;
+    m->dm.nbd = m->d_nbd;
     return AMRLBM_OK;
 }
 
@@ -569,40 +582,80 @@ int init_bubble(amrlbm_mesh *m, double R, double X0, double Y0)
     return AMRLBM_OK;
 }
 
-// one whole step with the fused per-level kernels
-void enqueue_step(amrlbm_mesh *m, int cur)
+// A segment of len whole steps with the fused per-level kernels, starting
+// from a state whose U and P are current (after set-up, a regrid or a
+// previous segment) and ending with U and P current again:
+//   step 1      : [C2 fill + collide](L)            [hg fill + stream + C=sum h](L)
+//   steps 2..len: [C2 fill + U,P + collide](L)      [hg fill + stream + C=sum h](L)
+//   end         : [C fill + U,P](L)
+// each bracket launched once per level, coarse to fine. after(kind, cells) is
+// called after every launch (kind 0: C2 pass, 1: hg pass, 2: C pass).
+template <class After>
+int enqueue_segment(amrlbm_mesh *m, int cur, int len, After after)
 {
-    for (int L = m->levelmin; L <= m->levelmax; L++) {
-        int n;
-        const int *lst = level_list(m, L, &n);
-        launch_fill_c(m->dm, cur, lst, n, 2, OP_COLLIDE, m->ph, true, m->stream);
+    for (int s = 0; s < len; s++) {
+        for (int L = m->levelmin; L <= m->levelmax; L++) {
+            int n;
+            const int *lst = level_list(m, L, &n);
+            launch_fill_c(m->dm, cur, lst, n, 2, s == 0 ? OP_COLLIDE : OP_MACRO_G_COLLIDE, m->ph, true, m->stream);
+            int rc = after(0, n * PS);
+            if (rc) return rc;
+        }
+        for (int L = m->levelmin; L <= m->levelmax; L++) {
+            int n;
+            const int *lst = level_list(m, L, &n);
+            launch_fill_hg(m->dm, cur, lst, n, true, true, true, m->stream);
+            int rc = after(1, n * PS);
+            if (rc) return rc;
+        }
+        cur ^= 1;
     }
     for (int L = m->levelmin; L <= m->levelmax; L++) {
         int n;
         const int *lst = level_list(m, L, &n);
-        launch_fill_hg(m->dm, cur, lst, n, true, true, true, m->stream);
+        launch_fill_c(m->dm, cur, lst, n, 1, OP_MACRO_G, m->ph, true, m->stream);
+        int rc = after(2, n * PS);
+        if (rc) return rc;
     }
-    for (int L = m->levelmin; L <= m->levelmax; L++) {
-        int n;
-        const int *lst = level_list(m, L, &n);
-        launch_fill_c(m->dm, cur ^ 1, lst, n, 1, OP_MACRO_G, m->ph, true, m->stream);
-    }
+    return AMRLBM_OK;
 }
 
-int build_graphs(amrlbm_mesh *m)
+void drop_graphs(amrlbm_mesh *m)
 {
-    for (int p = 0; p < 2; p++) {
-        if (m->graph[p]) HIPCHK(hipGraphExecDestroy(m->graph[p]));
-        m->graph[p] = nullptr;
-        hipGraph_t g;
-        HIPCHK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_step(m, p);
-        HIPCHK(hipStreamEndCapture(m->stream, &g));
-        HIPCHK(hipGraphInstantiate(&m->graph[p], g, nullptr, nullptr, 0));
-        HIPCHK(hipGraphDestroy(g));
+    for (auto &g : m->graphs) (void)hipGraphExecDestroy(g.exec);
+    m->graphs.clear();
+}
+
+int segment_graph(amrlbm_mesh *m, int len, hipGraphExec_t *out)
+{
+    if (m->graph_topo != m->topo_version) {
+        drop_graphs(m);
+        m->graph_topo = m->topo_version;
     }
-    m->graph_topo = m->topo_version;
+    for (auto &g : m->graphs)
+        if (g.len == len && g.parity == m->cur) {
+            *out = g.exec;
+            return AMRLBM_OK;
+        }
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_segment(m, m->cur, len, [](int, int) { return AMRLBM_OK; });
+    hipError_t e = hipStreamEndCapture(m->stream, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(AMRLBM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t ex;
+    HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    m->graphs.push_back({len, m->cur, ex});
+    *out = ex;
     return AMRLBM_OK;
+}
+
+// steps to the next regrid (or to the end of the call)
+int segment_length(const amrlbm_mesh *m, int remaining, int regrid_every)
+{
+    int to_regrid = regrid_every - (m->T % regrid_every);
+    return std::min(remaining, to_regrid);
 }
 
 }  // namespace
@@ -649,14 +702,13 @@ int amrlbm_destroy(amrlbm_mesh *m)
 {
     if (!m) return AMRLBM_OK;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
-    for (int p = 0; p < 2; p++)
-        if (m->graph[p]) (void)hipGraphExecDestroy(m->graph[p]);
+    drop_graphs(m);
     (void)hipFree(m->dm.pop[0]);
     (void)hipFree(m->dm.pop[1]);
     (void)hipFree(m->dm.C);
     (void)hipFree(m->dm.PU);
     (void)hipFree(m->d_blk);
-    (void)hipFree(m->d_nb);
+    (void)hipFree(m->d_nbd);
     (void)hipFree(m->d_err_flags);
     (void)hipFree(m->d_lists);
     (void)hipFree(m->d_pairs);
@@ -928,21 +980,18 @@ int amrlbm_step(amrlbm_mesh *m, int nsteps, int regrid_every)
     if (nsteps < 0 || regrid_every < 1) return fail(AMRLBM_ERR_ARG, "bad step arguments");
     int rc = AMRLBM_OK;
     HIPCHK(hipEventRecord(m->ev0, m->stream));
-    for (int s = 0; s < nsteps; s++) {
+    int done = 0;
+    while (done < nsteps) {
+        int len = segment_length(m, nsteps - done, regrid_every);
         determine_min_max_levels(m);
-        rc = upload_tables(m);
-        if (rc) return rc;
-        if (m->graph_topo != m->topo_version) {
-            rc = build_graphs(m);
-            if (rc) return rc;
-        }
-        HIPCHK(hipGraphLaunch(m->graph[m->cur], m->stream));
-        m->cur ^= 1;
-        if ((m->T + 1) % regrid_every == 0) {
-            rc = regrid(m, 0.002, 0.001);
-            if (rc) return rc;
-        }
-        m->T++;
+        if ((rc = upload_tables(m))) return rc;
+        hipGraphExec_t ex;
+        if ((rc = segment_graph(m, len, &ex))) return rc;
+        HIPCHK(hipGraphLaunch(ex, m->stream));
+        m->cur ^= (len & 1);
+        m->T += len;
+        done += len;
+        if (m->T % regrid_every == 0 && (rc = regrid(m, 0.002, 0.001))) return rc;
     }
     determine_min_max_levels(m);
     HIPCHK(hipEventRecord(m->ev1, m->stream));
@@ -959,8 +1008,7 @@ int amrlbm_profile_steps(amrlbm_mesh *m, int nsteps, int regrid_every, double st
     if (nsteps < 0 || regrid_every < 1 || !stats) return fail(AMRLBM_ERR_ARG, "bad profile arguments");
     for (int k = 0; k < 12; k++) stats[k] = 0.0;
     std::vector<hipEvent_t> ev;
-    std::vector<int> kind;
-    std::vector<int> cells;
+    std::vector<int> kind, cells;
     auto mark = [&](int kd, int nc) -> int {
         hipEvent_t e;
         HIPCHK(hipEventCreate(&e));
@@ -970,23 +1018,7 @@ int amrlbm_profile_steps(amrlbm_mesh *m, int nsteps, int regrid_every, double st
         cells.push_back(nc);
         return AMRLBM_OK;
     };
-    int rc = AMRLBM_OK;
-    for (int s = 0; s < nsteps && !rc; s++) {
-        determine_min_max_levels(m);
-        if ((rc = upload_tables(m))) break;
-        ev.clear(); kind.clear(); cells.clear();
-        if ((rc = mark(-1, 0))) break;
-        for (int pass = 0; pass < 3 && !rc; pass++)
-            for (int L = m->levelmin; L <= m->levelmax && !rc; L++) {
-                int n;
-                const int *lst = level_list(m, L, &n);
-                if (pass == 0) launch_fill_c(m->dm, m->cur, lst, n, 2, OP_COLLIDE, m->ph, true, m->stream);
-                else if (pass == 1) launch_fill_hg(m->dm, m->cur, lst, n, true, true, true, m->stream);
-                else launch_fill_c(m->dm, m->cur ^ 1, lst, n, 1, OP_MACRO_G, m->ph, true, m->stream);
-                rc = mark(pass, n * PS);
-            }
-        if (rc) break;
-        HIPCHK(hipGetLastError());
+    auto flush = [&]() -> int {
         HIPCHK(hipStreamSynchronize(m->stream));
         for (size_t k = 1; k < ev.size(); k++) {
             float ms = 0.f;
@@ -998,8 +1030,23 @@ int amrlbm_profile_steps(amrlbm_mesh *m, int nsteps, int regrid_every, double st
         }
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         ev.clear();
-        m->cur ^= 1;
-        if ((m->T + 1) % regrid_every == 0) {
+        kind.clear();
+        cells.clear();
+        return AMRLBM_OK;
+    };
+    int rc = AMRLBM_OK, done = 0;
+    while (done < nsteps && !rc) {
+        int len = segment_length(m, nsteps - done, regrid_every);
+        determine_min_max_levels(m);
+        if ((rc = upload_tables(m))) break;
+        if ((rc = mark(-1, 0))) break;
+        if ((rc = enqueue_segment(m, m->cur, len, mark))) break;
+        HIPCHK(hipGetLastError());
+        if ((rc = flush())) break;
+        m->cur ^= (len & 1);
+        m->T += len;
+        done += len;
+        if (m->T % regrid_every == 0) {
             hipEvent_t a, b;
             HIPCHK(hipEventCreate(&a));
             HIPCHK(hipEventCreate(&b));
@@ -1014,7 +1061,6 @@ int amrlbm_profile_steps(amrlbm_mesh *m, int nsteps, int regrid_every, double st
             (void)hipEventDestroy(a);
             (void)hipEventDestroy(b);
         }
-        m->T++;
     }
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     if (rc) return rc;
@@ -1059,7 +1105,8 @@ namespace {
 struct OneBlock {
     DevMesh dm{};
     BlockDev *blk = nullptr;
-    int *nb = nullptr, *list = nullptr, *flags = nullptr;
+    NbDesc *nb = nullptr;
+    int *list = nullptr, *flags = nullptr;
     hipStream_t s = nullptr;
     Phys ph{};
     ~OneBlock()
@@ -1087,7 +1134,7 @@ struct OneBlock {
         HIPCHK(hipMalloc(&dm.C, sizeof(double) * CS));
         HIPCHK(hipMalloc(&dm.PU, sizeof(double) * PUB));
         HIPCHK(hipMalloc(&blk, sizeof(BlockDev)));
-        HIPCHK(hipMalloc(&nb, sizeof(int) * 8));
+        HIPCHK(hipMalloc(&nb, sizeof(NbDesc) * 8));
         HIPCHK(hipMalloc(&list, sizeof(int)));
         HIPCHK(hipMalloc(&flags, sizeof(int)));
         HIPCHK(hipMemsetAsync(dm.pop[0], 0, sizeof(double) * POPB, s));
@@ -1097,12 +1144,13 @@ struct OneBlock {
         HIPCHK(hipMemsetAsync(list, 0, sizeof(int), s));
         HIPCHK(hipMemsetAsync(flags, 0, sizeof(int), s));
         BlockDev b{0, dx, 0, 0, 1, -1, 0, 0};
-        int nbh[8] = {-1, -2, -3, -4, -5, -6, -7, -8};
+        NbDesc nbh[8];
+        for (int k = 0; k < 8; k++) nbh[k] = NbDesc{-(k + 1), -1, 0, -1};
         HIPCHK(hipMemcpyAsync(blk, &b, sizeof(b), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(nb, nbh, sizeof(nbh), hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));
         dm.blk = blk;
-        dm.nb = nb;
+        dm.nbd = nb;
         dm.err_flags = flags;
         return AMRLBM_OK;
     }

@@ -38,6 +38,16 @@ KERNEL_NAMES = {"c2_collide": "k_fill_c<2,OP_COLLIDE>", "hg_stream": "k_fill_hg"
                 "c_macro_g": "k_fill_c<1,OP_MACRO_G>"}
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(workload, steps, regrid_every):
     """The reference Fortran/OpenMP build (oracle/_ref) on the host cores for a
     bounded sample of the same workload; falls back to the oracle's C
@@ -57,8 +67,8 @@ def cpu_baseline(workload, steps, regrid_every):
         return {"value": fine * steps / step_s / 1e6, "unit": "MLUPS (finest-equivalent)",
                 "cores": nthreads, "kind": "reference",
                 "sample": f"{steps} steps of the same workload, reference Fortran/OpenMP build "
-                          f"(amdflang -O3 -fopenmp), {nthreads} threads, step time = sum of the "
-                          f"reference's phase timers"}
+                          f"(amdflang -O3 -fopenmp), {nthreads} threads on {cpu_model()} "
+                          f"(nproc {os.cpu_count()}), step time = sum of the reference's phase timers"}
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     o = oracle.Oracle(workload)
