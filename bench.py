@@ -81,6 +81,21 @@ def cpu_baseline(workload, steps, regrid_every):
             "kind": "port", "sample": f"{n} steps of the same workload, C restatement, 1 thread"}
 
 
+def calibration_copy(nbytes):
+    """One hipMemcpy device-to-device of nbytes: the plain copy kernel whose
+    FETCH_SIZE / WRITE_SIZE calibrate the rocprofv3 traffic counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def pmc_traffic(kernel_key, workload):
     """HBM bytes per launch from committed rocprofv3 --pmc summaries, if present."""
     path = os.path.join(ROOT, "profiles", f"{kernel_key}_{workload}_pmc.json")
@@ -101,6 +116,8 @@ def main():
     ap.add_argument("--regrid-every", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this size (rocprof counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -124,6 +141,8 @@ def main():
     nleaf_mean = 0.5 * (c0["nleaf"] + c1["nleaf"])
 
     prof = m.profile_steps(args.steps, k)
+    if args.calibrate_copy_mb:
+        calibration_copy(args.calibrate_copy_mb << 20)
     diag = m.output()
     m.close()
 

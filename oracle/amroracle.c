@@ -1218,6 +1218,8 @@ void orc_step(orc_mesh *m, int regrid_every)
     orc_determine_min_max_levels(m);
 }
 
+void orc_set_step(orc_mesh *m, int T) { m->T = T; }
+
 void orc_run(orc_mesh *m, int nsteps, int regrid_every)
 {
     for (int s = 0; s < nsteps; s++) orc_step(m, regrid_every);

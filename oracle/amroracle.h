@@ -79,6 +79,7 @@ void orc_refine_derefine_mesh(orc_mesh *m, double refin_tol, double deref_tol);
 /* whole step; regrid after step T when (T+1) % regrid_every == 0 */
 void orc_step(orc_mesh *m, int regrid_every);
 void orc_run(orc_mesh *m, int nsteps, int regrid_every);
+void orc_set_step(orc_mesh *m, int T);   /* the step counter T written to snapshots */
 
 /* FP64 diagnostics over leaf interiors: rho_min, rho_max, max|Uy|, mass
  * (AMR_Rising_2D_OMP.f90:464-483) and bubble area, centroid x, y, rise velocity

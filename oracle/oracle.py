@@ -57,6 +57,7 @@ def lib():
         L.orc_set_bubble_c.argtypes = [vp, dp, dp, dp]
         L.orc_step.argtypes = [vp, ip]
         L.orc_run.argtypes = [vp, ip, ip]
+        L.orc_set_step.argtypes = [vp, ip]
         L.orc_diagnostics.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.orc_load_snapshot.argtypes = [vp, ctypes.c_char_p]
         L.orc_save_snapshot.argtypes = [vp, ctypes.c_char_p]
@@ -143,6 +144,9 @@ class Oracle:
 
     def step(self, regrid_every=1):
         lib().orc_step(self.h, regrid_every)
+
+    def set_step(self, T):
+        lib().orc_set_step(self.h, T)
 
     def run(self, nsteps, regrid_every=1):
         lib().orc_run(self.h, nsteps, regrid_every)

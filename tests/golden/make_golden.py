@@ -1,0 +1,86 @@
+"""Writes the golden vectors of the reference build (oracle/_ref, compiled from
+the reference sources by oracle/build_ref.sh) into tests/golden/.
+
+    bash oracle/build_ref.sh && python tests/golden/make_golden.py
+
+Outputs
+  reference_hashes.json   SHA-256 of every binary snapshot the reference
+                          harness writes in the sequences below (the file
+                          layout of oracle/ref_harness.f90)
+  tiny_init.bin           the tiny workload's state after set-up (full data)
+  tiny_s1_p3.bin          the tiny workload after the hg ghost fill of step 1
+  cfg1_diag.txt           reference FP64 diagnostics, cfg1, 300 steps
+  cfg2_diag.txt           reference FP64 diagnostics, cfg2, 40 steps, regrid every 10
+Sequences
+  tiny   20 steps, regrid every step, snapshots after every phase of steps 0, 1, 10
+  cfg1   300 steps, regrid every step (init + final)
+  cfg2   40 steps, regrid every 10 (init + final)
+  walk   cfg1 set-up, then 14 moves of the analytic bubble by (3, 5) cells with a
+         Refine_Derefine_Mesh after each move
+"""
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+
+def sha(path):
+    return hashlib.sha256(open(path, "rb").read()).hexdigest()
+
+
+def run(cfg, d, args, env=None):
+    e = dict(os.environ)
+    e.update(env or {})
+    subprocess.run([os.path.join(REF, cfg, "ref_harness"), d] + [str(a) for a in args], check=True,
+                   env=e, stdout=subprocess.DEVNULL)
+
+
+def main():
+    for cfg in ("tiny", "cfg1", "cfg2"):
+        if not os.path.exists(os.path.join(REF, cfg, "ref_harness")):
+            sys.exit(f"reference build missing: run oracle/build_ref.sh ({cfg})")
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        d = os.path.join(tmp, "tiny")
+        os.makedirs(d)
+        run("tiny", d, [20, 1, 1, 1, 0, 1, 10])
+        out["tiny"] = {"steps": 20, "regrid_every": 1, "phase_steps": [0, 1, 10],
+                       "init": sha(os.path.join(d, "init.bin")), "final": sha(os.path.join(d, "final.bin")),
+                       "phases": {f"s{t}_p{k}": sha(os.path.join(d, f"s{t:06d}_p{k}.bin"))
+                                  for t in (0, 1, 10) for k in range(9)}}
+        shutil.copy(os.path.join(d, "init.bin"), os.path.join(HERE, "tiny_init.bin"))
+        shutil.copy(os.path.join(d, "s000001_p3.bin"), os.path.join(HERE, "tiny_s1_p3.bin"))
+
+        d = os.path.join(tmp, "cfg1")
+        os.makedirs(d)
+        run("cfg1", d, [300, 1, 50, 4, -1])
+        out["cfg1"] = {"steps": 300, "regrid_every": 1, "init": sha(os.path.join(d, "init.bin")),
+                       "final": sha(os.path.join(d, "final.bin"))}
+        shutil.copy(os.path.join(d, "diag.txt"), os.path.join(HERE, "cfg1_diag.txt"))
+
+        d = os.path.join(tmp, "cfg2")
+        os.makedirs(d)
+        run("cfg2", d, [40, 10, 10, 8, -1])
+        out["cfg2"] = {"steps": 40, "regrid_every": 10, "init": sha(os.path.join(d, "init.bin")),
+                       "final": sha(os.path.join(d, "final.bin"))}
+        shutil.copy(os.path.join(d, "diag.txt"), os.path.join(HERE, "cfg2_diag.txt"))
+
+        d = os.path.join(tmp, "walk")
+        os.makedirs(d)
+        run("cfg1", d, [0, 1, 1, 1, -1], env={"AMRLBM_WALK": "14 3.0 5.0"})
+        out["walk"] = {"workload": "cfg1", "moves": 14, "step": [3.0, 5.0],
+                       "snapshots": [sha(os.path.join(d, f"walk{k:03d}.bin")) for k in range(1, 15)]}
+    with open(os.path.join(HERE, "reference_hashes.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote", os.path.join(HERE, "reference_hashes.json"))
+
+
+if __name__ == "__main__":
+    main()
