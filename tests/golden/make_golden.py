@@ -17,6 +17,9 @@ Sequences
   cfg2   40 steps, regrid every 10 (init + final)
   walk   cfg1 set-up, then 14 moves of the analytic bubble by (3, 5) cells with a
          Refine_Derefine_Mesh after each move
+  dropin the reference's complete program (oracle/build_dropin.sh: 16x16 blocks,
+         L0=64, three roots, Tf=12800 steps, regrid every step): its Output table
+         (dropin_output.txt) and the SHA-256 of its 21 Tecplot files
 """
 import hashlib
 import json
@@ -33,6 +36,21 @@ REF = os.path.join(ROOT, "oracle", "_ref")
 
 def sha(path):
     return hashlib.sha256(open(path, "rb").read()).hexdigest()
+
+
+def output_table(stdout):
+    """The rows Output prints (AMR_Rising_2D_OMP.f90:486-487): T, Nblks, nLeaf,
+    KK, rho_min, rho_max, U_max, mass, levels."""
+    rows = []
+    for line in stdout.splitlines():
+        parts = line.split()
+        if len(parts) == 10:
+            try:
+                float(parts[0]); [int(v) for v in parts[1:4]]
+                rows.append(" ".join(parts))
+            except ValueError:
+                pass
+    return rows
 
 
 def run(cfg, d, args, env=None):
@@ -77,6 +95,17 @@ def main():
         run("cfg1", d, [0, 1, 1, 1, -1], env={"AMRLBM_WALK": "14 3.0 5.0"})
         out["walk"] = {"workload": "cfg1", "moves": 14, "step": [3.0, 5.0],
                        "snapshots": [sha(os.path.join(d, f"walk{k:03d}.bin")) for k in range(1, 15)]}
+        exe = os.path.join(REF, "dropin", "ref_program")
+        if os.path.exists(exe):
+            d = os.path.join(tmp, "dropin")
+            os.makedirs(d)
+            res = subprocess.run([exe], cwd=d, check=True, capture_output=True, text=True)
+            rows = output_table(res.stdout)
+            with open(os.path.join(HERE, "dropin_output.txt"), "w") as f:
+                f.write("\n".join(rows) + "\n")
+            out["dropin"] = {"tecplot": {f: sha(os.path.join(d, f)) for f in sorted(os.listdir(d))
+                                         if f.startswith("Rising_Bubble_2D_")},
+                             "rows": len(rows)}
     with open(os.path.join(HERE, "reference_hashes.json"), "w") as f:
         json.dump(out, f, indent=1)
     print("wrote", os.path.join(HERE, "reference_hashes.json"))
