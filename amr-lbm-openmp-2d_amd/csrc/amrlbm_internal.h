@@ -54,12 +54,43 @@ struct DevMesh {
     int *err_flags;          // [0]: bad level jump seen by a ghost fill
 };
 
+// Ghost program of one block for one fill kind: the block's own ghost cells
+// and the ghost cells of coarser neighbours they read ("aux" cells), each one
+// interpolation recipe over interior values or earlier cells, grouped into
+// stages that depend only on earlier stages. Built on the host whenever the
+// topology changes; evaluated in LDS by one workgroup per leaf.
+enum FillKind { KIND_C1 = 0, KIND_C2 = 1, KIND_HG = 2 };
+enum NodeType { NT_COPY = 0, NT_FINE = 1, NT_COARSE = 2, NT_BB_BOT = 3, NT_BB_TOP = 4 };
+constexpr int MAX_STAGES = 6;
+constexpr int NAUX_MAX = 128;       // aux cells per block and kind
+// LDS cell strides: own cells first (C: 20x20, populations: 18x18), aux after
+constexpr int LC_C = CS + NAUX_MAX;
+constexpr int LC_F = FS + NAUX_MAX;
+struct PNode {
+    int dest;     // LDS cell written (own cell index < CS / FS, else aux)
+    int info;     // type | wx pattern << 4 | wy pattern << 6
+    int ref[9];   // >= 0: element offset in the global field; < 0: -(1 + LDS cell)
+};
+struct PHdr {
+    int off;                  // first node
+    int nst;                  // stages
+    int end[MAX_STAGES];      // cumulative node count at the end of each stage
+};
+struct Prog {
+    const PHdr *hdr;          // one per leaf, leaf-list order
+    const PNode *nodes;
+};
+
 // kernel launchers (amrlbm_kernels.hip); all enqueue on stream s
 enum FillOp { OP_NONE = 0, OP_COLLIDE = 1, OP_MACRO_G = 2, OP_MACRO_G_COLLIDE = 3 };
-void launch_fill_c(const DevMesh &m, int cur, const int *list, int n, int rings, int op, const Phys &ph,
-                   bool do_fill, hipStream_t s);
-void launch_fill_hg(const DevMesh &m, int cur, const int *list, int n, bool stream_after, bool macro_h,
-                    bool do_fill, hipStream_t s, bool copy_ghosts = false);
+// C fill (kind C1 or C2) from a ghost program, then op; with prog.hdr == null
+// the block's ghosts are read from memory instead (no fill)
+void launch_prog_c(const DevMesh &m, int cur, const int *list, int n, int kind, int op, const Prog &prog,
+                   const Phys &ph, hipStream_t s);
+// hg fill from a ghost program (or ghosts from memory with prog.hdr == null),
+// then optionally the stream (+ C = sum h, + ghost ring carried over)
+void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Prog &prog, bool stream_after,
+                    bool macro_h, bool copy_ghosts, hipStream_t s);
 void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream_t s);
 void launch_err(const DevMesh &m, const int *list, int n, double *err, hipStream_t s);
 void launch_restrict(const DevMesh &m, int cur, const int *parents, int n, hipStream_t s);

@@ -197,153 +197,71 @@ __device__ __forceinline__ void macro_g_cell(const double *sC, int i, int j, int
 }
 
 // ---------------------------------------------------------------------------
-// Ghost fills (Fill_Ghost_Cells_{C,C2,hg}.f90, Interpolations.f90:2-138)
+// Ghost fills (Fill_Ghost_Cells_{C,C2,hg}.f90, Interpolations.f90:2-138,
+// BC_Cell.f90) by ghost program: every value a block's fill needs is built
+// from interior data of the previous kernel, the coarse neighbours' ghost
+// cells included, so one launch covers every level.
 // ---------------------------------------------------------------------------
-// Ghost cell t of a ring-R block: sides L,R,B,T (16R cells each, layer-major),
-// then the corners LB,LT,RB,RT (R x R each). d is the neighbour direction
-// 0..7 = nghbr(1..8): L R B T LB LT RB RT (AMR_Rising_2D_OMP.f90:97-101).
-template <int R>
-__device__ __forceinline__ void ghost_cell(int t, int &d, int &x, int &y)
+// Ghost cell t of the ring-1 population block: sides L,R,B,T then the corners.
+__device__ __forceinline__ int ring1_cell(int t)
 {
-    constexpr int SIDE = 16 * R;
-    if (t < 4 * SIDE) {
-        d = t / SIDE;
-        int k = t - d * SIDE, l = k / 16, pos = k % 16 + 1;
-        x = d == 0 ? -l : (d == 1 ? 17 + l : pos);
-        y = d == 2 ? -l : (d == 3 ? 17 + l : pos);
+    int x, y;
+    if (t < 64) {
+        int d = t / 16, pos = t % 16 + 1;
+        x = d == 0 ? 0 : (d == 1 ? 17 : pos);
+        y = d == 2 ? 0 : (d == 3 ? 17 : pos);
     } else {
-        int k = t - 4 * SIDE, c = k / (R * R), r = k % (R * R), lx = r % R, ly = r / R;
-        d = 4 + c;
-        x = c < 2 ? -lx : 17 + lx;
-        y = (c == 0 || c == 2) ? -ly : 17 + ly;
+        int c = t - 64;
+        x = c < 2 ? 0 : 17;
+        y = (c == 0 || c == 2) ? 0 : 17;
     }
+    return y * FW + x;
 }
 
-// One axis of Biquadratic_from_Fine as the directional fills use it: three
-// source cells in the 32-wide fine frame of the refined neighbour, listed from
-// the interface outward (normal axis) or upward (tangential axis), with the
-// weights (3,6,-1) or (-1,6,3) by the parity of the coarse index
-// (Interpolations.f90:13-31; Fill_Left_hg, Fill_Ghost_Cells_hg.f90:74-85).
-__device__ __forceinline__ void fine_axis(int s, int t, int f[3], int w[3])
+// axis weights of Biquadratic_from_Fine / _from_Coarse (Interpolations.f90:13-31, :85-101)
+__device__ __forceinline__ void wpat(int p, double w[3])
 {
-    int k = s == 0 ? t : (s < 0 ? 1 - t : t - NX);
-    int m0;
-    if (k & 1) {
-        m0 = 2 * k - 1;
-        w[0] = 3; w[1] = 6; w[2] = -1;
-    } else {
-        m0 = 2 * k - 2;
-        w[0] = -1; w[1] = 6; w[2] = 3;
-    }
-#pragma unroll
-    for (int a = 0; a < 3; a++) f[a] = s < 0 ? 2 * NX + 1 - (m0 + a) : m0 + a;
+    w[0] = p == 0 ? 3.0 : (p == 1 ? -1.0 : (p == 2 ? 5.0 : -3.0));
+    w[1] = p < 2 ? 6.0 : 30.0;
+    w[2] = p == 0 ? -1.0 : (p == 1 ? 3.0 : (p == 2 ? -3.0 : 5.0));
 }
 
-// One axis of Biquadratic_from_Coarse: three coarse cells of the coarser
-// neighbour around the fine cell, the first lying toward the block being
-// filled on a normal axis (Fill_Left_hg:91-96, Fill_Left_Bottom_hg:311-315),
-// weights (5,30,-3) on the side the fine cell leans to (Interpolations.f90:85-101).
-__device__ __forceinline__ void coarse_axis(int s, int t, int pos, int c[3], int w[3])
+// one recipe: copy, or the tensor product (pass along y, then x) divided by
+// 64 (from finer) or 1024 (from coarser)
+__device__ __forceinline__ double eval_node(const PNode &nd, const double *__restrict__ G, int gs,
+                                            const double *L, int ls)
 {
-    int fp = pos * NX + t;
-    int cp = (fp + 1) >> 1;
-    int sn = cp < 1 ? -1 : (cp > NX ? 1 : 0);
-    int cc = cp - sn * NX;
-    bool upper = (fp & 1) == 0;
-    if (s < 0) {
-        c[0] = cc + 1; c[1] = cc; c[2] = cc - 1;
-        if (upper) { w[0] = 5; w[1] = 30; w[2] = -3; } else { w[0] = -3; w[1] = 30; w[2] = 5; }
-    } else {
-        c[0] = cc - 1; c[1] = cc; c[2] = cc + 1;
-        if (upper) { w[0] = -3; w[1] = 30; w[2] = 5; } else { w[0] = 5; w[1] = 30; w[2] = -3; }
-    }
-}
-
-// Interpolation recipe of one ghost cell: mode 0 copy (case (i)), 1 from the
-// finer children (case (ii), /64), 2 from the coarser neighbour (case (iii),
-// /1024); off[a][b] = element offsets of the source cells for x-point a and
-// y-point b, valid for component 0 of a field laid out with cell index
-// y*W + x inside a slot of SLOT elements.
-struct Recipe {
-    int mode;
-    int off[3][3];
-    int wx[3], wy[3];
-};
-
-template <int SLOT, int W, int ORG>
-__device__ __forceinline__ bool make_recipe(const NbDesc &nb, const BlockDev &me, int d, int x, int y, Recipe &r)
-{
-    const int sx = c_dsx[d], sy = c_dsy[d];
-    const int ns = nb.id - 1;
-    if (nb.level == me.level) {
-        if (nb.isleaf) {
-            r.mode = 0;
-            r.off[0][0] = ns * SLOT + (y - sy * NY + ORG) * W + (x - sx * NX + ORG);
-            return true;
-        }
-        r.mode = 1;
-        int fx[3], fy[3];
-        fine_axis(sx, x, fx, r.wx);
-        fine_axis(sy, y, fy, r.wy);
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            int ix = fx[a] > NX, lx = fx[a] - ix * NX;
-#pragma unroll
-            for (int b = 0; b < 3; b++) {
-                int iy = fy[b] > NY, ly = fy[b] - iy * NY;
-                r.off[a][b] = (nb.child0 + ix + 2 * iy) * SLOT + (ly + ORG) * W + (lx + ORG);
-            }
-        }
-        return true;
-    }
-    if (me.level == nb.level + 1) {
-        r.mode = 2;
-        int cx[3], cy[3];
-        coarse_axis(sx, x, me.I & 1, cx, r.wx);
-        coarse_axis(sy, y, me.J & 1, cy, r.wy);
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-            for (int b = 0; b < 3; b++) r.off[a][b] = ns * SLOT + (cy[b] + ORG) * W + (cx[a] + ORG);
-        return true;
-    }
-    return false;
-}
-
-// apply a recipe to the field base + shift (tensor product: y pass, then x)
-__device__ __forceinline__ double apply_recipe(const Recipe &r, const double *__restrict__ base, int shift)
-{
-    if (r.mode == 0) return base[r.off[0][0] + shift];
+    const int type = nd.info & 15;
+#define PV(r) ((r) >= 0 ? G[(r) + gs] : L[-1 - (r) + ls])
+    if (type == NT_COPY) return PV(nd.ref[0]);
+    double wx[3], wy[3];
+    wpat((nd.info >> 4) & 3, wx);
+    wpat((nd.info >> 6) & 3, wy);
     double t[3];
 #pragma unroll
-    for (int a = 0; a < 3; a++)
-        t[a] = (double)r.wy[0] * base[r.off[a][0] + shift] + (double)r.wy[1] * base[r.off[a][1] + shift] +
-               (double)r.wy[2] * base[r.off[a][2] + shift];
-    double o = (double)r.wx[0] * t[0] + (double)r.wx[1] * t[1] + (double)r.wx[2] * t[2];
-    return r.mode == 1 ? o / 64.0 : o / 1024.0;
+    for (int a = 0; a < 3; a++) t[a] = wy[0] * PV(nd.ref[3 * a]) + wy[1] * PV(nd.ref[3 * a + 1]) + wy[2] * PV(nd.ref[3 * a + 2]);
+    double o = wx[0] * t[0] + wx[1] * t[1] + wx[2] * t[2];
+#undef PV
+    return type == NT_FINE ? o / 64.0 : o / 1024.0;
 }
 
-// Fill_Ghost_Cells_C (R=1) / Fill_Ghost_Cells_C2 (R=2) for the leaves in
-// list (one level), then optionally the block's collision (after C2) or
-// Macroscopic_Values_g_AMR (after C): both read only this block's C, which
-// is final once its ghosts are filled, and no later fill pass reads what
-// they write. OP_MACRO_G_COLLIDE runs the previous step's
-// Macroscopic_Values_g_AMR and then this step's collision after a C2 fill:
-// between two steps with no regrid, the C fill that ends a step and the C2
-// fill that starts the next compute the same first ghost ring from the same
-// C, so U and P come out identical. The block's C is staged in LDS; its own
-// loads are issued before the neighbour gathers so the two overlap.
-template <int R, int OP>
-__global__ __launch_bounds__(256) void k_fill_c(DevMesh m, int cur, const int *__restrict__ list, Phys ph,
-                                                int do_fill)
+// Fill_Ghost_Cells_C (kind C1) / Fill_Ghost_Cells_C2 (kind C2) of every leaf
+// in list by its ghost program, then the op on the block: the collision
+// (after C2), Macroscopic_Values_g_AMR (after C), or both (OP_MACRO_G_COLLIDE:
+// U, P of the previous step then this step's collision; between two steps
+// with no regrid, the C fill that ends a step and the C2 fill that starts the
+// next compute the same first ghost ring from the same C, so U, P come out
+// identical). The collision and U,P read only this block's C. With
+// prog.hdr == null the ghosts are read from memory (no fill).
+template <int OP>
+__global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *__restrict__ list, Prog prog, Phys ph)
 {
-    const int slot = list[blockIdx.x];
+    const int k = blockIdx.x;
+    const int slot = list[k];
     const int tid = threadIdx.x;
     const BlockDev me = m.blk[slot];
     double *C = m.C + (size_t)slot * CS;
-    __shared__ NbDesc nbd[8];
-    __shared__ double sC[CS];
-    if (tid < 8) nbd[tid] = m.nbd[slot * 8 + tid];
+    __shared__ double sC[LC_C];
     const int i = tid % NX + 1, j = tid / NX + 1;
     double *F = m.pop[cur] + (size_t)slot * POPB;
     double *PU = m.PU + (size_t)slot * PUB;
@@ -361,58 +279,29 @@ __global__ __launch_bounds__(256) void k_fill_c(DevMesh m, int cur, const int *_
 #pragma unroll
         for (int q = 0; q < 9; q++) g[q] = F[fidx(1, q, i, j)];
     }
-    if (do_fill) {
+    const bool fill = prog.hdr != nullptr;
+    if (fill)
         sC[cidx(i, j)] = C[cidx(i, j)];
-    } else {
+    else
         for (int t = tid; t < CS; t += 256) sC[t] = C[t];
-    }
     __syncthreads();
-    if (do_fill) {
-        constexpr int NG = 64 * R + 4 * R * R;
-        if (tid < NG) {
-            int d, x, y;
-            ghost_cell<R>(tid, d, x, y);
-            const NbDesc nb = nbd[d];
-            if (nb.id > 0) {
-                Recipe rc;
-                if (make_recipe<CS, CW, 1>(nb, me, d, x, y, rc)) {
-                    double v = apply_recipe(rc, m.C, 0);
-                    sC[cidx(x, y)] = v;
-                    C[cidx(x, y)] = v;
-                } else {
-                    atomicOr(m.err_flags, 1);
-                }
+    if (fill) {
+        const PHdr hd = prog.hdr[k];
+        const PNode *nodes = prog.nodes + hd.off;
+        int begin = 0;
+        for (int s = 0; s < hd.nst; s++) {
+            const int end = hd.end[s];
+            for (int t = begin + tid; t < end; t += 256) {
+                const PNode nd = nodes[t];
+                const double v = eval_node(nd, m.C, 0, sC, 0);
+                sC[nd.dest] = v;
+                if (nd.dest < CS) C[nd.dest] = v;
             }
+            begin = end;
+            __syncthreads();
         }
-        __syncthreads();
-        // wall mirror: Symmetric_C_3/4 (BC_Cell.f90:46-72), Symmetric_C2_3/4 (:76-104)
-        if (tid < 64) {
-            int top = tid >= 32, X = (tid & 31) - (R - 1);
-            if (nbd[2 + top].id < 0 && X <= 17 + (R - 1)) {
-                if (!top) {
-                    if (R == 2) {
-                        double v = sC[cidx(X, 2)];
-                        sC[cidx(X, -1)] = v;
-                        C[cidx(X, -1)] = v;
-                    }
-                    double v = sC[cidx(X, 1)];
-                    sC[cidx(X, 0)] = v;
-                    C[cidx(X, 0)] = v;
-                } else {
-                    double v = sC[cidx(X, NY)];
-                    sC[cidx(X, NY + 1)] = v;
-                    C[cidx(X, NY + 1)] = v;
-                    if (R == 2) {
-                        v = sC[cidx(X, NY - 1)];
-                        sC[cidx(X, NY + 2)] = v;
-                        C[cidx(X, NY + 2)] = v;
-                    }
-                }
-            }
-        }
-        __syncthreads();
     }
-    if (OP == OP_MACRO_G_COLLIDE) {
+    if (OP == OP_MACRO_G_COLLIDE || OP == OP_MACRO_G) {
         macro_g_cell(sC, i, j, me.dx, g, ph, P, Ux, Uy);
         PU[pidx(i, j)] = P;
         PU[PS + pidx(i, j)] = Ux;
@@ -425,18 +314,15 @@ __global__ __launch_bounds__(256) void k_fill_c(DevMesh m, int cur, const int *_
             F[fidx(0, q, i, j)] = h[q];
             F[fidx(1, q, i, j)] = g[q];
         }
-    } else if (OP == OP_MACRO_G) {
-        macro_g_cell(sC, i, j, me.dx, g, ph, P, Ux, Uy);
-        PU[pidx(i, j)] = P;
-        PU[PS + pidx(i, j)] = Ux;
-        PU[2 * PS + pidx(i, j)] = Uy;
     }
 }
 
+// LDS layout of a population block: component c, cell (x,y) at c*LC_F + y*FW + x
+__device__ __forceinline__ int lidx(int c, int x, int y) { return c * LC_F + y * FW + x; }
+
 // Perfect_Shift_AMR / Streaming_LaxWendroff_AMR (LBM_Subroutines.f90:41-90)
 // of one cell from the LDS copy of the block into G, and C = sum(h) (:193-201)
-__device__ __forceinline__ void stream_cell(const double *__restrict__ sF, double *__restrict__ G, int i, int j,
-                                            int dx, double *csum)
+__device__ __forceinline__ double stream_cell(const double *sF, double *__restrict__ G, int i, int j, int dx)
 {
     const double cfl = 1.0 / (double)dx;
     const double a = cfl / 2.0 * (1.0 + cfl), b = 1.0 - cfl * cfl, c = cfl / 2.0 * (1.0 - cfl);
@@ -445,105 +331,89 @@ __device__ __forceinline__ void stream_cell(const double *__restrict__ sF, doubl
     for (int set = 0; set < 2; set++) {
 #pragma unroll
         for (int q = 0; q < 9; q++) {
+            const int cp = set * 9 + q;
             double v;
             if (q == 0)
-                v = sF[fidx(set, 0, i, j)];
+                v = sF[lidx(cp, i, j)];
             else if (cfl > 0.9)
-                v = sF[fidx(set, q, i - c_ex[q], j - c_ey[q])];
+                v = sF[lidx(cp, i - c_ex[q], j - c_ey[q])];
             else
-                v = a * sF[fidx(set, q, i - c_ex[q], j - c_ey[q])] + b * sF[fidx(set, q, i, j)] -
-                    c * sF[fidx(set, q, i + c_ex[q], j + c_ey[q])];
+                v = a * sF[lidx(cp, i - c_ex[q], j - c_ey[q])] + b * sF[lidx(cp, i, j)] -
+                    c * sF[lidx(cp, i + c_ex[q], j + c_ey[q])];
             G[fidx(set, q, i, j)] = v;
             if (set == 0) s += v;
         }
     }
-    *csum = s;
+    return s;
 }
 
-// Fill_Ghost_Cells_hg for the leaves in list (one level), bounce-back walls
-// (BC_Cell.f90:2-42), then optionally the stream of the same blocks into the
-// other population buffer and C = sum(h). The stream writes only the other
-// buffer, so finer passes still read the post-collision populations and
-// ghosts of this level. The block (18 populations with ghosts) is staged in
-// LDS; each of 136 threads fills one ghost cell of one set for all 9
-// populations from one interpolation recipe.
-// flags: 1 fill, 2 stream, 4 C = sum h, 8 carry the ghost ring into the new buffer
-__global__ __launch_bounds__(256) void k_fill_hg(DevMesh m, int cur, const int *__restrict__ list, int flags)
+// Fill_Ghost_Cells_hg of every leaf in list by its ghost program (bounce-back
+// rows included), then optionally the stream into the other population buffer
+// and C = sum(h). Each of the block's cells and its coarse neighbours' needed
+// ghost cells is one recipe applied to all 9 populations of one set.
+// flags: 1 fill (else ghosts from memory), 2 stream, 4 C = sum h,
+// 8 carry the ghost ring into the new buffer (as the in-place reference stream)
+__global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *__restrict__ list, Prog prog,
+                                                 int flags)
 {
-    const int slot = list[blockIdx.x];
+    const int k = blockIdx.x;
+    const int slot = list[k];
     const int tid = threadIdx.x;
     const BlockDev me = m.blk[slot];
     const double *__restrict__ Fc = m.pop[cur];
     double *F = m.pop[cur] + (size_t)slot * POPB;
-    __shared__ NbDesc nbd[8];
-    __shared__ double sF[POPB];
-    if (tid < 8) nbd[tid] = m.nbd[slot * 8 + tid];
+    __shared__ double sF[18 * LC_F];
     const int i = tid % NX + 1, j = tid / NX + 1;
 #pragma unroll
-    for (int c = 0; c < 18; c++) sF[c * FS + j * FW + i] = F[c * FS + j * FW + i];
+    for (int c = 0; c < 18; c++) sF[lidx(c, i, j)] = F[c * FS + j * FW + i];
     if (!(flags & 1)) {
         for (int t = tid; t < 68 * 18; t += 256) {
-            int comp = t / 68, cell = t - comp * 68, d, x, y;
-            ghost_cell<1>(cell, d, x, y);
-            sF[comp * FS + y * FW + x] = F[comp * FS + y * FW + x];
+            int c = t / 68, cell = ring1_cell(t - c * 68);
+            sF[c * LC_F + cell] = F[c * FS + cell];
         }
     }
     __syncthreads();
     if (flags & 1) {
-        if (tid < 136) {
-            int set = tid / 68, cell = tid - set * 68, d, x, y;
-            ghost_cell<1>(cell, d, x, y);
-            const NbDesc nb = nbd[d];
-            if (nb.id > 0) {
-                Recipe rc;
-                if (make_recipe<POPB, FW, 0>(nb, me, d, x, y, rc)) {
-                    double v[9];
+        const PHdr hd = prog.hdr[k];
+        const PNode *nodes = prog.nodes + hd.off;
+        int begin = 0;
+        for (int s = 0; s < hd.nst; s++) {
+            const int end = hd.end[s];
+            for (int t = 2 * begin + tid; t < 2 * end; t += 256) {
+                const PNode nd = nodes[t >> 1];
+                const int set = t & 1, type = nd.info & 15;
+                double v[9];
+                if (type < NT_BB_BOT) {
 #pragma unroll
-                    for (int q = 0; q < 9; q++) v[q] = apply_recipe(rc, Fc, (set * 9 + q) * FS);
+                    for (int q = 0; q < 9; q++) v[q] = eval_node(nd, Fc, (set * 9 + q) * FS, sF, (set * 9 + q) * LC_F);
+                } else {
+                    // Bounce_Back_3/4 (BC_Cell.f90:2-42): f_q(X,wall) = f_opp(q)(Xn, first row)
 #pragma unroll
                     for (int q = 0; q < 9; q++) {
-                        sF[fidx(set, q, x, y)] = v[q];
-                        F[fidx(set, q, x, y)] = v[q];
+                        int e = c_ex[q] * c_ey[q];
+                        int r = nd.ref[1 + (type == NT_BB_TOP ? -e : e)];
+                        int cs = set * 9 + c_opp[q];
+                        v[q] = r >= 0 ? Fc[r + cs * FS] : sF[-1 - r + cs * LC_F];
                     }
-                } else {
-                    atomicOr(m.err_flags, 1);
-                }
-            }
-        }
-        __syncthreads();
-        // Bounce_Back_3 (bottom, row 0) and Bounce_Back_4 (top, row ny+1)
-        if (tid < 2 * 2 * FW) {
-            int top = tid / (2 * FW), r = tid - top * (2 * FW), set = r / FW, X = r - set * FW;
-            if (nbd[2 + top].id < 0) {
-                int Y = top ? NY + 1 : 0, Yi = top ? NY : 1;
-                double v[9];
-                v[0] = sF[fidx(set, 0, X, Yi)];
-#pragma unroll
-                for (int q = 1; q < 9; q++) {
-                    int e = c_ex[q] * c_ey[q];
-                    int Xn = top ? X - e : X + e;
-                    Xn = Xn < 0 ? 0 : (Xn > NX + 1 ? NX + 1 : Xn);
-                    v[q] = sF[fidx(set, c_opp[q], Xn, Yi)];
                 }
 #pragma unroll
                 for (int q = 0; q < 9; q++) {
-                    sF[fidx(set, q, X, Y)] = v[q];
-                    F[fidx(set, q, X, Y)] = v[q];
+                    sF[(set * 9 + q) * LC_F + nd.dest] = v[q];
+                    if (nd.dest < FS) F[(set * 9 + q) * FS + nd.dest] = v[q];
                 }
             }
+            begin = end;
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (!(flags & 2)) return;
     double *G = m.pop[cur ^ 1] + (size_t)slot * POPB;
-    double csum;
-    stream_cell(sF, G, i, j, me.dx, &csum);
+    const double csum = stream_cell(sF, G, i, j, me.dx);
     if (flags & 4) m.C[(size_t)slot * CS + cidx(i, j)] = csum;
     if (flags & 8) {
         for (int t = tid; t < 68 * 18; t += 256) {
-            int comp = t / 68, cell = t - comp * 68, d, x, y;
-            ghost_cell<1>(cell, d, x, y);
-            G[comp * FS + y * FW + x] = sF[comp * FS + y * FW + x];
+            int c = t / 68, cell = ring1_cell(t - c * 68);
+            G[c * FS + cell] = sF[c * LC_F + cell];
         }
     }
 }
@@ -786,31 +656,24 @@ __global__ __launch_bounds__(256) void k_diag_final(const double *part, int n, d
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-void launch_fill_c(const DevMesh &m, int cur, const int *list, int n, int rings, int op, const Phys &ph, bool do_fill,
-                   hipStream_t s)
+void launch_prog_c(const DevMesh &m, int cur, const int *list, int n, int kind, int op, const Prog &prog,
+                   const Phys &ph, hipStream_t s)
 {
     if (n <= 0) return;
+    (void)kind;   // the program decides the ring; the op needs C with the rings it reads
     dim3 g(n), b(256);
-    int f = do_fill ? 1 : 0;
-    if (rings == 2) {
-        if (op == OP_MACRO_G_COLLIDE)
-            hipLaunchKernelGGL((k_fill_c<2, OP_MACRO_G_COLLIDE>), g, b, 0, s, m, cur, list, ph, f);
-        else if (op == OP_COLLIDE) hipLaunchKernelGGL((k_fill_c<2, OP_COLLIDE>), g, b, 0, s, m, cur, list, ph, f);
-        else if (op == OP_MACRO_G) hipLaunchKernelGGL((k_fill_c<2, OP_MACRO_G>), g, b, 0, s, m, cur, list, ph, f);
-        else hipLaunchKernelGGL((k_fill_c<2, OP_NONE>), g, b, 0, s, m, cur, list, ph, f);
-    } else {
-        if (op == OP_COLLIDE) hipLaunchKernelGGL((k_fill_c<1, OP_COLLIDE>), g, b, 0, s, m, cur, list, ph, f);
-        else if (op == OP_MACRO_G) hipLaunchKernelGGL((k_fill_c<1, OP_MACRO_G>), g, b, 0, s, m, cur, list, ph, f);
-        else hipLaunchKernelGGL((k_fill_c<1, OP_NONE>), g, b, 0, s, m, cur, list, ph, f);
-    }
+    if (op == OP_MACRO_G_COLLIDE) hipLaunchKernelGGL(k_prog_c<OP_MACRO_G_COLLIDE>, g, b, 0, s, m, cur, list, prog, ph);
+    else if (op == OP_COLLIDE) hipLaunchKernelGGL(k_prog_c<OP_COLLIDE>, g, b, 0, s, m, cur, list, prog, ph);
+    else if (op == OP_MACRO_G) hipLaunchKernelGGL(k_prog_c<OP_MACRO_G>, g, b, 0, s, m, cur, list, prog, ph);
+    else hipLaunchKernelGGL(k_prog_c<OP_NONE>, g, b, 0, s, m, cur, list, prog, ph);
 }
 
-void launch_fill_hg(const DevMesh &m, int cur, const int *list, int n, bool stream_after, bool macro_h, bool do_fill,
-                    hipStream_t s, bool copy_ghosts)
+void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Prog &prog, bool stream_after,
+                    bool macro_h, bool copy_ghosts, hipStream_t s)
 {
     if (n <= 0) return;
-    int flags = (do_fill ? 1 : 0) | (stream_after ? 2 : 0) | (macro_h ? 4 : 0) | (copy_ghosts ? 8 : 0);
-    hipLaunchKernelGGL(k_fill_hg, dim3(n), dim3(256), 0, s, m, cur, list, flags);
+    int flags = (prog.hdr ? 1 : 0) | (stream_after ? 2 : 0) | (macro_h ? 4 : 0) | (copy_ghosts ? 8 : 0);
+    hipLaunchKernelGGL(k_prog_hg, dim3(n), dim3(256), 0, s, m, cur, list, prog, flags);
 }
 
 void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream_t s)

@@ -79,6 +79,12 @@ struct amrlbm_mesh {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_step_ms = 0.0;
 
+    // ghost programs per fill kind (C1, C2, HG), leaf-list order
+    PHdr *d_phdr[3] = {nullptr, nullptr, nullptr};
+    PNode *d_pnode[3] = {nullptr, nullptr, nullptr};
+    size_t phdr_cap[3] = {0, 0, 0}, pnode_cap[3] = {0, 0, 0};
+    Prog prog[3] = {};
+
     // one HIP graph per (segment length, population-buffer parity) for the
     // current topology; a segment is the run of steps between two regrids
     struct GraphEntry {
@@ -208,6 +214,246 @@ void remove_children(amrlbm_mesh *m, int par)
 }
 
 // ---------------------------------------------------------------------------
+// ghost programs
+// ---------------------------------------------------------------------------
+// One axis of Biquadratic_from_Fine as the directional fills use it: source
+// cells in the 32-wide fine frame of the refined neighbour, listed from the
+// interface outward (normal axis) or upward (tangential axis); weight pattern
+// 0 = (3,6,-1), 1 = (-1,6,3) by the parity of the coarse index
+// (Interpolations.f90:13-31; Fill_Left_hg, Fill_Ghost_Cells_hg.f90:74-85).
+int fine_axis(int s, int t, int f[3])
+{
+    int k = s == 0 ? t : (s < 0 ? 1 - t : t - NX);
+    int m0 = (k & 1) ? 2 * k - 1 : 2 * k - 2;
+    for (int a = 0; a < 3; a++) f[a] = s < 0 ? 2 * NX + 1 - (m0 + a) : m0 + a;
+    return (k & 1) ? 0 : 1;
+}
+
+// One axis of Biquadratic_from_Coarse: coarse cells around the fine cell,
+// the first lying toward the block being filled on a normal axis
+// (Fill_Left_hg:91-96, Fill_Left_Bottom_hg:311-315); weight pattern
+// 2 = (5,30,-3), 3 = (-3,30,5), the 5 on the side the fine cell leans to
+// (Interpolations.f90:85-101). pos = the block's position in its parent.
+int coarse_axis(int s, int t, int pos, int c[3])
+{
+    int fp = pos * NX + t;
+    int cp = (fp + 1) >> 1;
+    int sn = cp < 1 ? -1 : (cp > NX ? 1 : 0);
+    int cc = cp - sn * NX;
+    bool upper = (fp & 1) == 0;
+    if (s < 0) {
+        c[0] = cc + 1; c[1] = cc; c[2] = cc - 1;
+        return upper ? 2 : 3;
+    }
+    c[0] = cc - 1; c[1] = cc; c[2] = cc + 1;
+    return upper ? 3 : 2;
+}
+
+int nb_value(amrlbm_mesh *m, int n, int k);
+
+// Builds the ghost program of leaf n for one fill kind. Every ghost cell the
+// reference fills for n (Fill_Ghost_Cells_{C,C2,hg}: the eight directional
+// fills, then the wall rows by BC_Cell.f90) becomes one node; a case (iii)
+// fill reads the coarser neighbour's ghost cells, which in the reference were
+// filled in the coarser level's pass, so those cells become nodes too,
+// recursively, in the same operation order. Interior values are read from
+// memory (left by the previous kernel), everything else from LDS.
+struct ProgBuild {
+    amrlbm_mesh *m;
+    int kind, R, n;
+    std::vector<PNode> nodes;
+    std::vector<int> nstage;                 // stage of each node
+    std::vector<int> cell_stage;             // stage of the node that writes LDS cell c
+    std::vector<std::pair<long long, int>> memo;
+    int naux = 0;
+    std::string err;
+
+    int own_cells() const { return kind == KIND_HG ? FS : CS; }
+    int cell_index(int x, int y) const { return kind == KIND_HG ? y * FW + x : (y + 1) * CW + (x + 1); }
+    static bool interior(int x, int y) { return x >= 1 && x <= NX && y >= 1 && y <= NY; }
+    int global_ref(int b, int x, int y) const
+    {
+        return (b - 1) * (kind == KIND_HG ? POPB : CS) + cell_index(x, y);
+    }
+    int find(long long key) const
+    {
+        for (auto &e : memo)
+            if (e.first == key) return e.second;
+        return -1;
+    }
+    int ref(int b, int x, int y, int &st)
+    {
+        st = 0;
+        if (interior(x, y)) return b == n ? -(1 + cell_index(x, y)) : global_ref(b, x, y);
+        int dest = node(b, x, y);
+        if (dest < 0) return 0;
+        st = cell_stage[dest];
+        return -(1 + dest);
+    }
+    int node(int b, int x, int y)
+    {
+        const long long key = ((long long)b << 16) | ((long long)(x + 8) << 8) | (long long)(y + 8);
+        int hit = find(key);
+        if (hit == -2) {
+            err = "cyclic ghost dependency";
+            return -1;
+        }
+        if (hit >= 0) return hit;
+        memo.push_back({key, -2});
+        const size_t slot = memo.size() - 1;
+        PNode nd{};
+        int st = 0, s2 = 0;
+        const Block &B = m->blk[b];
+        const bool bot = y <= 0 && nb_value(m, b, 3) < 0;
+        const bool top = y >= NY + 1 && nb_value(m, b, 4) < 0;
+        if (bot || top) {
+            if (kind == KIND_HG) {   // Bounce_Back_3/4 (BC_Cell.f90:2-42)
+                const int yi = bot ? 1 : NY;
+                nd.info = bot ? NT_BB_BOT : NT_BB_TOP;
+                for (int a = 0; a < 3; a++) {
+                    int xn = std::min(std::max(x - 1 + a, 0), NX + 1);
+                    nd.ref[a] = ref(b, xn, yi, s2);
+                    st = std::max(st, s2);
+                }
+            } else {                 // Symmetric_C_3/4, Symmetric_C2_3/4 (BC_Cell.f90:46-104)
+                const int ys = bot ? 1 - y : 2 * NY + 1 - y;
+                nd.info = NT_COPY;
+                nd.ref[0] = ref(b, x, ys, st);
+            }
+        } else {
+            static const int dtab[3][3] = {{4, 0, 5}, {2, -1, 3}, {6, 1, 7}};   // [sx+1][sy+1]
+            const int sx = x < 1 ? -1 : (x > NX ? 1 : 0), sy = y < 1 ? -1 : (y > NY ? 1 : 0);
+            const int d = dtab[sx + 1][sy + 1];
+            const int N1 = nb_value(m, b, d + 1);
+            if (d < 0 || N1 <= 0) {
+                err = "ghost cell (" + std::to_string(x) + "," + std::to_string(y) + ") of block " + std::to_string(b) +
+                      " has no neighbour to fill it";
+                return -1;
+            }
+            const Block &Q = m->blk[N1];
+            if (Q.level == B.level && Q.isleaf) {   // case (i): copy (Fill_Left_hg:66-70)
+                nd.info = NT_COPY;
+                nd.ref[0] = ref(N1, x - sx * NX, y - sy * NY, st);
+            } else if (Q.level == B.level) {        // case (ii): from the children (Fill_Left_hg:72-85)
+                int fx[3], fy[3];
+                int px = fine_axis(sx, x, fx), py = fine_axis(sy, y, fy);
+                nd.info = NT_FINE | (px << 4) | (py << 6);
+                for (int a = 0; a < 3; a++) {
+                    int ix = fx[a] > NX, lx = fx[a] - ix * NX;
+                    for (int c = 0; c < 3; c++) {
+                        int iy = fy[c] > NY, ly = fy[c] - iy * NY;
+                        nd.ref[3 * a + c] = ref(Q.child[0] + ix + 2 * iy, lx, ly, s2);
+                        st = std::max(st, s2);
+                    }
+                }
+            } else if (B.level == Q.level + 1) {    // case (iii): from the coarser block (Fill_Left_hg:89-99)
+                int cx[3], cy[3];
+                int px = coarse_axis(sx, x, B.I & 1, cx), py = coarse_axis(sy, y, B.J & 1, cy);
+                nd.info = NT_COARSE | (px << 4) | (py << 6);
+                for (int a = 0; a < 3; a++)
+                    for (int c = 0; c < 3; c++) {
+                        nd.ref[3 * a + c] = ref(N1, cx[a], cy[c], s2);
+                        st = std::max(st, s2);
+                    }
+            } else {
+                err = "level jump between blocks " + std::to_string(b) + " and " + std::to_string(N1);
+                return -1;
+            }
+        }
+        if (!err.empty()) return -1;
+        int dest;
+        if (b == n) {
+            dest = cell_index(x, y);
+        } else {
+            if (naux == NAUX_MAX) {
+                err = "more than " + std::to_string(NAUX_MAX) + " auxiliary ghost cells";
+                return -1;
+            }
+            dest = own_cells() + naux++;
+        }
+        nd.dest = dest;
+        if ((int)cell_stage.size() <= dest) cell_stage.resize(dest + 1, 0);
+        cell_stage[dest] = st + 1;
+        nodes.push_back(nd);
+        nstage.push_back(st + 1);
+        memo[slot].second = dest;
+        return dest;
+    }
+    int build()
+    {
+        for (int y = 1 - R; y <= NY + R; y++)
+            for (int x = 1 - R; x <= NX + R; x++)
+                if (!interior(x, y) && node(n, x, y) < 0) return -1;
+        return 0;
+    }
+};
+
+int build_programs(amrlbm_mesh *m)
+{
+    for (int kind = 0; kind < 3; kind++) {
+        std::vector<PHdr> hdr(m->leaf.size());
+        std::vector<PNode> all;
+        for (size_t k = 0; k < m->leaf.size(); k++) {
+            ProgBuild pb;
+            pb.m = m;
+            pb.kind = kind;
+            pb.R = kind == KIND_C2 ? 2 : 1;
+            pb.n = m->leaf[k];
+            if (pb.build()) return fail(AMRLBM_ERR_TOPOLOGY, "ghost program of block " + std::to_string(pb.n) + ": " + pb.err);
+            // every reference must land on an interior cell of a live block or on a staged LDS cell
+            const int slotsz = kind == KIND_HG ? POPB : CS, own = pb.own_cells();
+            for (const PNode &nd : pb.nodes) {
+                const int type = nd.info & 15;
+                const int nref = type == NT_COPY ? 1 : (type >= NT_BB_BOT ? 3 : 9);
+                bool ok = nd.dest >= 0 && nd.dest < own + NAUX_MAX;
+                for (int r = 0; r < nref && ok; r++) {
+                    int v = nd.ref[r];
+                    if (v < 0) {
+                        ok = -1 - v < own + pb.naux;
+                    } else {
+                        int b = v / slotsz + 1, c = v % slotsz;
+                        int x = kind == KIND_HG ? c % FW : c % CW - 1, y = kind == KIND_HG ? c / FW : c / CW - 1;
+                        ok = b <= m->nblks && m->blk[b].level >= 0 && m->blk[b].isleaf && ProgBuild::interior(x, y);
+                    }
+                }
+                if (!ok) return fail(AMRLBM_ERR_TOPOLOGY, "ghost program of block " + std::to_string(pb.n) + " references a cell outside the mesh");
+            }
+            int nst = 0;
+            for (int v : pb.nstage) nst = std::max(nst, v);
+            if (nst > MAX_STAGES) return fail(AMRLBM_ERR_TOPOLOGY, "ghost program deeper than MAX_STAGES");
+            PHdr &h = hdr[k];
+            h.off = (int)all.size();
+            h.nst = nst;
+            int cum = 0;
+            for (int st = 1; st <= nst; st++) {
+                for (size_t i = 0; i < pb.nodes.size(); i++)
+                    if (pb.nstage[i] == st) all.push_back(pb.nodes[i]);
+                for (size_t i = 0; i < pb.nodes.size(); i++) cum += pb.nstage[i] == st;
+                h.end[st - 1] = cum;
+            }
+        }
+        if (hdr.size() > m->phdr_cap[kind]) {
+            if (m->d_phdr[kind]) HIPCHK(hipFree(m->d_phdr[kind]));
+            m->phdr_cap[kind] = hdr.size() + 64;
+            HIPCHK(hipMalloc(&m->d_phdr[kind], sizeof(PHdr) * m->phdr_cap[kind]));
+        }
+        if (all.size() > m->pnode_cap[kind]) {
+            if (m->d_pnode[kind]) HIPCHK(hipFree(m->d_pnode[kind]));
+            m->pnode_cap[kind] = all.size() + all.size() / 4 + 256;
+            HIPCHK(hipMalloc(&m->d_pnode[kind], sizeof(PNode) * m->pnode_cap[kind]));
+        }
+        if (!hdr.empty())
+            HIPCHK(hipMemcpyAsync(m->d_phdr[kind], hdr.data(), sizeof(PHdr) * hdr.size(), hipMemcpyHostToDevice, m->stream));
+        if (!all.empty())
+            HIPCHK(hipMemcpyAsync(m->d_pnode[kind], all.data(), sizeof(PNode) * all.size(), hipMemcpyHostToDevice, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+        m->prog[kind].hdr = m->d_phdr[kind];
+        m->prog[kind].nodes = m->d_pnode[kind];
+    }
+    return AMRLBM_OK;
+}
+
+// ---------------------------------------------------------------------------
 // device tables
 // ---------------------------------------------------------------------------
 int ensure_int_buffer(int **buf, int *cap, int need)
@@ -273,19 +519,16 @@ int upload_tables(amrlbm_mesh *m)
     HIPCHK(hipStreamSynchronize(m->stream));   // host vectors go out of scope
     m->dm.blk = m->d_blk;
     m->dm.nbd = m->d_nbd;
+    if (!m->leaf.empty()) {
+        int rc2 = build_programs(m);
+        if (rc2) return rc2;
+    }
     m->tables_dirty = false;
     return AMRLBM_OK;
 }
 
-const int *level_list(amrlbm_mesh *m, int L, int *n)
-{
-    if (L < 0 || L >= m->nlev()) {
-        *n = 0;
-        return nullptr;
-    }
-    *n = m->lvl_cnt[L];
-    return m->d_lists + m->lvl_off[L];
-}
+const int *all_leaves(amrlbm_mesh *m) { return m->d_lists + m->all_off; }
+int nleaves(const amrlbm_mesh *m) { return (int)m->leaf.size(); }
 
 int check_err_flags(amrlbm_mesh *m)
 {
@@ -300,17 +543,16 @@ int check_err_flags(amrlbm_mesh *m)
 }
 
 // ---------------------------------------------------------------------------
-// phases
+// phases: one launch over every leaf each (the ghost programs make the
+// reference's coarse-to-fine level passes unnecessary)
 // ---------------------------------------------------------------------------
-int phase_fill_c(amrlbm_mesh *m, int rings, int op, bool do_fill)
+const Prog kNoProg = {nullptr, nullptr};
+
+int phase_fill_c(amrlbm_mesh *m, int kind, int op)
 {
     int rc = upload_tables(m);
     if (rc) return rc;
-    for (int L = m->levelmin; L <= m->levelmax; L++) {
-        int n;
-        const int *lst = level_list(m, L, &n);
-        launch_fill_c(m->dm, m->cur, lst, n, rings, op, m->ph, do_fill, m->stream);
-    }
+    launch_prog_c(m->dm, m->cur, all_leaves(m), nleaves(m), kind, op, m->prog[kind], m->ph, m->stream);
     HIPCHK(hipGetLastError());
     return AMRLBM_OK;
 }
@@ -319,22 +561,17 @@ int phase_collide(amrlbm_mesh *m)
 {
     int rc = upload_tables(m);
     if (rc) return rc;
-    launch_fill_c(m->dm, m->cur, m->d_lists + m->all_off, (int)m->leaf.size(), 2, OP_COLLIDE, m->ph, false, m->stream);
+    launch_prog_c(m->dm, m->cur, all_leaves(m), nleaves(m), KIND_C2, OP_COLLIDE, kNoProg, m->ph, m->stream);
     HIPCHK(hipGetLastError());
     return AMRLBM_OK;
 }
 
-int phase_fill_hg(amrlbm_mesh *m, bool stream_after, bool macro_h)
+int phase_fill_hg(amrlbm_mesh *m)
 {
     int rc = upload_tables(m);
     if (rc) return rc;
-    for (int L = m->levelmin; L <= m->levelmax; L++) {
-        int n;
-        const int *lst = level_list(m, L, &n);
-        launch_fill_hg(m->dm, m->cur, lst, n, stream_after, macro_h, true, m->stream);
-    }
+    launch_prog_hg(m->dm, m->cur, all_leaves(m), nleaves(m), m->prog[KIND_HG], false, false, false, m->stream);
     HIPCHK(hipGetLastError());
-    if (stream_after) m->cur ^= 1;
     return AMRLBM_OK;
 }
 
@@ -344,7 +581,7 @@ int phase_stream(amrlbm_mesh *m, bool macro_h)
 {
     int rc = upload_tables(m);
     if (rc) return rc;
-    launch_fill_hg(m->dm, m->cur, m->d_lists + m->all_off, (int)m->leaf.size(), true, macro_h, false, m->stream, true);
+    launch_prog_hg(m->dm, m->cur, all_leaves(m), nleaves(m), kNoProg, true, macro_h, true, m->stream);
     HIPCHK(hipGetLastError());
     m->cur ^= 1;
     return AMRLBM_OK;
@@ -354,7 +591,7 @@ int phase_macro_h(amrlbm_mesh *m)
 {
     int rc = upload_tables(m);
     if (rc) return rc;
-    launch_macro_h(m->dm, m->cur, m->d_lists + m->all_off, (int)m->leaf.size(), m->stream);
+    launch_macro_h(m->dm, m->cur, all_leaves(m), nleaves(m), m->stream);
     HIPCHK(hipGetLastError());
     return AMRLBM_OK;
 }
@@ -363,7 +600,7 @@ int phase_macro_g(amrlbm_mesh *m)
 {
     int rc = upload_tables(m);
     if (rc) return rc;
-    launch_fill_c(m->dm, m->cur, m->d_lists + m->all_off, (int)m->leaf.size(), 1, OP_MACRO_G, m->ph, false, m->stream);
+    launch_prog_c(m->dm, m->cur, all_leaves(m), nleaves(m), KIND_C1, OP_MACRO_G, kNoProg, m->ph, m->stream);
     HIPCHK(hipGetLastError());
     return AMRLBM_OK;
 }
@@ -582,42 +819,31 @@ int init_bubble(amrlbm_mesh *m, double R, double X0, double Y0)
     return AMRLBM_OK;
 }
 
-// A segment of len whole steps with the fused per-level kernels, starting
-// from a state whose U and P are current (after set-up, a regrid or a
-// previous segment) and ending with U and P current again:
-//   step 1      : [C2 fill + collide](L)            [hg fill + stream + C=sum h](L)
-//   steps 2..len: [C2 fill + U,P + collide](L)      [hg fill + stream + C=sum h](L)
-//   end         : [C fill + U,P](L)
-// each bracket launched once per level, coarse to fine. after(kind, cells) is
-// called after every launch (kind 0: C2 pass, 1: hg pass, 2: C pass).
+// A segment of len whole steps, starting from a state whose U and P are
+// current (after set-up, a regrid or a previous segment) and ending with U
+// and P current again, two launches per step over every leaf:
+//   step 1      : [C2 fill + collide]          [hg fill + stream + C = sum h]
+//   steps 2..len: [C2 fill + U,P + collide]    [hg fill + stream + C = sum h]
+//   end         : [C fill + U,P]
+// after(kind, cells) is called after every launch (kind 0: C2 kernel,
+// 1: hg kernel, 2: C kernel).
 template <class After>
 int enqueue_segment(amrlbm_mesh *m, int cur, int len, After after)
 {
+    const int n = nleaves(m);
+    const int *lst = all_leaves(m);
     for (int s = 0; s < len; s++) {
-        for (int L = m->levelmin; L <= m->levelmax; L++) {
-            int n;
-            const int *lst = level_list(m, L, &n);
-            launch_fill_c(m->dm, cur, lst, n, 2, s == 0 ? OP_COLLIDE : OP_MACRO_G_COLLIDE, m->ph, true, m->stream);
-            int rc = after(0, n * PS);
-            if (rc) return rc;
-        }
-        for (int L = m->levelmin; L <= m->levelmax; L++) {
-            int n;
-            const int *lst = level_list(m, L, &n);
-            launch_fill_hg(m->dm, cur, lst, n, true, true, true, m->stream);
-            int rc = after(1, n * PS);
-            if (rc) return rc;
-        }
+        launch_prog_c(m->dm, cur, lst, n, KIND_C2, s == 0 ? OP_COLLIDE : OP_MACRO_G_COLLIDE, m->prog[KIND_C2], m->ph,
+                      m->stream);
+        int rc = after(0, n * PS);
+        if (rc) return rc;
+        launch_prog_hg(m->dm, cur, lst, n, m->prog[KIND_HG], true, true, false, m->stream);
+        rc = after(1, n * PS);
+        if (rc) return rc;
         cur ^= 1;
     }
-    for (int L = m->levelmin; L <= m->levelmax; L++) {
-        int n;
-        const int *lst = level_list(m, L, &n);
-        launch_fill_c(m->dm, cur, lst, n, 1, OP_MACRO_G, m->ph, true, m->stream);
-        int rc = after(2, n * PS);
-        if (rc) return rc;
-    }
-    return AMRLBM_OK;
+    launch_prog_c(m->dm, cur, lst, n, KIND_C1, OP_MACRO_G, m->prog[KIND_C1], m->ph, m->stream);
+    return after(2, n * PS);
 }
 
 void drop_graphs(amrlbm_mesh *m)
@@ -709,6 +935,10 @@ int amrlbm_destroy(amrlbm_mesh *m)
     (void)hipFree(m->dm.PU);
     (void)hipFree(m->d_blk);
     (void)hipFree(m->d_nbd);
+    for (int k = 0; k < 3; k++) {
+        (void)hipFree(m->d_phdr[k]);
+        (void)hipFree(m->d_pnode[k]);
+    }
     (void)hipFree(m->d_err_flags);
     (void)hipFree(m->d_lists);
     (void)hipFree(m->d_pairs);
@@ -899,7 +1129,7 @@ int amrlbm_determine_min_max_levels(amrlbm_mesh *m, int *levelmin, int *levelmax
 int amrlbm_fill_ghost_cells_c2(amrlbm_mesh *m)
 {
     NEED_READY(m);
-    int rc = phase_fill_c(m, 2, OP_NONE, true);
+    int rc = phase_fill_c(m, KIND_C2, OP_NONE);
     return rc ? rc : check_err_flags(m);
 }
 
@@ -912,7 +1142,7 @@ int amrlbm_collide(amrlbm_mesh *m)
 int amrlbm_fill_ghost_cells_hg(amrlbm_mesh *m)
 {
     NEED_READY(m);
-    int rc = phase_fill_hg(m, false, false);
+    int rc = phase_fill_hg(m);
     return rc ? rc : check_err_flags(m);
 }
 
@@ -932,7 +1162,7 @@ int amrlbm_phase_field_lbm_h_g(amrlbm_mesh *m, double ms[3])
     int rc = phase_collide(m);
     if (!rc) {
         HIPCHK(hipEventRecord(e[1], m->stream));
-        rc = phase_fill_hg(m, false, false);
+        rc = phase_fill_hg(m);
     }
     if (!rc) {
         HIPCHK(hipEventRecord(e[2], m->stream));
@@ -958,7 +1188,7 @@ int amrlbm_macroscopic_values_h(amrlbm_mesh *m)
 int amrlbm_fill_ghost_cells_c(amrlbm_mesh *m)
 {
     NEED_READY(m);
-    int rc = phase_fill_c(m, 1, OP_NONE, true);
+    int rc = phase_fill_c(m, KIND_C1, OP_NONE);
     return rc ? rc : check_err_flags(m);
 }
 
@@ -1185,7 +1415,7 @@ int amrlbm_collision_block(const amrlbm_params *p, int dx, const double *C, cons
     HIPCHK(hipMemcpyAsync(o.dm.PU, P, sizeof(double) * PS, hipMemcpyHostToDevice, o.s));
     HIPCHK(hipMemcpyAsync(o.dm.PU + PS, Ux, sizeof(double) * PS, hipMemcpyHostToDevice, o.s));
     HIPCHK(hipMemcpyAsync(o.dm.PU + 2 * PS, Uy, sizeof(double) * PS, hipMemcpyHostToDevice, o.s));
-    launch_fill_c(o.dm, 0, o.list, 1, 2, OP_COLLIDE, o.ph, false, o.s);
+    launch_prog_c(o.dm, 0, o.list, 1, KIND_C2, OP_COLLIDE, kNoProg, o.ph, o.s);
     HIPCHK(hipGetLastError());
     if ((rc = o.get_pop(0, 0, h)) || (rc = o.get_pop(0, 1, g))) return rc;
     return AMRLBM_OK;
@@ -1200,7 +1430,7 @@ int amrlbm_macro_g_block(const amrlbm_params *p, int dx, const double *g, const 
     if (rc) return rc;
     if ((rc = o.put_pop(1, g))) return rc;
     HIPCHK(hipMemcpyAsync(o.dm.C, C, sizeof(double) * CS, hipMemcpyHostToDevice, o.s));
-    launch_fill_c(o.dm, 0, o.list, 1, 1, OP_MACRO_G, o.ph, false, o.s);
+    launch_prog_c(o.dm, 0, o.list, 1, KIND_C1, OP_MACRO_G, kNoProg, o.ph, o.s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(P, o.dm.PU, sizeof(double) * PS, hipMemcpyDeviceToHost, o.s));
     HIPCHK(hipMemcpyAsync(Ux, o.dm.PU + PS, sizeof(double) * PS, hipMemcpyDeviceToHost, o.s));
@@ -1216,7 +1446,7 @@ int amrlbm_stream_block(const amrlbm_params *p, int dx, double *f)
     int rc = o.init(p, dx);
     if (rc) return rc;
     if ((rc = o.put_pop(0, f))) return rc;
-    launch_fill_hg(o.dm, 0, o.list, 1, true, false, false, o.s, true);
+    launch_prog_hg(o.dm, 0, o.list, 1, kNoProg, true, false, true, o.s);
     HIPCHK(hipGetLastError());
     return o.get_pop(1, 0, f);
 }
