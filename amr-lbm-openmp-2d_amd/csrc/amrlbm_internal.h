@@ -66,10 +66,11 @@ constexpr int NAUX_MAX = 128;       // aux cells per block and kind
 // LDS cell strides: own cells first (C: 20x20, populations: 18x18), aux after
 constexpr int LC_C = CS + NAUX_MAX;
 constexpr int LC_F = FS + NAUX_MAX;
-struct PNode {
+struct __attribute__((aligned(16))) PNode {
     int dest;     // LDS cell written (own cell index < CS / FS, else aux)
     int info;     // type | wx pattern << 4 | wy pattern << 6
     int ref[9];   // >= 0: element offset in the global field; < 0: -(1 + LDS cell)
+    int pad;
 };
 struct PHdr {
     int off;                  // first node

@@ -226,6 +226,45 @@ __device__ __forceinline__ void wpat(int p, double w[3])
     w[2] = p == 0 ? -1.0 : (p == 1 ? 3.0 : (p == 2 ? -3.0 : 5.0));
 }
 
+// node load as three 16-byte loads
+__device__ __forceinline__ PNode load_node(const PNode *p)
+{
+    const int4 *q = reinterpret_cast<const int4 *>(p);
+    int4 a = q[0], b = q[1], c = q[2];
+    PNode n;
+    n.dest = a.x; n.info = a.y; n.ref[0] = a.z; n.ref[1] = a.w;
+    n.ref[2] = b.x; n.ref[3] = b.y; n.ref[4] = b.z; n.ref[5] = b.w;
+    n.ref[6] = c.x; n.ref[7] = c.y; n.ref[8] = c.z; n.pad = c.w;
+    return n;
+}
+
+// the recipe's operands that live in global memory, fetched before any stage
+// barrier (they are interior values left by the previous kernel)
+__device__ __forceinline__ void prefetch_node(const PNode &nd, const double *__restrict__ G, int gs, double gv[9])
+{
+    const int type = nd.info & 15;
+    const int nref = type == NT_COPY ? 1 : (type >= NT_BB_BOT ? 0 : 9);
+#pragma unroll
+    for (int r = 0; r < 9; r++) gv[r] = (r < nref && nd.ref[r] >= 0) ? G[nd.ref[r] + gs] : 0.0;
+}
+
+// one recipe from prefetched global operands gv and LDS L (+ shift ls)
+__device__ __forceinline__ double eval_pref(const PNode &nd, const double gv[9], const double *L, int ls)
+{
+    const int type = nd.info & 15;
+#define PV(r) (nd.ref[r] >= 0 ? gv[r] : L[-1 - nd.ref[r] + ls])
+    if (type == NT_COPY) return PV(0);
+    double wx[3], wy[3];
+    wpat((nd.info >> 4) & 3, wx);
+    wpat((nd.info >> 6) & 3, wy);
+    double t[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) t[a] = wy[0] * PV(3 * a) + wy[1] * PV(3 * a + 1) + wy[2] * PV(3 * a + 2);
+    double o = wx[0] * t[0] + wx[1] * t[1] + wx[2] * t[2];
+#undef PV
+    return type == NT_FINE ? o / 64.0 : o / 1024.0;
+}
+
 // one recipe: copy, or the tensor product (pass along y, then x) divided by
 // 64 (from finer) or 1024 (from coarser)
 __device__ __forceinline__ double eval_node(const PNode &nd, const double *__restrict__ G, int gs,
@@ -280,19 +319,35 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *_
         for (int q = 0; q < 9; q++) g[q] = F[fidx(1, q, i, j)];
     }
     const bool fill = prog.hdr != nullptr;
-    if (fill)
+    // this thread's first node and its global operands, fetched up front
+    PHdr hd;
+    PNode nd0;
+    double gv[9];
+    int total = 0;
+    if (fill) {
+        hd = prog.hdr[k];
+        total = hd.nst > 0 ? hd.end[hd.nst - 1] : 0;
+        if (tid < total) {
+            nd0 = load_node(prog.nodes + hd.off + tid);
+            prefetch_node(nd0, m.C, 0, gv);
+        }
         sC[cidx(i, j)] = C[cidx(i, j)];
-    else
+    } else {
         for (int t = tid; t < CS; t += 256) sC[t] = C[t];
+    }
     __syncthreads();
     if (fill) {
-        const PHdr hd = prog.hdr[k];
-        const PNode *nodes = prog.nodes + hd.off;
         int begin = 0;
         for (int s = 0; s < hd.nst; s++) {
             const int end = hd.end[s];
-            for (int t = begin + tid; t < end; t += 256) {
-                const PNode nd = nodes[t];
+            if (tid >= begin && tid < end) {
+                const double v = eval_pref(nd0, gv, sC, 0);
+                sC[nd0.dest] = v;
+                if (nd0.dest < CS) C[nd0.dest] = v;
+            }
+            for (int t = tid + 256; t < total; t += 256) {
+                if (t < begin || t >= end) continue;
+                const PNode nd = load_node(prog.nodes + hd.off + t);
                 const double v = eval_node(nd, m.C, 0, sC, 0);
                 sC[nd.dest] = v;
                 if (nd.dest < CS) C[nd.dest] = v;
@@ -364,6 +419,15 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *
     double *F = m.pop[cur] + (size_t)slot * POPB;
     __shared__ double sF[18 * LC_F];
     const int i = tid % NX + 1, j = tid / NX + 1;
+    // this thread's first (node, set) item, fetched up front
+    PHdr hd;
+    PNode nd0;
+    int total = 0;
+    if (flags & 1) {
+        hd = prog.hdr[k];
+        total = hd.nst > 0 ? 2 * hd.end[hd.nst - 1] : 0;
+        if (tid < total) nd0 = load_node(prog.nodes + hd.off + (tid >> 1));
+    }
 #pragma unroll
     for (int c = 0; c < 18; c++) sF[lidx(c, i, j)] = F[c * FS + j * FW + i];
     if (!(flags & 1)) {
@@ -374,32 +438,38 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *
     }
     __syncthreads();
     if (flags & 1) {
-        const PHdr hd = prog.hdr[k];
         const PNode *nodes = prog.nodes + hd.off;
         int begin = 0;
         for (int s = 0; s < hd.nst; s++) {
             const int end = hd.end[s];
-            for (int t = 2 * begin + tid; t < 2 * end; t += 256) {
-                const PNode nd = nodes[t >> 1];
+            for (int t = tid; t < total; t += 256) {
+                if (t < 2 * begin || t >= 2 * end) continue;
+                const PNode nd = t == tid ? nd0 : load_node(nodes + (t >> 1));
                 const int set = t & 1, type = nd.info & 15;
-                double v[9];
-                if (type < NT_BB_BOT) {
+                // three populations at a time: 27 operands in flight, then their stores
+#pragma unroll 1
+                for (int qg = 0; qg < 9; qg += 3) {
+                    double v[3];
+                    if (type < NT_BB_BOT) {
 #pragma unroll
-                    for (int q = 0; q < 9; q++) v[q] = eval_node(nd, Fc, (set * 9 + q) * FS, sF, (set * 9 + q) * LC_F);
-                } else {
-                    // Bounce_Back_3/4 (BC_Cell.f90:2-42): f_q(X,wall) = f_opp(q)(Xn, first row)
+                        for (int u = 0; u < 3; u++)
+                            v[u] = eval_node(nd, Fc, (set * 9 + qg + u) * FS, sF, (set * 9 + qg + u) * LC_F);
+                    } else {
+                        // Bounce_Back_3/4 (BC_Cell.f90:2-42): f_q(X,wall) = f_opp(q)(Xn, first row)
 #pragma unroll
-                    for (int q = 0; q < 9; q++) {
-                        int e = c_ex[q] * c_ey[q];
-                        int r = nd.ref[1 + (type == NT_BB_TOP ? -e : e)];
-                        int cs = set * 9 + c_opp[q];
-                        v[q] = r >= 0 ? Fc[r + cs * FS] : sF[-1 - r + cs * LC_F];
+                        for (int u = 0; u < 3; u++) {
+                            const int q = qg + u;
+                            int e = c_ex[q] * c_ey[q];
+                            int r = nd.ref[1 + (type == NT_BB_TOP ? -e : e)];
+                            int cs = set * 9 + c_opp[q];
+                            v[u] = r >= 0 ? Fc[r + cs * FS] : sF[-1 - r + cs * LC_F];
+                        }
                     }
-                }
 #pragma unroll
-                for (int q = 0; q < 9; q++) {
-                    sF[(set * 9 + q) * LC_F + nd.dest] = v[q];
-                    if (nd.dest < FS) F[(set * 9 + q) * FS + nd.dest] = v[q];
+                    for (int u = 0; u < 3; u++) {
+                        sF[(set * 9 + qg + u) * LC_F + nd.dest] = v[u];
+                        if (nd.dest < FS) F[(set * 9 + qg + u) * FS + nd.dest] = v[u];
+                    }
                 }
             }
             begin = end;
