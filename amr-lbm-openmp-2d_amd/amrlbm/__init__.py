@@ -90,6 +90,8 @@ def lib():
             "amrlbm_output": [vp, _DP],
             "amrlbm_synchronize": [vp],
             "amrlbm_last_step_ms": [vp, _DP],
+            "amrlbm_regrid_stats": [vp, _DP],
+            "amrlbm_time_kernels": [vp, ip, _DP],
             "amrlbm_collision_block": [PP, ip, _DP, _DP, _DP, _DP, _DP, _DP],
             "amrlbm_macro_g_block": [PP, ip, _DP, _DP, _DP, _DP, _DP],
             "amrlbm_stream_block": [PP, ip, _DP],
@@ -243,6 +245,24 @@ class Mesh:
         v = ctypes.c_double()
         _check(lib().amrlbm_last_step_ms(self.h, ctypes.byref(v)))
         return v.value
+
+    def time_kernels(self, reps=200):
+        """Average ms per launch of the three step kernels (back-to-back launches,
+        HIP events on the mesh's stream); the state is left unchanged."""
+        out = (ctypes.c_double * 3)()
+        _check(lib().amrlbm_time_kernels(self.h, int(reps), out))
+        return {"c2_collide": out[0], "hg_stream": out[1], "c_macro_g": out[2]}
+
+    def regrid_stats(self):
+        """Device regrid decisions taken by step(), how many changed the mesh,
+        how many segments launched ahead of a change were taken back, and the
+        host time spent on regrids, tables, ghost programs and graphs."""
+        out = (ctypes.c_double * 8)()
+        _check(lib().amrlbm_regrid_stats(self.h, out))
+        return {"checks": int(out[0]), "changes": int(out[1]), "segments_taken_back": int(out[2]),
+                "host_ms": {"regrid": round(out[3], 3), "tables": round(out[4], 3),
+                            "programs": round(out[5], 3), "graphs": round(out[6], 3)},
+                "graphs_built": int(out[7])}
 
     def output(self, allow_nonfinite=False):
         """rho_min, rho_max, max|Uy|, mass, bubble area, centroid x, y, rise velocity, non-finite."""

@@ -32,7 +32,8 @@ struct BlockDev {
     int I, J;     // block coordinates at its level
     int isleaf;
     int child0;   // slot of the first child when refined, else -1
-    int pad0, pad1;
+    int hasleaf;  // children exist and are all leaves (derefinement candidate)
+    int pad1;
 };
 
 // Neighbour descriptor: the resolved nghbr(k)%n plus what a ghost fill needs
@@ -52,6 +53,9 @@ struct DevMesh {
     const BlockDev *blk;     // [slot]
     const NbDesc *nbd;       // [slot][8] neighbours in nghbr(1:8) order
     int *err_flags;          // [0]: bad level jump seen by a ghost fill
+    const int *skip;         // [0] != 0: the step kernels return at once (a
+                             // segment launched ahead of a regrid that changes
+                             // the mesh)
 };
 
 // Ghost program of one block for one fill kind: the block's own ghost cells
@@ -66,6 +70,10 @@ constexpr int NAUX_MAX = 128;       // aux cells per block and kind
 // LDS cell strides: own cells first (C: 20x20, populations: 18x18), aux after
 constexpr int LC_C = CS + NAUX_MAX;
 constexpr int LC_F = FS + NAUX_MAX;
+// most recipes in one block's program: its ghost cells plus NAUX_MAX aux cells
+constexpr int NODES_HG = (FS - NX * NY) + NAUX_MAX;          // 1-ring of h, g
+constexpr int NODES_C1 = (CS - NX * NY - 76) + NAUX_MAX;     // 1-ring of C (ring 2 not filled)
+constexpr int NODES_C2 = (CS - NX * NY) + NAUX_MAX;          // 2 rings of C
 struct __attribute__((aligned(16))) PNode {
     int dest;     // LDS cell written (own cell index < CS / FS, else aux)
     int info;     // type | wx pattern << 4 | wy pattern << 6
@@ -93,7 +101,10 @@ void launch_prog_c(const DevMesh &m, int cur, const int *list, int n, int kind, 
 void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Prog &prog, bool stream_after,
                     bool macro_h, bool copy_ghosts, hipStream_t s);
 void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream_t s);
-void launch_err(const DevMesh &m, const int *list, int n, double *err, hipStream_t s);
+void launch_err(const DevMesh &m, const int *list, int n, double *err, double *err_slot, hipStream_t s);
+void launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, int min_levels, int max_levels,
+                         int levelmin, int levelmax, double refin_tol, double deref_tol, int *changed,
+                         hipStream_t s);
 void launch_restrict(const DevMesh &m, int cur, const int *parents, int n, hipStream_t s);
 void launch_prolong(const DevMesh &m, int cur, const int *pairs, int n, const Phys &ph, hipStream_t s);
 void launch_init_fields(const DevMesh &m, int cur, const int *list, int n, const Phys &ph, double R,

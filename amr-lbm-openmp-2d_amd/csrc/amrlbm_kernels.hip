@@ -295,6 +295,7 @@ __device__ __forceinline__ double eval_node(const PNode &nd, const double *__res
 template <int OP>
 __global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *__restrict__ list, Prog prog, Phys ph)
 {
+    if (*m.skip) return;
     const int k = blockIdx.x;
     const int slot = list[k];
     const int tid = threadIdx.x;
@@ -411,6 +412,7 @@ __device__ __forceinline__ double stream_cell(const double *sF, double *__restri
 __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *__restrict__ list, Prog prog,
                                                  int flags)
 {
+    if (*m.skip) return;
     const int k = blockIdx.x;
     const int slot = list[k];
     const int tid = threadIdx.x;
@@ -418,15 +420,17 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *
     const double *__restrict__ Fc = m.pop[cur];
     double *F = m.pop[cur] + (size_t)slot * POPB;
     __shared__ double sF[18 * LC_F];
+    __shared__ PNode sN[NODES_HG];
     const int i = tid % NX + 1, j = tid / NX + 1;
-    // this thread's first (node, set) item, fetched up front
+    // the block's recipes, staged in LDS beside its populations
     PHdr hd;
-    PNode nd0;
-    int total = 0;
+    hd.nst = 0;
     if (flags & 1) {
         hd = prog.hdr[k];
-        total = hd.nst > 0 ? 2 * hd.end[hd.nst - 1] : 0;
-        if (tid < total) nd0 = load_node(prog.nodes + hd.off + (tid >> 1));
+        const int nn = hd.nst > 0 ? hd.end[hd.nst - 1] : 0;
+        const int4 *src = reinterpret_cast<const int4 *>(prog.nodes + hd.off);
+        int4 *dst = reinterpret_cast<int4 *>(sN);
+        for (int t = tid; t < 3 * nn; t += 256) dst[t] = src[t];
     }
 #pragma unroll
     for (int c = 0; c < 18; c++) sF[lidx(c, i, j)] = F[c * FS + j * FW + i];
@@ -437,44 +441,40 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *
         }
     }
     __syncthreads();
-    if (flags & 1) {
-        const PNode *nodes = prog.nodes + hd.off;
-        int begin = 0;
-        for (int s = 0; s < hd.nst; s++) {
-            const int end = hd.end[s];
-            for (int t = tid; t < total; t += 256) {
-                if (t < 2 * begin || t >= 2 * end) continue;
-                const PNode nd = t == tid ? nd0 : load_node(nodes + (t >> 1));
-                const int set = t & 1, type = nd.info & 15;
-                // three populations at a time: 27 operands in flight, then their stores
-#pragma unroll 1
-                for (int qg = 0; qg < 9; qg += 3) {
-                    double v[3];
-                    if (type < NT_BB_BOT) {
+    // stage by stage; an item is one recipe applied to three populations of
+    // one set, so all 27 operands of an item are in flight together
+    int begin = 0;
+    for (int s = 0; s < hd.nst; s++) {
+        const int end = hd.end[s];
+        for (int t = 6 * begin + tid; t < 6 * end; t += 256) {
+            const int nix = t / 6, sub = t - 6 * nix;
+            const int set = sub >= 3 ? 1 : 0, q0 = 3 * (sub - 3 * set);
+            const PNode nd = sN[nix];
+            const int type = nd.info & 15;
+            double v[3];
+            if (type < NT_BB_BOT) {
 #pragma unroll
-                        for (int u = 0; u < 3; u++)
-                            v[u] = eval_node(nd, Fc, (set * 9 + qg + u) * FS, sF, (set * 9 + qg + u) * LC_F);
-                    } else {
-                        // Bounce_Back_3/4 (BC_Cell.f90:2-42): f_q(X,wall) = f_opp(q)(Xn, first row)
+                for (int u = 0; u < 3; u++)
+                    v[u] = eval_node(nd, Fc, (set * 9 + q0 + u) * FS, sF, (set * 9 + q0 + u) * LC_F);
+            } else {
+                // Bounce_Back_3/4 (BC_Cell.f90:2-42): f_q(X,wall) = f_opp(q)(Xn, first row)
 #pragma unroll
-                        for (int u = 0; u < 3; u++) {
-                            const int q = qg + u;
-                            int e = c_ex[q] * c_ey[q];
-                            int r = nd.ref[1 + (type == NT_BB_TOP ? -e : e)];
-                            int cs = set * 9 + c_opp[q];
-                            v[u] = r >= 0 ? Fc[r + cs * FS] : sF[-1 - r + cs * LC_F];
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < 3; u++) {
-                        sF[(set * 9 + qg + u) * LC_F + nd.dest] = v[u];
-                        if (nd.dest < FS) F[(set * 9 + qg + u) * FS + nd.dest] = v[u];
-                    }
+                for (int u = 0; u < 3; u++) {
+                    const int q = q0 + u;
+                    int e = c_ex[q] * c_ey[q];
+                    int r = nd.ref[1 + (type == NT_BB_TOP ? -e : e)];
+                    int cs = set * 9 + c_opp[q];
+                    v[u] = r >= 0 ? Fc[r + cs * FS] : sF[-1 - r + cs * LC_F];
                 }
             }
-            begin = end;
-            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 3; u++) {
+                sF[(set * 9 + q0 + u) * LC_F + nd.dest] = v[u];
+                if (nd.dest < FS) F[(set * 9 + q0 + u) * FS + nd.dest] = v[u];
+            }
         }
+        begin = end;
+        __syncthreads();
     }
     if (!(flags & 2)) return;
     double *G = m.pop[cur ^ 1] + (size_t)slot * POPB;
@@ -501,7 +501,7 @@ __global__ __launch_bounds__(256) void k_macro_h(DevMesh m, int cur, const int *
 }
 
 // Err_Interface (Refine_Derefine.f90:327-352): sqrt(max |C(i+1)-C(i-1)|^2 ...)/(2dx)
-__global__ __launch_bounds__(256) void k_err(DevMesh m, const int *__restrict__ list, double *err)
+__global__ __launch_bounds__(256) void k_err(DevMesh m, const int *__restrict__ list, double *err, double *err_slot)
 {
     const int slot = list[blockIdx.x];
     const int i = threadIdx.x % NX + 1, j = threadIdx.x / NX + 1;
@@ -519,7 +519,76 @@ __global__ __launch_bounds__(256) void k_err(DevMesh m, const int *__restrict__ 
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) err[blockIdx.x] = sqrt(red[0]) / (2.0 * (double)m.blk[slot].dx);
+    if (threadIdx.x == 0) {
+        const double e = sqrt(red[0]) / (2.0 * (double)m.blk[slot].dx);
+        err[blockIdx.x] = e;
+        if (err_slot) err_slot[slot] = e;
+    }
+}
+
+// Whether Refine_Derefine_Mesh (Refine_Derefine.f90:2-14) would change the
+// mesh, decided on the device from Err_Interface of every leaf: true iff
+// Flag_for_refinement (:18-54) flags any leaf (the 2:1 closure only adds
+// flags), or a derefinement survives Flag_for_derefinement (:97-145) and its
+// level-by-level neighbour check (:149-194). With no refinement flag, the
+// check's Refin terms are false and a pass over level L reads only the final
+// flags of level L+1, so each pass is one barrier-separated sweep.
+// One workgroup; flags of the nblks blocks are bits in LDS.
+__global__ __launch_bounds__(1024) void k_regrid_check(DevMesh m, int nblks, const double *__restrict__ err_slot,
+                                                      int min_levels, int max_levels, int levelmin, int levelmax,
+                                                      double refin_tol, double deref_tol, int *changed)
+{
+    extern __shared__ unsigned dmask[];
+    const int tid = threadIdx.x, nw = (nblks + 31) >> 5;
+    for (int w = tid; w < nw; w += 1024) dmask[w] = 0u;
+    int ref = 0;
+    for (int n = tid; n < nblks; n += 1024) {
+        const BlockDev b = m.blk[n];
+        if (b.level >= 0 && b.isleaf &&
+            (b.level < min_levels || (b.level < max_levels && err_slot[n] > refin_tol)))
+            ref = 1;
+    }
+    if (__syncthreads_or(ref)) {
+        if (tid == 0) *changed = 1;
+        return;
+    }
+    for (int n = tid; n < nblks; n += 1024) {
+        const BlockDev b = m.blk[n];
+        if (!b.hasleaf || b.level < min_levels || b.child0 < 0) continue;
+        bool d = true;
+        for (int c = b.child0; c < b.child0 + 4; c++) {
+            const double e = m.blk[c].isleaf ? err_slot[c] : 0.0;
+            if (e > deref_tol) d = false;
+        }
+        if (d) atomicOr(&dmask[n >> 5], 1u << (n & 31));
+    }
+    __syncthreads();
+    // children adjacent to direction k (1-based, Refine_Derefine.f90:155-163)
+    const int pair_c1[8] = {2, 1, 3, 1, 4, 2, 3, 1};
+    const int pair_c2[8] = {4, 3, 4, 2, 4, 2, 3, 1};
+    for (int L = levelmax - 1; L >= levelmin - 1; L--) {
+        for (int n = tid; n < nblks; n += 1024) {
+            if (!((dmask[n >> 5] >> (n & 31)) & 1u) || m.blk[n].level != L) continue;
+            bool keep = true;
+            for (int k = 0; k < 8 && keep; k++) {
+                const int v = m.nbd[(size_t)n * 8 + k].id;
+                if (v <= 0) continue;
+                const BlockDev q = m.blk[v - 1];
+                if (q.isleaf || q.child0 < 0) continue;
+                const int cc[2] = {q.child0 + pair_c1[k] - 1, q.child0 + pair_c2[k] - 1};
+                for (int t = 0; t < 2; t++) {
+                    const int c = cc[t];
+                    if (!m.blk[c].isleaf && !((dmask[c >> 5] >> (c & 31)) & 1u)) keep = false;
+                }
+            }
+            if (!keep) atomicAnd(&dmask[n >> 5], ~(1u << (n & 31)));
+        }
+        __syncthreads();
+    }
+    int any = 0;
+    for (int w = tid; w < nw; w += 1024) any |= dmask[w] != 0u;
+    any = __syncthreads_or(any);
+    if (tid == 0) *changed = any;
 }
 
 // Restrict_Bilinear (Restrict.f90:14-65) with Bilinear_from_Fine(_f)
@@ -751,9 +820,18 @@ void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream
     if (n > 0) hipLaunchKernelGGL(k_macro_h, dim3(n), dim3(256), 0, s, m, cur, list);
 }
 
-void launch_err(const DevMesh &m, const int *list, int n, double *err, hipStream_t s)
+void launch_err(const DevMesh &m, const int *list, int n, double *err, double *err_slot, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_err, dim3(n), dim3(256), 0, s, m, list, err);
+    if (n > 0) hipLaunchKernelGGL(k_err, dim3(n), dim3(256), 0, s, m, list, err, err_slot);
+}
+
+void launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, int min_levels, int max_levels,
+                         int levelmin, int levelmax, double refin_tol, double deref_tol, int *changed,
+                         hipStream_t s)
+{
+    const size_t lds = sizeof(unsigned) * (size_t)((nblks + 31) / 32 + 1);
+    hipLaunchKernelGGL(k_regrid_check, dim3(1), dim3(1024), lds, s, m, nblks, err_slot, min_levels, max_levels,
+                       levelmin, levelmax, refin_tol, deref_tol, changed);
 }
 
 void launch_restrict(const DevMesh &m, int cur, const int *pairs, int n, hipStream_t s)

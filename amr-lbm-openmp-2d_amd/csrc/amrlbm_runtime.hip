@@ -11,11 +11,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <deque>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "amrlbm_internal.h"
@@ -68,6 +70,9 @@ struct amrlbm_mesh {
     int *d_pairs = nullptr;          // restrict / prolong work lists
     int pairs_cap = 0;
     double *d_err = nullptr;         // Err_Interface per leaf
+    double *d_err_slot = nullptr;    // Err_Interface per block slot (leaves)
+    int *d_changed = nullptr;        // the device regrid decision; the step kernels' skip word
+    int *h_changed = nullptr;        // pinned copy of it
     double *d_part = nullptr;        // diagnostics partials
     double *d_diag = nullptr;
     double *h_pinned = nullptr;      // staging (err, diagnostics)
@@ -76,8 +81,12 @@ struct amrlbm_mesh {
     std::vector<int> lvl_cnt;
     int all_off = 0;
     bool tables_dirty = true;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_check = nullptr;
     double last_step_ms = 0.0;
+    int regrid_checks = 0, regrid_changes = 0, segments_skipped = 0, graphs_built = 0;
+    // host wall time (ms): Refine_Derefine_Mesh, table upload, ghost-program
+    // build (part of the upload), graph capture and instantiation
+    double host_ms_regrid = 0.0, host_ms_tables = 0.0, host_ms_programs = 0.0, host_ms_graphs = 0.0;
 
     // ghost programs per fill kind (C1, C2, HG), leaf-list order
     PHdr *d_phdr[3] = {nullptr, nullptr, nullptr};
@@ -102,6 +111,17 @@ struct amrlbm_mesh {
 };
 
 namespace {
+
+// adds the wall time of its scope to acc (ms)
+struct HostTimer {
+    double &acc;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    explicit HostTimer(double &a) : acc(a) {}
+    ~HostTimer()
+    {
+        acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
 
 // ---------------------------------------------------------------------------
 // topology
@@ -264,7 +284,10 @@ struct ProgBuild {
     std::vector<PNode> nodes;
     std::vector<int> nstage;                 // stage of each node
     std::vector<int> cell_stage;             // stage of the node that writes LDS cell c
-    std::vector<std::pair<long long, int>> memo;
+    // node memo: open addressing on (block, x, y); -2 marks a node being built
+    static constexpr int MEMO = 2048;
+    std::vector<long long> memo_key = std::vector<long long>(MEMO, -1);
+    std::vector<int> memo_val = std::vector<int>(MEMO, -1);
     int naux = 0;
     std::string err;
 
@@ -275,11 +298,11 @@ struct ProgBuild {
     {
         return (b - 1) * (kind == KIND_HG ? POPB : CS) + cell_index(x, y);
     }
-    int find(long long key) const
+    int memo_slot(long long key) const
     {
-        for (auto &e : memo)
-            if (e.first == key) return e.second;
-        return -1;
+        unsigned h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> 40) & (MEMO - 1);
+        while (memo_key[h] != -1 && memo_key[h] != key) h = (h + 1) & (MEMO - 1);
+        return (int)h;
     }
     int ref(int b, int x, int y, int &st)
     {
@@ -293,14 +316,16 @@ struct ProgBuild {
     int node(int b, int x, int y)
     {
         const long long key = ((long long)b << 16) | ((long long)(x + 8) << 8) | (long long)(y + 8);
-        int hit = find(key);
-        if (hit == -2) {
-            err = "cyclic ghost dependency";
-            return -1;
+        const int slot = memo_slot(key);
+        if (memo_key[slot] == key) {
+            if (memo_val[slot] == -2) {
+                err = "cyclic ghost dependency";
+                return -1;
+            }
+            return memo_val[slot];
         }
-        if (hit >= 0) return hit;
-        memo.push_back({key, -2});
-        const size_t slot = memo.size() - 1;
+        memo_key[slot] = key;
+        memo_val[slot] = -2;
         PNode nd{};
         int st = 0, s2 = 0;
         const Block &B = m->blk[b];
@@ -376,7 +401,7 @@ struct ProgBuild {
         cell_stage[dest] = st + 1;
         nodes.push_back(nd);
         nstage.push_back(st + 1);
-        memo[slot].second = dest;
+        memo_val[slot] = dest;
         return dest;
     }
     int build()
@@ -388,49 +413,101 @@ struct ProgBuild {
     }
 };
 
+// Ghost program of one leaf for one fill kind, nodes grouped by stage.
+struct LeafProg {
+    std::vector<PNode> nodes;
+    int nst = 0;
+    int end[MAX_STAGES] = {};
+    std::string err;
+};
+
+void build_leaf_program(amrlbm_mesh *m, int kind, int n, LeafProg &out)
+{
+    ProgBuild pb;
+    pb.m = m;
+    pb.kind = kind;
+    pb.R = kind == KIND_C2 ? 2 : 1;
+    pb.n = n;
+    if (pb.build()) {
+        out.err = "ghost program of block " + std::to_string(n) + ": " + pb.err;
+        return;
+    }
+    // every reference must land on an interior cell of a live block or on a staged LDS cell
+    const int slotsz = kind == KIND_HG ? POPB : CS, own = pb.own_cells();
+    for (const PNode &nd : pb.nodes) {
+        const int type = nd.info & 15;
+        const int nref = type == NT_COPY ? 1 : (type >= NT_BB_BOT ? 3 : 9);
+        bool ok = nd.dest >= 0 && nd.dest < own + NAUX_MAX;
+        for (int r = 0; r < nref && ok; r++) {
+            int v = nd.ref[r];
+            if (v < 0) {
+                ok = -1 - v < own + pb.naux;
+            } else {
+                int b = v / slotsz + 1, c = v % slotsz;
+                int x = kind == KIND_HG ? c % FW : c % CW - 1, y = kind == KIND_HG ? c / FW : c / CW - 1;
+                ok = b <= m->nblks && m->blk[b].level >= 0 && m->blk[b].isleaf && ProgBuild::interior(x, y);
+            }
+        }
+        if (!ok) {
+            out.err = "ghost program of block " + std::to_string(n) + " references a cell outside the mesh";
+            return;
+        }
+    }
+    int nst = 0;
+    for (int v : pb.nstage) nst = std::max(nst, v);
+    if (nst > MAX_STAGES) {
+        out.err = "ghost program of block " + std::to_string(n) + " deeper than MAX_STAGES";
+        return;
+    }
+    const size_t cap = kind == KIND_HG ? NODES_HG : (kind == KIND_C2 ? NODES_C2 : NODES_C1);
+    if (pb.nodes.size() > cap) {
+        out.err = "ghost program of block " + std::to_string(n) + " has more recipes than a kernel stages";
+        return;
+    }
+    out.nst = nst;
+    out.nodes.reserve(pb.nodes.size());
+    for (int st = 1; st <= nst; st++) {
+        for (size_t i = 0; i < pb.nodes.size(); i++)
+            if (pb.nstage[i] == st) out.nodes.push_back(pb.nodes[i]);
+        out.end[st - 1] = (int)out.nodes.size();
+    }
+}
+
+// Programs of every leaf for the three kinds; the builds only read the
+// topology, so they run on several host threads.
 int build_programs(amrlbm_mesh *m)
 {
+    HostTimer timer(m->host_ms_programs);
+    const int nl = (int)m->leaf.size(), ntask = 3 * nl;
+    std::vector<LeafProg> progs(ntask);
+    auto work = [&](int t0, int t1) {
+        for (int t = t0; t < t1; t++) build_leaf_program(m, t / nl, m->leaf[t % nl], progs[t]);
+    };
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int nthr = std::max(1, std::min({hw, 16, ntask / 32}));
+    if (nthr == 1) {
+        work(0, ntask);
+    } else {
+        std::vector<std::thread> pool;
+        for (int k = 0; k < nthr; k++)
+            pool.emplace_back(work, (int)((long long)ntask * k / nthr), (int)((long long)ntask * (k + 1) / nthr));
+        for (auto &th : pool) th.join();
+    }
+    for (const LeafProg &lp : progs)
+        if (!lp.err.empty()) return fail(AMRLBM_ERR_TOPOLOGY, lp.err);
     for (int kind = 0; kind < 3; kind++) {
-        std::vector<PHdr> hdr(m->leaf.size());
+        std::vector<PHdr> hdr(nl);
+        size_t total = 0;
+        for (int k = 0; k < nl; k++) total += progs[kind * nl + k].nodes.size();
         std::vector<PNode> all;
-        for (size_t k = 0; k < m->leaf.size(); k++) {
-            ProgBuild pb;
-            pb.m = m;
-            pb.kind = kind;
-            pb.R = kind == KIND_C2 ? 2 : 1;
-            pb.n = m->leaf[k];
-            if (pb.build()) return fail(AMRLBM_ERR_TOPOLOGY, "ghost program of block " + std::to_string(pb.n) + ": " + pb.err);
-            // every reference must land on an interior cell of a live block or on a staged LDS cell
-            const int slotsz = kind == KIND_HG ? POPB : CS, own = pb.own_cells();
-            for (const PNode &nd : pb.nodes) {
-                const int type = nd.info & 15;
-                const int nref = type == NT_COPY ? 1 : (type >= NT_BB_BOT ? 3 : 9);
-                bool ok = nd.dest >= 0 && nd.dest < own + NAUX_MAX;
-                for (int r = 0; r < nref && ok; r++) {
-                    int v = nd.ref[r];
-                    if (v < 0) {
-                        ok = -1 - v < own + pb.naux;
-                    } else {
-                        int b = v / slotsz + 1, c = v % slotsz;
-                        int x = kind == KIND_HG ? c % FW : c % CW - 1, y = kind == KIND_HG ? c / FW : c / CW - 1;
-                        ok = b <= m->nblks && m->blk[b].level >= 0 && m->blk[b].isleaf && ProgBuild::interior(x, y);
-                    }
-                }
-                if (!ok) return fail(AMRLBM_ERR_TOPOLOGY, "ghost program of block " + std::to_string(pb.n) + " references a cell outside the mesh");
-            }
-            int nst = 0;
-            for (int v : pb.nstage) nst = std::max(nst, v);
-            if (nst > MAX_STAGES) return fail(AMRLBM_ERR_TOPOLOGY, "ghost program deeper than MAX_STAGES");
+        all.reserve(total);
+        for (int k = 0; k < nl; k++) {
+            const LeafProg &lp = progs[kind * nl + k];
             PHdr &h = hdr[k];
             h.off = (int)all.size();
-            h.nst = nst;
-            int cum = 0;
-            for (int st = 1; st <= nst; st++) {
-                for (size_t i = 0; i < pb.nodes.size(); i++)
-                    if (pb.nstage[i] == st) all.push_back(pb.nodes[i]);
-                for (size_t i = 0; i < pb.nodes.size(); i++) cum += pb.nstage[i] == st;
-                h.end[st - 1] = cum;
-            }
+            h.nst = lp.nst;
+            for (int st = 0; st < MAX_STAGES; st++) h.end[st] = lp.end[st];
+            all.insert(all.end(), lp.nodes.begin(), lp.nodes.end());
         }
         if (hdr.size() > m->phdr_cap[kind]) {
             if (m->d_phdr[kind]) HIPCHK(hipFree(m->d_phdr[kind]));
@@ -469,6 +546,7 @@ int ensure_int_buffer(int **buf, int *cap, int need)
 int upload_tables(amrlbm_mesh *m)
 {
     if (!m->tables_dirty) return AMRLBM_OK;
+    HostTimer timer(m->host_ms_tables);
     const int nb = m->nblks;
     std::vector<BlockDev> bd(nb);
     std::vector<NbDesc> nbt((size_t)nb * 8, NbDesc{0, -1, 0, -1});
@@ -481,7 +559,8 @@ int upload_tables(amrlbm_mesh *m)
         d.J = b.J;
         d.isleaf = b.isleaf ? 1 : 0;
         d.child0 = (b.level >= 0 && !b.isleaf && b.has_child) ? b.child[0] - 1 : -1;
-        d.pad0 = d.pad1 = 0;
+        d.hasleaf = (b.level >= 0 && b.hasleaf) ? 1 : 0;
+        d.pad1 = 0;
         if (b.level >= 0)
             for (int k = 1; k <= 8; k++) {
                 int v = nb_value(m, n, k);
@@ -612,9 +691,10 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol)
 {
     int rc = upload_tables(m);
     if (rc) return rc;
+    HostTimer timer(m->host_ms_regrid);
     const int nleaf = (int)m->leaf.size();
     // Err_Interface of every leaf on the device (Refine_Derefine.f90:36, :122)
-    launch_err(m->dm, m->d_lists + m->all_off, nleaf, m->d_err, m->stream);
+    launch_err(m->dm, m->d_lists + m->all_off, nleaf, m->d_err, nullptr, m->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(m->h_pinned, m->d_err, sizeof(double) * nleaf, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
@@ -764,6 +844,11 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipMalloc(&m->d_err_flags, sizeof(int)));
     HIPCHK(hipMemsetAsync(m->d_err_flags, 0, sizeof(int), m->stream));
     HIPCHK(hipMalloc(&m->d_err, sizeof(double) * S));
+    HIPCHK(hipMalloc(&m->d_err_slot, sizeof(double) * S));
+    HIPCHK(hipMalloc(&m->d_changed, sizeof(int)));
+    HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int), m->stream));
+    HIPCHK(hipHostMalloc(&m->h_changed, sizeof(int), hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&m->ev_check, hipEventDisableTiming));
     HIPCHK(hipMalloc(&m->d_part, sizeof(double) * 9 * S));
     HIPCHK(hipMalloc(&m->d_diag, sizeof(double) * 16));
     HIPCHK(hipHostMalloc(&m->h_pinned, sizeof(double) * std::max<size_t>(S, 16), hipHostMallocDefault));
@@ -771,6 +856,7 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipEventCreate(&m->ev1));
     HIPCHK(hipStreamSynchronize(m->stream));
     m->dm.err_flags = m->d_err_flags;
+    m->dm.skip = m->d_changed;
     m->dm.blk = m->d_blk;
     m->dm.nbd = m->d_nbd;
     return AMRLBM_OK;
@@ -863,6 +949,8 @@ int segment_graph(amrlbm_mesh *m, int len, hipGraphExec_t *out)
             *out = g.exec;
             return AMRLBM_OK;
         }
+    HostTimer timer(m->host_ms_graphs);
+    m->graphs_built++;
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
     int rc = enqueue_segment(m, m->cur, len, [](int, int) { return AMRLBM_OK; });
@@ -875,6 +963,30 @@ int segment_graph(amrlbm_mesh *m, int len, hipGraphExec_t *out)
     m->graphs.push_back({len, m->cur, ex});
     *out = ex;
     return AMRLBM_OK;
+}
+
+// Enqueue the device regrid decision at the current step: Err_Interface of
+// every leaf, then k_regrid_check writes 1 to d_changed iff the host regrid
+// would change the mesh, and its copy reaches h_changed when ev_check fires.
+// Until the next check, a nonzero d_changed makes the step kernels return at
+// once, so a segment launched behind the check leaves the state as it is.
+int enqueue_regrid_check(amrlbm_mesh *m)
+{
+    const int nleaf = nleaves(m);
+    launch_err(m->dm, all_leaves(m), nleaf, m->d_err, m->d_err_slot, m->stream);
+    launch_regrid_check(m->dm, m->nblks, m->d_err_slot, m->prm.min_levels, m->prm.max_levels, m->levelmin,
+                        m->levelmax, 0.002, 0.001, m->d_changed, m->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(m->h_changed, m->d_changed, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipEventRecord(m->ev_check, m->stream));
+    m->regrid_checks++;
+    return AMRLBM_OK;
+}
+
+// the flags a regrid that changes nothing leaves: Refin and Deref all false
+void clear_regrid_flags(amrlbm_mesh *m)
+{
+    for (int n = 1; n <= m->nblks; n++) m->blk[n].refin = m->blk[n].deref = false;
 }
 
 // steps to the next regrid (or to the end of the call)
@@ -943,6 +1055,10 @@ int amrlbm_destroy(amrlbm_mesh *m)
     (void)hipFree(m->d_lists);
     (void)hipFree(m->d_pairs);
     (void)hipFree(m->d_err);
+    (void)hipFree(m->d_err_slot);
+    (void)hipFree(m->d_changed);
+    if (m->h_changed) (void)hipHostFree(m->h_changed);
+    if (m->ev_check) (void)hipEventDestroy(m->ev_check);
     (void)hipFree(m->d_part);
     (void)hipFree(m->d_diag);
     if (m->h_pinned) (void)hipHostFree(m->h_pinned);
@@ -1210,18 +1326,51 @@ int amrlbm_step(amrlbm_mesh *m, int nsteps, int regrid_every)
     if (nsteps < 0 || regrid_every < 1) return fail(AMRLBM_ERR_ARG, "bad step arguments");
     int rc = AMRLBM_OK;
     HIPCHK(hipEventRecord(m->ev0, m->stream));
+    // Stream order: segment r, regrid check r, segment r+1, ... The host reads
+    // check r while segment r+1 runs. When check r says the mesh changes,
+    // segment r+1 has skipped all its work: the host takes its steps back,
+    // runs Refine_Derefine_Mesh on the state of check r and goes on from there.
     int done = 0;
-    while (done < nsteps) {
-        int len = segment_length(m, nsteps - done, regrid_every);
-        determine_min_max_levels(m);
-        if ((rc = upload_tables(m))) return rc;
-        hipGraphExec_t ex;
-        if ((rc = segment_graph(m, len, &ex))) return rc;
-        HIPCHK(hipGraphLaunch(ex, m->stream));
-        m->cur ^= (len & 1);
-        m->T += len;
-        done += len;
-        if (m->T % regrid_every == 0 && (rc = regrid(m, 0.002, 0.001))) return rc;
+    bool pending = false;
+    while (done < nsteps || pending) {
+        bool launched = false;
+        int len = 0, cur0 = m->cur;
+        if (done < nsteps) {
+            len = segment_length(m, nsteps - done, regrid_every);
+            if (!pending) {
+                determine_min_max_levels(m);
+                if ((rc = upload_tables(m))) return rc;
+            }
+            hipGraphExec_t ex;
+            if ((rc = segment_graph(m, len, &ex))) return rc;
+            HIPCHK(hipGraphLaunch(ex, m->stream));
+            m->cur ^= (len & 1);
+            m->T += len;
+            done += len;
+            launched = true;
+        }
+        if (pending) {
+            pending = false;
+            HIPCHK(hipEventSynchronize(m->ev_check));
+            if (*m->h_changed) {
+                HIPCHK(hipStreamSynchronize(m->stream));
+                if (launched) {
+                    m->cur = cur0;
+                    m->T -= len;
+                    done -= len;
+                    m->segments_skipped++;
+                }
+                HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int), m->stream));
+                m->regrid_changes++;
+                if ((rc = regrid(m, 0.002, 0.001))) return rc;
+                continue;
+            }
+            clear_regrid_flags(m);
+        }
+        if (launched && m->T % regrid_every == 0) {
+            if ((rc = enqueue_regrid_check(m))) return rc;
+            pending = true;
+        }
     }
     determine_min_max_levels(m);
     HIPCHK(hipEventRecord(m->ev1, m->stream));
@@ -1298,6 +1447,50 @@ int amrlbm_profile_steps(amrlbm_mesh *m, int nsteps, int regrid_every, double st
     return check_err_flags(m);
 }
 
+int amrlbm_time_kernels(amrlbm_mesh *m, int reps, double out[3])
+{
+    NEED_READY(m);
+    if (reps < 1 || !out) return fail(AMRLBM_ERR_ARG, "bad timing arguments");
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    determine_min_max_levels(m);
+    const int n = nleaves(m);
+    const size_t nb = (size_t)m->nblks;
+    // the launches below overwrite populations, C and U,P: keep a copy
+    double *save = nullptr;
+    const size_t npop = POPB * nb, nc = CS * nb, npu = PUB * nb;
+    HIPCHK(hipMalloc(&save, sizeof(double) * (npop + nc + npu)));
+    HIPCHK(hipMemcpyAsync(save, m->dm.pop[m->cur], sizeof(double) * npop, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(save + npop, m->dm.C, sizeof(double) * nc, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(save + npop + nc, m->dm.PU, sizeof(double) * npu, hipMemcpyDeviceToDevice, m->stream));
+    const int *lst = all_leaves(m);
+    for (int kind = 0; kind < 3 && !rc; kind++) {
+        auto launch = [&]() {
+            if (kind == 0)
+                launch_prog_c(m->dm, m->cur, lst, n, KIND_C2, OP_MACRO_G_COLLIDE, m->prog[KIND_C2], m->ph, m->stream);
+            else if (kind == 1)
+                launch_prog_hg(m->dm, m->cur, lst, n, m->prog[KIND_HG], true, true, false, m->stream);
+            else
+                launch_prog_c(m->dm, m->cur, lst, n, KIND_C1, OP_MACRO_G, m->prog[KIND_C1], m->ph, m->stream);
+        };
+        launch();   // warm
+        HIPCHK(hipEventRecord(m->ev0, m->stream));
+        for (int r = 0; r < reps; r++) launch();
+        HIPCHK(hipEventRecord(m->ev1, m->stream));
+        HIPCHK(hipEventSynchronize(m->ev1));
+        HIPCHK(hipGetLastError());
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+        out[kind] = ms / reps;
+    }
+    HIPCHK(hipMemcpyAsync(m->dm.pop[m->cur], save, sizeof(double) * npop, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->dm.C, save + npop, sizeof(double) * nc, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->dm.PU, save + npop + nc, sizeof(double) * npu, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipFree(save));
+    return check_err_flags(m);
+}
+
 int amrlbm_output(amrlbm_mesh *m, double out[9])
 {
     NEED_READY(m);
@@ -1327,6 +1520,20 @@ int amrlbm_last_step_ms(amrlbm_mesh *m, double *ms)
 {
     if (!m || !ms) return fail(AMRLBM_ERR_ARG, "null argument");
     *ms = m->last_step_ms;
+    return AMRLBM_OK;
+}
+
+int amrlbm_regrid_stats(amrlbm_mesh *m, double out[8])
+{
+    if (!m || !out) return fail(AMRLBM_ERR_ARG, "null argument");
+    out[0] = m->regrid_checks;
+    out[1] = m->regrid_changes;
+    out[2] = m->segments_skipped;
+    out[3] = m->host_ms_regrid;
+    out[4] = m->host_ms_tables;
+    out[5] = m->host_ms_programs;
+    out[6] = m->host_ms_graphs;
+    out[7] = m->graphs_built;
     return AMRLBM_OK;
 }
 
@@ -1366,13 +1573,13 @@ struct OneBlock {
         HIPCHK(hipMalloc(&blk, sizeof(BlockDev)));
         HIPCHK(hipMalloc(&nb, sizeof(NbDesc) * 8));
         HIPCHK(hipMalloc(&list, sizeof(int)));
-        HIPCHK(hipMalloc(&flags, sizeof(int)));
+        HIPCHK(hipMalloc(&flags, sizeof(int) * 2));   // [0] error flags, [1] skip word (zero)
         HIPCHK(hipMemsetAsync(dm.pop[0], 0, sizeof(double) * POPB, s));
         HIPCHK(hipMemsetAsync(dm.pop[1], 0, sizeof(double) * POPB, s));
         HIPCHK(hipMemsetAsync(dm.C, 0, sizeof(double) * CS, s));
         HIPCHK(hipMemsetAsync(dm.PU, 0, sizeof(double) * PUB, s));
         HIPCHK(hipMemsetAsync(list, 0, sizeof(int), s));
-        HIPCHK(hipMemsetAsync(flags, 0, sizeof(int), s));
+        HIPCHK(hipMemsetAsync(flags, 0, sizeof(int) * 2, s));
         BlockDev b{0, dx, 0, 0, 1, -1, 0, 0};
         NbDesc nbh[8];
         for (int k = 0; k < 8; k++) nbh[k] = NbDesc{-(k + 1), -1, 0, -1};
@@ -1382,6 +1589,7 @@ struct OneBlock {
         dm.blk = blk;
         dm.nbd = nb;
         dm.err_flags = flags;
+        dm.skip = flags + 1;
         return AMRLBM_OK;
     }
     int put_pop(int set, const double *ref)

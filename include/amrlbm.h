@@ -98,6 +98,13 @@ int amrlbm_step(amrlbm_mesh *m, int nsteps, int regrid_every);
  * measurement hook for the roofline report. */
 int amrlbm_profile_steps(amrlbm_mesh *m, int nsteps, int regrid_every, double stats[12]);
 
+/* Average device time (ms) of one launch of each step kernel over every leaf,
+ * from reps back-to-back launches between two HIP events on the mesh's
+ * stream: out[0] C2 fill + U,P + collision, out[1] hg fill + stream + C = sum h,
+ * out[2] C fill + U,P. The mesh state is saved before and restored after.
+ * No counterpart in the reference: the measurement hook for the roofline. */
+int amrlbm_time_kernels(amrlbm_mesh *m, int reps, double out[3]);
+
 /* FP64 diagnostics over the leaf interiors (Output, AMR_Rising_2D_OMP.f90:456-489):
  * out[0..8] = rho_min, rho_max, max|Uy|, mass (sum rho dx^2), bubble area
  * (sum (1-C) dx^2), bubble centroid x, y, bubble rise velocity, non-finite count.
@@ -108,6 +115,12 @@ int amrlbm_output(amrlbm_mesh *m, double out[9]);
 int amrlbm_synchronize(amrlbm_mesh *m);
 /* device time (ms) of the last amrlbm_step call, measured with HIP events */
 int amrlbm_last_step_ms(amrlbm_mesh *m, double *ms);
+/* regrid bookkeeping since amrlbm_create (no reference counterpart; instrumentation):
+ * out[0] device regrid decisions taken by amrlbm_step, out[1] of them that changed the mesh,
+ * out[2] segments launched ahead of such a change and taken back, host wall ms spent in
+ * out[3] Refine_Derefine_Mesh (excluding table upload), out[4] table upload (including
+ * out[5] ghost-program build), out[6] graph capture and instantiation; out[7] graphs built */
+int amrlbm_regrid_stats(amrlbm_mesh *m, double out[8]);
 
 /* ---- per-block kernels on caller (host) arrays, reference layouts --------- */
 int amrlbm_collision_block(const amrlbm_params *p, int dx, const double *C, const double *P,

@@ -157,6 +157,37 @@ def test_deterministic(amrlbm, workloads):
     assert np.array_equal(states[0][-1], states[1][-1])
 
 
+@pytest.mark.parametrize("name,steps,k,min_changes", [("tiny", 300, 1, 0), ("cfg1", 400, 5, 0),
+                                                     ("cfg2", 1000, 10, 1), ("cfg2", 2400, 7, 1)])
+def test_device_regrid_decision_matches_host_regrid(amrlbm, workloads, name, steps, k, min_changes):
+    """step() decides each regrid on the device and launches the next segment
+    before the host knows the answer; profile_steps() runs the host's
+    Refine_Derefine_Mesh after every segment with the device idle. Both must
+    give the same topology, flags, free list and bitwise equal fields, and the
+    runs must include regrids that change the mesh (taken-back segments)."""
+    from conftest import mesh_state
+    w = workloads[name]
+    print(f"[spec] {name} {steps} steps regrid every {k}", flush=True)
+    with amrlbm.Mesh(w) as a, amrlbm.Mesh(w) as b:
+        a.setup_rising_bubble()
+        b.setup_rising_bubble()
+        a.step(steps, k)
+        b.profile_steps(steps, k)
+        st = a.regrid_stats()
+        print(f"[spec] {name}: {st}", flush=True)
+        assert st["checks"] == steps // k
+        assert st["changes"] >= min_changes
+        ra, fa, ka, xa = mesh_state(a)
+        rb, fb, kb, xb = mesh_state(b)
+        assert np.array_equal(ra, rb)
+        assert list(fa) == list(fb) and ka == kb
+        assert xa.keys() == xb.keys()
+        for n in xa:
+            for f in xa[n]:
+                assert np.array_equal(xa[n][f], xb[n][f]), (n, f)
+        assert np.array_equal(a.output(), b.output())
+
+
 def test_errors_reported(amrlbm, workloads):
     """Argument errors come back as codes with a message, not crashes."""
     w = workloads["tiny"]
