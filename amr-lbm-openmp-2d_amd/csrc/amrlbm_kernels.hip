@@ -284,6 +284,68 @@ __device__ __forceinline__ double eval_node(const PNode &nd, const double *__res
     return type == NT_FINE ? o / 64.0 : o / 1024.0;
 }
 
+// a first-stage recipe: every operand is an interior value in memory
+__device__ __forceinline__ double eval_global(const PNode &nd, const double *__restrict__ G, int gs)
+{
+    const int type = nd.info & 15;
+    if (type == NT_COPY) return G[nd.ref[0] + gs];
+    double wx[3], wy[3];
+    wpat((nd.info >> 4) & 3, wx);
+    wpat((nd.info >> 6) & 3, wy);
+    double t[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        t[a] = wy[0] * G[nd.ref[3 * a] + gs] + wy[1] * G[nd.ref[3 * a + 1] + gs] + wy[2] * G[nd.ref[3 * a + 2] + gs];
+    double o = wx[0] * t[0] + wx[1] * t[1] + wx[2] * t[2];
+    return type == NT_FINE ? o / 64.0 : o / 1024.0;
+}
+
+// population component c (set * 9 + q) of one recipe; bounce-back rows take
+// the opposite population of the first row (Bounce_Back_3/4, BC_Cell.f90:2-42)
+template <bool GLOBAL_ONLY>
+__device__ __forceinline__ double eval_hg(const PNode &nd, int c, const double *__restrict__ Fc, const double *sF)
+{
+    const int type = nd.info & 15;
+    if (type < NT_BB_BOT) return GLOBAL_ONLY ? eval_global(nd, Fc, c * FS) : eval_node(nd, Fc, c * FS, sF, c * LC_F);
+    const int set = c >= 9 ? 1 : 0, q = c - 9 * set;
+    const int e = c_ex[q] * c_ey[q];
+    const int r = nd.ref[1 + (type == NT_BB_TOP ? -e : e)];
+    const int cs = set * 9 + c_opp[q];
+    return (GLOBAL_ONLY || r >= 0) ? Fc[r + cs * FS] : sF[-1 - r + cs * LC_F];
+}
+
+// One stage of an hg ghost program: recipes [b, e) of the block (staged in
+// sN) applied to all 18 population components. Items run recipe-fastest, so
+// the lanes of a wavefront read neighbouring cells of one component; U items
+// per thread have their operands in flight together.
+template <bool GLOBAL_ONLY, int U>
+__device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, const double *__restrict__ Fc, double *sF,
+                                         double *F, int tid)
+{
+    const int ns = e - b;
+    for (int base = 0; base < 18 * ns; base += 256 * U) {
+        double v[U];
+#pragma unroll
+        for (int r = 0; r < U; r++) {
+            const int t = base + tid + 256 * r;
+            if (t < 18 * ns) {
+                const int c = t / ns;
+                v[r] = eval_hg<GLOBAL_ONLY>(sN[b + t - c * ns], c, Fc, sF);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < U; r++) {
+            const int t = base + tid + 256 * r;
+            if (t < 18 * ns) {
+                const int c = t / ns;
+                const int dest = sN[b + t - c * ns].dest;
+                sF[c * LC_F + dest] = v[r];
+                if (dest < FS) F[c * FS + dest] = v[r];
+            }
+        }
+    }
+}
+
 // Fill_Ghost_Cells_C (kind C1) / Fill_Ghost_Cells_C2 (kind C2) of every leaf
 // in list by its ghost program, then the op on the block: the collision
 // (after C2), Macroscopic_Values_g_AMR (after C), or both (OP_MACRO_G_COLLIDE:
@@ -295,10 +357,16 @@ __device__ __forceinline__ double eval_node(const PNode &nd, const double *__res
 template <int OP>
 __global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *__restrict__ list, Prog prog, Phys ph)
 {
-    if (*m.skip) return;
     const int k = blockIdx.x;
-    const int slot = list[k];
     const int tid = threadIdx.x;
+    const int skip = *m.skip;
+    const int slot = list[k];
+    const bool fill = prog.hdr != nullptr;
+    PHdr hd;
+    hd.off = 0;
+    hd.nst = 0;
+    if (fill) hd = prog.hdr[k];
+    if (skip) return;
     const BlockDev me = m.blk[slot];
     double *C = m.C + (size_t)slot * CS;
     __shared__ double sC[LC_C];
@@ -319,20 +387,24 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *_
 #pragma unroll
         for (int q = 0; q < 9; q++) g[q] = F[fidx(1, q, i, j)];
     }
-    const bool fill = prog.hdr != nullptr;
-    // this thread's first node and its global operands, fetched up front
-    PHdr hd;
+    // this thread's first recipe; first-stage recipes read only interior
+    // values in memory, so they are done before the first barrier
+    const int total = hd.nst > 0 ? hd.end[hd.nst - 1] : 0;
+    const int n1 = hd.nst > 0 ? hd.end[0] : 0;
     PNode nd0;
     double gv[9];
-    int total = 0;
+    if (tid < total) {
+        nd0 = load_node(prog.nodes + hd.off + tid);
+        prefetch_node(nd0, m.C, 0, gv);
+    }
     if (fill) {
-        hd = prog.hdr[k];
-        total = hd.nst > 0 ? hd.end[hd.nst - 1] : 0;
-        if (tid < total) {
-            nd0 = load_node(prog.nodes + hd.off + tid);
-            prefetch_node(nd0, m.C, 0, gv);
+        const double cin = C[cidx(i, j)];
+        if (tid < n1) {
+            const double v = eval_pref(nd0, gv, sC, 0);
+            sC[nd0.dest] = v;
+            if (nd0.dest < CS) C[nd0.dest] = v;
         }
-        sC[cidx(i, j)] = C[cidx(i, j)];
+        sC[cidx(i, j)] = cin;
     } else {
         for (int t = tid; t < CS; t += 256) sC[t] = C[t];
     }
@@ -341,7 +413,7 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *_
         int begin = 0;
         for (int s = 0; s < hd.nst; s++) {
             const int end = hd.end[s];
-            if (tid >= begin && tid < end) {
+            if (s > 0 && tid >= begin && tid < end) {
                 const double v = eval_pref(nd0, gv, sC, 0);
                 sC[nd0.dest] = v;
                 if (nd0.dest < CS) C[nd0.dest] = v;
@@ -412,22 +484,23 @@ __device__ __forceinline__ double stream_cell(const double *sF, double *__restri
 __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *__restrict__ list, Prog prog,
                                                  int flags)
 {
-    if (*m.skip) return;
     const int k = blockIdx.x;
-    const int slot = list[k];
     const int tid = threadIdx.x;
+    const int skip = *m.skip;
+    const int slot = list[k];
+    PHdr hd;
+    hd.off = 0;
+    hd.nst = 0;
+    if (flags & 1) hd = prog.hdr[k];
+    if (skip) return;
     const BlockDev me = m.blk[slot];
     const double *__restrict__ Fc = m.pop[cur];
     double *F = m.pop[cur] + (size_t)slot * POPB;
     __shared__ double sF[18 * LC_F];
     __shared__ PNode sN[NODES_HG];
     const int i = tid % NX + 1, j = tid / NX + 1;
-    // the block's recipes, staged in LDS beside its populations
-    PHdr hd;
-    hd.nst = 0;
-    if (flags & 1) {
-        hd = prog.hdr[k];
-        const int nn = hd.nst > 0 ? hd.end[hd.nst - 1] : 0;
+    const int nn = hd.nst > 0 ? hd.end[hd.nst - 1] : 0;
+    {   // the block's recipes, staged in LDS
         const int4 *src = reinterpret_cast<const int4 *>(prog.nodes + hd.off);
         int4 *dst = reinterpret_cast<int4 *>(sN);
         for (int t = tid; t < 3 * nn; t += 256) dst[t] = src[t];
@@ -441,38 +514,15 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *
         }
     }
     __syncthreads();
-    // stage by stage; an item is one recipe applied to three populations of
-    // one set, so all 27 operands of an item are in flight together
+    // the first stage reads interior values in memory only; later stages
+    // also read cells of earlier stages
     int begin = 0;
     for (int s = 0; s < hd.nst; s++) {
         const int end = hd.end[s];
-        for (int t = 6 * begin + tid; t < 6 * end; t += 256) {
-            const int nix = t / 6, sub = t - 6 * nix;
-            const int set = sub >= 3 ? 1 : 0, q0 = 3 * (sub - 3 * set);
-            const PNode nd = sN[nix];
-            const int type = nd.info & 15;
-            double v[3];
-            if (type < NT_BB_BOT) {
-#pragma unroll
-                for (int u = 0; u < 3; u++)
-                    v[u] = eval_node(nd, Fc, (set * 9 + q0 + u) * FS, sF, (set * 9 + q0 + u) * LC_F);
-            } else {
-                // Bounce_Back_3/4 (BC_Cell.f90:2-42): f_q(X,wall) = f_opp(q)(Xn, first row)
-#pragma unroll
-                for (int u = 0; u < 3; u++) {
-                    const int q = q0 + u;
-                    int e = c_ex[q] * c_ey[q];
-                    int r = nd.ref[1 + (type == NT_BB_TOP ? -e : e)];
-                    int cs = set * 9 + c_opp[q];
-                    v[u] = r >= 0 ? Fc[r + cs * FS] : sF[-1 - r + cs * LC_F];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 3; u++) {
-                sF[(set * 9 + q0 + u) * LC_F + nd.dest] = v[u];
-                if (nd.dest < FS) F[(set * 9 + q0 + u) * FS + nd.dest] = v[u];
-            }
-        }
+        if (s == 0)
+            hg_stage<true, 4>(sN, begin, end, Fc, sF, F, tid);
+        else
+            hg_stage<false, 2>(sN, begin, end, Fc, sF, F, tid);
         begin = end;
         __syncthreads();
     }

@@ -276,8 +276,10 @@ int nb_value(amrlbm_mesh *m, int n, int k);
 // fills, then the wall rows by BC_Cell.f90) becomes one node; a case (iii)
 // fill reads the coarser neighbour's ghost cells, which in the reference were
 // filled in the coarser level's pass, so those cells become nodes too,
-// recursively, in the same operation order. Interior values are read from
-// memory (left by the previous kernel), everything else from LDS.
+// recursively, in the same operation order. Interior values, the block's own
+// included, are read from memory (left by the previous kernel), everything
+// else from LDS; so the first stage reads memory only and a kernel can
+// evaluate it before its first barrier.
 struct ProgBuild {
     amrlbm_mesh *m;
     int kind, R, n;
@@ -307,7 +309,7 @@ struct ProgBuild {
     int ref(int b, int x, int y, int &st)
     {
         st = 0;
-        if (interior(x, y)) return b == n ? -(1 + cell_index(x, y)) : global_ref(b, x, y);
+        if (interior(x, y)) return global_ref(b, x, y);
         int dest = node(b, x, y);
         if (dest < 0) return 0;
         st = cell_stage[dest];
