@@ -95,7 +95,8 @@ enum FillOp { OP_NONE = 0, OP_COLLIDE = 1, OP_MACRO_G = 2, OP_MACRO_G_COLLIDE = 
 // C fill (kind C1 or C2) from a ghost program, then op; with prog.hdr == null
 // the block's ghosts are read from memory instead (no fill)
 void launch_prog_c(const DevMesh &m, int cur, const int *list, int n, int kind, int op, const Prog &prog,
-                   const Phys &ph, hipStream_t s);
+                   const Phys &ph, hipStream_t s,
+                   double *err_slot = nullptr);
 // hg fill from a ghost program (or ghosts from memory with prog.hdr == null),
 // then optionally the stream (+ C = sum h, + ghost ring carried over)
 void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Prog &prog, bool stream_after,

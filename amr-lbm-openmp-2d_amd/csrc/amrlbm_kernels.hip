@@ -355,7 +355,8 @@ __device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, const do
 // identical). The collision and U,P read only this block's C. With
 // prog.hdr == null the ghosts are read from memory (no fill).
 template <int OP>
-__global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *__restrict__ list, Prog prog, Phys ph)
+__global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *__restrict__ list, Prog prog, Phys ph,
+                                                double *err_slot)
 {
     const int k = blockIdx.x;
     const int tid = threadIdx.x;
@@ -427,6 +428,26 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *_
             }
             begin = end;
             __syncthreads();
+        }
+    }
+    if (err_slot) {
+        // Err_Interface (Refine_Derefine.f90:327-352) from the filled ring
+        const double a = sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)];
+        const double b = sC[cidx(i, j + 1)] - sC[cidx(i, j - 1)];
+        double dc = a * a + b * b;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double x = __shfl_xor(dc, o);
+            if (x > dc) dc = x;
+        }
+        __shared__ double wmax[4];
+        if ((tid & 63) == 0) wmax[tid >> 6] = dc;
+        __syncthreads();
+        if (tid == 0) {
+            double e = wmax[0];
+            for (int w = 1; w < 4; w++)
+                if (wmax[w] > e) e = wmax[w];
+            err_slot[slot] = sqrt(e) / (2.0 * (double)me.dx);
         }
     }
     if (OP == OP_MACRO_G_COLLIDE || OP == OP_MACRO_G) {
@@ -581,56 +602,82 @@ __global__ __launch_bounds__(256) void k_err(DevMesh m, const int *__restrict__ 
 // Flag_for_refinement (:18-54) flags any leaf (the 2:1 closure only adds
 // flags), or a derefinement survives Flag_for_derefinement (:97-145) and its
 // level-by-level neighbour check (:149-194). With no refinement flag, the
-// check's Refin terms are false and a pass over level L reads only the final
-// flags of level L+1, so each pass is one barrier-separated sweep.
-// One workgroup; flags of the nblks blocks are bits in LDS.
+// check's Refin terms are false, and a candidate at level L is cancelled iff
+// one of the two children next to it of a non-leaf neighbour (pairs
+// :155-163) is itself a non-leaf without a surviving Deref flag; those flags
+// are final once the pass over level L+1 is done. So each candidate gathers
+// its non-leaf watch blocks once, and the passes only test LDS bits.
+// One workgroup. More than RC_CAND candidates: answers "changed" and leaves
+// the decision to the host.
+constexpr int RC_CAND = 512;
 __global__ __launch_bounds__(1024) void k_regrid_check(DevMesh m, int nblks, const double *__restrict__ err_slot,
                                                       int min_levels, int max_levels, int levelmin, int levelmax,
                                                       double refin_tol, double deref_tol, int *changed)
 {
     extern __shared__ unsigned dmask[];
+    __shared__ int cand[RC_CAND], clev[RC_CAND], watch[RC_CAND * 16];
+    __shared__ int ncand;
     const int tid = threadIdx.x, nw = (nblks + 31) >> 5;
     for (int w = tid; w < nw; w += 1024) dmask[w] = 0u;
+    if (tid == 0) ncand = 0;
+    __syncthreads();
     int ref = 0;
     for (int n = tid; n < nblks; n += 1024) {
         const BlockDev b = m.blk[n];
-        if (b.level >= 0 && b.isleaf &&
-            (b.level < min_levels || (b.level < max_levels && err_slot[n] > refin_tol)))
-            ref = 1;
-    }
-    if (__syncthreads_or(ref)) {
-        if (tid == 0) *changed = 1;
-        return;
-    }
-    for (int n = tid; n < nblks; n += 1024) {
-        const BlockDev b = m.blk[n];
+        if (b.level < 0) continue;
+        if (b.isleaf && (b.level < min_levels || (b.level < max_levels && err_slot[n] > refin_tol))) ref = 1;
         if (!b.hasleaf || b.level < min_levels || b.child0 < 0) continue;
         bool d = true;
         for (int c = b.child0; c < b.child0 + 4; c++) {
             const double e = m.blk[c].isleaf ? err_slot[c] : 0.0;
             if (e > deref_tol) d = false;
         }
-        if (d) atomicOr(&dmask[n >> 5], 1u << (n & 31));
+        if (d) {
+            atomicOr(&dmask[n >> 5], 1u << (n & 31));
+            const int ix = atomicAdd(&ncand, 1);
+            if (ix < RC_CAND) {
+                cand[ix] = n;
+                clev[ix] = b.level;
+            }
+        }
     }
-    __syncthreads();
+    if (__syncthreads_or(ref)) {
+        if (tid == 0) *changed = 1;
+        return;
+    }
+    const int nc = ncand;
+    if (nc > RC_CAND) {
+        if (tid == 0) *changed = 1;
+        return;
+    }
     // children adjacent to direction k (1-based, Refine_Derefine.f90:155-163)
     const int pair_c1[8] = {2, 1, 3, 1, 4, 2, 3, 1};
     const int pair_c2[8] = {4, 3, 4, 2, 4, 2, 3, 1};
-    for (int L = levelmax - 1; L >= levelmin - 1; L--) {
-        for (int n = tid; n < nblks; n += 1024) {
-            if (!((dmask[n >> 5] >> (n & 31)) & 1u) || m.blk[n].level != L) continue;
-            bool keep = true;
-            for (int k = 0; k < 8 && keep; k++) {
-                const int v = m.nbd[(size_t)n * 8 + k].id;
-                if (v <= 0) continue;
-                const BlockDev q = m.blk[v - 1];
-                if (q.isleaf || q.child0 < 0) continue;
-                const int cc[2] = {q.child0 + pair_c1[k] - 1, q.child0 + pair_c2[k] - 1};
-                for (int t = 0; t < 2; t++) {
-                    const int c = cc[t];
-                    if (!m.blk[c].isleaf && !((dmask[c >> 5] >> (c & 31)) & 1u)) keep = false;
-                }
+    for (int t = tid; t < 8 * nc; t += 1024) {
+        const int ci = t >> 3, k = t & 7, n = cand[ci];
+        int w0 = -1, w1 = -1;
+        const int v = m.nbd[(size_t)n * 8 + k].id;
+        if (v > 0) {
+            const BlockDev q = m.blk[v - 1];
+            if (!q.isleaf && q.child0 >= 0) {
+                const int c1 = q.child0 + pair_c1[k] - 1, c2 = q.child0 + pair_c2[k] - 1;
+                if (!m.blk[c1].isleaf) w0 = c1;
+                if (!m.blk[c2].isleaf) w1 = c2;
             }
+        }
+        watch[ci * 16 + 2 * k] = w0;
+        watch[ci * 16 + 2 * k + 1] = w1;
+    }
+    __syncthreads();
+    for (int L = levelmax - 1; L >= levelmin - 1; L--) {
+        for (int ci = tid; ci < nc; ci += 1024) {
+            if (clev[ci] != L) continue;
+            bool keep = true;
+            for (int r = 0; r < 16; r++) {
+                const int c = watch[ci * 16 + r];
+                if (c >= 0 && !((dmask[c >> 5] >> (c & 31)) & 1u)) keep = false;
+            }
+            const int n = cand[ci];
             if (!keep) atomicAnd(&dmask[n >> 5], ~(1u << (n & 31)));
         }
         __syncthreads();
@@ -846,15 +893,19 @@ __global__ __launch_bounds__(256) void k_diag_final(const double *part, int n, d
 // launchers
 // ---------------------------------------------------------------------------
 void launch_prog_c(const DevMesh &m, int cur, const int *list, int n, int kind, int op, const Prog &prog,
-                   const Phys &ph, hipStream_t s)
+                   const Phys &ph, hipStream_t s, double *err_slot)
 {
     if (n <= 0) return;
     (void)kind;   // the program decides the ring; the op needs C with the rings it reads
     dim3 g(n), b(256);
-    if (op == OP_MACRO_G_COLLIDE) hipLaunchKernelGGL(k_prog_c<OP_MACRO_G_COLLIDE>, g, b, 0, s, m, cur, list, prog, ph);
-    else if (op == OP_COLLIDE) hipLaunchKernelGGL(k_prog_c<OP_COLLIDE>, g, b, 0, s, m, cur, list, prog, ph);
-    else if (op == OP_MACRO_G) hipLaunchKernelGGL(k_prog_c<OP_MACRO_G>, g, b, 0, s, m, cur, list, prog, ph);
-    else hipLaunchKernelGGL(k_prog_c<OP_NONE>, g, b, 0, s, m, cur, list, prog, ph);
+    if (op == OP_MACRO_G_COLLIDE)
+        hipLaunchKernelGGL(k_prog_c<OP_MACRO_G_COLLIDE>, g, b, 0, s, m, cur, list, prog, ph, err_slot);
+    else if (op == OP_COLLIDE)
+        hipLaunchKernelGGL(k_prog_c<OP_COLLIDE>, g, b, 0, s, m, cur, list, prog, ph, err_slot);
+    else if (op == OP_MACRO_G)
+        hipLaunchKernelGGL(k_prog_c<OP_MACRO_G>, g, b, 0, s, m, cur, list, prog, ph, err_slot);
+    else
+        hipLaunchKernelGGL(k_prog_c<OP_NONE>, g, b, 0, s, m, cur, list, prog, ph, err_slot);
 }
 
 void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Prog &prog, bool stream_after,

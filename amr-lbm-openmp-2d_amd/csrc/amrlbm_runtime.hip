@@ -17,7 +17,7 @@
 #include <cstring>
 #include <deque>
 #include <string>
-#include <thread>
+#include <omp.h>
 #include <vector>
 
 #include "amrlbm_internal.h"
@@ -92,6 +92,8 @@ struct amrlbm_mesh {
     PHdr *d_phdr[3] = {nullptr, nullptr, nullptr};
     PNode *d_pnode[3] = {nullptr, nullptr, nullptr};
     size_t phdr_cap[3] = {0, 0, 0}, pnode_cap[3] = {0, 0, 0};
+    void *h_stage[3] = {nullptr, nullptr, nullptr};   // pinned upload staging per kind
+    size_t stage_cap[3] = {0, 0, 0};
     Prog prog[3] = {};
 
     // one HIP graph per (segment length, population-buffer parity) for the
@@ -482,53 +484,52 @@ int build_programs(amrlbm_mesh *m)
     HostTimer timer(m->host_ms_programs);
     const int nl = (int)m->leaf.size(), ntask = 3 * nl;
     std::vector<LeafProg> progs(ntask);
-    auto work = [&](int t0, int t1) {
-        for (int t = t0; t < t1; t++) build_leaf_program(m, t / nl, m->leaf[t % nl], progs[t]);
-    };
-    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
-    const int nthr = std::max(1, std::min({hw, 16, ntask / 32}));
-    if (nthr == 1) {
-        work(0, ntask);
-    } else {
-        std::vector<std::thread> pool;
-        for (int k = 0; k < nthr; k++)
-            pool.emplace_back(work, (int)((long long)ntask * k / nthr), (int)((long long)ntask * (k + 1) / nthr));
-        for (auto &th : pool) th.join();
-    }
+    const int nthr = std::max(1, std::min(omp_get_max_threads(), ntask / 32));
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nthr)
+    for (int t = 0; t < ntask; t++) build_leaf_program(m, t / nl, m->leaf[t % nl], progs[t]);
     for (const LeafProg &lp : progs)
         if (!lp.err.empty()) return fail(AMRLBM_ERR_TOPOLOGY, lp.err);
     for (int kind = 0; kind < 3; kind++) {
-        std::vector<PHdr> hdr(nl);
         size_t total = 0;
         for (int k = 0; k < nl; k++) total += progs[kind * nl + k].nodes.size();
-        std::vector<PNode> all;
-        all.reserve(total);
+        // staged in pinned host memory, so the upload runs at full link speed
+        const size_t need = sizeof(PHdr) * nl + sizeof(PNode) * total;
+        if (need > m->stage_cap[kind]) {
+            if (m->h_stage[kind]) HIPCHK(hipHostFree(m->h_stage[kind]));
+            m->h_stage[kind] = nullptr;
+            m->stage_cap[kind] = need + need / 4 + 4096;
+            HIPCHK(hipHostMalloc(&m->h_stage[kind], m->stage_cap[kind], hipHostMallocDefault));
+        }
+        PNode *all = reinterpret_cast<PNode *>(m->h_stage[kind]);
+        PHdr *hdr = reinterpret_cast<PHdr *>(all + total);
+        size_t pos = 0;
         for (int k = 0; k < nl; k++) {
             const LeafProg &lp = progs[kind * nl + k];
             PHdr &h = hdr[k];
-            h.off = (int)all.size();
+            h.off = (int)pos;
             h.nst = lp.nst;
             for (int st = 0; st < MAX_STAGES; st++) h.end[st] = lp.end[st];
-            all.insert(all.end(), lp.nodes.begin(), lp.nodes.end());
+            std::memcpy(all + pos, lp.nodes.data(), sizeof(PNode) * lp.nodes.size());
+            pos += lp.nodes.size();
         }
-        if (hdr.size() > m->phdr_cap[kind]) {
+        if ((size_t)nl > m->phdr_cap[kind]) {
             if (m->d_phdr[kind]) HIPCHK(hipFree(m->d_phdr[kind]));
-            m->phdr_cap[kind] = hdr.size() + 64;
+            m->phdr_cap[kind] = nl + 64;
             HIPCHK(hipMalloc(&m->d_phdr[kind], sizeof(PHdr) * m->phdr_cap[kind]));
         }
-        if (all.size() > m->pnode_cap[kind]) {
+        if (total > m->pnode_cap[kind]) {
             if (m->d_pnode[kind]) HIPCHK(hipFree(m->d_pnode[kind]));
-            m->pnode_cap[kind] = all.size() + all.size() / 4 + 256;
+            m->pnode_cap[kind] = total + total / 4 + 256;
             HIPCHK(hipMalloc(&m->d_pnode[kind], sizeof(PNode) * m->pnode_cap[kind]));
         }
-        if (!hdr.empty())
-            HIPCHK(hipMemcpyAsync(m->d_phdr[kind], hdr.data(), sizeof(PHdr) * hdr.size(), hipMemcpyHostToDevice, m->stream));
-        if (!all.empty())
-            HIPCHK(hipMemcpyAsync(m->d_pnode[kind], all.data(), sizeof(PNode) * all.size(), hipMemcpyHostToDevice, m->stream));
-        HIPCHK(hipStreamSynchronize(m->stream));
+        if (nl > 0)
+            HIPCHK(hipMemcpyAsync(m->d_phdr[kind], hdr, sizeof(PHdr) * nl, hipMemcpyHostToDevice, m->stream));
+        if (total > 0)
+            HIPCHK(hipMemcpyAsync(m->d_pnode[kind], all, sizeof(PNode) * total, hipMemcpyHostToDevice, m->stream));
         m->prog[kind].hdr = m->d_phdr[kind];
         m->prog[kind].nodes = m->d_pnode[kind];
     }
+    HIPCHK(hipStreamSynchronize(m->stream));   // the pinned stages are reused by the next build
     return AMRLBM_OK;
 }
 
@@ -930,7 +931,9 @@ int enqueue_segment(amrlbm_mesh *m, int cur, int len, After after)
         if (rc) return rc;
         cur ^= 1;
     }
-    launch_prog_c(m->dm, cur, lst, n, KIND_C1, OP_MACRO_G, m->prog[KIND_C1], m->ph, m->stream);
+    // Err_Interface of every leaf comes out of the last kernel, for a regrid
+    // decision that may follow the segment
+    launch_prog_c(m->dm, cur, lst, n, KIND_C1, OP_MACRO_G, m->prog[KIND_C1], m->ph, m->stream, m->d_err_slot);
     return after(2, n * PS);
 }
 
@@ -967,15 +970,14 @@ int segment_graph(amrlbm_mesh *m, int len, hipGraphExec_t *out)
     return AMRLBM_OK;
 }
 
-// Enqueue the device regrid decision at the current step: Err_Interface of
-// every leaf, then k_regrid_check writes 1 to d_changed iff the host regrid
-// would change the mesh, and its copy reaches h_changed when ev_check fires.
+// Enqueue the device regrid decision at the current step, behind a segment
+// whose last kernel left Err_Interface of every leaf in d_err_slot:
+// k_regrid_check writes 1 to d_changed iff the host regrid would change the
+// mesh (or it cannot tell), and its copy reaches h_changed when ev_check fires.
 // Until the next check, a nonzero d_changed makes the step kernels return at
 // once, so a segment launched behind the check leaves the state as it is.
 int enqueue_regrid_check(amrlbm_mesh *m)
 {
-    const int nleaf = nleaves(m);
-    launch_err(m->dm, all_leaves(m), nleaf, m->d_err, m->d_err_slot, m->stream);
     launch_regrid_check(m->dm, m->nblks, m->d_err_slot, m->prm.min_levels, m->prm.max_levels, m->levelmin,
                         m->levelmax, 0.002, 0.001, m->d_changed, m->stream);
     HIPCHK(hipGetLastError());
@@ -1052,6 +1054,7 @@ int amrlbm_destroy(amrlbm_mesh *m)
     for (int k = 0; k < 3; k++) {
         (void)hipFree(m->d_phdr[k]);
         (void)hipFree(m->d_pnode[k]);
+        if (m->h_stage[k]) (void)hipHostFree(m->h_stage[k]);
     }
     (void)hipFree(m->d_err_flags);
     (void)hipFree(m->d_lists);
