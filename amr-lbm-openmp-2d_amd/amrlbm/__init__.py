@@ -70,6 +70,7 @@ def lib():
             "amrlbm_create": [PP, ctypes.POINTER(vp)],
             "amrlbm_destroy": [vp],
             "amrlbm_setup_rising_bubble": [vp, dp, dp, dp],
+            "amrlbm_setup_replicas": [vp, ip, dp, dp, dp],
             "amrlbm_get_counts": [vp, _IP],
             "amrlbm_get_topology": [vp, _IP, _IP],
             "amrlbm_set_topology": [vp, ip, _IP, ip, ip, _IP, ip],
@@ -153,10 +154,16 @@ class Mesh:
 
     # ---- set-up and state exchange -------------------------------------
     def setup_rising_bubble(self, radius=None, x0=None, y0=None):
+        """The reference's set-up; a workload of several instances gets one
+        bubble per instance (amrlbm_setup_replicas)."""
         w = self.w
-        _check(lib().amrlbm_setup_rising_bubble(
-            self.h, w.radius if radius is None else radius, w.x0 if x0 is None else x0,
-            w.y0 if y0 is None else y0))
+        R = w.radius if radius is None else radius
+        x = w.x0 if x0 is None else x0
+        y = w.y0 if y0 is None else y0
+        if w.instances > 1:
+            _check(lib().amrlbm_setup_replicas(self.h, w.instances, R, x, y))
+        else:
+            _check(lib().amrlbm_setup_rising_bubble(self.h, R, x, y))
 
     def counts(self):
         c = np.zeros(7, dtype=np.int32)

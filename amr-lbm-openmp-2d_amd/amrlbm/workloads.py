@@ -3,8 +3,8 @@
 The fluid parameters are the reference's own set (Set_Rising_Bubble_Parameters,
 AMR_Rising_2D_OMP.f90:241-259: Bo=40, Ar=40, g=5e-6, W=5, rho_h=1, rho_l=0.001,
 viscosity ratio 100) with M0=0.03 (D2Q9_Mod.f90:10). The geometries are
-16x16-cell blocks under two root blocks stacked in y, x periodic, bounce-back
-walls in y. The derived quantities are computed in the reference's operation
+16x16-cell blocks under two root blocks stacked in y (per instance), x periodic,
+bounce-back walls in y. The derived quantities are computed in the reference's operation
 order so that they agree bit for bit with the reference build.
 """
 import math
@@ -31,6 +31,8 @@ class Workload:
     visc_ratio: float = 100.0
     refin_tol: float = 0.002
     deref_tol: float = 0.001
+    instances: int = 1      # independent copies stacked in y (amrlbm_setup_replicas)
+    block_cap: int = 0      # MaxBlks when set; else the reference's formula
 
     @property
     def ny(self):
@@ -42,7 +44,7 @@ class Workload:
 
     @property
     def max_blocks(self):      # Parameters.f90:19 (per root pair: 6 (L0/nx)^2)
-        return 6 * (self.L0 // self.nx) ** 2
+        return self.block_cap or 6 * (self.L0 // self.nx) ** 2
 
     @property
     def fine_shape(self):      # finest-equivalent lattice (x, y)
@@ -91,4 +93,10 @@ WORKLOADS = {
     "cfg1": Workload("cfg1", L0=128, min_levels=1, radius=16.0, x0=64.0, y0=64.0),
     # 1024 x 2048, levels 3..6 (dx = 8,4,2,1), D=128 at (512,512): the benchmark case
     "cfg2": Workload("cfg2", L0=1024, min_levels=3, radius=64.0, x0=512.0, y0=512.0),
+    # 16 independent copies of cfg2 in one mesh (1024 x 32768 finest-equivalent, ~4,600 leaves, ~0.5 GB
+    # of leaf state): the step kernels at a working set beyond the 256 MB Infinity Cache
+    "cfg2x16": Workload("cfg2x16", L0=1024, min_levels=3, radius=64.0, x0=512.0, y0=512.0, nroots=32,
+                        instances=16, block_cap=16384),
+    # 2 copies of cfg1: the replicated-instance parity case
+    "cfg1x2": Workload("cfg1x2", L0=128, min_levels=1, radius=16.0, x0=64.0, y0=64.0, nroots=4, instances=2),
 }

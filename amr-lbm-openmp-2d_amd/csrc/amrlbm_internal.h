@@ -12,13 +12,37 @@ namespace amrlbm {
 // Block geometry of this build: 16 x 16 cells (the workloads' blocks).
 constexpr int NX = 16;
 constexpr int NY = 16;
-constexpr int FW = NX + 2;            // population row width with the 1-cell ghost ring
+constexpr int FW = NX + 2;            // population row width with the 1-cell ghost ring (LDS frame)
 constexpr int FS = FW * (NY + 2);     // 324 cells per population per block
-constexpr int POPB = 2 * 9 * FS;      // doubles per block per population buffer: h(9) then g(9)
+// One population array of a block in memory: the 256 interior cells as 4 x 4
+// tiles of 4 x 4 cells (a tile is one 128-byte line, so a row or a column of
+// the block touches 4 lines), then the 68 ghost-ring cells in ring order (see
+// pcell), padded to a multiple of 128 bytes.
+constexpr int GS = 336;
+constexpr int POPB = 2 * 9 * GS;      // doubles per block per population buffer: h(9) then g(9)
 constexpr int CW = NX + 4;            // C row width with 2 ghost rings
 constexpr int CS = CW * (NY + 4);     // 400
 constexpr int PS = NX * NY;           // 256
 constexpr int PUB = 3 * PS;           // P, Ux, Uy
+
+// Offset of cell (x, y), x, y in 0..17, inside a population array in memory.
+// Ring order: left column y=1..16, right column, bottom row x=1..16, top row,
+// then the corners (0,0), (0,17), (17,0), (17,17).
+__host__ __device__ inline int pcell(int x, int y)
+{
+    if (x >= 1 && x <= NX && y >= 1 && y <= NY) {
+        const int xi = x - 1, yi = y - 1;
+        return (((yi >> 2) << 2) + (xi >> 2)) * 16 + ((yi & 3) << 2) + (xi & 3);
+    }
+    int t;
+    if (y >= 1 && y <= NY)
+        t = (x == 0 ? 0 : 16) + y - 1;
+    else if (x >= 1 && x <= NX)
+        t = (y == 0 ? 32 : 48) + x - 1;
+    else
+        t = 64 + (x == 0 ? 0 : 2) + (y == 0 ? 0 : 1);
+    return NX * NY + t;
+}
 
 // Physics constants of D2Q9_Mod.f90 plus wc (D2Q9_Mod.f90:11).
 struct Phys {
@@ -47,7 +71,7 @@ struct NbDesc {
 
 // Device view of a mesh, passed by value to the kernels.
 struct DevMesh {
-    double *pop[2];          // [slot][set 0:h 1:g][q][FS], ping-pong buffers
+    double *pop[2];          // [slot][set 0:h 1:g][q][GS] (pcell order), ping-pong buffers
     double *C;               // [slot][CS], C(-1:nx+2,-1:ny+2)
     double *PU;              // [slot][3][PS]
     const BlockDev *blk;     // [slot]
@@ -78,7 +102,7 @@ struct __attribute__((aligned(16))) PNode {
     int dest;     // LDS cell written (own cell index < CS / FS, else aux)
     int info;     // type | wx pattern << 4 | wy pattern << 6
     int ref[9];   // >= 0: element offset in the global field; < 0: -(1 + LDS cell)
-    int pad;
+    int own;      // populations: pcell offset of an own ghost cell, -1 for aux cells
 };
 struct PHdr {
     int off;                  // first node

@@ -26,7 +26,7 @@ __constant__ int c_dsy[8] = {0, 0, -1, 1, -1, 1, -1, 1};
 // lattice weights Wa = [16,4,4,4,4,1,1,1,1]/36 (D2Q9_Mod.f90:8)
 __device__ __forceinline__ double wa(int q) { return q == 0 ? 16.0 / 36.0 : (q < 5 ? 4.0 / 36.0 : 1.0 / 36.0); }
 
-__device__ __forceinline__ int fidx(int set, int q, int x, int y) { return (set * 9 + q) * FS + y * FW + x; }
+__device__ __forceinline__ int fidx(int set, int q, int x, int y) { return (set * 9 + q) * GS + pcell(x, y); }
 __device__ __forceinline__ int cidx(int x, int y) { return (y + 1) * CW + (x + 1); }
 __device__ __forceinline__ int pidx(int i, int j) { return (i - 1) + NX * (j - 1); }
 
@@ -234,7 +234,7 @@ __device__ __forceinline__ PNode load_node(const PNode *p)
     PNode n;
     n.dest = a.x; n.info = a.y; n.ref[0] = a.z; n.ref[1] = a.w;
     n.ref[2] = b.x; n.ref[3] = b.y; n.ref[4] = b.z; n.ref[5] = b.w;
-    n.ref[6] = c.x; n.ref[7] = c.y; n.ref[8] = c.z; n.pad = c.w;
+    n.ref[6] = c.x; n.ref[7] = c.y; n.ref[8] = c.z; n.own = c.w;
     return n;
 }
 
@@ -300,47 +300,49 @@ __device__ __forceinline__ double eval_global(const PNode &nd, const double *__r
     return type == NT_FINE ? o / 64.0 : o / 1024.0;
 }
 
-// population component c (set * 9 + q) of one recipe; bounce-back rows take
-// the opposite population of the first row (Bounce_Back_3/4, BC_Cell.f90:2-42)
+// population q of set `set` for one recipe; sF holds that set's 9 arrays.
+// Bounce-back rows take the opposite population of the first row
+// (Bounce_Back_3/4, BC_Cell.f90:2-42).
 template <bool GLOBAL_ONLY>
-__device__ __forceinline__ double eval_hg(const PNode &nd, int c, const double *__restrict__ Fc, const double *sF)
+__device__ __forceinline__ double eval_hg(const PNode &nd, int set, int q, const double *__restrict__ Fc,
+                                          const double *sF)
 {
     const int type = nd.info & 15;
-    if (type < NT_BB_BOT) return GLOBAL_ONLY ? eval_global(nd, Fc, c * FS) : eval_node(nd, Fc, c * FS, sF, c * LC_F);
-    const int set = c >= 9 ? 1 : 0, q = c - 9 * set;
+    const int c = set * 9 + q;
+    if (type < NT_BB_BOT) return GLOBAL_ONLY ? eval_global(nd, Fc, c * GS) : eval_node(nd, Fc, c * GS, sF, q * LC_F);
     const int e = c_ex[q] * c_ey[q];
     const int r = nd.ref[1 + (type == NT_BB_TOP ? -e : e)];
-    const int cs = set * 9 + c_opp[q];
-    return (GLOBAL_ONLY || r >= 0) ? Fc[r + cs * FS] : sF[-1 - r + cs * LC_F];
+    const int qo = c_opp[q];
+    return (GLOBAL_ONLY || r >= 0) ? Fc[r + (set * 9 + qo) * GS] : sF[-1 - r + qo * LC_F];
 }
 
-// One stage of an hg ghost program: recipes [b, e) of the block (staged in
-// sN) applied to all 18 population components. Items run recipe-fastest, so
-// the lanes of a wavefront read neighbouring cells of one component; U items
-// per thread have their operands in flight together.
+// One stage of an hg ghost program for one population set: recipes [b, e)
+// of the block (staged in sN) applied to the set's 9 populations. Items run
+// recipe-fastest, so the lanes of a wavefront read neighbouring cells of one
+// population array; U items per thread have their operands in flight together.
 template <bool GLOBAL_ONLY, int U>
-__device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, const double *__restrict__ Fc, double *sF,
-                                         double *F, int tid)
+__device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, int set, const double *__restrict__ Fc,
+                                         double *sF, double *F, int tid)
 {
     const int ns = e - b;
-    for (int base = 0; base < 18 * ns; base += 256 * U) {
+    for (int base = 0; base < 9 * ns; base += 256 * U) {
         double v[U];
 #pragma unroll
         for (int r = 0; r < U; r++) {
             const int t = base + tid + 256 * r;
-            if (t < 18 * ns) {
-                const int c = t / ns;
-                v[r] = eval_hg<GLOBAL_ONLY>(sN[b + t - c * ns], c, Fc, sF);
+            if (t < 9 * ns) {
+                const int q = t / ns;
+                v[r] = eval_hg<GLOBAL_ONLY>(sN[b + t - q * ns], set, q, Fc, sF);
             }
         }
 #pragma unroll
         for (int r = 0; r < U; r++) {
             const int t = base + tid + 256 * r;
-            if (t < 18 * ns) {
-                const int c = t / ns;
-                const int dest = sN[b + t - c * ns].dest;
-                sF[c * LC_F + dest] = v[r];
-                if (dest < FS) F[c * FS + dest] = v[r];
+            if (t < 9 * ns) {
+                const int q = t / ns;
+                const PNode &nd = sN[b + t - q * ns];
+                sF[q * LC_F + nd.dest] = v[r];
+                if (nd.own >= 0) F[(set * 9 + q) * GS + nd.own] = v[r];
             }
         }
     }
@@ -471,41 +473,39 @@ __device__ __forceinline__ int lidx(int c, int x, int y) { return c * LC_F + y *
 
 // Perfect_Shift_AMR / Streaming_LaxWendroff_AMR (LBM_Subroutines.f90:41-90)
 // of one cell from the LDS copy of the block into G, and C = sum(h) (:193-201)
-__device__ __forceinline__ double stream_cell(const double *sF, double *__restrict__ G, int i, int j, int dx)
+__device__ __forceinline__ double stream_cell(const double *sF, double *__restrict__ G, int set, int i, int j,
+                                              int dx)
 {
     const double cfl = 1.0 / (double)dx;
     const double a = cfl / 2.0 * (1.0 + cfl), b = 1.0 - cfl * cfl, c = cfl / 2.0 * (1.0 - cfl);
     double s = 0.0;
 #pragma unroll
-    for (int set = 0; set < 2; set++) {
-#pragma unroll
-        for (int q = 0; q < 9; q++) {
-            const int cp = set * 9 + q;
-            double v;
-            if (q == 0)
-                v = sF[lidx(cp, i, j)];
-            else if (cfl > 0.9)
-                v = sF[lidx(cp, i - c_ex[q], j - c_ey[q])];
-            else
-                v = a * sF[lidx(cp, i - c_ex[q], j - c_ey[q])] + b * sF[lidx(cp, i, j)] -
-                    c * sF[lidx(cp, i + c_ex[q], j + c_ey[q])];
-            G[fidx(set, q, i, j)] = v;
-            if (set == 0) s += v;
-        }
+    for (int q = 0; q < 9; q++) {
+        double v;
+        if (q == 0)
+            v = sF[lidx(q, i, j)];
+        else if (cfl > 0.9)
+            v = sF[lidx(q, i - c_ex[q], j - c_ey[q])];
+        else
+            v = a * sF[lidx(q, i - c_ex[q], j - c_ey[q])] + b * sF[lidx(q, i, j)] -
+                c * sF[lidx(q, i + c_ex[q], j + c_ey[q])];
+        G[fidx(set, q, i, j)] = v;
+        s += v;
     }
     return s;
 }
 
 // Fill_Ghost_Cells_hg of every leaf in list by its ghost program (bounce-back
 // rows included), then optionally the stream into the other population buffer
-// and C = sum(h). Each of the block's cells and its coarse neighbours' needed
-// ghost cells is one recipe applied to all 9 populations of one set.
+// and C = sum(h). The two population sets never read each other here, so a
+// block is two workgroups, blockIdx.y = set (0: h, 1: g). Each recipe covers a
+// ghost cell of the block or a coarse neighbour's ghost cell it needs.
 // flags: 1 fill (else ghosts from memory), 2 stream, 4 C = sum h,
 // 8 carry the ghost ring into the new buffer (as the in-place reference stream)
 __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *__restrict__ list, Prog prog,
                                                  int flags)
 {
-    const int k = blockIdx.x;
+    const int k = blockIdx.x, set = blockIdx.y;
     const int tid = threadIdx.x;
     const int skip = *m.skip;
     const int slot = list[k];
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *
     const BlockDev me = m.blk[slot];
     const double *__restrict__ Fc = m.pop[cur];
     double *F = m.pop[cur] + (size_t)slot * POPB;
-    __shared__ double sF[18 * LC_F];
+    __shared__ double sF[9 * LC_F];
     __shared__ PNode sN[NODES_HG];
     const int i = tid % NX + 1, j = tid / NX + 1;
     const int nn = hd.nst > 0 ? hd.end[hd.nst - 1] : 0;
@@ -527,11 +527,11 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *
         for (int t = tid; t < 3 * nn; t += 256) dst[t] = src[t];
     }
 #pragma unroll
-    for (int c = 0; c < 18; c++) sF[lidx(c, i, j)] = F[c * FS + j * FW + i];
+    for (int q = 0; q < 9; q++) sF[lidx(q, i, j)] = F[fidx(set, q, i, j)];
     if (!(flags & 1)) {
-        for (int t = tid; t < 68 * 18; t += 256) {
-            int c = t / 68, cell = ring1_cell(t - c * 68);
-            sF[c * LC_F + cell] = F[c * FS + cell];
+        for (int t = tid; t < 68 * 9; t += 256) {   // the ring is stored in ring1_cell order
+            int q = t / 68, r = t - q * 68;
+            sF[q * LC_F + ring1_cell(r)] = F[(set * 9 + q) * GS + NX * NY + r];
         }
     }
     __syncthreads();
@@ -541,20 +541,20 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *
     for (int s = 0; s < hd.nst; s++) {
         const int end = hd.end[s];
         if (s == 0)
-            hg_stage<true, 4>(sN, begin, end, Fc, sF, F, tid);
+            hg_stage<true, 4>(sN, begin, end, set, Fc, sF, F, tid);
         else
-            hg_stage<false, 2>(sN, begin, end, Fc, sF, F, tid);
+            hg_stage<false, 2>(sN, begin, end, set, Fc, sF, F, tid);
         begin = end;
         __syncthreads();
     }
     if (!(flags & 2)) return;
     double *G = m.pop[cur ^ 1] + (size_t)slot * POPB;
-    const double csum = stream_cell(sF, G, i, j, me.dx);
-    if (flags & 4) m.C[(size_t)slot * CS + cidx(i, j)] = csum;
+    const double csum = stream_cell(sF, G, set, i, j, me.dx);
+    if ((flags & 4) && set == 0) m.C[(size_t)slot * CS + cidx(i, j)] = csum;
     if (flags & 8) {
-        for (int t = tid; t < 68 * 18; t += 256) {
-            int c = t / 68, cell = ring1_cell(t - c * 68);
-            G[c * FS + cell] = sF[c * LC_F + cell];
+        for (int t = tid; t < 68 * 9; t += 256) {
+            int q = t / 68, r = t - q * 68;
+            G[(set * 9 + q) * GS + NX * NY + r] = sF[q * LC_F + ring1_cell(r)];
         }
     }
 }
@@ -913,7 +913,7 @@ void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Pro
 {
     if (n <= 0) return;
     int flags = (prog.hdr ? 1 : 0) | (stream_after ? 2 : 0) | (macro_h ? 4 : 0) | (copy_ghosts ? 8 : 0);
-    hipLaunchKernelGGL(k_prog_hg, dim3(n), dim3(256), 0, s, m, cur, list, prog, flags);
+    hipLaunchKernelGGL(k_prog_hg, dim3(n, 2), dim3(256), 0, s, m, cur, list, prog, flags);
 }
 
 void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream_t s)

@@ -54,6 +54,7 @@ struct amrlbm_mesh {
     Phys ph{};
     std::vector<Block> blk;          // 1-based, size max_blocks + 1
     int nblks = 0, kk = 0, T = 0, levelmin = 0, levelmax = 0;
+    int tile_roots = 0;              // roots per independent instance (walls between instances)
     std::deque<int> freeq;
     std::vector<int> leaf;           // Array_of_Leaf_Blocks order (ascending id)
     std::vector<std::vector<int>> at;  // per level: position -> id (0 = none)
@@ -109,6 +110,7 @@ struct amrlbm_mesh {
     int nlev() const { return prm.max_levels + 1; }
     int width(int L) const { return 1 << L; }
     int height(int L) const { return prm.nroots << L; }
+    int tile_height(int L) const { return tile_roots << L; }
     int &cell_at(int L, int I, int J) { return at[L][(size_t)J * width(L) + I]; }
 };
 
@@ -135,7 +137,7 @@ int nb_value(amrlbm_mesh *m, int n, int k)   // k = 1..8
     const Block &b = m->blk[n];
     int L = b.level;
     int tJ = b.J + dsy[k - 1];
-    if (tJ < 0 || tJ >= m->height(L)) return -k;
+    if (tJ < 0 || tJ >= m->height(L) || tJ / m->tile_height(L) != b.J / m->tile_height(L)) return -k;
     int W = m->width(L);
     int tI = ((b.I + dsx[k - 1]) % W + W) % W;
     int id = m->cell_at(L, tI, tJ);
@@ -300,7 +302,7 @@ struct ProgBuild {
     static bool interior(int x, int y) { return x >= 1 && x <= NX && y >= 1 && y <= NY; }
     int global_ref(int b, int x, int y) const
     {
-        return (b - 1) * (kind == KIND_HG ? POPB : CS) + cell_index(x, y);
+        return kind == KIND_HG ? (b - 1) * POPB + pcell(x, y) : (b - 1) * CS + cell_index(x, y);
     }
     int memo_slot(long long key) const
     {
@@ -401,6 +403,7 @@ struct ProgBuild {
             dest = own_cells() + naux++;
         }
         nd.dest = dest;
+        nd.own = (kind == KIND_HG && b == n) ? pcell(x, y) : -1;
         if ((int)cell_stage.size() <= dest) cell_stage.resize(dest + 1, 0);
         cell_stage[dest] = st + 1;
         nodes.push_back(nd);
@@ -448,8 +451,8 @@ void build_leaf_program(amrlbm_mesh *m, int kind, int n, LeafProg &out)
                 ok = -1 - v < own + pb.naux;
             } else {
                 int b = v / slotsz + 1, c = v % slotsz;
-                int x = kind == KIND_HG ? c % FW : c % CW - 1, y = kind == KIND_HG ? c / FW : c / CW - 1;
-                ok = b <= m->nblks && m->blk[b].level >= 0 && m->blk[b].isleaf && ProgBuild::interior(x, y);
+                bool in = kind == KIND_HG ? c < NX * NY : ProgBuild::interior(c % CW - 1, c / CW - 1);
+                ok = b <= m->nblks && m->blk[b].level >= 0 && m->blk[b].isleaf && in;
             }
         }
         if (!ok) {
@@ -820,14 +823,14 @@ void ref_to_dev_pop(const double *ref, double *dev)   // one set
 {
     for (int y = 0; y < NY + 2; y++)
         for (int x = 0; x < NX + 2; x++)
-            for (int q = 0; q < 9; q++) dev[q * FS + y * FW + x] = ref[q + 9 * (x + FW * y)];
+            for (int q = 0; q < 9; q++) dev[q * GS + pcell(x, y)] = ref[q + 9 * (x + FW * y)];
 }
 
 void dev_to_ref_pop(const double *dev, double *ref)
 {
     for (int y = 0; y < NY + 2; y++)
         for (int x = 0; x < NX + 2; x++)
-            for (int q = 0; q < 9; q++) ref[q + 9 * (x + FW * y)] = dev[q * FS + y * FW + x];
+            for (int q = 0; q < 9; q++) ref[q + 9 * (x + FW * y)] = dev[q * GS + pcell(x, y)];
 }
 
 int alloc_device(amrlbm_mesh *m)
@@ -883,10 +886,11 @@ void reset_topology(amrlbm_mesh *m)
 void bubble_c(const amrlbm_mesh *m, const Block &b, double R, double X0, double Y0, double *C)
 {
     const int dx = b.dx;
+    const int J = b.J % m->tile_height(b.level);   // the instance's own frame
     for (int j = -1; j <= NY + 2; j++)
         for (int i = -1; i <= NX + 2; i++) {
             double Xc = (double)(b.I * dx * NX) + (double)dx * ((double)i - 0.5);
-            double Yc = (double)(b.J * dx * NY) + (double)dx * ((double)j - 0.5);
+            double Yc = (double)(J * dx * NY) + (double)dx * ((double)j - 0.5);
             double Ri = std::sqrt((Xc - X0) * (Xc - X0) + (Yc - Y0) * (Yc - Y0));
             C[(j + 1) * CW + (i + 1)] = 0.5 + 0.5 * std::tanh(2.0 * (Ri - R) / m->ph.W);
         }
@@ -1019,6 +1023,7 @@ int amrlbm_create(const amrlbm_params *p, amrlbm_mesh **out)
         return fail(AMRLBM_ERR_ARG, "inconsistent mesh parameters (L0 must be nx * 2^max_levels)");
     amrlbm_mesh *m = new amrlbm_mesh();
     m->prm = *p;
+    m->tile_roots = p->nroots;
     m->ph.M0 = p->M0;
     m->ph.wc = 1.0 / (0.5 + p->M0 * 3.0);
     m->ph.W = p->W;
@@ -1076,7 +1081,15 @@ int amrlbm_destroy(amrlbm_mesh *m)
 
 int amrlbm_setup_rising_bubble(amrlbm_mesh *m, double R, double x0, double y0)
 {
+    return amrlbm_setup_replicas(m, 1, R, x0, y0);
+}
+
+int amrlbm_setup_replicas(amrlbm_mesh *m, int ninst, double R, double x0, double y0)
+{
     if (!m) return fail(AMRLBM_ERR_ARG, "null mesh");
+    if (ninst < 1 || m->prm.nroots % ninst)
+        return fail(AMRLBM_ERR_ARG, "the roots must split evenly into the instances");
+    m->tile_roots = m->prm.nroots / ninst;
     reset_topology(m);
     m->cur = 0;
     const amrlbm_params &p = m->prm;
@@ -1158,6 +1171,7 @@ int amrlbm_set_topology(amrlbm_mesh *m, int nblks, const int *blocks, int kk, in
 {
     if (!m || !blocks || nblks < 1 || nblks > m->prm.max_blocks || nfree < 0 || (nfree > 0 && !freeq))
         return fail(AMRLBM_ERR_ARG, "bad topology arguments");
+    m->tile_roots = m->prm.nroots;   // a caller's mesh is one instance
     reset_topology(m);
     m->nblks = nblks;
     for (int n = 1; n <= nblks; n++) {
@@ -1199,12 +1213,12 @@ int amrlbm_put_block(amrlbm_mesh *m, int id, const double *h, const double *g, c
     double *F = m->dm.pop[m->cur] + s * POPB;
     if (h) {
         ref_to_dev_pop(h, buf.data());
-        HIPCHK(hipMemcpyAsync(F, buf.data(), sizeof(double) * 9 * FS, hipMemcpyHostToDevice, m->stream));
+        HIPCHK(hipMemcpyAsync(F, buf.data(), sizeof(double) * 9 * GS, hipMemcpyHostToDevice, m->stream));
         HIPCHK(hipStreamSynchronize(m->stream));
     }
     if (g) {
         ref_to_dev_pop(g, buf.data());
-        HIPCHK(hipMemcpyAsync(F + 9 * FS, buf.data(), sizeof(double) * 9 * FS, hipMemcpyHostToDevice, m->stream));
+        HIPCHK(hipMemcpyAsync(F + 9 * GS, buf.data(), sizeof(double) * 9 * GS, hipMemcpyHostToDevice, m->stream));
         HIPCHK(hipStreamSynchronize(m->stream));
     }
     if (C) HIPCHK(hipMemcpyAsync(m->dm.C + s * CS, C, sizeof(double) * CS, hipMemcpyHostToDevice, m->stream));
@@ -1228,7 +1242,7 @@ int amrlbm_get_block(amrlbm_mesh *m, int id, double *h, double *g, double *C, do
     if (Uy) HIPCHK(hipMemcpyAsync(Uy, m->dm.PU + s * PUB + 2 * PS, sizeof(double) * PS, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
     if (h) dev_to_ref_pop(buf.data(), h);
-    if (g) dev_to_ref_pop(buf.data() + 9 * FS, g);
+    if (g) dev_to_ref_pop(buf.data() + 9 * GS, g);
     return AMRLBM_OK;
 }
 
@@ -1599,16 +1613,16 @@ struct OneBlock {
     }
     int put_pop(int set, const double *ref)
     {
-        std::vector<double> buf(9 * FS);
+        std::vector<double> buf(9 * GS);
         ref_to_dev_pop(ref, buf.data());
-        HIPCHK(hipMemcpyAsync(dm.pop[0] + set * 9 * FS, buf.data(), sizeof(double) * 9 * FS, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(dm.pop[0] + set * 9 * GS, buf.data(), sizeof(double) * 9 * GS, hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));
         return AMRLBM_OK;
     }
     int get_pop(int b, int set, double *ref)
     {
-        std::vector<double> buf(9 * FS);
-        HIPCHK(hipMemcpyAsync(buf.data(), dm.pop[b] + set * 9 * FS, sizeof(double) * 9 * FS, hipMemcpyDeviceToHost, s));
+        std::vector<double> buf(9 * GS);
+        HIPCHK(hipMemcpyAsync(buf.data(), dm.pop[b] + set * 9 * GS, sizeof(double) * 9 * GS, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         dev_to_ref_pop(buf.data(), ref);
         return AMRLBM_OK;

@@ -54,6 +54,14 @@ const char *amrlbm_last_error(void);
  * Initial_Mesh (:493-513), Initialize_AMR_Rising_Bubble (:142-231) and the
  * init-regrid loop, with bubble radius R centred at (x0,y0). */
 int amrlbm_setup_rising_bubble(amrlbm_mesh *m, double R, double x0, double y0);
+/* ninst independent copies of that set-up in one mesh: the nroots roots split
+ * into ninst groups stacked in y, each group a rising-bubble domain of its
+ * own with bounce-back walls at its ends and its bubble at (x0, y0) in its
+ * own frame. Every step launch covers the leaves of all copies. No
+ * counterpart in the reference (one domain per program); it is the
+ * replicated-instance workload for measurements at a large working set.
+ * ninst = 1 is amrlbm_setup_rising_bubble. */
+int amrlbm_setup_replicas(amrlbm_mesh *m, int ninst, double R, double x0, double y0);
 
 /* ---- state exchange with the caller's Blk(:) mirror ----------------------- */
 /* counts: Nblks, nLeaf, LevelMin, LevelMax, KK, free-list length, step T */
