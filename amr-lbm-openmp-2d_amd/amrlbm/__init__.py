@@ -68,6 +68,7 @@ def lib():
         PP = ctypes.POINTER(Params)
         sig = {
             "amrlbm_create": [PP, ctypes.POINTER(vp)],
+            "amrlbm_create_fp32": [PP, ctypes.POINTER(vp)],
             "amrlbm_destroy": [vp],
             "amrlbm_setup_rising_bubble": [vp, dp, dp, dp],
             "amrlbm_setup_replicas": [vp, ip, dp, dp, dp],
@@ -126,13 +127,19 @@ def pop_array():
 
 
 class Mesh:
-    """One AMR mesh resident on the GPU."""
+    """One AMR mesh resident on the GPU. precision=64 is the reference's FP64
+    arithmetic; precision=32 the FP32 production form (amrlbm_create_fp32),
+    with the same host-side interface (FP64 arrays in, FP64 arrays out)."""
 
-    def __init__(self, workload):
+    def __init__(self, workload, precision=64):
+        if precision not in (32, 64):
+            raise ValueError("precision is 32 or 64")
         self.w = workload
+        self.precision = precision
         self.params = params_from_workload(workload)
         h = ctypes.c_void_p()
-        _check(lib().amrlbm_create(ctypes.byref(self.params), ctypes.byref(h)))
+        create = lib().amrlbm_create_fp32 if precision == 32 else lib().amrlbm_create
+        _check(create(ctypes.byref(self.params), ctypes.byref(h)))
         self.h = h
 
     def close(self):

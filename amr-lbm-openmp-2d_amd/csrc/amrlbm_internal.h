@@ -49,6 +49,13 @@ struct Phys {
     double M0, wc, W, sigma, beta, kappa, taul, tauh, rhol, rhoh, gy;
 };
 
+// The same constants for the FP32 path, with the quotients it uses as
+// products precomputed.
+struct PhysF {
+    float M0, wc, beta, kappa, taul, tauh, rhol, rhoh, gy;
+    float drho, drho3, drho6, inv_tauh, inv_taul, k12W;   // k12W = 12 M0 / W
+};
+
 // Per-slot block descriptor on the device (slot = reference id - 1).
 struct BlockDev {
     int level;    // -1 for free slots
@@ -69,11 +76,14 @@ struct NbDesc {
     int child0;   // slot of its first child when refined, else -1
 };
 
-// Device view of a mesh, passed by value to the kernels.
-struct DevMesh {
-    double *pop[2];          // [slot][set 0:h 1:g][q][GS] (pcell order), ping-pong buffers
-    double *C;               // [slot][CS], C(-1:nx+2,-1:ny+2)
-    double *PU;              // [slot][3][PS]
+// Device view of a mesh, passed by value to the kernels. T is the storage
+// type of the fields: double (the reference's arithmetic) or float (the FP32
+// path, which stores h - w_q and C - 1, deviations from the liquid at rest).
+template <class T>
+struct DevMeshT {
+    T *pop[2];               // [slot][set 0:h 1:g][q][GS] (pcell order), ping-pong buffers
+    T *C;                    // [slot][CS], C(-1:nx+2,-1:ny+2)
+    T *PU;                   // [slot][3][PS]
     const BlockDev *blk;     // [slot]
     const NbDesc *nbd;       // [slot][8] neighbours in nghbr(1:8) order
     int *err_flags;          // [0]: bad level jump seen by a ghost fill
@@ -81,6 +91,28 @@ struct DevMesh {
                              // segment launched ahead of a regrid that changes
                              // the mesh)
 };
+
+// The host's handle: the FP64 view, whose field pointers hold float arrays
+// when f32 is set; the launchers pick the kernel instantiation.
+struct DevMesh : DevMeshT<double> {
+    int f32 = 0;
+    PhysF phf{};
+};
+
+template <class T>
+inline DevMeshT<T> typed(const DevMesh &m)
+{
+    DevMeshT<T> t;
+    t.pop[0] = reinterpret_cast<T *>(m.pop[0]);
+    t.pop[1] = reinterpret_cast<T *>(m.pop[1]);
+    t.C = reinterpret_cast<T *>(m.C);
+    t.PU = reinterpret_cast<T *>(m.PU);
+    t.blk = m.blk;
+    t.nbd = m.nbd;
+    t.err_flags = m.err_flags;
+    t.skip = m.skip;
+    return t;
+}
 
 // Ghost program of one block for one fill kind: the block's own ghost cells
 // and the ghost cells of coarser neighbours they read ("aux" cells), each one

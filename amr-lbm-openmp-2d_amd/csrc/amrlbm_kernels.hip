@@ -1,7 +1,8 @@
-// gfx950 kernels of the two-phase AMR-LBM step, FP64.
+// gfx950 kernels of the two-phase AMR-LBM step: FP64 (the reference's
+// arithmetic) and FP32 (the production form) instantiations.
 //
 // Layout (one slot per reference block id, slot = id - 1):
-//   pop[b][slot][set][q][y][x]   x,y in 0..17 (1-cell ghost ring), set 0 = h, 1 = g
+//   pop[b][slot][set][q][pcell(x,y)]  x,y in 0..17 (1-cell ghost ring), set 0 = h, 1 = g
 //   C[slot][y+1][x+1]            x,y in -1..18 (2 ghost rings)
 //   PU[slot][k][j-1][i-1]        k = P, Ux, Uy
 // One workgroup of 256 threads owns one 16 x 16 block: thread t <-> cell
@@ -12,6 +13,14 @@
 // divisions) and the library is built with -ffp-contract=off, so results
 // match the reference build to the last bit wherever the GPU's IEEE division
 // and square root round as the host's do.
+//
+// FP32 stores h - w_q and C - 1 (deviations from the liquid at rest; g and P
+// are deviations from zero already). Every weight set of the fills, the
+// stream, prolong and restrict sums to 1 and bounce-back swaps populations of
+// equal weight, so those kernels are the same code on either storage; only
+// the collision and U, P have FP32 forms of their own (deviation form).
+#include <type_traits>
+
 #include "amrlbm_internal.h"
 
 namespace amrlbm {
@@ -197,6 +206,185 @@ __device__ __forceinline__ void macro_g_cell(const double *sC, int i, int j, int
 }
 
 // ---------------------------------------------------------------------------
+// FP32 cell numerics in deviation form: sC holds C~ = C - 1, h holds h - w_q,
+// g, P, U are stored as they are. Same formulas as above with C = C~ + 1
+// substituted where C enters absolutely (chemical potential, density,
+// equilibrium, relaxation time) and constant quotients taken as products.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float waf(int q) { return q == 0 ? 16.0f / 36.0f : (q < 5 ? 4.0f / 36.0f : 1.0f / 36.0f); }
+
+// 4 beta C (C - 1)(C - 1/2) - kappa lap(C) / dx^2 (the Laplacian's weights sum to 0)
+__device__ __forceinline__ float chem_pot(const float *sC, int i, int j, float inv_dx2, const PhysF &ph)
+{
+    float d2c = (sC[cidx(i - 1, j - 1)] + sC[cidx(i + 1, j - 1)] + sC[cidx(i - 1, j + 1)] + sC[cidx(i + 1, j + 1)] +
+                 4.0f * (sC[cidx(i, j - 1)] + sC[cidx(i - 1, j)] + sC[cidx(i + 1, j)] + sC[cidx(i, j + 1)]) -
+                 20.0f * sC[cidx(i, j)]) *
+                (1.0f / 6.0f);
+    float ct = sC[cidx(i, j)];
+    return 4.0f * ph.beta * (ct + 1.0f) * ct * (ct + 0.5f) - ph.kappa * d2c * inv_dx2;
+}
+
+__device__ __forceinline__ void gradient_ed(const float e[9], float &gx, float &gy)
+{
+    gx = (e[1] - e[3]) * (1.0f / 3.0f) + (e[5] - e[6] - e[7] + e[8]) * (1.0f / 12.0f);
+    gy = (e[2] - e[4]) * (1.0f / 3.0f) + (e[5] + e[6] - e[7] - e[8]) * (1.0f / 12.0f);
+}
+
+__device__ __forceinline__ void ed_c(const float *sC, int i, int j, float inv_dx, float e[9])
+{
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+        e[k] = (sC[cidx(i + c_ex[k], j + c_ey[k])] - sC[cidx(i - c_ex[k], j - c_ey[k])]) * (0.5f * inv_dx);
+}
+
+__device__ __forceinline__ void ed_m(const float *sC, int i, int j, float inv_dx, float e[9])
+{
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+        e[k] = (-sC[cidx(i - c_ex[k], j - c_ey[k])] - 3.0f * sC[cidx(i, j)] + 5.0f * sC[cidx(i + c_ex[k], j + c_ey[k])] -
+                sC[cidx(i + 2 * c_ex[k], j + 2 * c_ey[k])]) *
+               (0.25f * inv_dx);
+}
+
+// Gamma - w and heq - w (Equilibrium_h_g_AMR, LBM_Subroutines.f90:2-21):
+// heq - w = C~ Gamma + (Gamma - w) + C (12 (1 - C) / W) M0 w e.n
+__device__ __forceinline__ void equilibrium(float ct, float U, float V, float ni, float nj, const PhysF &ph,
+                                            float gmw[9], float heqd[9])
+{
+    const float C = ct + 1.0f;
+    const float tmp = -ct * ph.k12W;
+    const float U2 = U * U + V * V;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const float eU = (float)c_ex[i] * U + (float)c_ey[i] * V;
+        const float eN = (float)c_ex[i] * ni + (float)c_ey[i] * nj;
+        gmw[i] = waf(i) * (eU * (3.0f + 4.5f * eU) - 1.5f * U2);
+        heqd[i] = ct * (waf(i) + gmw[i]) + gmw[i] + C * tmp * waf(i) * eN;
+    }
+}
+
+__device__ __forceinline__ void forcing(const float gmw[9], float drho3, float mu, const float e[9], float Ux,
+                                        float Uy, float gx, float gy, float Fb, float hlp[9])
+{
+    const float ug = Ux * gx + Uy * gy;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const float G = waf(i) + gmw[i];
+        hlp[i] = (gmw[i] * drho3 + G * mu) * (e[i] - ug) + ((float)c_ey[i] - Uy) * G * Fb;
+    }
+}
+
+__device__ __forceinline__ void mrt(float s9, const float geq[9], float g[9])
+{
+    float f[9], o[9], t[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) f[i] = g[i] - geq[i];
+    o[0] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+    o[1] = -4.0f * f[0] - f[1] - f[2] - f[3] - f[4] + 2.0f * (f[5] + f[6] + f[7] + f[8]);
+    o[2] = 4.0f * f[0] - 2.0f * (f[1] + f[2] + f[3] + f[4]) + f[5] + f[6] + f[7] + f[8];
+    o[3] = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
+    o[4] = -2.0f * (f[1] - f[3]) + f[5] - f[6] - f[7] + f[8];
+    o[5] = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
+    o[6] = -2.0f * (f[2] - f[4]) + f[5] + f[6] - f[7] - f[8];
+    o[7] = f[1] - f[2] + f[3] - f[4];
+    o[8] = f[5] - f[6] + f[7] - f[8];
+#pragma unroll
+    for (int i = 0; i < 7; i++) t[i] = o[i];
+    t[7] = o[7] * s9;
+    t[8] = o[8] * s9;
+    o[0] = 4.0f * (t[0] - t[1] + t[2]);
+    o[1] = 4.0f * t[0] - t[1] - 2.0f * t[2] + 6.0f * (t[3] - t[4]) + 9.0f * t[7];
+    o[2] = 4.0f * t[0] - t[1] - 2.0f * t[2] + 6.0f * (t[5] - t[6]) - 9.0f * t[7];
+    o[3] = 4.0f * t[0] - t[1] - 2.0f * t[2] - 6.0f * (t[3] - t[4]) + 9.0f * t[7];
+    o[4] = 4.0f * t[0] - t[1] - 2.0f * t[2] - 6.0f * (t[5] - t[6]) - 9.0f * t[7];
+    o[5] = 4.0f * t[0] + 2.0f * t[1] + t[2] + 6.0f * t[3] + 3.0f * t[4] + 6.0f * t[5] + 3.0f * t[6] + 9.0f * t[8];
+    o[6] = 4.0f * t[0] + 2.0f * t[1] + t[2] - 6.0f * t[3] - 3.0f * t[4] + 6.0f * t[5] + 3.0f * t[6] - 9.0f * t[8];
+    o[7] = 4.0f * t[0] + 2.0f * t[1] + t[2] - 6.0f * t[3] - 3.0f * t[4] - 6.0f * t[5] - 3.0f * t[6] + 9.0f * t[8];
+    o[8] = 4.0f * t[0] + 2.0f * t[1] + t[2] + 6.0f * t[3] + 3.0f * t[4] - 6.0f * t[5] - 3.0f * t[6] - 9.0f * t[8];
+#pragma unroll
+    for (int i = 0; i < 9; i++) g[i] = g[i] - o[i] * (1.0f / 36.0f);
+}
+
+// Collision_h_g_AMR (LBM_Subroutines.f90:116-189), deviation form
+__device__ __forceinline__ void collide_cell(const float *sC, int i, int j, int dx, float P, float Ux, float Uy,
+                                             const PhysF &ph, float h[9], float g[9])
+{
+    const float inv_dx = 1.0f / (float)dx;
+    float e[9], hlp[9], gmw[9], heqd[9], geq[9], gx, gy;
+    const float mu = chem_pot(sC, i, j, inv_dx * inv_dx, ph);
+    const float ct = sC[cidx(i, j)];
+    ed_c(sC, i, j, inv_dx, e);
+    gradient_ed(e, gx, gy);
+    const float rn = rsqrtf(gx * gx + gy * gy + 1.0e-32f);   // normal_AMR (:25-37)
+    equilibrium(ct, Ux, Uy, gx * rn, gy * rn, ph, gmw, heqd);
+#pragma unroll
+    for (int k = 0; k < 9; k++) h[k] = h[k] * (1.0f - ph.wc) + heqd[k] * ph.wc;
+    const float Rho = ph.rhoh + ct * ph.drho;
+    const float Fb = -ct * ph.drho * ph.gy;
+    forcing(gmw, ph.drho3, mu, e, Ux, Uy, gx, gy, Fb, hlp);
+    const float rho3 = Rho * (1.0f / 3.0f);
+#pragma unroll
+    for (int k = 0; k < 9; k++) geq[k] = P * waf(k) + gmw[k] * rho3 - 0.5f * hlp[k];
+    ed_m(sC, i, j, inv_dx, e);
+    gradient_ed(e, gx, gy);
+    forcing(gmw, ph.drho3, mu, e, Ux, Uy, gx, gy, Fb, hlp);
+    float tau;
+    if (ct < -1.0f)
+        tau = ph.taul + 0.5f;
+    else if (ct > 0.0f)
+        tau = ph.tauh + 0.5f;
+    else
+        tau = 1.0f / (ct * (ph.inv_tauh - ph.inv_taul) + ph.inv_tauh) + 0.5f;
+    mrt(1.0f / tau, geq, g);
+#pragma unroll
+    for (int k = 0; k < 9; k++) g[k] = g[k] + hlp[k];
+}
+
+// Macroscopic_Values_g_AMR (LBM_Subroutines.f90:205-240), deviation form
+__device__ __forceinline__ void macro_g_cell(const float *sC, int i, int j, int dx, const float g[9], const PhysF &ph,
+                                             float &P, float &Ux, float &Uy)
+{
+    const float inv_dx = 1.0f / (float)dx;
+    const float gx = ((sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)]) * (1.0f / 3.0f) +
+                      (sC[cidx(i + 1, j + 1)] + sC[cidx(i + 1, j - 1)] - sC[cidx(i - 1, j + 1)] - sC[cidx(i - 1, j - 1)]) *
+                          (1.0f / 12.0f)) *
+                     inv_dx;
+    const float gy = ((sC[cidx(i, j + 1)] - sC[cidx(i, j - 1)]) * (1.0f / 3.0f) +
+                      (sC[cidx(i + 1, j + 1)] + sC[cidx(i - 1, j + 1)] - sC[cidx(i + 1, j - 1)] - sC[cidx(i - 1, j - 1)]) *
+                          (1.0f / 12.0f)) *
+                     inv_dx;
+    const float mu = chem_pot(sC, i, j, inv_dx * inv_dx, ph);
+    const float ct = sC[cidx(i, j)];
+    const float Rho = ph.rhoh + ct * ph.drho;
+    const float Fb = -ct * ph.drho * ph.gy;
+    const float Fx = mu * gx;
+    const float Fy = mu * gy + Fb;
+    float sx = 0.0f, sy = 0.0f, s = 0.0f;
+#pragma unroll
+    for (int k = 1; k < 9; k++) sx += g[k] * (float)c_ex[k];
+#pragma unroll
+    for (int k = 1; k < 9; k++) sy += g[k] * (float)c_ey[k];
+#pragma unroll
+    for (int k = 0; k < 9; k++) s += g[k];
+    const float ir = 1.0f / Rho;
+    Ux = (3.0f * sx + 0.5f * Fx) * ir;
+    Uy = (3.0f * sy + 0.5f * Fy) * ir;
+    P = s + ph.drho6 * (Ux * gx + Uy * gy);
+}
+
+// the physics arguments of either precision, so one kernel template serves both
+template <class T> struct PhysOf;
+template <> struct PhysOf<double> {
+    static __device__ __forceinline__ const Phys &get(const Phys &p, const PhysF &) { return p; }
+};
+template <> struct PhysOf<float> {
+    static __device__ __forceinline__ const PhysF &get(const Phys &, const PhysF &p) { return p; }
+};
+
+// storage shift of the FP32 path: C is stored as C - 1, h_q as h_q - w_q
+template <class T> __device__ __forceinline__ double c_shift() { return std::is_same<T, float>::value ? 1.0 : 0.0; }
+
+// ---------------------------------------------------------------------------
 // Ghost fills (Fill_Ghost_Cells_{C,C2,hg}.f90, Interpolations.f90:2-138,
 // BC_Cell.f90) by ghost program: every value a block's fill needs is built
 // from interior data of the previous kernel, the coarse neighbours' ghost
@@ -219,11 +407,12 @@ __device__ __forceinline__ int ring1_cell(int t)
 }
 
 // axis weights of Biquadratic_from_Fine / _from_Coarse (Interpolations.f90:13-31, :85-101)
-__device__ __forceinline__ void wpat(int p, double w[3])
+template <class T>
+__device__ __forceinline__ void wpat(int p, T w[3])
 {
-    w[0] = p == 0 ? 3.0 : (p == 1 ? -1.0 : (p == 2 ? 5.0 : -3.0));
-    w[1] = p < 2 ? 6.0 : 30.0;
-    w[2] = p == 0 ? -1.0 : (p == 1 ? 3.0 : (p == 2 ? -3.0 : 5.0));
+    w[0] = p == 0 ? T(3) : (p == 1 ? T(-1) : (p == 2 ? T(5) : T(-3)));
+    w[1] = p < 2 ? T(6) : T(30);
+    w[2] = p == 0 ? T(-1) : (p == 1 ? T(3) : (p == 2 ? T(-3) : T(5)));
 }
 
 // node load as three 16-byte loads
@@ -240,72 +429,74 @@ __device__ __forceinline__ PNode load_node(const PNode *p)
 
 // the recipe's operands that live in global memory, fetched before any stage
 // barrier (they are interior values left by the previous kernel)
-__device__ __forceinline__ void prefetch_node(const PNode &nd, const double *__restrict__ G, int gs, double gv[9])
+template <class T>
+__device__ __forceinline__ void prefetch_node(const PNode &nd, const T *__restrict__ G, int gs, T gv[9])
 {
     const int type = nd.info & 15;
     const int nref = type == NT_COPY ? 1 : (type >= NT_BB_BOT ? 0 : 9);
 #pragma unroll
-    for (int r = 0; r < 9; r++) gv[r] = (r < nref && nd.ref[r] >= 0) ? G[nd.ref[r] + gs] : 0.0;
+    for (int r = 0; r < 9; r++) gv[r] = (r < nref && nd.ref[r] >= 0) ? G[nd.ref[r] + gs] : T(0);
 }
 
 // one recipe from prefetched global operands gv and LDS L (+ shift ls)
-__device__ __forceinline__ double eval_pref(const PNode &nd, const double gv[9], const double *L, int ls)
+template <class T>
+__device__ __forceinline__ T eval_pref(const PNode &nd, const T gv[9], const T *L, int ls)
 {
     const int type = nd.info & 15;
 #define PV(r) (nd.ref[r] >= 0 ? gv[r] : L[-1 - nd.ref[r] + ls])
     if (type == NT_COPY) return PV(0);
-    double wx[3], wy[3];
+    T wx[3], wy[3];
     wpat((nd.info >> 4) & 3, wx);
     wpat((nd.info >> 6) & 3, wy);
-    double t[3];
+    T t[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) t[a] = wy[0] * PV(3 * a) + wy[1] * PV(3 * a + 1) + wy[2] * PV(3 * a + 2);
-    double o = wx[0] * t[0] + wx[1] * t[1] + wx[2] * t[2];
+    T o = wx[0] * t[0] + wx[1] * t[1] + wx[2] * t[2];
 #undef PV
-    return type == NT_FINE ? o / 64.0 : o / 1024.0;
+    return type == NT_FINE ? o / T(64) : o / T(1024);
 }
 
 // one recipe: copy, or the tensor product (pass along y, then x) divided by
 // 64 (from finer) or 1024 (from coarser)
-__device__ __forceinline__ double eval_node(const PNode &nd, const double *__restrict__ G, int gs,
-                                            const double *L, int ls)
+template <class T>
+__device__ __forceinline__ T eval_node(const PNode &nd, const T *__restrict__ G, int gs, const T *L, int ls)
 {
     const int type = nd.info & 15;
 #define PV(r) ((r) >= 0 ? G[(r) + gs] : L[-1 - (r) + ls])
     if (type == NT_COPY) return PV(nd.ref[0]);
-    double wx[3], wy[3];
+    T wx[3], wy[3];
     wpat((nd.info >> 4) & 3, wx);
     wpat((nd.info >> 6) & 3, wy);
-    double t[3];
+    T t[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) t[a] = wy[0] * PV(nd.ref[3 * a]) + wy[1] * PV(nd.ref[3 * a + 1]) + wy[2] * PV(nd.ref[3 * a + 2]);
-    double o = wx[0] * t[0] + wx[1] * t[1] + wx[2] * t[2];
+    T o = wx[0] * t[0] + wx[1] * t[1] + wx[2] * t[2];
 #undef PV
-    return type == NT_FINE ? o / 64.0 : o / 1024.0;
+    return type == NT_FINE ? o / T(64) : o / T(1024);
 }
 
 // a first-stage recipe: every operand is an interior value in memory
-__device__ __forceinline__ double eval_global(const PNode &nd, const double *__restrict__ G, int gs)
+template <class T>
+__device__ __forceinline__ T eval_global(const PNode &nd, const T *__restrict__ G, int gs)
 {
     const int type = nd.info & 15;
     if (type == NT_COPY) return G[nd.ref[0] + gs];
-    double wx[3], wy[3];
+    T wx[3], wy[3];
     wpat((nd.info >> 4) & 3, wx);
     wpat((nd.info >> 6) & 3, wy);
-    double t[3];
+    T t[3];
 #pragma unroll
     for (int a = 0; a < 3; a++)
         t[a] = wy[0] * G[nd.ref[3 * a] + gs] + wy[1] * G[nd.ref[3 * a + 1] + gs] + wy[2] * G[nd.ref[3 * a + 2] + gs];
-    double o = wx[0] * t[0] + wx[1] * t[1] + wx[2] * t[2];
-    return type == NT_FINE ? o / 64.0 : o / 1024.0;
+    T o = wx[0] * t[0] + wx[1] * t[1] + wx[2] * t[2];
+    return type == NT_FINE ? o / T(64) : o / T(1024);
 }
 
 // population q of set `set` for one recipe; sF holds that set's 9 arrays.
 // Bounce-back rows take the opposite population of the first row
 // (Bounce_Back_3/4, BC_Cell.f90:2-42).
-template <bool GLOBAL_ONLY>
-__device__ __forceinline__ double eval_hg(const PNode &nd, int set, int q, const double *__restrict__ Fc,
-                                          const double *sF)
+template <bool GLOBAL_ONLY, class T>
+__device__ __forceinline__ T eval_hg(const PNode &nd, int set, int q, const T *__restrict__ Fc, const T *sF)
 {
     const int type = nd.info & 15;
     const int c = set * 9 + q;
@@ -320,13 +511,13 @@ __device__ __forceinline__ double eval_hg(const PNode &nd, int set, int q, const
 // of the block (staged in sN) applied to the set's 9 populations. Items run
 // recipe-fastest, so the lanes of a wavefront read neighbouring cells of one
 // population array; U items per thread have their operands in flight together.
-template <bool GLOBAL_ONLY, int U>
-__device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, int set, const double *__restrict__ Fc,
-                                         double *sF, double *F, int tid)
+template <bool GLOBAL_ONLY, int U, class T>
+__device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, int set, const T *__restrict__ Fc, T *sF,
+                                         T *F, int tid)
 {
     const int ns = e - b;
     for (int base = 0; base < 9 * ns; base += 256 * U) {
-        double v[U];
+        T v[U];
 #pragma unroll
         for (int r = 0; r < U; r++) {
             const int t = base + tid + 256 * r;
@@ -356,10 +547,11 @@ __device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, int set,
 // next compute the same first ghost ring from the same C, so U, P come out
 // identical). The collision and U,P read only this block's C. With
 // prog.hdr == null the ghosts are read from memory (no fill).
-template <int OP>
-__global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *__restrict__ list, Prog prog, Phys ph,
-                                                double *err_slot)
+template <class T, int OP>
+__global__ __launch_bounds__(256) void k_prog_c(DevMeshT<T> m, int cur, const int *__restrict__ list, Prog prog,
+                                                Phys ph64, PhysF ph32, double *err_slot)
 {
+    const auto &ph = PhysOf<T>::get(ph64, ph32);
     const int k = blockIdx.x;
     const int tid = threadIdx.x;
     const int skip = *m.skip;
@@ -371,12 +563,12 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *_
     if (fill) hd = prog.hdr[k];
     if (skip) return;
     const BlockDev me = m.blk[slot];
-    double *C = m.C + (size_t)slot * CS;
-    __shared__ double sC[LC_C];
+    T *C = m.C + (size_t)slot * CS;
+    __shared__ T sC[LC_C];
     const int i = tid % NX + 1, j = tid / NX + 1;
-    double *F = m.pop[cur] + (size_t)slot * POPB;
-    double *PU = m.PU + (size_t)slot * PUB;
-    double h[9], g[9], P = 0.0, Ux = 0.0, Uy = 0.0;
+    T *F = m.pop[cur] + (size_t)slot * POPB;
+    T *PU = m.PU + (size_t)slot * PUB;
+    T h[9], g[9], P = T(0), Ux = T(0), Uy = T(0);
     if (OP == OP_COLLIDE || OP == OP_MACRO_G_COLLIDE) {
 #pragma unroll
         for (int q = 0; q < 9; q++) h[q] = F[fidx(0, q, i, j)];
@@ -395,15 +587,15 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *_
     const int total = hd.nst > 0 ? hd.end[hd.nst - 1] : 0;
     const int n1 = hd.nst > 0 ? hd.end[0] : 0;
     PNode nd0;
-    double gv[9];
+    T gv[9];
     if (tid < total) {
         nd0 = load_node(prog.nodes + hd.off + tid);
-        prefetch_node(nd0, m.C, 0, gv);
+        prefetch_node(nd0, (const T *)m.C, 0, gv);
     }
     if (fill) {
-        const double cin = C[cidx(i, j)];
+        const T cin = C[cidx(i, j)];
         if (tid < n1) {
-            const double v = eval_pref(nd0, gv, sC, 0);
+            const T v = eval_pref(nd0, gv, (const T *)sC, 0);
             sC[nd0.dest] = v;
             if (nd0.dest < CS) C[nd0.dest] = v;
         }
@@ -417,14 +609,14 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *_
         for (int s = 0; s < hd.nst; s++) {
             const int end = hd.end[s];
             if (s > 0 && tid >= begin && tid < end) {
-                const double v = eval_pref(nd0, gv, sC, 0);
+                const T v = eval_pref(nd0, gv, (const T *)sC, 0);
                 sC[nd0.dest] = v;
                 if (nd0.dest < CS) C[nd0.dest] = v;
             }
             for (int t = tid + 256; t < total; t += 256) {
                 if (t < begin || t >= end) continue;
                 const PNode nd = load_node(prog.nodes + hd.off + t);
-                const double v = eval_node(nd, m.C, 0, sC, 0);
+                const T v = eval_node(nd, (const T *)m.C, 0, (const T *)sC, 0);
                 sC[nd.dest] = v;
                 if (nd.dest < CS) C[nd.dest] = v;
             }
@@ -434,22 +626,22 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMesh m, int cur, const int *_
     }
     if (err_slot) {
         // Err_Interface (Refine_Derefine.f90:327-352) from the filled ring
-        const double a = sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)];
-        const double b = sC[cidx(i, j + 1)] - sC[cidx(i, j - 1)];
-        double dc = a * a + b * b;
+        const T a = sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)];
+        const T b = sC[cidx(i, j + 1)] - sC[cidx(i, j - 1)];
+        T dc = a * a + b * b;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
-            const double x = __shfl_xor(dc, o);
+            const T x = __shfl_xor(dc, o);
             if (x > dc) dc = x;
         }
-        __shared__ double wmax[4];
+        __shared__ T wmax[4];
         if ((tid & 63) == 0) wmax[tid >> 6] = dc;
         __syncthreads();
         if (tid == 0) {
-            double e = wmax[0];
+            T e = wmax[0];
             for (int w = 1; w < 4; w++)
                 if (wmax[w] > e) e = wmax[w];
-            err_slot[slot] = sqrt(e) / (2.0 * (double)me.dx);
+            err_slot[slot] = (double)(sqrt(e) / (T(2) * (T)me.dx));
         }
     }
     if (OP == OP_MACRO_G_COLLIDE || OP == OP_MACRO_G) {
@@ -473,18 +665,18 @@ __device__ __forceinline__ int lidx(int c, int x, int y) { return c * LC_F + y *
 
 // Perfect_Shift_AMR / Streaming_LaxWendroff_AMR (LBM_Subroutines.f90:41-90)
 // of one cell from the LDS copy of the block into G, and C = sum(h) (:193-201)
-__device__ __forceinline__ double stream_cell(const double *sF, double *__restrict__ G, int set, int i, int j,
-                                              int dx)
+template <class T>
+__device__ __forceinline__ T stream_cell(const T *sF, T *__restrict__ G, int set, int i, int j, int dx)
 {
-    const double cfl = 1.0 / (double)dx;
-    const double a = cfl / 2.0 * (1.0 + cfl), b = 1.0 - cfl * cfl, c = cfl / 2.0 * (1.0 - cfl);
-    double s = 0.0;
+    const T cfl = T(1) / (T)dx;
+    const T a = cfl / T(2) * (T(1) + cfl), b = T(1) - cfl * cfl, c = cfl / T(2) * (T(1) - cfl);
+    T s = T(0);
 #pragma unroll
     for (int q = 0; q < 9; q++) {
-        double v;
+        T v;
         if (q == 0)
             v = sF[lidx(q, i, j)];
-        else if (cfl > 0.9)
+        else if (cfl > T(0.9))
             v = sF[lidx(q, i - c_ex[q], j - c_ey[q])];
         else
             v = a * sF[lidx(q, i - c_ex[q], j - c_ey[q])] + b * sF[lidx(q, i, j)] -
@@ -502,7 +694,8 @@ __device__ __forceinline__ double stream_cell(const double *sF, double *__restri
 // ghost cell of the block or a coarse neighbour's ghost cell it needs.
 // flags: 1 fill (else ghosts from memory), 2 stream, 4 C = sum h,
 // 8 carry the ghost ring into the new buffer (as the in-place reference stream)
-__global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *__restrict__ list, Prog prog,
+template <class T>
+__global__ __launch_bounds__(256) void k_prog_hg(DevMeshT<T> m, int cur, const int *__restrict__ list, Prog prog,
                                                  int flags)
 {
     const int k = blockIdx.x, set = blockIdx.y;
@@ -515,9 +708,9 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *
     if (flags & 1) hd = prog.hdr[k];
     if (skip) return;
     const BlockDev me = m.blk[slot];
-    const double *__restrict__ Fc = m.pop[cur];
-    double *F = m.pop[cur] + (size_t)slot * POPB;
-    __shared__ double sF[9 * LC_F];
+    const T *__restrict__ Fc = m.pop[cur];
+    T *F = m.pop[cur] + (size_t)slot * POPB;
+    __shared__ T sF[9 * LC_F];
     __shared__ PNode sN[NODES_HG];
     const int i = tid % NX + 1, j = tid / NX + 1;
     const int nn = hd.nst > 0 ? hd.end[hd.nst - 1] : 0;
@@ -548,8 +741,8 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *
         __syncthreads();
     }
     if (!(flags & 2)) return;
-    double *G = m.pop[cur ^ 1] + (size_t)slot * POPB;
-    const double csum = stream_cell(sF, G, set, i, j, me.dx);
+    T *G = m.pop[cur ^ 1] + (size_t)slot * POPB;
+    const T csum = stream_cell((const T *)sF, G, set, i, j, me.dx);
     if ((flags & 4) && set == 0) m.C[(size_t)slot * CS + cidx(i, j)] = csum;
     if (flags & 8) {
         for (int t = tid; t < 68 * 9; t += 256) {
@@ -560,38 +753,40 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMesh m, int cur, const int *
 }
 
 // Macroscopic_Values_h_AMR (LBM_Subroutines.f90:193-201)
-__global__ __launch_bounds__(256) void k_macro_h(DevMesh m, int cur, const int *__restrict__ list)
+template <class T>
+__global__ __launch_bounds__(256) void k_macro_h(DevMeshT<T> m, int cur, const int *__restrict__ list)
 {
     const int slot = list[blockIdx.x];
     const int i = threadIdx.x % NX + 1, j = threadIdx.x / NX + 1;
-    const double *F = m.pop[cur] + (size_t)slot * POPB;
-    double s = 0.0;
+    const T *F = m.pop[cur] + (size_t)slot * POPB;
+    T s = T(0);
 #pragma unroll
     for (int q = 0; q < 9; q++) s += F[fidx(0, q, i, j)];
     m.C[(size_t)slot * CS + cidx(i, j)] = s;
 }
 
 // Err_Interface (Refine_Derefine.f90:327-352): sqrt(max |C(i+1)-C(i-1)|^2 ...)/(2dx)
-__global__ __launch_bounds__(256) void k_err(DevMesh m, const int *__restrict__ list, double *err, double *err_slot)
+template <class T>
+__global__ __launch_bounds__(256) void k_err(DevMeshT<T> m, const int *__restrict__ list, double *err, double *err_slot)
 {
     const int slot = list[blockIdx.x];
     const int i = threadIdx.x % NX + 1, j = threadIdx.x / NX + 1;
-    const double *C = m.C + (size_t)slot * CS;
-    double a = C[cidx(i + 1, j)] - C[cidx(i - 1, j)];
-    double b = C[cidx(i, j + 1)] - C[cidx(i, j - 1)];
-    double dc = a * a + b * b;
-    __shared__ double red[256];
+    const T *C = m.C + (size_t)slot * CS;
+    T a = C[cidx(i + 1, j)] - C[cidx(i - 1, j)];
+    T b = C[cidx(i, j + 1)] - C[cidx(i, j - 1)];
+    T dc = a * a + b * b;
+    __shared__ T red[256];
     red[threadIdx.x] = dc;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
-            double o = red[threadIdx.x + s];
+            T o = red[threadIdx.x + s];
             if (o > red[threadIdx.x]) red[threadIdx.x] = o;
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const double e = sqrt(red[0]) / (2.0 * (double)m.blk[slot].dx);
+        const double e = (double)(sqrt(red[0]) / (T(2) * (T)m.blk[slot].dx));
         err[blockIdx.x] = e;
         if (err_slot) err_slot[slot] = e;
     }
@@ -610,7 +805,7 @@ __global__ __launch_bounds__(256) void k_err(DevMesh m, const int *__restrict__ 
 // One workgroup. More than RC_CAND candidates: answers "changed" and leaves
 // the decision to the host.
 constexpr int RC_CAND = 512;
-__global__ __launch_bounds__(1024) void k_regrid_check(DevMesh m, int nblks, const double *__restrict__ err_slot,
+__global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int nblks, const double *__restrict__ err_slot,
                                                       int min_levels, int max_levels, int levelmin, int levelmax,
                                                       double refin_tol, double deref_tol, int *changed)
 {
@@ -691,32 +886,33 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMesh m, int nblks, con
 // Restrict_Bilinear (Restrict.f90:14-65) with Bilinear_from_Fine(_f)
 // (Interpolations.f90:142-194): parent interior = 2x2 mean of the children,
 // parent ghosts zeroed. pairs[2k] = parent slot, pairs[2k+1] = first child slot.
-__global__ __launch_bounds__(256) void k_restrict(DevMesh m, int cur, const int *__restrict__ pairs)
+template <class T>
+__global__ __launch_bounds__(256) void k_restrict(DevMeshT<T> m, int cur, const int *__restrict__ pairs)
 {
     const int ps = pairs[2 * blockIdx.x], c0 = pairs[2 * blockIdx.x + 1];
     const int tid = threadIdx.x;
-    double *F = m.pop[cur] + (size_t)ps * POPB;
-    double *C = m.C + (size_t)ps * CS;
-    double *PU = m.PU + (size_t)ps * PUB;
-    for (int t = tid; t < POPB; t += 256) F[t] = 0.0;
-    for (int t = tid; t < CS; t += 256) C[t] = 0.0;
-    for (int t = tid; t < PUB; t += 256) PU[t] = 0.0;
+    T *F = m.pop[cur] + (size_t)ps * POPB;
+    T *C = m.C + (size_t)ps * CS;
+    T *PU = m.PU + (size_t)ps * PUB;
+    for (int t = tid; t < POPB; t += 256) F[t] = T(0);
+    for (int t = tid; t < CS; t += 256) C[t] = T(0);
+    for (int t = tid; t < PUB; t += 256) PU[t] = T(0);
     __syncthreads();
     const int X = tid % NX + 1, Y = tid / NX + 1;
     const int cs = c0 + (X > NX / 2) + 2 * (Y > NY / 2);
     const int lx = 2 * X - 1 - NX * (X > NX / 2), ly = 2 * Y - 1 - NY * (Y > NY / 2);   // (2X-1, 2Y-1) in the child
-    auto mean4 = [](double a, double b, double c, double d) {   // in(i,2Y-1), in(i,2Y), in(i+1,2Y-1), in(i+1,2Y)
-        return 0.5 * (0.5 * (a + b) + 0.5 * (c + d));
+    auto mean4 = [](T a, T b, T c, T d) {   // in(i,2Y-1), in(i,2Y), in(i+1,2Y-1), in(i+1,2Y)
+        return T(0.5) * (T(0.5) * (a + b) + T(0.5) * (c + d));
     };
-    const double *Fch = m.pop[cur] + (size_t)cs * POPB;
+    const T *Fch = m.pop[cur] + (size_t)cs * POPB;
     for (int comp = 0; comp < 18; comp++) {
         int set = comp / 9, q = comp % 9;
         F[fidx(set, q, X, Y)] = mean4(Fch[fidx(set, q, lx, ly)], Fch[fidx(set, q, lx, ly + 1)],
                                       Fch[fidx(set, q, lx + 1, ly)], Fch[fidx(set, q, lx + 1, ly + 1)]);
     }
-    const double *Cch = m.C + (size_t)cs * CS;
+    const T *Cch = m.C + (size_t)cs * CS;
     C[cidx(X, Y)] = mean4(Cch[cidx(lx, ly)], Cch[cidx(lx, ly + 1)], Cch[cidx(lx + 1, ly)], Cch[cidx(lx + 1, ly + 1)]);
-    const double *Pch = m.PU + (size_t)cs * PUB;
+    const T *Pch = m.PU + (size_t)cs * PUB;
     for (int k = 0; k < 3; k++)
         PU[k * PS + pidx(X, Y)] = mean4(Pch[k * PS + pidx(lx, ly)], Pch[k * PS + pidx(lx, ly + 1)],
                                         Pch[k * PS + pidx(lx + 1, ly)], Pch[k * PS + pidx(lx + 1, ly + 1)]);
@@ -727,20 +923,23 @@ __global__ __launch_bounds__(256) void k_restrict(DevMesh m, int cur, const int 
 // C with one ghost ring (:74-101), then U, P by Macroscopic_Values_g_AMR
 // (Fill_Children_Data, :105-130). New children start zeroed
 // (AMR_Rising_2D_OMP.f90:572-577). pairs as in k_restrict.
-__global__ __launch_bounds__(256) void k_prolong(DevMesh m, int cur, const int *__restrict__ pairs, Phys ph)
+template <class T>
+__global__ __launch_bounds__(256) void k_prolong(DevMeshT<T> m, int cur, const int *__restrict__ pairs, Phys ph64,
+                                                 PhysF ph32)
 {
+    const auto &ph = PhysOf<T>::get(ph64, ph32);
     const int ps = pairs[2 * blockIdx.x], c0 = pairs[2 * blockIdx.x + 1];
     const int tid = threadIdx.x;
-    const double *Fp = m.pop[cur] + (size_t)ps * POPB;
-    const double *Cp = m.C + (size_t)ps * CS;
-    __shared__ double sC[CS];
+    const T *Fp = m.pop[cur] + (size_t)ps * POPB;
+    const T *Cp = m.C + (size_t)ps * CS;
+    __shared__ T sC[CS];
     for (int k = 0; k < 4; k++) {
-        double *F = m.pop[cur] + (size_t)(c0 + k) * POPB;
-        for (int t = tid; t < POPB; t += 256) F[t] = 0.0;
-        double *C = m.C + (size_t)(c0 + k) * CS;
-        for (int t = tid; t < CS; t += 256) C[t] = 0.0;
-        double *PU = m.PU + (size_t)(c0 + k) * PUB;
-        for (int t = tid; t < PUB; t += 256) PU[t] = 0.0;
+        T *F = m.pop[cur] + (size_t)(c0 + k) * POPB;
+        for (int t = tid; t < POPB; t += 256) F[t] = T(0);
+        T *C = m.C + (size_t)(c0 + k) * CS;
+        for (int t = tid; t < CS; t += 256) C[t] = T(0);
+        T *PU = m.PU + (size_t)(c0 + k) * PUB;
+        for (int t = tid; t < PUB; t += 256) PU[t] = T(0);
     }
     __syncthreads();
     const int I = tid % NX + 1, J = tid / NX + 1;
@@ -748,14 +947,14 @@ __global__ __launch_bounds__(256) void k_prolong(DevMesh m, int cur, const int *
         const int i = I + NX * (k & 1), j = J + NY * (k >> 1);   // fine index 1..2n
         int Xi = i / 2; Xi = Xi > NX - 1 ? NX - 1 : (Xi < 1 ? 1 : Xi);
         int Yj = j / 2; Yj = Yj > NY - 1 ? NY - 1 : (Yj < 1 ? 1 : Yj);
-        const double ai = (double)(2 * i - 4 * Xi + 1), bi = 4.0 - ai;
-        const double aj = (double)(2 * j - 4 * Yj + 1), bj = 4.0 - aj;
-        double *F = m.pop[cur] + (size_t)(c0 + k) * POPB;
+        const T ai = (T)(2 * i - 4 * Xi + 1), bi = T(4) - ai;
+        const T aj = (T)(2 * j - 4 * Yj + 1), bj = T(4) - aj;
+        T *F = m.pop[cur] + (size_t)(c0 + k) * POPB;
         for (int comp = 0; comp < 18; comp++) {
             int set = comp / 9, q = comp % 9;
-            double t0 = bi * Fp[fidx(set, q, Xi, Yj)] + ai * Fp[fidx(set, q, Xi + 1, Yj)];
-            double t1 = bi * Fp[fidx(set, q, Xi, Yj + 1)] + ai * Fp[fidx(set, q, Xi + 1, Yj + 1)];
-            F[fidx(set, q, I, J)] = (bj * t0 + aj * t1) / 16.0;
+            T t0 = bi * Fp[fidx(set, q, Xi, Yj)] + ai * Fp[fidx(set, q, Xi + 1, Yj)];
+            T t1 = bi * Fp[fidx(set, q, Xi, Yj + 1)] + ai * Fp[fidx(set, q, Xi + 1, Yj + 1)];
+            F[fidx(set, q, I, J)] = (bj * t0 + aj * t1) / T(16);
         }
     }
     // C on (0:nx+1)^2 of each child from the parent's (0:nx+1)^2
@@ -763,23 +962,23 @@ __global__ __launch_bounds__(256) void k_prolong(DevMesh m, int cur, const int *
         int k = t / FS, r = t - k * FS, Ic = r % FW, Jc = r / FW;
         int i = Ic + NX * (k & 1), j = Jc + NY * (k >> 1);      // 0..2n+1
         int Xi = i / 2, Yj = j / 2;
-        double ai = (double)(2 * i - 4 * Xi + 1), bi = 4.0 - ai;
-        double aj = (double)(2 * j - 4 * Yj + 1), bj = 4.0 - aj;
-        double t0 = bi * Cp[cidx(Xi, Yj)] + ai * Cp[cidx(Xi + 1, Yj)];
-        double t1 = bi * Cp[cidx(Xi, Yj + 1)] + ai * Cp[cidx(Xi + 1, Yj + 1)];
-        m.C[(size_t)(c0 + k) * CS + cidx(Ic, Jc)] = (bj * t0 + aj * t1) / 16.0;
+        T ai = (T)(2 * i - 4 * Xi + 1), bi = T(4) - ai;
+        T aj = (T)(2 * j - 4 * Yj + 1), bj = T(4) - aj;
+        T t0 = bi * Cp[cidx(Xi, Yj)] + ai * Cp[cidx(Xi + 1, Yj)];
+        T t1 = bi * Cp[cidx(Xi, Yj + 1)] + ai * Cp[cidx(Xi + 1, Yj + 1)];
+        m.C[(size_t)(c0 + k) * CS + cidx(Ic, Jc)] = (bj * t0 + aj * t1) / T(16);
     }
     __syncthreads();
     for (int k = 0; k < 4; k++) {
-        const double *Cc = m.C + (size_t)(c0 + k) * CS;
+        const T *Cc = m.C + (size_t)(c0 + k) * CS;
         for (int t = tid; t < CS; t += 256) sC[t] = Cc[t];
         __syncthreads();
-        const double *F = m.pop[cur] + (size_t)(c0 + k) * POPB;
-        double g[9], P, Ux, Uy;
+        const T *F = m.pop[cur] + (size_t)(c0 + k) * POPB;
+        T g[9], P, Ux, Uy;
 #pragma unroll
         for (int q = 0; q < 9; q++) g[q] = F[fidx(1, q, I, J)];
-        macro_g_cell(sC, I, J, m.blk[ps].dx / 2, g, ph, P, Ux, Uy);
-        double *PU = m.PU + (size_t)(c0 + k) * PUB;
+        macro_g_cell((const T *)sC, I, J, m.blk[ps].dx / 2, g, ph, P, Ux, Uy);
+        T *PU = m.PU + (size_t)(c0 + k) * PUB;
         PU[pidx(I, J)] = P;
         PU[PS + pidx(I, J)] = Ux;
         PU[2 * PS + pidx(I, J)] = Uy;
@@ -789,23 +988,25 @@ __global__ __launch_bounds__(256) void k_prolong(DevMesh m, int cur, const int *
 
 // Initialize_AMR_Rising_Bubble (AMR_Rising_2D_OMP.f90:186-225): with C set
 // (both rings), U = 0, P = -sigma/R C, h = heq, g = geq with Fb = 0.
-__global__ __launch_bounds__(256) void k_init_fields(DevMesh m, int cur, const int *__restrict__ list, Phys ph,
+// Computed in FP64 for either storage.
+template <class T>
+__global__ __launch_bounds__(256) void k_init_fields(DevMeshT<T> m, int cur, const int *__restrict__ list, Phys ph,
                                                      double R)
 {
     const int slot = list[blockIdx.x];
     const int tid = threadIdx.x;
     __shared__ double sC[CS];
-    const double *C = m.C + (size_t)slot * CS;
-    for (int t = tid; t < CS; t += 256) sC[t] = C[t];
+    const T *C = m.C + (size_t)slot * CS;
+    for (int t = tid; t < CS; t += 256) sC[t] = (double)C[t] + c_shift<T>();
     __syncthreads();
     const int i = tid % NX + 1, j = tid / NX + 1, dx = m.blk[slot].dx;
     const double dRho3 = (ph.rhoh - ph.rhol) / 3.0;
     double c = sC[cidx(i, j)];
     double P = -(ph.sigma / R * c), Ux = 0.0, Uy = 0.0;
-    double *PU = m.PU + (size_t)slot * PUB;
-    PU[pidx(i, j)] = P;
-    PU[PS + pidx(i, j)] = Ux;
-    PU[2 * PS + pidx(i, j)] = Uy;
+    T *PU = m.PU + (size_t)slot * PUB;
+    PU[pidx(i, j)] = (T)P;
+    PU[PS + pidx(i, j)] = (T)Ux;
+    PU[2 * PS + pidx(i, j)] = (T)Uy;
     double mu = chem_pot(sC, i, j, dx, ph);
     double e[9], hlp[9], Gamma[9], heq[9], gx, gy;
     ed_c(sC, i, j, dx, e);
@@ -815,24 +1016,29 @@ __global__ __launch_bounds__(256) void k_init_fields(DevMesh m, int cur, const i
     double Rho = ph.rhol + c * (ph.rhoh - ph.rhol);
     equilibrium(c, Ux, Uy, ni, nj, ph, Gamma, heq);
     forcing(Gamma, dRho3, mu, e, Ux, Uy, gx, gy, 0.0, hlp);
-    double *F = m.pop[cur] + (size_t)slot * POPB;
+    T *F = m.pop[cur] + (size_t)slot * POPB;
 #pragma unroll
     for (int q = 0; q < 9; q++) {
-        F[fidx(0, q, i, j)] = heq[q];
-        F[fidx(1, q, i, j)] = P * wa(q) + (Gamma[q] - wa(q)) * Rho / 3.0 - 0.5 * hlp[q];
+        if (std::is_same<T, float>::value)
+            F[fidx(0, q, i, j)] = (T)(heq[q] - wa(q));
+        else
+            F[fidx(0, q, i, j)] = (T)heq[q];
+        F[fidx(1, q, i, j)] = (T)(P * wa(q) + (Gamma[q] - wa(q)) * Rho / 3.0 - 0.5 * hlp[q]);
     }
 }
 
 // FP64 diagnostics, stage 1: per-leaf partials in a fixed reduction order
 // (min rho, max rho, max|Uy|, mass, area, x-moment, y-moment, Uy-moment, non-finite)
-__global__ __launch_bounds__(256) void k_diag_partial(DevMesh m, const int *__restrict__ list, Phys ph, double *part)
+template <class T>
+__global__ __launch_bounds__(256) void k_diag_partial(DevMeshT<T> m, const int *__restrict__ list, Phys ph,
+                                                      double *part)
 {
     const int slot = list[blockIdx.x];
     const int tid = threadIdx.x;
     const int i = tid % NX + 1, j = tid / NX + 1;
     const BlockDev b = m.blk[slot];
-    double c = m.C[(size_t)slot * CS + cidx(i, j)];
-    double uy = m.PU[(size_t)slot * PUB + 2 * PS + pidx(i, j)];
+    double c = (double)m.C[(size_t)slot * CS + cidx(i, j)] + c_shift<T>();
+    double uy = (double)m.PU[(size_t)slot * PUB + 2 * PS + pidx(i, j)];
     double rho = ph.rhol + c * (ph.rhoh - ph.rhol);
     double dx2 = (double)(b.dx * b.dx);
     double w = (1.0 - c) * dx2;
@@ -892,20 +1098,31 @@ __global__ __launch_bounds__(256) void k_diag_final(const double *part, int n, d
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+template <class T>
+static void prog_c(const DevMesh &dm, int cur, const int *list, int n, int op, const Prog &prog, const Phys &ph,
+                   hipStream_t s, double *err_slot)
+{
+    const DevMeshT<T> m = typed<T>(dm);
+    dim3 g(n), b(256);
+    if (op == OP_MACRO_G_COLLIDE)
+        hipLaunchKernelGGL((k_prog_c<T, OP_MACRO_G_COLLIDE>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
+    else if (op == OP_COLLIDE)
+        hipLaunchKernelGGL((k_prog_c<T, OP_COLLIDE>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
+    else if (op == OP_MACRO_G)
+        hipLaunchKernelGGL((k_prog_c<T, OP_MACRO_G>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
+    else
+        hipLaunchKernelGGL((k_prog_c<T, OP_NONE>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
+}
+
 void launch_prog_c(const DevMesh &m, int cur, const int *list, int n, int kind, int op, const Prog &prog,
                    const Phys &ph, hipStream_t s, double *err_slot)
 {
     if (n <= 0) return;
     (void)kind;   // the program decides the ring; the op needs C with the rings it reads
-    dim3 g(n), b(256);
-    if (op == OP_MACRO_G_COLLIDE)
-        hipLaunchKernelGGL(k_prog_c<OP_MACRO_G_COLLIDE>, g, b, 0, s, m, cur, list, prog, ph, err_slot);
-    else if (op == OP_COLLIDE)
-        hipLaunchKernelGGL(k_prog_c<OP_COLLIDE>, g, b, 0, s, m, cur, list, prog, ph, err_slot);
-    else if (op == OP_MACRO_G)
-        hipLaunchKernelGGL(k_prog_c<OP_MACRO_G>, g, b, 0, s, m, cur, list, prog, ph, err_slot);
+    if (m.f32)
+        prog_c<float>(m, cur, list, n, op, prog, ph, s, err_slot);
     else
-        hipLaunchKernelGGL(k_prog_c<OP_NONE>, g, b, 0, s, m, cur, list, prog, ph, err_slot);
+        prog_c<double>(m, cur, list, n, op, prog, ph, s, err_slot);
 }
 
 void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Prog &prog, bool stream_after,
@@ -913,17 +1130,28 @@ void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Pro
 {
     if (n <= 0) return;
     int flags = (prog.hdr ? 1 : 0) | (stream_after ? 2 : 0) | (macro_h ? 4 : 0) | (copy_ghosts ? 8 : 0);
-    hipLaunchKernelGGL(k_prog_hg, dim3(n, 2), dim3(256), 0, s, m, cur, list, prog, flags);
+    if (m.f32)
+        hipLaunchKernelGGL(k_prog_hg<float>, dim3(n, 2), dim3(256), 0, s, typed<float>(m), cur, list, prog, flags);
+    else
+        hipLaunchKernelGGL(k_prog_hg<double>, dim3(n, 2), dim3(256), 0, s, typed<double>(m), cur, list, prog, flags);
 }
 
 void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_macro_h, dim3(n), dim3(256), 0, s, m, cur, list);
+    if (n <= 0) return;
+    if (m.f32)
+        hipLaunchKernelGGL(k_macro_h<float>, dim3(n), dim3(256), 0, s, typed<float>(m), cur, list);
+    else
+        hipLaunchKernelGGL(k_macro_h<double>, dim3(n), dim3(256), 0, s, typed<double>(m), cur, list);
 }
 
 void launch_err(const DevMesh &m, const int *list, int n, double *err, double *err_slot, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_err, dim3(n), dim3(256), 0, s, m, list, err, err_slot);
+    if (n <= 0) return;
+    if (m.f32)
+        hipLaunchKernelGGL(k_err<float>, dim3(n), dim3(256), 0, s, typed<float>(m), list, err, err_slot);
+    else
+        hipLaunchKernelGGL(k_err<double>, dim3(n), dim3(256), 0, s, typed<double>(m), list, err, err_slot);
 }
 
 void launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, int min_levels, int max_levels,
@@ -931,29 +1159,44 @@ void launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, in
                          hipStream_t s)
 {
     const size_t lds = sizeof(unsigned) * (size_t)((nblks + 31) / 32 + 1);
-    hipLaunchKernelGGL(k_regrid_check, dim3(1), dim3(1024), lds, s, m, nblks, err_slot, min_levels, max_levels,
-                       levelmin, levelmax, refin_tol, deref_tol, changed);
+    hipLaunchKernelGGL(k_regrid_check, dim3(1), dim3(1024), lds, s, typed<double>(m), nblks, err_slot, min_levels,
+                       max_levels, levelmin, levelmax, refin_tol, deref_tol, changed);
 }
 
 void launch_restrict(const DevMesh &m, int cur, const int *pairs, int n, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_restrict, dim3(n), dim3(256), 0, s, m, cur, pairs);
+    if (n <= 0) return;
+    if (m.f32)
+        hipLaunchKernelGGL(k_restrict<float>, dim3(n), dim3(256), 0, s, typed<float>(m), cur, pairs);
+    else
+        hipLaunchKernelGGL(k_restrict<double>, dim3(n), dim3(256), 0, s, typed<double>(m), cur, pairs);
 }
 
 void launch_prolong(const DevMesh &m, int cur, const int *pairs, int n, const Phys &ph, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_prolong, dim3(n), dim3(256), 0, s, m, cur, pairs, ph);
+    if (n <= 0) return;
+    if (m.f32)
+        hipLaunchKernelGGL(k_prolong<float>, dim3(n), dim3(256), 0, s, typed<float>(m), cur, pairs, ph, m.phf);
+    else
+        hipLaunchKernelGGL(k_prolong<double>, dim3(n), dim3(256), 0, s, typed<double>(m), cur, pairs, ph, m.phf);
 }
 
 void launch_init_fields(const DevMesh &m, int cur, const int *list, int n, const Phys &ph, double R, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_init_fields, dim3(n), dim3(256), 0, s, m, cur, list, ph, R);
+    if (n <= 0) return;
+    if (m.f32)
+        hipLaunchKernelGGL(k_init_fields<float>, dim3(n), dim3(256), 0, s, typed<float>(m), cur, list, ph, R);
+    else
+        hipLaunchKernelGGL(k_init_fields<double>, dim3(n), dim3(256), 0, s, typed<double>(m), cur, list, ph, R);
 }
 
 void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, double *partial, double *out, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_diag_partial, dim3(n), dim3(256), 0, s, m, list, ph, partial);
+    if (m.f32)
+        hipLaunchKernelGGL(k_diag_partial<float>, dim3(n), dim3(256), 0, s, typed<float>(m), list, ph, partial);
+    else
+        hipLaunchKernelGGL(k_diag_partial<double>, dim3(n), dim3(256), 0, s, typed<double>(m), list, ph, partial);
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, partial, n, out);
 }
 

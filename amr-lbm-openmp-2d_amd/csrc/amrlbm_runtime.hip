@@ -819,6 +819,48 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol)
 // layout conversion between the reference's h(9,0:nx+1,0:ny+1) and the
 // device's [set][q][y][x]
 // ---------------------------------------------------------------------------
+// element size of the mesh's fields and the address of element `off` of one
+size_t esz(const amrlbm_mesh *m) { return m->dm.f32 ? sizeof(float) : sizeof(double); }
+void *fptr(const amrlbm_mesh *m, void *base, size_t off) { return static_cast<char *>(base) + off * esz(m); }
+
+// FP32 storage shifts: h_q - w_q, C - 1 (deviations from the liquid at rest)
+const double kW[9] = {16.0 / 36.0, 4.0 / 36.0, 4.0 / 36.0, 4.0 / 36.0, 4.0 / 36.0,
+                      1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+
+// n values at element offset off of a device field to host doubles, adding
+// shift(k) (k = element index) on an FP32 mesh
+template <class Shift>
+int download(amrlbm_mesh *m, void *base, size_t off, size_t n, double *out, Shift shift)
+{
+    if (!m->dm.f32) {
+        HIPCHK(hipMemcpyAsync(out, fptr(m, base, off), sizeof(double) * n, hipMemcpyDeviceToHost, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+        return AMRLBM_OK;
+    }
+    std::vector<float> f(n);
+    HIPCHK(hipMemcpyAsync(f.data(), fptr(m, base, off), sizeof(float) * n, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    for (size_t k = 0; k < n; k++) out[k] = (double)f[k] + shift(k);
+    return AMRLBM_OK;
+}
+
+template <class Shift>
+int upload(amrlbm_mesh *m, void *base, size_t off, size_t n, const double *in, Shift shift)
+{
+    if (!m->dm.f32) {
+        HIPCHK(hipMemcpyAsync(fptr(m, base, off), in, sizeof(double) * n, hipMemcpyHostToDevice, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+        return AMRLBM_OK;
+    }
+    std::vector<float> f(n);
+    for (size_t k = 0; k < n; k++) f[k] = (float)(in[k] - shift(k));
+    HIPCHK(hipMemcpyAsync(fptr(m, base, off), f.data(), sizeof(float) * n, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return AMRLBM_OK;
+}
+
+auto no_shift = [](size_t) { return 0.0; };
+
 void ref_to_dev_pop(const double *ref, double *dev)   // one set
 {
     for (int y = 0; y < NY + 2; y++)
@@ -835,16 +877,16 @@ void dev_to_ref_pop(const double *dev, double *ref)
 
 int alloc_device(amrlbm_mesh *m)
 {
-    const size_t S = (size_t)m->prm.max_blocks;
+    const size_t S = (size_t)m->prm.max_blocks, e = esz(m);
     HIPCHK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
-    HIPCHK(hipMalloc(&m->dm.pop[0], sizeof(double) * POPB * S));
-    HIPCHK(hipMalloc(&m->dm.pop[1], sizeof(double) * POPB * S));
-    HIPCHK(hipMalloc(&m->dm.C, sizeof(double) * CS * S));
-    HIPCHK(hipMalloc(&m->dm.PU, sizeof(double) * PUB * S));
-    HIPCHK(hipMemsetAsync(m->dm.pop[0], 0, sizeof(double) * POPB * S, m->stream));
-    HIPCHK(hipMemsetAsync(m->dm.pop[1], 0, sizeof(double) * POPB * S, m->stream));
-    HIPCHK(hipMemsetAsync(m->dm.C, 0, sizeof(double) * CS * S, m->stream));
-    HIPCHK(hipMemsetAsync(m->dm.PU, 0, sizeof(double) * PUB * S, m->stream));
+    HIPCHK(hipMalloc(&m->dm.pop[0], e * POPB * S));
+    HIPCHK(hipMalloc(&m->dm.pop[1], e * POPB * S));
+    HIPCHK(hipMalloc(&m->dm.C, e * CS * S));
+    HIPCHK(hipMalloc(&m->dm.PU, e * PUB * S));
+    HIPCHK(hipMemsetAsync(m->dm.pop[0], 0, e * POPB * S, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.pop[1], 0, e * POPB * S, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.C, 0, e * CS * S, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.PU, 0, e * PUB * S, m->stream));
     HIPCHK(hipMalloc(&m->d_blk, sizeof(BlockDev) * S));
     HIPCHK(hipMalloc(&m->d_nbd, sizeof(NbDesc) * 8 * S));
     HIPCHK(hipMalloc(&m->d_err_flags, sizeof(int)));
@@ -903,9 +945,11 @@ int init_bubble(amrlbm_mesh *m, double R, double X0, double Y0)
     const size_t nl = m->leaf.size();
     std::vector<double> cbuf(nl * CS);
     for (size_t k = 0; k < nl; k++) bubble_c(m, m->blk[m->leaf[k]], R, X0, Y0, &cbuf[k * CS]);
-    for (size_t k = 0; k < nl; k++)
-        HIPCHK(hipMemcpyAsync(m->dm.C + (size_t)(m->leaf[k] - 1) * CS, &cbuf[k * CS], sizeof(double) * CS,
-                              hipMemcpyHostToDevice, m->stream));
+    const double cs = m->dm.f32 ? 1.0 : 0.0;
+    for (size_t k = 0; k < nl; k++) {
+        int rc2 = upload(m, m->dm.C, (size_t)(m->leaf[k] - 1) * CS, CS, &cbuf[k * CS], [cs](size_t) { return cs; });
+        if (rc2) return rc2;
+    }
     launch_init_fields(m->dm, m->cur, m->d_lists + m->all_off, (int)nl, m->ph, R, m->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(m->stream));
@@ -1013,7 +1057,7 @@ extern "C" {
 
 const char *amrlbm_last_error(void) { return g_last_error.c_str(); }
 
-int amrlbm_create(const amrlbm_params *p, amrlbm_mesh **out)
+static int create_mesh(const amrlbm_params *p, bool f32, amrlbm_mesh **out)
 {
     if (!p || !out) return fail(AMRLBM_ERR_ARG, "null argument");
     *out = nullptr;
@@ -1035,6 +1079,23 @@ int amrlbm_create(const amrlbm_params *p, amrlbm_mesh **out)
     m->ph.rhol = p->rhol;
     m->ph.rhoh = p->rhoh;
     m->ph.gy = p->gy;
+    m->dm.f32 = f32 ? 1 : 0;
+    PhysF &f = m->dm.phf;
+    f.M0 = (float)p->M0;
+    f.wc = (float)m->ph.wc;
+    f.beta = (float)p->beta;
+    f.kappa = (float)p->kappa;
+    f.taul = (float)p->taul;
+    f.tauh = (float)p->tauh;
+    f.rhol = (float)p->rhol;
+    f.rhoh = (float)p->rhoh;
+    f.gy = (float)p->gy;
+    f.drho = (float)(p->rhoh - p->rhol);
+    f.drho3 = (float)((p->rhoh - p->rhol) / 3.0);
+    f.drho6 = (float)((p->rhoh - p->rhol) / 6.0);
+    f.inv_tauh = (float)(1.0 / p->tauh);
+    f.inv_taul = (float)(1.0 / p->taul);
+    f.k12W = (float)(12.0 * p->M0 / p->W);
     int rc = alloc_device(m);
     if (rc) {
         amrlbm_destroy(m);
@@ -1044,6 +1105,10 @@ int amrlbm_create(const amrlbm_params *p, amrlbm_mesh **out)
     *out = m;
     return AMRLBM_OK;
 }
+
+int amrlbm_create(const amrlbm_params *p, amrlbm_mesh **out) { return create_mesh(p, false, out); }
+
+int amrlbm_create_fp32(const amrlbm_params *p, amrlbm_mesh **out) { return create_mesh(p, true, out); }
 
 int amrlbm_destroy(amrlbm_mesh *m)
 {
@@ -1109,9 +1174,9 @@ int amrlbm_setup_replicas(amrlbm_mesh *m, int ninst, double R, double x0, double
         place(m, N);
     }
     // Initial_Mesh (AMR_Rising_2D_OMP.f90:493-513); device slots start zeroed
-    HIPCHK(hipMemsetAsync(m->dm.pop[0], 0, sizeof(double) * POPB * p.max_blocks, m->stream));
-    HIPCHK(hipMemsetAsync(m->dm.C, 0, sizeof(double) * CS * p.max_blocks, m->stream));
-    HIPCHK(hipMemsetAsync(m->dm.PU, 0, sizeof(double) * PUB * p.max_blocks, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.pop[0], 0, esz(m) * POPB * p.max_blocks, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.C, 0, esz(m) * CS * p.max_blocks, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.PU, 0, esz(m) * PUB * p.max_blocks, m->stream));
     for (int L = 1; L <= m->levelmin; L++) {
         int nend = m->nblks;
         for (int n = 1; n <= nend; n++)
@@ -1209,23 +1274,23 @@ int amrlbm_put_block(amrlbm_mesh *m, int id, const double *h, const double *g, c
 {
     if (!m || id < 1 || id > m->prm.max_blocks) return fail(AMRLBM_ERR_ARG, "bad block id");
     const size_t s = (size_t)(id - 1);
-    std::vector<double> buf(POPB);
-    double *F = m->dm.pop[m->cur] + s * POPB;
+    const bool f32 = m->dm.f32;
+    std::vector<double> buf(9 * GS);
+    int rc = AMRLBM_OK;
     if (h) {
         ref_to_dev_pop(h, buf.data());
-        HIPCHK(hipMemcpyAsync(F, buf.data(), sizeof(double) * 9 * GS, hipMemcpyHostToDevice, m->stream));
-        HIPCHK(hipStreamSynchronize(m->stream));
+        rc = upload(m, m->dm.pop[m->cur], s * POPB, 9 * GS, buf.data(),
+                    [f32](size_t k) { return f32 ? kW[k / GS] : 0.0; });
+        if (rc) return rc;
     }
     if (g) {
         ref_to_dev_pop(g, buf.data());
-        HIPCHK(hipMemcpyAsync(F + 9 * GS, buf.data(), sizeof(double) * 9 * GS, hipMemcpyHostToDevice, m->stream));
-        HIPCHK(hipStreamSynchronize(m->stream));
+        if ((rc = upload(m, m->dm.pop[m->cur], s * POPB + 9 * GS, 9 * GS, buf.data(), no_shift))) return rc;
     }
-    if (C) HIPCHK(hipMemcpyAsync(m->dm.C + s * CS, C, sizeof(double) * CS, hipMemcpyHostToDevice, m->stream));
-    if (P) HIPCHK(hipMemcpyAsync(m->dm.PU + s * PUB, P, sizeof(double) * PS, hipMemcpyHostToDevice, m->stream));
-    if (Ux) HIPCHK(hipMemcpyAsync(m->dm.PU + s * PUB + PS, Ux, sizeof(double) * PS, hipMemcpyHostToDevice, m->stream));
-    if (Uy) HIPCHK(hipMemcpyAsync(m->dm.PU + s * PUB + 2 * PS, Uy, sizeof(double) * PS, hipMemcpyHostToDevice, m->stream));
-    HIPCHK(hipStreamSynchronize(m->stream));
+    if (C && (rc = upload(m, m->dm.C, s * CS, CS, C, [f32](size_t) { return f32 ? 1.0 : 0.0; }))) return rc;
+    if (P && (rc = upload(m, m->dm.PU, s * PUB, PS, P, no_shift))) return rc;
+    if (Ux && (rc = upload(m, m->dm.PU, s * PUB + PS, PS, Ux, no_shift))) return rc;
+    if (Uy && (rc = upload(m, m->dm.PU, s * PUB + 2 * PS, PS, Uy, no_shift))) return rc;
     return AMRLBM_OK;
 }
 
@@ -1233,16 +1298,23 @@ int amrlbm_get_block(amrlbm_mesh *m, int id, double *h, double *g, double *C, do
 {
     if (!m || id < 1 || id > m->prm.max_blocks) return fail(AMRLBM_ERR_ARG, "bad block id");
     const size_t s = (size_t)(id - 1);
-    std::vector<double> buf(POPB);
-    HIPCHK(hipMemcpyAsync(buf.data(), m->dm.pop[m->cur] + s * POPB, sizeof(double) * POPB, hipMemcpyDeviceToHost,
-                          m->stream));
-    if (C) HIPCHK(hipMemcpyAsync(C, m->dm.C + s * CS, sizeof(double) * CS, hipMemcpyDeviceToHost, m->stream));
-    if (P) HIPCHK(hipMemcpyAsync(P, m->dm.PU + s * PUB, sizeof(double) * PS, hipMemcpyDeviceToHost, m->stream));
-    if (Ux) HIPCHK(hipMemcpyAsync(Ux, m->dm.PU + s * PUB + PS, sizeof(double) * PS, hipMemcpyDeviceToHost, m->stream));
-    if (Uy) HIPCHK(hipMemcpyAsync(Uy, m->dm.PU + s * PUB + 2 * PS, sizeof(double) * PS, hipMemcpyDeviceToHost, m->stream));
-    HIPCHK(hipStreamSynchronize(m->stream));
-    if (h) dev_to_ref_pop(buf.data(), h);
-    if (g) dev_to_ref_pop(buf.data() + 9 * GS, g);
+    const bool f32 = m->dm.f32;
+    std::vector<double> buf(9 * GS);
+    int rc = AMRLBM_OK;
+    if (h) {
+        rc = download(m, m->dm.pop[m->cur], s * POPB, 9 * GS, buf.data(),
+                      [f32](size_t k) { return f32 ? kW[k / GS] : 0.0; });
+        if (rc) return rc;
+        dev_to_ref_pop(buf.data(), h);
+    }
+    if (g) {
+        if ((rc = download(m, m->dm.pop[m->cur], s * POPB + 9 * GS, 9 * GS, buf.data(), no_shift))) return rc;
+        dev_to_ref_pop(buf.data(), g);
+    }
+    if (C && (rc = download(m, m->dm.C, s * CS, CS, C, [f32](size_t) { return f32 ? 1.0 : 0.0; }))) return rc;
+    if (P && (rc = download(m, m->dm.PU, s * PUB, PS, P, no_shift))) return rc;
+    if (Ux && (rc = download(m, m->dm.PU, s * PUB + PS, PS, Ux, no_shift))) return rc;
+    if (Uy && (rc = download(m, m->dm.PU, s * PUB + 2 * PS, PS, Uy, no_shift))) return rc;
     return AMRLBM_OK;
 }
 
@@ -1476,12 +1548,12 @@ int amrlbm_time_kernels(amrlbm_mesh *m, int reps, double out[3])
     const int n = nleaves(m);
     const size_t nb = (size_t)m->nblks;
     // the launches below overwrite populations, C and U,P: keep a copy
-    double *save = nullptr;
-    const size_t npop = POPB * nb, nc = CS * nb, npu = PUB * nb;
-    HIPCHK(hipMalloc(&save, sizeof(double) * (npop + nc + npu)));
-    HIPCHK(hipMemcpyAsync(save, m->dm.pop[m->cur], sizeof(double) * npop, hipMemcpyDeviceToDevice, m->stream));
-    HIPCHK(hipMemcpyAsync(save + npop, m->dm.C, sizeof(double) * nc, hipMemcpyDeviceToDevice, m->stream));
-    HIPCHK(hipMemcpyAsync(save + npop + nc, m->dm.PU, sizeof(double) * npu, hipMemcpyDeviceToDevice, m->stream));
+    char *save = nullptr;
+    const size_t e = esz(m), npop = e * POPB * nb, nc = e * CS * nb, npu = e * PUB * nb;
+    HIPCHK(hipMalloc(&save, npop + nc + npu));
+    HIPCHK(hipMemcpyAsync(save, m->dm.pop[m->cur], npop, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(save + npop, m->dm.C, nc, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(save + npop + nc, m->dm.PU, npu, hipMemcpyDeviceToDevice, m->stream));
     const int *lst = all_leaves(m);
     for (int kind = 0; kind < 3 && !rc; kind++) {
         auto launch = [&]() {
@@ -1502,9 +1574,9 @@ int amrlbm_time_kernels(amrlbm_mesh *m, int reps, double out[3])
         HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
         out[kind] = ms / reps;
     }
-    HIPCHK(hipMemcpyAsync(m->dm.pop[m->cur], save, sizeof(double) * npop, hipMemcpyDeviceToDevice, m->stream));
-    HIPCHK(hipMemcpyAsync(m->dm.C, save + npop, sizeof(double) * nc, hipMemcpyDeviceToDevice, m->stream));
-    HIPCHK(hipMemcpyAsync(m->dm.PU, save + npop + nc, sizeof(double) * npu, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->dm.pop[m->cur], save, npop, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->dm.C, save + npop, nc, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->dm.PU, save + npop + nc, npu, hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
     HIPCHK(hipFree(save));
     return check_err_flags(m);

@@ -47,6 +47,13 @@ typedef struct {
 
 /* ---- life cycle ---------------------------------------------------------- */
 int amrlbm_create(const amrlbm_params *p, amrlbm_mesh **out);   /* module parameters (Parameters.f90) */
+/* The FP32 production form of the same mesh (no counterpart in the reference,
+ * which is FP64 throughout): fields stored in FP32 as deviations from the
+ * liquid at rest (h_q - w_q, C - 1; g, P, U as they are), collision and U, P
+ * in FP32, the fills, stream, prolong and restrict unchanged in form. Every
+ * entry point works on it; host arrays stay FP64 in the reference layouts
+ * (converted on the way in and out), and amrlbm_output reduces in FP64. */
+int amrlbm_create_fp32(const amrlbm_params *p, amrlbm_mesh **out);
 int amrlbm_destroy(amrlbm_mesh *m);
 const char *amrlbm_last_error(void);
 
