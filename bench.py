@@ -21,9 +21,15 @@ must read and write once (FP64):
   C fill + U,P        : read g (9) + C (1), write P,Ux,Uy (3)          = 104 B
 Ghost-cell, neighbour-table and regrid traffic is overhead, not counted.
 
+fp32 repeats the measurement with the FP32 production form (fields stored in
+FP32 as deviations from the liquid at rest, FP32 collision; bytes per leaf
+cell halved) and states its error against the FP64 run after the same steps.
+The learned regrid predictor is not built: learned_regrid is null.
+
 large_working_set repeats the measurement on 16 independent copies of the
-same domain in one mesh (about 4,600 leaf blocks, 0.5 GB of leaf state, past
-the 256 MB Infinity Cache); it is a second measurement, not the headline.
+same domain in one mesh (about 4,600 leaf blocks, 0.5 GB of FP64 leaf state,
+past the 256 MB Infinity Cache), FP64 and FP32; second measurements, not the
+headline.
 """
 import argparse
 import json
@@ -102,9 +108,10 @@ def calibration_copy(nbytes):
     hip.hipFree(b)
 
 
-def pmc_traffic(kernel_key, workload):
+def pmc_traffic(kernel_key, workload, precision=64):
     """HBM bytes per launch from committed rocprofv3 --pmc summaries, if present."""
-    path = os.path.join(ROOT, "profiles", f"{kernel_key}_{workload}_pmc.json")
+    tag = "_fp32" if precision == 32 else ""
+    path = os.path.join(ROOT, "profiles", f"{kernel_key}_{workload}{tag}_pmc.json")
     if not os.path.exists(path):
         return None
     try:
@@ -113,10 +120,10 @@ def pmc_traffic(kernel_key, workload):
         return None
 
 
-def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0):
+def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64):
     """Time `steps` steps of workload w after `warmup`, then the step kernels
     back to back; returns the numbers of one bench entry."""
-    m = amrlbm.Mesh(w)
+    m = amrlbm.Mesh(w, precision=precision)
     m.setup_rising_bubble()
     m.step(warmup, k)
     m.synchronize()
@@ -135,13 +142,15 @@ def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0):
     share = {kk: kt[kk] * LAUNCHES_PER_STEP.get(kk, 1.0 / k) for kk in kt}
     dom = max(share, key=share.get)
 
+    scale = 0.5 if precision == 32 else 1.0   # 4-byte values
+
     def roof(key):
-        b = ALG_BYTES[key] * c1["nleaf"] * 256
+        b = ALG_BYTES[key] * scale * c1["nleaf"] * 256
         ach = b / (kt[key] * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(key, w.name),
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(key, w.name, precision),
                 "kernel": KERNEL_NAMES[key], "avg_launch_us": round(kt[key] * 1e3, 3),
-                "algorithmic_bytes_per_leaf_cell": ALG_BYTES[key], "leaf_blocks": c1["nleaf"],
+                "algorithmic_bytes_per_leaf_cell": ALG_BYTES[key] * scale, "leaf_blocks": c1["nleaf"],
                 "bytes_per_launch": round(b)}
     return {
         "value": w.fine_cells * steps / wall / 1e6, "wall": wall, "c0": c0, "c1": c1, "regrids": rstats,
@@ -152,9 +161,10 @@ def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0):
     }
 
 
-def describe(w, k):
+def describe(w, k, precision=64):
+    path = "plain FP64 path" if precision == 64 else "FP32 path (deviation storage)"
     what = (f"rising bubble {w.fine_shape[0]}x{w.fine_shape[1]} finest-equivalent, 16x16 blocks, "
-            f"levels {w.min_levels}..{w.max_levels}, D={int(w.D)}, regrid every {k}, plain FP64 path")
+            f"levels {w.min_levels}..{w.max_levels}, D={int(w.D)}, regrid every {k}, {path}")
     if w.instances > 1:
         what = f"{w.instances} independent copies of the {w.L0}x{w.nroots // w.instances * w.L0} " + what
     return what
@@ -169,6 +179,7 @@ def main():
     ap.add_argument("--regrid-every", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 path's entries")
     ap.add_argument("--large-workload", default="cfg2x16",
                     help="replicated-instance workload measured after the main one ('' to skip)")
     ap.add_argument("--large-steps", type=int, default=200)
@@ -201,20 +212,40 @@ def main():
         "regrids": r["regrids"],
         "diagnostics": {"mass": diag[3], "rise_velocity": diag[7], "centroid_y": diag[6]},
     }
+    if not args.no_fp32:
+        r32 = measure(amrlbm, w, args.steps, args.warmup, k, precision=32)
+        d32 = r32["diag"]
+        out["fp32"] = {
+            "workload": describe(w, k, 32), "value": round(r32["value"], 1), "unit": "MLUPS (finest-equivalent)",
+            "speedup_vs_fp64": round(r32["value"] / r["value"], 3),
+            "ms_per_step": round(r32["wall"] * 1e3 / args.steps, 4), "dtype": "f32",
+            "roofline": r32["roofline"], "kernel_avg_us": r32["kernel_us"],
+            "error_vs_fp64": {
+                "after_steps": args.warmup + args.steps,
+                "mass_rel": abs(d32[3] - diag[3]) / abs(diag[3]),
+                "rise_velocity_rel": abs(d32[7] - diag[7]) / abs(diag[7]),
+                "centroid_abs_cells": max(abs(d32[5] - diag[5]), abs(d32[6] - diag[6])),
+                "same_mesh": r32["c1"] == r["c1"]},
+            "learned_regrid": None,
+        }
     if args.large_workload:
         wl = amrlbm.WORKLOADS[args.large_workload]
         warm = max(args.warmup // 5, 10)
-        rl = measure(amrlbm, wl, args.large_steps, warm, k)
-        ll = rl["c1"]["nleaf"]
-        out["large_working_set"] = {
-            "workload": describe(wl, k), "steps": args.large_steps, "warmup": warm,
-            "leaf_blocks": ll, "leaf_state_mb": round(ll * (2 * 18 * 324 + 400 + 3 * 256) * 8 / 1e6, 1),
-            "value": round(rl["value"], 1), "unit": "MLUPS (finest-equivalent)",
-            "ms_per_step": round(rl["wall"] * 1e3 / args.large_steps, 4),
-            "roofline": {kk: {f: v[f] for f in ("kernel", "achieved", "frac", "avg_launch_us", "traffic")}
-                         for kk, v in rl["roofline_all"].items()},
-            "regrids": {f: rl["regrids"][f] for f in ("checks", "changes")},
-        }
+        entries = {}
+        for prec in ((64,) if args.no_fp32 else (64, 32)):
+            rl = measure(amrlbm, wl, args.large_steps, warm, k, precision=prec)
+            ll = rl["c1"]["nleaf"]
+            entries[f"fp{prec}"] = {
+                "workload": describe(wl, k, prec), "steps": args.large_steps, "warmup": warm,
+                "leaf_blocks": ll,
+                "leaf_state_mb": round(ll * (2 * 18 * 324 + 400 + 3 * 256) * (prec // 8) / 1e6, 1),
+                "value": round(rl["value"], 1), "unit": "MLUPS (finest-equivalent)",
+                "ms_per_step": round(rl["wall"] * 1e3 / args.large_steps, 4),
+                "roofline": {kk: {f: v[f] for f in ("kernel", "achieved", "frac", "avg_launch_us", "traffic")}
+                             for kk, v in rl["roofline_all"].items()},
+                "regrids": {f: rl["regrids"][f] for f in ("checks", "changes")},
+            }
+        out["large_working_set"] = entries
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, k)
     print(json.dumps(out), flush=True)
