@@ -700,15 +700,20 @@ __device__ __forceinline__ T stream_cell(const T *sF, T *__restrict__ G, int set
 // Fill_Ghost_Cells_hg of every leaf in list by its ghost program (bounce-back
 // rows included), then optionally the stream into the other population buffer
 // and C = sum(h). The two population sets never read each other here, so a
-// block is two workgroups, blockIdx.y = set (0: h, 1: g). Each recipe covers a
-// ghost cell of the block or a coarse neighbour's ghost cell it needs.
+// block is two workgroups, one per set (0: h, 1: g). Workgroup w takes list
+// entry 8 (w / 16) + w % 8 and set (w / 8) % 2, so both of entry k's
+// workgroups run on XCD k % 8, where every other launch runs entry k. Each
+// recipe covers a ghost cell of the block or a coarse neighbour's ghost cell
+// it needs.
 // flags: 1 fill (else ghosts from memory), 2 stream, 4 C = sum h,
 // 8 carry the ghost ring into the new buffer (as the in-place reference stream)
 template <class T>
-__global__ __launch_bounds__(256) void k_prog_hg(DevMeshT<T> m, int cur, const int *__restrict__ list, Prog prog,
-                                                 int flags)
+__global__ __launch_bounds__(256) void k_prog_hg(DevMeshT<T> m, int cur, const int *__restrict__ list, int n,
+                                                 Prog prog, int flags)
 {
-    const int k = blockIdx.x, set = blockIdx.y;
+    const int w = blockIdx.x;
+    const int k = 8 * (w >> 4) + (w & 7), set = (w >> 3) & 1;
+    if (k >= n) return;
     const int tid = threadIdx.x;
     const int skip = *m.skip;
     const int slot = list[k];
@@ -1140,10 +1145,11 @@ void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Pro
 {
     if (n <= 0) return;
     int flags = (prog.hdr ? 1 : 0) | (stream_after ? 2 : 0) | (macro_h ? 4 : 0) | (copy_ghosts ? 8 : 0);
+    const dim3 grid(16 * ((n + 7) / 8));   // two sets per list entry, entries padded to a multiple of 8
     if (m.f32)
-        hipLaunchKernelGGL(k_prog_hg<float>, dim3(n, 2), dim3(256), 0, s, typed<float>(m), cur, list, prog, flags);
+        hipLaunchKernelGGL(k_prog_hg<float>, grid, dim3(256), 0, s, typed<float>(m), cur, list, n, prog, flags);
     else
-        hipLaunchKernelGGL(k_prog_hg<double>, dim3(n, 2), dim3(256), 0, s, typed<double>(m), cur, list, prog, flags);
+        hipLaunchKernelGGL(k_prog_hg<double>, grid, dim3(256), 0, s, typed<double>(m), cur, list, n, prog, flags);
 }
 
 void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream_t s)

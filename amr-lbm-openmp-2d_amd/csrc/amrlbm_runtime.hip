@@ -57,6 +57,7 @@ struct amrlbm_mesh {
     int tile_roots = 0;              // roots per independent instance (walls between instances)
     std::deque<int> freeq;
     std::vector<int> leaf;           // Array_of_Leaf_Blocks order (ascending id)
+    std::vector<int> disp;           // the leaves in launch order (see dispatch_order)
     std::vector<std::vector<int>> at;  // per level: position -> id (0 = none)
     bool ready = false;
 
@@ -489,7 +490,7 @@ int build_programs(amrlbm_mesh *m)
     std::vector<LeafProg> progs(ntask);
     const int nthr = std::max(1, std::min(omp_get_max_threads(), ntask / 32));
 #pragma omp parallel for schedule(dynamic, 8) num_threads(nthr)
-    for (int t = 0; t < ntask; t++) build_leaf_program(m, t / nl, m->leaf[t % nl], progs[t]);
+    for (int t = 0; t < ntask; t++) build_leaf_program(m, t / nl, m->disp[t % nl], progs[t]);
     for (const LeafProg &lp : progs)
         if (!lp.err.empty()) return fail(AMRLBM_ERR_TOPOLOGY, lp.err);
     for (int kind = 0; kind < 3; kind++) {
@@ -549,6 +550,34 @@ int ensure_int_buffer(int **buf, int *cap, int need)
     return AMRLBM_OK;
 }
 
+// Launch order of the leaves. A launch hands workgroup k to XCD k % 8, and each
+// XCD has its own L2, so the leaves go along a Morton curve (finest-cell
+// origin of the block), the curve is cut into 8 runs of neighbouring blocks,
+// and run c takes the positions k = c mod 8: a block and most of its
+// neighbours then run on one XCD in every launch.
+void dispatch_order(amrlbm_mesh *m)
+{
+    const int n = (int)m->leaf.size();
+    std::vector<std::pair<unsigned long long, int>> key(n);
+    for (int k = 0; k < n; k++) {
+        const Block &b = m->blk[m->leaf[k]];
+        const unsigned x = (unsigned)(b.I * b.dx * NX), y = (unsigned)(b.J * b.dx * NY);
+        unsigned long long z = 0;
+        for (int bit = 0; bit < 31; bit++)
+            z |= (unsigned long long)((x >> bit) & 1u) << (2 * bit) | (unsigned long long)((y >> bit) & 1u) << (2 * bit + 1);
+        key[k] = {z, m->leaf[k]};
+    }
+    std::sort(key.begin(), key.end());
+    const int q = n / 8, r = n % 8;   // runs 0..r-1 hold q + 1 blocks, the others q
+    m->disp.assign(n, 0);
+    int start = 0;
+    for (int c = 0; c < 8; c++) {
+        const int len = q + (c < r ? 1 : 0);
+        for (int i = 0; i < len; i++) m->disp[8 * i + c] = key[start + i].second;
+        start += len;
+    }
+}
+
 int upload_tables(amrlbm_mesh *m)
 {
     if (!m->tables_dirty) return AMRLBM_OK;
@@ -592,7 +621,8 @@ int upload_tables(amrlbm_mesh *m)
     std::vector<int> fillpos(m->lvl_off.begin(), m->lvl_off.end() - 1);
     for (int n : m->leaf) lists[fillpos[m->blk[n].level]++] = n - 1;
     m->all_off = m->lvl_off[NL];
-    for (size_t k = 0; k < m->leaf.size(); k++) lists[m->all_off + k] = m->leaf[k] - 1;
+    dispatch_order(m);
+    for (size_t k = 0; k < m->disp.size(); k++) lists[m->all_off + k] = m->disp[k] - 1;
     int rc = ensure_int_buffer(&m->d_lists, &m->lists_cap, (int)lists.size());
     if (rc) return rc;
     if (nb > 0) {
@@ -705,7 +735,7 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol)
     HIPCHK(hipMemcpyAsync(m->h_pinned, m->d_err, sizeof(double) * nleaf, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
     std::vector<double> err(m->nblks + 1, 0.0);
-    for (int k = 0; k < nleaf; k++) err[m->leaf[k]] = m->h_pinned[k];
+    for (int k = 0; k < nleaf; k++) err[m->disp[k]] = m->h_pinned[k];
 
     const amrlbm_params &p = m->prm;
     // Flag_for_refinement (Refine_Derefine.f90:18-54)
