@@ -523,7 +523,7 @@ __device__ __forceinline__ T eval_hg(const PNode &nd, int set, int q, const T *_
 // population array; U items per thread have their operands in flight together.
 template <bool GLOBAL_ONLY, int U, class T>
 __device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, int set, const T *__restrict__ Fc, T *sF,
-                                         T *F, int tid)
+                                         T *F, bool store, int tid)
 {
     const int ns = e - b;
     for (int base = 0; base < 9 * ns; base += 256 * U) {
@@ -543,7 +543,7 @@ __device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, int set,
                 const int q = t / ns;
                 const PNode &nd = sN[b + t - q * ns];
                 sF[q * LC_F + nd.dest] = v[r];
-                if (nd.own >= 0) F[(set * 9 + q) * GS + nd.own] = v[r];
+                if (store && nd.own >= 0) F[(set * 9 + q) * GS + nd.own] = v[r];
             }
         }
     }
@@ -706,7 +706,10 @@ __device__ __forceinline__ T stream_cell(const T *sF, T *__restrict__ G, int set
 // recipe covers a ghost cell of the block or a coarse neighbour's ghost cell
 // it needs.
 // flags: 1 fill (else ghosts from memory), 2 stream, 4 C = sum h,
-// 8 carry the ghost ring into the new buffer (as the in-place reference stream)
+// 8 carry the ghost ring into the new buffer (as the in-place reference stream),
+// 16 store the filled ghost ring in memory (the phase-by-phase fill; a fill
+// followed by the stream in the same kernel keeps it in LDS only: no kernel
+// reads a population ghost cell from memory, and the next fill rewrites it)
 template <class T>
 __global__ __launch_bounds__(256) void k_prog_hg(DevMeshT<T> m, int cur, const int *__restrict__ list, int n,
                                                  Prog prog, int flags)
@@ -749,9 +752,9 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMeshT<T> m, int cur, const i
     for (int s = 0; s < hd.nst; s++) {
         const int end = hd.end[s];
         if (s == 0)
-            hg_stage<true, 4>(sN, begin, end, set, Fc, sF, F, tid);
+            hg_stage<true, 4>(sN, begin, end, set, Fc, sF, F, (flags & 16) != 0, tid);
         else
-            hg_stage<false, 2>(sN, begin, end, set, Fc, sF, F, tid);
+            hg_stage<false, 2>(sN, begin, end, set, Fc, sF, F, (flags & 16) != 0, tid);
         begin = end;
         __syncthreads();
     }
@@ -1144,7 +1147,8 @@ void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Pro
                     bool macro_h, bool copy_ghosts, hipStream_t s)
 {
     if (n <= 0) return;
-    int flags = (prog.hdr ? 1 : 0) | (stream_after ? 2 : 0) | (macro_h ? 4 : 0) | (copy_ghosts ? 8 : 0);
+    int flags = (prog.hdr ? 1 : 0) | (stream_after ? 2 : 0) | (macro_h ? 4 : 0) | (copy_ghosts ? 8 : 0) |
+                (prog.hdr && !stream_after ? 16 : 0);
     const dim3 grid(16 * ((n + 7) / 8));   // two sets per list entry, entries padded to a multiple of 8
     if (m.f32)
         hipLaunchKernelGGL(k_prog_hg<float>, grid, dim3(256), 0, s, typed<float>(m), cur, list, n, prog, flags);
