@@ -94,6 +94,8 @@ def lib():
             "amrlbm_last_step_ms": [vp, _DP],
             "amrlbm_regrid_stats": [vp, _DP],
             "amrlbm_time_kernels": [vp, ip, _DP],
+            "amrlbm_save_snapshot": [vp, ctypes.c_char_p],
+            "amrlbm_load_snapshot": [vp, ctypes.c_char_p],
             "amrlbm_collision_block": [PP, ip, _DP, _DP, _DP, _DP, _DP, _DP],
             "amrlbm_macro_g_block": [PP, ip, _DP, _DP, _DP, _DP, _DP],
             "amrlbm_stream_block": [PP, ip, _DP],
@@ -259,6 +261,14 @@ class Mesh:
         v = ctypes.c_double()
         _check(lib().amrlbm_last_step_ms(self.h, ctypes.byref(v)))
         return v.value
+
+    def save_snapshot(self, path):
+        """Restart file in the reference dump harness's layout."""
+        _check(lib().amrlbm_save_snapshot(self.h, os.fsencode(path)))
+
+    def load_snapshot(self, path):
+        """Replace the mesh by a snapshot file (the library's or the reference harness's)."""
+        _check(lib().amrlbm_load_snapshot(self.h, os.fsencode(path)))
 
     def time_kernels(self, reps=200):
         """Average ms per launch of the three step kernels (back-to-back launches,

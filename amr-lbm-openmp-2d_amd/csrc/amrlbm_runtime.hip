@@ -1262,11 +1262,13 @@ int amrlbm_get_topology(amrlbm_mesh *m, int *blocks, int *freeq)
     return AMRLBM_OK;
 }
 
-int amrlbm_set_topology(amrlbm_mesh *m, int nblks, const int *blocks, int kk, int nfree, const int *freeq, int T)
+static int set_topology(amrlbm_mesh *m, int nblks, const int *blocks, int kk, int nfree, const int *freeq, int T,
+                        int ninst)
 {
     if (!m || !blocks || nblks < 1 || nblks > m->prm.max_blocks || nfree < 0 || (nfree > 0 && !freeq))
         return fail(AMRLBM_ERR_ARG, "bad topology arguments");
-    m->tile_roots = m->prm.nroots;   // a caller's mesh is one instance
+    if (ninst < 1 || m->prm.nroots % ninst) return fail(AMRLBM_ERR_ARG, "the roots must split evenly into the instances");
+    m->tile_roots = m->prm.nroots / ninst;
     reset_topology(m);
     m->nblks = nblks;
     for (int n = 1; n <= nblks; n++) {
@@ -1299,12 +1301,117 @@ int amrlbm_set_topology(amrlbm_mesh *m, int nblks, const int *blocks, int kk, in
     return upload_tables(m);
 }
 
-int amrlbm_put_block(amrlbm_mesh *m, int id, const double *h, const double *g, const double *C, const double *P,
-                     const double *Ux, const double *Uy)
+int amrlbm_set_topology(amrlbm_mesh *m, int nblks, const int *blocks, int kk, int nfree, const int *freeq, int T)
+{
+    return set_topology(m, nblks, blocks, kk, nfree, freeq, T, 1);   // a caller's mesh is one instance
+}
+
+// Snapshot file: the layout of the reference dump harness and the oracle
+// (oracle/ref_harness.f90, oracle/amroracle.c): 16-int header, the free list,
+// then per block 22 ints (the topology row and a has-arrays flag) and, for
+// leaves, h, g, C, P, Ux, Uy in the reference's array layouts, FP64.
+// Header word 15 (unused there) holds the instance count of a replicated mesh.
+constexpr int kSnapMagic = 1095586380;
+static int put_block(amrlbm_mesh *m, int id, const double *h, const double *g, const double *C, const double *P,
+                     const double *Ux, const double *Uy, bool shifted);
+static int get_block(amrlbm_mesh *m, int id, double *h, double *g, double *C, double *P, double *Ux, double *Uy,
+                     bool shifted);
+
+int amrlbm_save_snapshot(amrlbm_mesh *m, const char *path)
+{
+    if (!m || !path) return fail(AMRLBM_ERR_ARG, "null argument");
+    if (!m->ready) return fail(AMRLBM_ERR_STATE, "mesh not set up");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    std::vector<int> rows((size_t)m->nblks * 21), freeq(m->freeq.size() + 1);
+    int rc = amrlbm_get_topology(m, rows.data(), freeq.data());
+    if (rc) return rc;
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return fail(AMRLBM_ERR_ARG, std::string("cannot write ") + path);
+    const amrlbm_params &p = m->prm;
+    const bool dev = m->dm.f32 != 0;   // an FP32 mesh writes its deviations, exactly
+    int hdr[16] = {kSnapMagic, dev ? 2 : 1, p.nx, p.ny, p.L0, p.min_levels, p.max_levels, m->nblks, (int)m->leaf.size(), m->T,
+                   m->levelmin, m->levelmax, m->kk, (int)m->freeq.size(), p.max_blocks,
+                   m->tile_roots < p.nroots ? p.nroots / m->tile_roots : 0};
+    bool ok = std::fwrite(hdr, sizeof(int), 16, fp) == 16;
+    ok = ok && std::fwrite(freeq.data(), sizeof(int), m->freeq.size(), fp) == m->freeq.size();
+    const size_t nf = 9 * FS, nc = CS, np = PS;
+    std::vector<double> a(2 * nf + nc + 3 * np);
+    for (int n = 1; n <= m->nblks && ok; n++) {
+        int iv[22];
+        std::memcpy(iv, &rows[(size_t)(n - 1) * 21], sizeof(int) * 21);
+        const Block &b = m->blk[n];
+        iv[21] = (b.level >= 0 && b.isleaf) ? 1 : 0;
+        ok = std::fwrite(iv, sizeof(int), 22, fp) == 22;
+        if (!ok || !iv[21]) continue;
+        double *d = a.data();
+        rc = get_block(m, n, d, d + nf, d + 2 * nf, d + 2 * nf + nc, d + 2 * nf + nc + np, d + 2 * nf + nc + 2 * np,
+                       !dev);
+        if (rc) {
+            std::fclose(fp);
+            return rc;
+        }
+        ok = std::fwrite(d, sizeof(double), a.size(), fp) == a.size();
+    }
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok ? AMRLBM_OK : fail(AMRLBM_ERR_ARG, std::string("short write to ") + path);
+}
+
+int amrlbm_load_snapshot(amrlbm_mesh *m, const char *path)
+{
+    if (!m || !path) return fail(AMRLBM_ERR_ARG, "null argument");
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return fail(AMRLBM_ERR_ARG, std::string("cannot read ") + path);
+    auto bad = [&](const std::string &why) {
+        std::fclose(fp);
+        return fail(AMRLBM_ERR_ARG, std::string(path) + ": " + why);
+    };
+    int hdr[16];
+    if (std::fread(hdr, sizeof(int), 16, fp) != 16 || hdr[0] != kSnapMagic || hdr[1] < 1 || hdr[1] > 2)
+        return bad("not a snapshot");
+    const bool file_dev = hdr[1] == 2;   // values are deviations: h - w_q, C - 1
+    const amrlbm_params &p = m->prm;
+    if (hdr[2] != p.nx || hdr[3] != p.ny || hdr[4] != p.L0 || hdr[5] != p.min_levels || hdr[6] != p.max_levels)
+        return bad("mesh parameters differ from this mesh's");
+    const int nblks = hdr[7], nfree = hdr[13], ninst = hdr[15] > 1 ? hdr[15] : 1;
+    if (nblks < 1 || nblks > p.max_blocks || nfree < 0 || nfree > p.max_blocks) return bad("block counts out of range");
+    std::vector<int> freeq(nfree + 1), rows((size_t)nblks * 21), leafs;
+    if ((int)std::fread(freeq.data(), sizeof(int), nfree, fp) != nfree) return bad("truncated free list");
+    const size_t nf = 9 * FS, nc = CS, np = PS, na = 2 * nf + nc + 3 * np;
+    std::vector<double> arrays;
+    for (int n = 1; n <= nblks; n++) {
+        int iv[22];
+        if (std::fread(iv, sizeof(int), 22, fp) != 22) return bad("truncated block table");
+        std::memcpy(&rows[(size_t)(n - 1) * 21], iv, sizeof(int) * 21);
+        if (!iv[21]) continue;
+        leafs.push_back(n);
+        arrays.resize(arrays.size() + na);
+        if (std::fread(&arrays[arrays.size() - na], sizeof(double), na, fp) != na) return bad("truncated arrays");
+    }
+    std::fclose(fp);
+    int rc = set_topology(m, nblks, rows.data(), hdr[12], nfree, freeq.data(), hdr[9], ninst);
+    if (rc) return rc;
+    const bool raw = file_dev && m->dm.f32;   // same storage: no conversion at all
+    for (size_t k = 0; k < leafs.size(); k++) {
+        double *d = &arrays[k * na];
+        if (file_dev && !raw) {   // deviations into an FP64 mesh: absolute values
+            for (size_t e = 0; e < nf; e++) d[e] += kW[e % 9];   // h(9, 0:nx+1, 0:ny+1): q fastest
+            for (size_t e = 0; e < nc; e++) d[2 * nf + e] += 1.0;
+        }
+        rc = put_block(m, leafs[k], d, d + nf, d + 2 * nf, d + 2 * nf + nc, d + 2 * nf + nc + np,
+                       d + 2 * nf + nc + 2 * np, !raw);
+        if (rc) return rc;
+    }
+    return AMRLBM_OK;
+}
+
+// shifted: the host arrays hold absolute values (the FP32 mesh's storage
+// shift is applied on the way); else they hold what the mesh stores
+static int put_block(amrlbm_mesh *m, int id, const double *h, const double *g, const double *C, const double *P,
+                     const double *Ux, const double *Uy, bool shifted)
 {
     if (!m || id < 1 || id > m->prm.max_blocks) return fail(AMRLBM_ERR_ARG, "bad block id");
     const size_t s = (size_t)(id - 1);
-    const bool f32 = m->dm.f32;
+    const bool f32 = m->dm.f32 && shifted;
     std::vector<double> buf(9 * GS);
     int rc = AMRLBM_OK;
     if (h) {
@@ -1324,11 +1431,12 @@ int amrlbm_put_block(amrlbm_mesh *m, int id, const double *h, const double *g, c
     return AMRLBM_OK;
 }
 
-int amrlbm_get_block(amrlbm_mesh *m, int id, double *h, double *g, double *C, double *P, double *Ux, double *Uy)
+static int get_block(amrlbm_mesh *m, int id, double *h, double *g, double *C, double *P, double *Ux, double *Uy,
+                     bool shifted)
 {
     if (!m || id < 1 || id > m->prm.max_blocks) return fail(AMRLBM_ERR_ARG, "bad block id");
     const size_t s = (size_t)(id - 1);
-    const bool f32 = m->dm.f32;
+    const bool f32 = m->dm.f32 && shifted;
     std::vector<double> buf(9 * GS);
     int rc = AMRLBM_OK;
     if (h) {
@@ -1346,6 +1454,17 @@ int amrlbm_get_block(amrlbm_mesh *m, int id, double *h, double *g, double *C, do
     if (Ux && (rc = download(m, m->dm.PU, s * PUB + PS, PS, Ux, no_shift))) return rc;
     if (Uy && (rc = download(m, m->dm.PU, s * PUB + 2 * PS, PS, Uy, no_shift))) return rc;
     return AMRLBM_OK;
+}
+
+int amrlbm_put_block(amrlbm_mesh *m, int id, const double *h, const double *g, const double *C, const double *P,
+                     const double *Ux, const double *Uy)
+{
+    return put_block(m, id, h, g, C, P, Ux, Uy, true);
+}
+
+int amrlbm_get_block(amrlbm_mesh *m, int id, double *h, double *g, double *C, double *P, double *Ux, double *Uy)
+{
+    return get_block(m, id, h, g, C, P, Ux, Uy, true);
 }
 
 int amrlbm_determine_min_max_levels(amrlbm_mesh *m, int *levelmin, int *levelmax)

@@ -81,6 +81,18 @@ int amrlbm_get_topology(amrlbm_mesh *m, int *blocks, int *freeq);
  * are recomputed from the quadtree and checked against the given ones */
 int amrlbm_set_topology(amrlbm_mesh *m, int nblks, const int *blocks, int kk, int nfree,
                         const int *freeq, int T);
+
+/* Restart files in the layout of the reference dump harness
+ * (oracle/ref_harness.f90): header, free list, every block's topology row
+ * and, for leaves, h, g, C, P, Ux, Uy in the reference's array layouts (FP64;
+ * an FP32 mesh converts). No counterpart in the reference's own program, whose
+ * only output is the Tecplot writer; the format lets a run restart and be
+ * exchanged with the reference harness's snapshots. An FP32 mesh writes its
+ * stored deviations (h_q - w_q, C - 1) exactly, marked by header word 1 = 2;
+ * load converts between the forms as needed. load replaces the mesh; the
+ * file's nx, ny, L0 and level range must equal the mesh's. */
+int amrlbm_save_snapshot(amrlbm_mesh *m, const char *path);
+int amrlbm_load_snapshot(amrlbm_mesh *m, const char *path);
 /* arrays of one live block (NULL pointers skip a field) */
 int amrlbm_put_block(amrlbm_mesh *m, int id, const double *h, const double *g, const double *C,
                      const double *P, const double *Ux, const double *Uy);
