@@ -95,6 +95,10 @@ def lib():
             "amrlbm_regrid_stats": [vp, _DP],
             "amrlbm_time_kernels": [vp, ip, _DP],
             "amrlbm_save_snapshot": [vp, ctypes.c_char_p],
+            "amrlbm_cnn_forward": [ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
+            "amrlbm_cnn_gradient": [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_float), ctypes.c_float, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
+            "amrlbm_cnn_train": [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_float), ip, ip, ctypes.c_float, ctypes.c_float, ctypes.c_uint, ctypes.POINTER(ctypes.c_float)],
+            "amrlbm_cnn_features": [vp, ctypes.POINTER(ctypes.c_float), _IP],
             "amrlbm_load_snapshot": [vp, ctypes.c_char_p],
             "amrlbm_collision_block": [PP, ip, _DP, _DP, _DP, _DP, _DP, _DP],
             "amrlbm_macro_g_block": [PP, ip, _DP, _DP, _DP, _DP, _DP],
@@ -126,6 +130,57 @@ def _i(a):
 def pop_array():
     """Empty array in the reference layout h(9, 0:nx+1, 0:ny+1)."""
     return np.zeros((9, NX + 2, NY + 2), order="F")
+
+
+CNN_NPARAM = 3058
+
+
+def _f(a):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def cnn_forward(x, weights):
+    """Logits [n, 2] of the regrid predictor for inputs x [n, 3, 18, 18]."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    out = np.zeros((x.shape[0], 2), dtype=np.float32)
+    _check(lib().amrlbm_cnn_forward(_f(x), x.shape[0], _f(w), _f(out)))
+    return out
+
+
+def cnn_gradient(x, y, weights, pos_weight=1.0):
+    """(gradient [CNN_NPARAM], loss) of the weighted logistic loss."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    g = np.zeros(CNN_NPARAM, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    _check(lib().amrlbm_cnn_gradient(_f(x), _f(y), x.shape[0], _f(w), float(pos_weight), _f(g), _f(loss)))
+    return g, float(loss[0])
+
+
+def cnn_train(x, y, weights, epochs, batch=256, lr=1e-3, pos_weight=1.0, seed=1):
+    """Train on the GPU; returns (weights, per-epoch mean loss)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    w = np.array(weights, dtype=np.float32)
+    hist = np.zeros(max(epochs, 1), dtype=np.float32)
+    _check(lib().amrlbm_cnn_train(_f(x), _f(y), x.shape[0], _f(w), int(epochs), int(batch), float(lr),
+                                  float(pos_weight), int(seed), _f(hist)))
+    return w, hist[:epochs]
+
+
+def cnn_init(seed=0):
+    """He-uniform initial weights, zero biases."""
+    rng = np.random.default_rng(seed)
+    w = np.zeros(CNN_NPARAM, dtype=np.float32)
+    o = 0
+    for cout, fan in ((16, 27), (16, 144), (2, 144)):
+        lim = np.sqrt(6.0 / fan)
+        w[o:o + cout * fan] = rng.uniform(-lim, lim, cout * fan)
+        o += cout * fan + cout
+    return w
 
 
 class Mesh:
@@ -269,6 +324,14 @@ class Mesh:
     def load_snapshot(self, path):
         """Replace the mesh by a snapshot file (the library's or the reference harness's)."""
         _check(lib().amrlbm_load_snapshot(self.h, os.fsencode(path)))
+
+    def cnn_features(self):
+        """Network input of every leaf: (x [nLeaf, 3, 18, 18] float32, block ids)."""
+        n = self.counts()["nleaf"]
+        x = np.zeros((n, 3, 18, 18), dtype=np.float32)
+        ids = np.zeros(n, dtype=np.int32)
+        _check(lib().amrlbm_cnn_features(self.h, _f(x), _i(ids)))
+        return x, ids
 
     def time_kernels(self, reps=200):
         """Average ms per launch of the three step kernels (back-to-back launches,

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libamrlbm.so")
-SOURCES = ["amrlbm_kernels.hip", "amrlbm_runtime.hip"]
+SOURCES = ["amrlbm_kernels.hip", "amrlbm_cnn.hip", "amrlbm_runtime.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AMRLBM_ARCH", "gfx950")
 

@@ -150,6 +150,14 @@ struct Prog {
 enum FillOp { OP_NONE = 0, OP_COLLIDE = 1, OP_MACRO_G = 2, OP_MACRO_G_COLLIDE = 3 };
 // C fill (kind C1 or C2) from a ghost program, then op; with prog.hdr == null
 // the block's ghosts are read from memory instead (no fill)
+// learned regrid predictor (amrlbm_cnn.hip)
+constexpr int CNN_NPARAM = 16 * 27 + 16 + 16 * 144 + 16 + 2 * 144 + 2;   // 3058
+constexpr float kUScale = 10.0f;   // network input: velocity times this
+void launch_cnn(const float *X, const int *idx, int n, const float *w, float *logits, const float *Y,
+                float pos_weight, float *grad, float *loss, hipStream_t s);
+void launch_cnn_adam(float *w, float *m, float *v, float *g, float lr, int t, hipStream_t s);
+void launch_cnn_features(const DevMesh &m, const int *list, int n, float *X, hipStream_t s);
+
 void launch_prog_c(const DevMesh &m, int cur, const int *list, int n, int kind, int op, const Prog &prog,
                    const Phys &ph, hipStream_t s,
                    double *err_slot = nullptr);

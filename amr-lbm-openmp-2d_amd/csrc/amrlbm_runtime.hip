@@ -1731,6 +1731,147 @@ int amrlbm_time_kernels(amrlbm_mesh *m, int reps, double out[3])
     return check_err_flags(m);
 }
 
+// ---- learned regrid predictor --------------------------------------------
+namespace {
+// device buffers of the predictor entry points (grown on demand, kept until
+// the process ends); the entry points run on their own stream
+struct CnnCtx {
+    hipStream_t s = nullptr;
+    float *x = nullptr, *y = nullptr, *logit = nullptr, *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr,
+          *loss = nullptr;
+    int *idx = nullptr;
+    size_t cap = 0;
+};
+CnnCtx g_cnn;
+
+int cnn_reserve(size_t n)
+{
+    CnnCtx &c = g_cnn;
+    if (!c.s) {
+        HIPCHK(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
+        HIPCHK(hipMalloc(&c.w, sizeof(float) * CNN_NPARAM));
+        HIPCHK(hipMalloc(&c.m, sizeof(float) * CNN_NPARAM));
+        HIPCHK(hipMalloc(&c.v, sizeof(float) * CNN_NPARAM));
+        HIPCHK(hipMalloc(&c.g, sizeof(float) * CNN_NPARAM));
+        HIPCHK(hipMalloc(&c.loss, sizeof(float)));
+    }
+    if (n > c.cap) {
+        (void)hipFree(c.x);
+        (void)hipFree(c.y);
+        (void)hipFree(c.logit);
+        (void)hipFree(c.idx);
+        c.x = c.y = c.logit = nullptr;
+        c.idx = nullptr;
+        c.cap = 0;
+        HIPCHK(hipMalloc(&c.x, sizeof(float) * 3 * 18 * 18 * n));
+        HIPCHK(hipMalloc(&c.y, sizeof(float) * 2 * n));
+        HIPCHK(hipMalloc(&c.logit, sizeof(float) * 2 * n));
+        HIPCHK(hipMalloc(&c.idx, sizeof(int) * n));
+        c.cap = n;
+    }
+    return AMRLBM_OK;
+}
+}  // namespace
+
+int amrlbm_cnn_forward(const float *x, int n, const float *weights, float *logits)
+{
+    if (!x || !weights || !logits || n < 0) return fail(AMRLBM_ERR_ARG, "bad predictor arguments");
+    if (n == 0) return AMRLBM_OK;
+    int rc = cnn_reserve(n);
+    if (rc) return rc;
+    CnnCtx &c = g_cnn;
+    HIPCHK(hipMemcpyAsync(c.x, x, sizeof(float) * 3 * 18 * 18 * n, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.w, weights, sizeof(float) * CNN_NPARAM, hipMemcpyHostToDevice, c.s));
+    launch_cnn(c.x, nullptr, n, c.w, c.logit, nullptr, 0.f, nullptr, nullptr, c.s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(logits, c.logit, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
+    return AMRLBM_OK;
+}
+
+int amrlbm_cnn_gradient(const float *x, const float *y, int n, const float *weights, float pos_weight, float *grad,
+                        float *loss)
+{
+    if (!x || !y || !weights || !grad || !loss || n < 1) return fail(AMRLBM_ERR_ARG, "bad predictor arguments");
+    int rc = cnn_reserve(n);
+    if (rc) return rc;
+    CnnCtx &c = g_cnn;
+    HIPCHK(hipMemcpyAsync(c.x, x, sizeof(float) * 3 * 18 * 18 * n, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.y, y, sizeof(float) * 2 * n, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.w, weights, sizeof(float) * CNN_NPARAM, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemsetAsync(c.g, 0, sizeof(float) * CNN_NPARAM, c.s));
+    HIPCHK(hipMemsetAsync(c.loss, 0, sizeof(float), c.s));
+    launch_cnn(c.x, nullptr, n, c.w, nullptr, c.y, pos_weight, c.g, c.loss, c.s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(grad, c.g, sizeof(float) * CNN_NPARAM, hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipMemcpyAsync(loss, c.loss, sizeof(float), hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
+    return AMRLBM_OK;
+}
+
+int amrlbm_cnn_train(const float *x, const float *y, int n, float *weights, int epochs, int batch, float lr,
+                     float pos_weight, unsigned seed, float *epoch_loss)
+{
+    if (!x || !y || !weights || n < 1 || epochs < 0 || batch < 1 || !(lr > 0.f))
+        return fail(AMRLBM_ERR_ARG, "bad training arguments");
+    int rc = cnn_reserve(n);
+    if (rc) return rc;
+    CnnCtx &c = g_cnn;
+    HIPCHK(hipMemcpyAsync(c.x, x, sizeof(float) * 3 * 18 * 18 * n, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.y, y, sizeof(float) * 2 * n, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.w, weights, sizeof(float) * CNN_NPARAM, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemsetAsync(c.m, 0, sizeof(float) * CNN_NPARAM, c.s));
+    HIPCHK(hipMemsetAsync(c.v, 0, sizeof(float) * CNN_NPARAM, c.s));
+    HIPCHK(hipMemsetAsync(c.g, 0, sizeof(float) * CNN_NPARAM, c.s));
+    std::vector<int> perm(n);
+    for (int k = 0; k < n; k++) perm[k] = k;
+    unsigned long long st = seed * 0x9E3779B97F4A7C15ull + 1;   // SplitMix64 shuffle
+    auto next = [&st]() {
+        unsigned long long z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    int t = 0;
+    for (int ep = 0; ep < epochs; ep++) {
+        for (int k = n - 1; k > 0; k--) std::swap(perm[k], perm[next() % (unsigned long long)(k + 1)]);
+        HIPCHK(hipMemcpyAsync(c.idx, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, c.s));
+        HIPCHK(hipMemsetAsync(c.loss, 0, sizeof(float), c.s));
+        int nb = 0;
+        for (int b0 = 0; b0 < n; b0 += batch, nb++) {
+            const int nbat = std::min(batch, n - b0);
+            launch_cnn(c.x, c.idx + b0, nbat, c.w, nullptr, c.y, pos_weight, c.g, c.loss, c.s);
+            launch_cnn_adam(c.w, c.m, c.v, c.g, lr, ++t, c.s);
+        }
+        HIPCHK(hipGetLastError());
+        float l = 0.f;
+        HIPCHK(hipMemcpyAsync(&l, c.loss, sizeof(float), hipMemcpyDeviceToHost, c.s));
+        HIPCHK(hipStreamSynchronize(c.s));   // perm is rewritten by the next epoch
+        if (epoch_loss) epoch_loss[ep] = l / (float)nb;
+    }
+    HIPCHK(hipMemcpyAsync(weights, c.w, sizeof(float) * CNN_NPARAM, hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
+    return AMRLBM_OK;
+}
+
+int amrlbm_cnn_features(amrlbm_mesh *m, float *x, int *ids)
+{
+    NEED_READY(m);
+    if (!x || !ids) return fail(AMRLBM_ERR_ARG, "null argument");
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    const int n = nleaves(m);
+    float *d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(float) * 3 * 18 * 18 * std::max(n, 1)));
+    launch_cnn_features(m->dm, all_leaves(m), n, d, m->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(x, d, sizeof(float) * 3 * 18 * 18 * n, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipFree(d));
+    for (int k = 0; k < n; k++) ids[k] = m->disp[k];
+    return AMRLBM_OK;
+}
+
 int amrlbm_output(amrlbm_mesh *m, double out[9])
 {
     NEED_READY(m);

@@ -149,6 +149,28 @@ int amrlbm_last_step_ms(amrlbm_mesh *m, double *ms);
  * out[5] ghost-program build), out[6] graph capture and instantiation; out[7] graphs built */
 int amrlbm_regrid_stats(amrlbm_mesh *m, double out[8]);
 
+/* ---- learned regrid predictor (no counterpart in the reference) ----------
+ * A 3-layer 3x3 convolutional network (16 channels, BF16 weights on MFMA,
+ * FP32 accumulate) over one leaf block's C (18 x 18, first ghost ring) and
+ * Ux, Uy (interior, times 10, zero ring): input rows of 3 x 18 x 18 floats;
+ * output two logits per block: its region refined / derefined k steps ahead.
+ * weights: AMRLBM_CNN_NPARAM floats (conv1 16x27, b1, conv2 16x144, b2,
+ * conv3 2x144, b3; row k of a 3x3 filter is tap * cin_count + cin, tap =
+ * 3 dy + dx). These entry points share one stream and device buffers of the
+ * process: call them from one thread. */
+#define AMRLBM_CNN_NPARAM 3058
+int amrlbm_cnn_forward(const float *x, int n, const float *weights, float *logits);
+/* gradient (AMRLBM_CNN_NPARAM) and value of the weighted logistic loss
+ * sum over samples and channels of -[pw y log p + (1-y) log(1-p)] / n */
+int amrlbm_cnn_gradient(const float *x, const float *y, int n, const float *weights, float pos_weight, float *grad,
+                        float *loss);
+/* Adam (lr, 0.9, 0.999) over epochs of shuffled minibatches, on the GPU;
+ * weights in/out; epoch_loss[e] = mean minibatch loss of epoch e (may be null) */
+int amrlbm_cnn_train(const float *x, const float *y, int n, float *weights, int epochs, int batch, float lr,
+                     float pos_weight, unsigned seed, float *epoch_loss);
+/* the network input of every leaf of the mesh: x (nLeaf rows), ids (block ids) */
+int amrlbm_cnn_features(amrlbm_mesh *m, float *x, int *ids);
+
 /* ---- per-block kernels on caller (host) arrays, reference layouts --------- */
 int amrlbm_collision_block(const amrlbm_params *p, int dx, const double *C, const double *P,
                            const double *Ux, const double *Uy, double *h, double *g);    /* LBM_Subroutines.f90:116-189 */
