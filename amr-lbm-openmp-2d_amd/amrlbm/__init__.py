@@ -99,6 +99,8 @@ def lib():
             "amrlbm_cnn_gradient": [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_float), ctypes.c_float, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
             "amrlbm_cnn_train": [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_float), ip, ip, ctypes.c_float, ctypes.c_float, ctypes.c_uint, ctypes.POINTER(ctypes.c_float)],
             "amrlbm_cnn_features": [vp, ctypes.POINTER(ctypes.c_float), _IP],
+            "amrlbm_regrid_check": [vp, _IP],
+            "amrlbm_refine_derefine_predicted": [vp, _IP, ip],
             "amrlbm_load_snapshot": [vp, ctypes.c_char_p],
             "amrlbm_collision_block": [PP, ip, _DP, _DP, _DP, _DP, _DP, _DP],
             "amrlbm_macro_g_block": [PP, ip, _DP, _DP, _DP, _DP, _DP],
@@ -324,6 +326,17 @@ class Mesh:
     def load_snapshot(self, path):
         """Replace the mesh by a snapshot file (the library's or the reference harness's)."""
         _check(lib().amrlbm_load_snapshot(self.h, os.fsencode(path)))
+
+    def regrid_check(self):
+        """(would change the mesh, would refine a block) at the current state."""
+        out = np.zeros(2, dtype=np.int32)
+        _check(lib().amrlbm_regrid_check(self.h, _i(out)))
+        return bool(out[0]), bool(out[1])
+
+    def refine_derefine_predicted(self, ids):
+        """Refine_Derefine_Mesh with the given blocks flagged for refinement too."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        _check(lib().amrlbm_refine_derefine_predicted(self.h, _i(ids), int(ids.size)))
 
     def cnn_features(self):
         """Network input of every leaf: (x [nLeaf, 3, 18, 18] float32, block ids)."""

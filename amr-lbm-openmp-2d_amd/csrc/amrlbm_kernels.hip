@@ -821,7 +821,7 @@ __global__ __launch_bounds__(256) void k_err(DevMeshT<T> m, const int *__restric
 // are final once the pass over level L+1 is done. So each candidate gathers
 // its non-leaf watch blocks once, and the passes only test LDS bits.
 // One workgroup. More than RC_CAND candidates: answers "changed" and leaves
-// the decision to the host.
+// the decision to the host. changed[1] = 1 iff any refinement flag is set.
 constexpr int RC_CAND = 512;
 __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int nblks, const double *__restrict__ err_slot,
                                                       int min_levels, int max_levels, int levelmin, int levelmax,
@@ -855,12 +855,16 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
         }
     }
     if (__syncthreads_or(ref)) {
-        if (tid == 0) *changed = 1;
+        if (tid == 0) {
+            changed[0] = 1;
+            changed[1] = 1;
+        }
         return;
     }
+    if (tid == 0) changed[1] = 0;
     const int nc = ncand;
     if (nc > RC_CAND) {
-        if (tid == 0) *changed = 1;
+        if (tid == 0) changed[0] = 1;
         return;
     }
     // children adjacent to direction k (1-based, Refine_Derefine.f90:155-163)
@@ -898,7 +902,7 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
     int any = 0;
     for (int w = tid; w < nw; w += 1024) any |= dmask[w] != 0u;
     any = __syncthreads_or(any);
-    if (tid == 0) *changed = any;
+    if (tid == 0) changed[0] = any;
 }
 
 // Restrict_Bilinear (Restrict.f90:14-65) with Bilinear_from_Fine(_f)

@@ -73,7 +73,8 @@ struct amrlbm_mesh {
     int pairs_cap = 0;
     double *d_err = nullptr;         // Err_Interface per leaf
     double *d_err_slot = nullptr;    // Err_Interface per block slot (leaves)
-    int *d_changed = nullptr;        // the device regrid decision; the step kernels' skip word
+    int *d_changed = nullptr;        // [0] the device regrid decision (the step kernels' skip word),
+                                     // [1] any refinement flag
     int *h_changed = nullptr;        // pinned copy of it
     double *d_part = nullptr;        // diagnostics partials
     double *d_diag = nullptr;
@@ -723,7 +724,7 @@ int phase_macro_g(amrlbm_mesh *m)
 // ---------------------------------------------------------------------------
 // regrid: Refine_Derefine_Mesh (Refine_Derefine.f90:2-14)
 // ---------------------------------------------------------------------------
-int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol)
+int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector<char> *extra = nullptr)
 {
     int rc = upload_tables(m);
     if (rc) return rc;
@@ -745,6 +746,7 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol)
         if (b.isleaf) {
             if (b.level < p.min_levels) b.refin = true;
             else if (b.level < p.max_levels && err[n] > refin_tol) b.refin = true;
+            else if (extra && b.level < p.max_levels && (*extra)[n]) b.refin = true;   // predicted
         }
     }
     // Enforce_1_Level_Jump / Check_Level_Jump (:58-93): least fixed point
@@ -923,9 +925,9 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipMemsetAsync(m->d_err_flags, 0, sizeof(int), m->stream));
     HIPCHK(hipMalloc(&m->d_err, sizeof(double) * S));
     HIPCHK(hipMalloc(&m->d_err_slot, sizeof(double) * S));
-    HIPCHK(hipMalloc(&m->d_changed, sizeof(int)));
-    HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int), m->stream));
-    HIPCHK(hipHostMalloc(&m->h_changed, sizeof(int), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&m->d_changed, sizeof(int) * 2));
+    HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int) * 2, m->stream));
+    HIPCHK(hipHostMalloc(&m->h_changed, sizeof(int) * 2, hipHostMallocDefault));
     HIPCHK(hipEventCreateWithFlags(&m->ev_check, hipEventDisableTiming));
     HIPCHK(hipMalloc(&m->d_part, sizeof(double) * 9 * S));
     HIPCHK(hipMalloc(&m->d_diag, sizeof(double) * 16));
@@ -1558,6 +1560,37 @@ int amrlbm_refine_derefine_mesh(amrlbm_mesh *m, double refin_tol, double deref_t
 {
     NEED_READY(m);
     return regrid(m, refin_tol, deref_tol);
+}
+
+int amrlbm_regrid_check(amrlbm_mesh *m, int out[2])
+{
+    NEED_READY(m);
+    if (!out) return fail(AMRLBM_ERR_ARG, "null argument");
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    determine_min_max_levels(m);
+    // C, its first ghost ring and U, P are current between steps
+    launch_err(m->dm, all_leaves(m), nleaves(m), m->d_err, m->d_err_slot, m->stream);
+    launch_regrid_check(m->dm, m->nblks, m->d_err_slot, m->prm.min_levels, m->prm.max_levels, m->levelmin,
+                        m->levelmax, 0.002, 0.001, m->d_changed, m->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(m->h_changed, m->d_changed, sizeof(int) * 2, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    out[0] = m->h_changed[0];
+    out[1] = m->h_changed[1];
+    HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int) * 2, m->stream));   // the skip word stays clear
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return AMRLBM_OK;
+}
+
+int amrlbm_refine_derefine_predicted(amrlbm_mesh *m, const int *ids, int n)
+{
+    NEED_READY(m);
+    if (n < 0 || (n > 0 && !ids)) return fail(AMRLBM_ERR_ARG, "bad predicted blocks");
+    std::vector<char> extra(m->nblks + 1, 0);
+    for (int k = 0; k < n; k++)
+        if (ids[k] >= 1 && ids[k] <= m->nblks) extra[ids[k]] = 1;
+    return regrid(m, 0.002, 0.001, &extra);
 }
 
 int amrlbm_step(amrlbm_mesh *m, int nsteps, int regrid_every)

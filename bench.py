@@ -24,7 +24,9 @@ Ghost-cell, neighbour-table and regrid traffic is overhead, not counted.
 fp32 repeats the measurement with the FP32 production form (fields stored in
 FP32 as deviations from the liquid at rest, FP32 collision; bytes per leaf
 cell halved) and states its error against the FP64 run after the same steps.
-The learned regrid predictor is not built: learned_regrid is null.
+fp32.learned_regrid runs the learned regrid (the committed network,
+amrlbm/regrid_cnn_weights.npy) on a held-out bubble against the same regrid
+cadence without the network and the plain regrid every 10 steps.
 
 large_working_set repeats the measurement on 16 independent copies of the
 same domain in one mesh (about 4,600 leaf blocks, 0.5 GB of FP64 leaf state,
@@ -161,6 +163,28 @@ def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64):
     }
 
 
+def learned_regrid(amrlbm, w, steps=2000):
+    """The learned regrid (committed network) on a held-out bubble of the same
+    geometry, FP32, against the same cadence without the network and the plain
+    regrid every 10 steps."""
+    import numpy as np
+    from amrlbm import learned
+    if not os.path.exists(learned.WEIGHTS):
+        return None
+    ev = learned.evaluate(w, np.load(learned.WEIGHTS), steps)
+    out = {"workload": f"held-out bubble R={learned.EVAL_VARIANT[0]} at {learned.EVAL_VARIANT[1:]}, {steps} steps, "
+                       f"full regrid every {learned.HORIZON} steps, plain judge every 10",
+           "precision": "f32"}
+    for k, v in ev.items():
+        out[k] = {f: (round(x, 4) if isinstance(x, float) else x) for f, x in v.items() if f != "diag"}
+    base = ev["plain_every_10"]["diag"]
+    out["learned"]["mass_rel_vs_plain"] = abs(ev["learned"]["diag"][3] - base[3]) / abs(base[3])
+    out["learned"]["rise_velocity_rel_vs_plain"] = abs(ev["learned"]["diag"][7] - base[7]) / abs(base[7])
+    out["gain_mlups_vs_no_network"] = round(ev["learned"]["mlups"] / ev["no_network"]["mlups"], 3)
+    out["gain_mlups_vs_plain_every_10"] = round(ev["learned"]["mlups"] / ev["plain_every_10"]["mlups"], 3)
+    return out
+
+
 def describe(w, k, precision=64):
     path = "plain FP64 path" if precision == 64 else "FP32 path (deviation storage)"
     what = (f"rising bubble {w.fine_shape[0]}x{w.fine_shape[1]} finest-equivalent, 16x16 blocks, "
@@ -226,7 +250,7 @@ def main():
                 "rise_velocity_rel": abs(d32[7] - diag[7]) / abs(diag[7]),
                 "centroid_abs_cells": max(abs(d32[5] - diag[5]), abs(d32[6] - diag[6])),
                 "same_mesh": r32["c1"] == r["c1"]},
-            "learned_regrid": None,
+            "learned_regrid": learned_regrid(amrlbm, w),
         }
     if args.large_workload:
         wl = amrlbm.WORKLOADS[args.large_workload]

@@ -110,7 +110,16 @@ int amrlbm_phase_field_lbm_h_g(amrlbm_mesh *m, double ms[3]);
 int amrlbm_macroscopic_values_h(amrlbm_mesh *m);      /* :328-335 -> LBM_Subroutines.f90:193-201 */
 int amrlbm_fill_ghost_cells_c(amrlbm_mesh *m);        /* Fill_Ghost_Cells_C.f90:2-40 */
 int amrlbm_macroscopic_values_g(amrlbm_mesh *m);      /* :346-354 -> LBM_Subroutines.f90:205-240 */
-int amrlbm_refine_derefine_mesh(amrlbm_mesh *m, double refin_tol, double deref_tol); /* Refine_Derefine.f90:2-14 */
+int amrlbm_refine_derefine_mesh(amrlbm_mesh *m, double refin_tol, double deref_tol);
+/* The device's regrid decision at the current state without regridding
+ * (no counterpart in the reference): out[0] = 1 iff Refine_Derefine_Mesh
+ * would change the mesh, out[1] = 1 iff it would refine a block (the plain
+ * criterion finds the interface on an under-refined block). */
+int amrlbm_regrid_check(amrlbm_mesh *m, int out[2]);
+/* Refine_Derefine_Mesh with the blocks ids[0..n-1] flagged for refinement in
+ * addition to the plain criterion (the learned regrid's predictions; the 2:1
+ * closure and the derefinement rules apply as usual). */
+int amrlbm_refine_derefine_predicted(amrlbm_mesh *m, const int *ids, int n); /* Refine_Derefine.f90:2-14 */
 
 /* nsteps whole steps (fused kernels, one HIP graph per mesh topology); the
  * regrid runs after step T when (T+1) % regrid_every == 0, with the
