@@ -51,7 +51,9 @@ __device__ __forceinline__ double chem_pot(const double *sC, int i, int j, int d
                   20.0 * sC[cidx(i, j)]) /
                  6.0;
     double c = sC[cidx(i, j)];
-    return 4.0 * ph.beta * c * (c - 1.0) * (c - 0.5) - ph.kappa * d2c / (double)(dx * dx);
+    // x / dx^2 as x * (1 / dx^2): dx is a power of two, so both are the correctly
+    // rounded x / dx^2 (the same bits, subnormals included)
+    return 4.0 * ph.beta * c * (c - 1.0) * (c - 0.5) - ph.kappa * d2c * (1.0 / (double)(dx * dx));
 }
 
 // Gradient_ED_AMR (LBM_Subroutines.f90:281-289)
@@ -61,22 +63,25 @@ __device__ __forceinline__ void gradient_ed(const double e[9], double &gx, doubl
     gy = (e[2] - e[4]) / 3.0 + (e[5] + e[6] - e[7] - e[8]) / 12.0;
 }
 
-// ED_C_AMR (LBM_Subroutines.f90:244-258)
+// ED_C_AMR (LBM_Subroutines.f90:244-258); / dx as * (1 / dx), exact for the
+// power-of-two dx
 __device__ __forceinline__ void ed_c(const double *sC, int i, int j, int dx, double e[9])
 {
+    const double inv_dx = 1.0 / (double)dx;
 #pragma unroll
     for (int k = 0; k < 9; k++)
-        e[k] = (sC[cidx(i + c_ex[k], j + c_ey[k])] - sC[cidx(i - c_ex[k], j - c_ey[k])]) * 0.5 / (double)dx;
+        e[k] = (sC[cidx(i + c_ex[k], j + c_ey[k])] - sC[cidx(i - c_ex[k], j - c_ey[k])]) * 0.5 * inv_dx;
 }
 
 // ED_M_AMR (LBM_Subroutines.f90:262-277)
 __device__ __forceinline__ void ed_m(const double *sC, int i, int j, int dx, double e[9])
 {
+    const double inv_dx = 1.0 / (double)dx;
 #pragma unroll
     for (int k = 0; k < 9; k++)
         e[k] = (-sC[cidx(i - c_ex[k], j - c_ey[k])] - 3.0 * sC[cidx(i, j)] + 5.0 * sC[cidx(i + c_ex[k], j + c_ey[k])] -
                 sC[cidx(i + 2 * c_ex[k], j + 2 * c_ey[k])]) *
-               0.25 / (double)dx;
+               0.25 * inv_dx;
 }
 
 // Equilibrium_h_g_AMR (LBM_Subroutines.f90:2-21)
@@ -180,14 +185,15 @@ __device__ __forceinline__ void macro_g_cell(const double *sC, int i, int j, int
                                              double &P, double &Ux, double &Uy)
 {
     const double dRho6 = (ph.rhoh - ph.rhol) / 6.0;
+    const double inv_dx = 1.0 / (double)dx;   // exact: dx is a power of two
     double gx = ((sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)]) / 3.0 +
                  (sC[cidx(i + 1, j + 1)] + sC[cidx(i + 1, j - 1)] - sC[cidx(i - 1, j + 1)] - sC[cidx(i - 1, j - 1)]) /
-                     12.0) /
-                (double)dx;
+                     12.0) *
+                inv_dx;
     double gy = ((sC[cidx(i, j + 1)] - sC[cidx(i, j - 1)]) / 3.0 +
                  (sC[cidx(i + 1, j + 1)] + sC[cidx(i - 1, j + 1)] - sC[cidx(i + 1, j - 1)] - sC[cidx(i - 1, j - 1)]) /
-                     12.0) /
-                (double)dx;
+                     12.0) *
+                inv_dx;
     double mu = chem_pot(sC, i, j, dx, ph);
     double Rho = ph.rhol + sC[cidx(i, j)] * (ph.rhoh - ph.rhol);
     double Fb = (ph.rhoh - Rho) * ph.gy;
