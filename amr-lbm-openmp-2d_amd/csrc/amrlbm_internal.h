@@ -47,6 +47,10 @@ __host__ __device__ inline int pcell(int x, int y)
 // Physics constants of D2Q9_Mod.f90 plus wc (D2Q9_Mod.f90:11).
 struct Phys {
     double M0, wc, W, sigma, beta, kappa, taul, tauh, rhol, rhoh, gy;
+    // quotients of the constants above, computed on the host with the same
+    // IEEE division the kernels would use: (rhoh - rhol) / 3, (rhoh - rhol) / 6,
+    // 1 / tauh, 1 / taul
+    double drho3, drho6, inv_tauh, inv_taul;
 };
 
 // The same constants for the FP32 path, with the quotients it uses as

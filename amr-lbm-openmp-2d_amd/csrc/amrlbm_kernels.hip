@@ -146,7 +146,7 @@ __device__ __forceinline__ void mrt(double s9, const double geq[9], double g[9])
 __device__ __forceinline__ void collide_cell(const double *sC, int i, int j, int dx, double P, double Ux, double Uy,
                                              const Phys &ph, double h[9], double g[9])
 {
-    const double dRho3 = (ph.rhoh - ph.rhol) / 3.0;
+    const double dRho3 = ph.drho3;
     double e[9], hlp[9], Gamma[9], heq[9], geq[9], gx, gy, ni, nj;
     double mu = chem_pot(sC, i, j, dx, ph);
     double c = sC[cidx(i, j)];
@@ -172,7 +172,7 @@ __device__ __forceinline__ void collide_cell(const double *sC, int i, int j, int
     else if (c > 1.0)
         tau = ph.tauh + 0.5;
     else {
-        tau = c * (1.0 / ph.tauh - 1.0 / ph.taul) + 1.0 / ph.taul;
+        tau = c * (ph.inv_tauh - ph.inv_taul) + ph.inv_taul;
         tau = 1.0 / tau + 0.5;
     }
     mrt(1.0 / tau, geq, g);
@@ -184,7 +184,7 @@ __device__ __forceinline__ void collide_cell(const double *sC, int i, int j, int
 __device__ __forceinline__ void macro_g_cell(const double *sC, int i, int j, int dx, const double g[9], const Phys &ph,
                                              double &P, double &Ux, double &Uy)
 {
-    const double dRho6 = (ph.rhoh - ph.rhol) / 6.0;
+    const double dRho6 = ph.drho6;
     const double inv_dx = 1.0 / (double)dx;   // exact: dx is a power of two
     double gx = ((sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)]) / 3.0 +
                  (sC[cidx(i + 1, j + 1)] + sC[cidx(i + 1, j - 1)] - sC[cidx(i - 1, j + 1)] - sC[cidx(i - 1, j - 1)]) /

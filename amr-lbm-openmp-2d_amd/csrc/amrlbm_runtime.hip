@@ -1111,6 +1111,10 @@ static int create_mesh(const amrlbm_params *p, bool f32, amrlbm_mesh **out)
     m->ph.rhol = p->rhol;
     m->ph.rhoh = p->rhoh;
     m->ph.gy = p->gy;
+    m->ph.drho3 = (p->rhoh - p->rhol) / 3.0;
+    m->ph.drho6 = (p->rhoh - p->rhol) / 6.0;
+    m->ph.inv_tauh = 1.0 / p->tauh;
+    m->ph.inv_taul = 1.0 / p->taul;
     m->dm.f32 = f32 ? 1 : 0;
     PhysF &f = m->dm.phf;
     f.M0 = (float)p->M0;
@@ -1975,10 +1979,16 @@ struct OneBlock {
     }
     int init(const amrlbm_params *p, int dx)
     {
-        if (!p || p->nx != NX || p->ny != NY || dx < 1) return fail(AMRLBM_ERR_ARG, "bad per-block arguments");
+        // dx is L0 / nx / 2^Level, a power of two (the kernels divide by it as a product)
+        if (!p || p->nx != NX || p->ny != NY || dx < 1 || (dx & (dx - 1)))
+            return fail(AMRLBM_ERR_ARG, "bad per-block arguments (16 x 16 blocks, dx a power of two)");
         ph.M0 = p->M0; ph.wc = 1.0 / (0.5 + p->M0 * 3.0); ph.W = p->W; ph.sigma = p->sigma; ph.beta = p->beta;
         ph.kappa = p->kappa; ph.taul = p->taul; ph.tauh = p->tauh; ph.rhol = p->rhol; ph.rhoh = p->rhoh;
         ph.gy = p->gy;
+        ph.drho3 = (p->rhoh - p->rhol) / 3.0;
+        ph.drho6 = (p->rhoh - p->rhol) / 6.0;
+        ph.inv_tauh = 1.0 / p->tauh;
+        ph.inv_taul = 1.0 / p->taul;
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         HIPCHK(hipMalloc(&dm.pop[0], sizeof(double) * POPB));
         HIPCHK(hipMalloc(&dm.pop[1], sizeof(double) * POPB));
