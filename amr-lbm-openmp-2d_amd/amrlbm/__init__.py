@@ -72,6 +72,7 @@ def lib():
             "amrlbm_destroy": [vp],
             "amrlbm_setup_rising_bubble": [vp, dp, dp, dp],
             "amrlbm_setup_replicas": [vp, ip, dp, dp, dp],
+            "amrlbm_setup_bubbles": [vp, ip, _DP, _DP, _DP],
             "amrlbm_get_counts": [vp, _IP],
             "amrlbm_get_topology": [vp, _IP, _IP],
             "amrlbm_set_topology": [vp, ip, _IP, ip, ip, _IP, ip],
@@ -230,6 +231,11 @@ class Mesh:
             _check(lib().amrlbm_setup_replicas(self.h, w.instances, R, x, y))
         else:
             _check(lib().amrlbm_setup_rising_bubble(self.h, R, x, y))
+
+    def setup_bubbles(self, R, x0, y0):
+        """Device-resident set-up of several bubbles (amrlbm_setup_bubbles)."""
+        R, x0, y0 = (np.ascontiguousarray(v, dtype=np.float64) for v in (R, x0, y0))
+        _check(lib().amrlbm_setup_bubbles(self.h, int(R.size), _d(R), _d(x0), _d(y0)))
 
     def counts(self):
         c = np.zeros(7, dtype=np.int32)

@@ -176,8 +176,8 @@ void launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, in
                          hipStream_t s);
 void launch_restrict(const DevMesh &m, int cur, const int *parents, int n, hipStream_t s);
 void launch_prolong(const DevMesh &m, int cur, const int *pairs, int n, const Phys &ph, hipStream_t s);
-void launch_init_fields(const DevMesh &m, int cur, const int *list, int n, const Phys &ph, double R,
-                        hipStream_t s);
+void launch_init_fields(const DevMesh &m, int cur, const int *list, int n, const Phys &ph, double R, hipStream_t s,
+                        const double *bub = nullptr, int nb = 0);
 void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, double *partial, double *out,
                  hipStream_t s);
 

@@ -1016,21 +1016,50 @@ __global__ __launch_bounds__(256) void k_prolong(DevMeshT<T> m, int cur, const i
 
 // Initialize_AMR_Rising_Bubble (AMR_Rising_2D_OMP.f90:186-225): with C set
 // (both rings), U = 0, P = -sigma/R C, h = heq, g = geq with Fb = 0.
-// Computed in FP64 for either storage.
+// Computed in FP64 for either storage. With nb > 0 bubbles (x0, y0, R per
+// bubble, finest cells) C itself is set here first, on both ghost rings:
+// 0.5 + 0.5 tanh(2 d / W) with d the distance to the nearest bubble surface,
+// min_k(|x - x_k| - R_k), and R in P is the nearest bubble's; else C comes
+// from memory and R is the argument.
 template <class T>
 __global__ __launch_bounds__(256) void k_init_fields(DevMeshT<T> m, int cur, const int *__restrict__ list, Phys ph,
-                                                     double R)
+                                                     double R, const double *__restrict__ bub, int nb)
 {
     const int slot = list[blockIdx.x];
     const int tid = threadIdx.x;
     __shared__ double sC[CS];
-    const T *C = m.C + (size_t)slot * CS;
-    for (int t = tid; t < CS; t += 256) sC[t] = (double)C[t] + c_shift<T>();
+    __shared__ double sR[CS];
+    T *C = m.C + (size_t)slot * CS;
+    const BlockDev b = m.blk[slot];
+    if (nb > 0) {
+        for (int t = tid; t < CS; t += 256) {
+            const int x = t % CW - 1, y = t / CW - 1;
+            // cell centre as the host set-up computes it (bubble_c)
+            const double Xc = (double)(b.I * b.dx * NX) + (double)b.dx * ((double)x - 0.5);
+            const double Yc = (double)(b.J * b.dx * NY) + (double)b.dx * ((double)y - 0.5);
+            double dmin = 0.0, rk = 0.0;
+            for (int q = 0; q < nb; q++) {
+                const double dxq = Xc - bub[3 * q], dyq = Yc - bub[3 * q + 1];
+                const double d = sqrt(dxq * dxq + dyq * dyq) - bub[3 * q + 2];
+                if (q == 0 || d < dmin) {
+                    dmin = d;
+                    rk = bub[3 * q + 2];
+                }
+            }
+            const double c = 0.5 + 0.5 * tanh(2.0 * dmin / ph.W);
+            sC[t] = c;
+            sR[t] = rk;
+            C[t] = (T)(c - c_shift<T>());
+        }
+    } else {
+        for (int t = tid; t < CS; t += 256) sC[t] = (double)C[t] + c_shift<T>();
+    }
     __syncthreads();
-    const int i = tid % NX + 1, j = tid / NX + 1, dx = m.blk[slot].dx;
+    const int i = tid % NX + 1, j = tid / NX + 1, dx = b.dx;
     const double dRho3 = (ph.rhoh - ph.rhol) / 3.0;
     double c = sC[cidx(i, j)];
-    double P = -(ph.sigma / R * c), Ux = 0.0, Uy = 0.0;
+    const double Rc = nb > 0 ? sR[cidx(i, j)] : R;
+    double P = -(ph.sigma / Rc * c), Ux = 0.0, Uy = 0.0;
     T *PU = m.PU + (size_t)slot * PUB;
     PU[pidx(i, j)] = (T)P;
     PU[PS + pidx(i, j)] = (T)Ux;
@@ -1211,13 +1240,15 @@ void launch_prolong(const DevMesh &m, int cur, const int *pairs, int n, const Ph
         hipLaunchKernelGGL(k_prolong<double>, dim3(n), dim3(256), 0, s, typed<double>(m), cur, pairs, ph, m.phf);
 }
 
-void launch_init_fields(const DevMesh &m, int cur, const int *list, int n, const Phys &ph, double R, hipStream_t s)
+void launch_init_fields(const DevMesh &m, int cur, const int *list, int n, const Phys &ph, double R, hipStream_t s,
+                        const double *bub, int nb)
 {
     if (n <= 0) return;
     if (m.f32)
-        hipLaunchKernelGGL(k_init_fields<float>, dim3(n), dim3(256), 0, s, typed<float>(m), cur, list, ph, R);
+        hipLaunchKernelGGL(k_init_fields<float>, dim3(n), dim3(256), 0, s, typed<float>(m), cur, list, ph, R, bub, nb);
     else
-        hipLaunchKernelGGL(k_init_fields<double>, dim3(n), dim3(256), 0, s, typed<double>(m), cur, list, ph, R);
+        hipLaunchKernelGGL(k_init_fields<double>, dim3(n), dim3(256), 0, s, typed<double>(m), cur, list, ph, R, bub,
+                           nb);
 }
 
 void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, double *partial, double *out, hipStream_t s)

@@ -1185,7 +1185,12 @@ int amrlbm_setup_rising_bubble(amrlbm_mesh *m, double R, double x0, double y0)
     return amrlbm_setup_replicas(m, 1, R, x0, y0);
 }
 
-int amrlbm_setup_replicas(amrlbm_mesh *m, int ninst, double R, double x0, double y0)
+}  // extern "C"
+
+// AMR_LBM_Rising_Bubble set-up (AMR_Rising_2D_OMP.f90:51-67) with init(m)
+// setting the fields of every leaf (Initialize_AMR_Rising_Bubble)
+template <class Init>
+static int setup_mesh(amrlbm_mesh *m, int ninst, Init init)
 {
     if (!m) return fail(AMRLBM_ERR_ARG, "null mesh");
     if (ninst < 1 || m->prm.nroots % ninst)
@@ -1222,12 +1227,12 @@ int amrlbm_setup_replicas(amrlbm_mesh *m, int ninst, double R, double x0, double
             }
     }
     array_of_leaf_blocks(m);
-    int rc = init_bubble(m, R, x0, y0);
+    int rc = init(m);
     if (rc) return rc;
     while (m->levelmax < p.max_levels) {   // AMR_Rising_2D_OMP.f90:63-67
         rc = regrid(m, 0.002, 0.001);
         if (rc) return rc;
-        rc = init_bubble(m, R, x0, y0);
+        rc = init(m);
         if (rc) return rc;
         m->levelmax++;
     }
@@ -1235,6 +1240,38 @@ int amrlbm_setup_replicas(amrlbm_mesh *m, int ninst, double R, double x0, double
     determine_min_max_levels(m);
     m->ready = true;
     return check_err_flags(m);
+}
+
+extern "C" {
+
+int amrlbm_setup_replicas(amrlbm_mesh *m, int ninst, double R, double x0, double y0)
+{
+    return setup_mesh(m, ninst, [=](amrlbm_mesh *mm) { return init_bubble(mm, R, x0, y0); });
+}
+
+int amrlbm_setup_bubbles(amrlbm_mesh *m, int n, const double *R, const double *x0, const double *y0)
+{
+    if (!m || n < 1 || !R || !x0 || !y0) return fail(AMRLBM_ERR_ARG, "bad bubble arguments");
+    std::vector<double> bub(3 * (size_t)n);
+    for (int k = 0; k < n; k++) {
+        if (!(R[k] > 0.0)) return fail(AMRLBM_ERR_ARG, "bubble radius must be positive");
+        bub[3 * k] = x0[k];
+        bub[3 * k + 1] = y0[k];
+        bub[3 * k + 2] = R[k];
+    }
+    double *d_bub = nullptr;
+    HIPCHK(hipMalloc(&d_bub, sizeof(double) * bub.size()));
+    HIPCHK(hipMemcpy(d_bub, bub.data(), sizeof(double) * bub.size(), hipMemcpyHostToDevice));
+    int rc = setup_mesh(m, 1, [&](amrlbm_mesh *mm) {
+        int r2 = upload_tables(mm);
+        if (r2) return r2;
+        launch_init_fields(mm->dm, mm->cur, mm->d_lists + mm->all_off, nleaves(mm), mm->ph, 0.0, mm->stream, d_bub, n);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(mm->stream));
+        return AMRLBM_OK;
+    });
+    (void)hipFree(d_bub);
+    return rc;
 }
 
 int amrlbm_get_counts(amrlbm_mesh *m, int counts[7])

@@ -69,6 +69,14 @@ int amrlbm_setup_rising_bubble(amrlbm_mesh *m, double R, double x0, double y0);
  * replicated-instance workload for measurements at a large working set.
  * ninst = 1 is amrlbm_setup_rising_bubble. */
 int amrlbm_setup_replicas(amrlbm_mesh *m, int ninst, double R, double x0, double y0);
+/* Device-resident set-up of n bubbles in one domain (radius R[k], centre
+ * (x0[k], y0[k]), finest cells): the same roots, Initial_Mesh and
+ * init-regrid loop as amrlbm_setup_rising_bubble, with C, its ghost rings and
+ * the initial fields evaluated on the GPU, C = 0.5 + 0.5 tanh(2 d / W), d the
+ * distance to the nearest bubble surface. No counterpart in the reference
+ * (one bubble, profile evaluated on the host: amrlbm_setup_rising_bubble keeps
+ * that, with the host's tanh, for bit-exact parity). */
+int amrlbm_setup_bubbles(amrlbm_mesh *m, int n, const double *R, const double *x0, const double *y0);
 
 /* ---- state exchange with the caller's Blk(:) mirror ----------------------- */
 /* counts: Nblks, nLeaf, LevelMin, LevelMax, KK, free-list length, step T */
