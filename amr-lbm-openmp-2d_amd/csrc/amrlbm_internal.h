@@ -49,8 +49,9 @@ struct Phys {
     double M0, wc, W, sigma, beta, kappa, taul, tauh, rhol, rhoh, gy;
     // quotients of the constants above, computed on the host with the same
     // IEEE division the kernels would use: (rhoh - rhol) / 3, (rhoh - rhol) / 6,
-    // 1 / tauh, 1 / taul
-    double drho3, drho6, inv_tauh, inv_taul;
+    // 1 / tauh, 1 / taul, and the correctly rounded reciprocal 1 / W that the
+    // kernels' Markstein division by W takes
+    double drho3, drho6, inv_tauh, inv_taul, inv_W;
 };
 
 // The same constants for the FP32 path, with the quotients it uses as

@@ -43,13 +43,34 @@ __device__ __forceinline__ int pidx(int i, int j) { return (i - 1) + NX * (j - 1
 // Cell-level numerics (LBM_Subroutines.f90)
 // ---------------------------------------------------------------------------
 
+// x / d, correctly rounded, from y = RN(1 / d) by two Markstein corrections
+// (P. W. Markstein, IBM J. Res. Dev. 34(1), 1990, and Muller et al., Handbook
+// of Floating-Point Arithmetic, Markstein's theorem): RN(x y) is within 1.5
+// ulp of x / d; one correction q + (x - d q) y makes it faithful (within one
+// ulp); with a faithful q the residual x - d q is exact in an FMA and
+// RN(q + (x - d q) y) is RN(x / d). The residual is taken negated, d q - x
+// (RN is odd, so the same value up to sign), which keeps the sign of a zero x.
+// One multiply and four FMAs instead of the generic eleven-instruction
+// division, with the same result for x = +-0 and every 2^-969 <= |x| < 2^1020
+// (below, the residual can fall under the subnormal grid); NaN stays NaN,
+// +-Inf gives NaN.
+__device__ __forceinline__ double div_rn(double x, double d, double y)
+{
+    double q = x * y;
+    double r = fma(q, d, -x);
+    q = fma(-r, y, q);
+    r = fma(q, d, -x);
+    return fma(-r, y, q);
+}
+template <int D> __device__ __forceinline__ double div_c(double x) { return div_rn(x, (double)D, 1.0 / (double)D); }
+
 // Laplacian_AMR (LBM_Subroutines.f90:316-333) + Chemical_Potential_AMR (:94-112)
 __device__ __forceinline__ double chem_pot(const double *sC, int i, int j, int dx, const Phys &ph)
 {
-    double d2c = (sC[cidx(i - 1, j - 1)] + sC[cidx(i + 1, j - 1)] + sC[cidx(i - 1, j + 1)] + sC[cidx(i + 1, j + 1)] +
-                  4.0 * (sC[cidx(i, j - 1)] + sC[cidx(i - 1, j)] + sC[cidx(i + 1, j)] + sC[cidx(i, j + 1)]) -
-                  20.0 * sC[cidx(i, j)]) /
-                 6.0;
+    double d2c = div_c<6>(sC[cidx(i - 1, j - 1)] + sC[cidx(i + 1, j - 1)] + sC[cidx(i - 1, j + 1)] +
+                          sC[cidx(i + 1, j + 1)] +
+                          4.0 * (sC[cidx(i, j - 1)] + sC[cidx(i - 1, j)] + sC[cidx(i + 1, j)] + sC[cidx(i, j + 1)]) -
+                          20.0 * sC[cidx(i, j)]);
     double c = sC[cidx(i, j)];
     // x / dx^2 as x * (1 / dx^2): dx is a power of two, so both are the correctly
     // rounded x / dx^2 (the same bits, subnormals included)
@@ -59,8 +80,8 @@ __device__ __forceinline__ double chem_pot(const double *sC, int i, int j, int d
 // Gradient_ED_AMR (LBM_Subroutines.f90:281-289)
 __device__ __forceinline__ void gradient_ed(const double e[9], double &gx, double &gy)
 {
-    gx = (e[1] - e[3]) / 3.0 + (e[5] - e[6] - e[7] + e[8]) / 12.0;
-    gy = (e[2] - e[4]) / 3.0 + (e[5] + e[6] - e[7] - e[8]) / 12.0;
+    gx = div_c<3>(e[1] - e[3]) + div_c<12>(e[5] - e[6] - e[7] + e[8]);
+    gy = div_c<3>(e[2] - e[4]) + div_c<12>(e[5] + e[6] - e[7] - e[8]);
 }
 
 // ED_C_AMR (LBM_Subroutines.f90:244-258); / dx as * (1 / dx), exact for the
@@ -88,7 +109,7 @@ __device__ __forceinline__ void ed_m(const double *sC, int i, int j, int dx, dou
 __device__ __forceinline__ void equilibrium(double C, double U, double V, double ni, double nj, const Phys &ph,
                                             double Gamma[9], double heq[9])
 {
-    double tmp = ((12.0 * (1.0 - C)) / ph.W) * ph.M0;
+    double tmp = div_rn(12.0 * (1.0 - C), ph.W, ph.inv_W) * ph.M0;
     double U2 = U * U + V * V;
 #pragma unroll
     for (int i = 0; i < 9; i++) {
@@ -139,7 +160,7 @@ __device__ __forceinline__ void mrt(double s9, const double geq[9], double g[9])
     o[7] = 4.0 * t[0] + 2.0 * t[1] + t[2] - 6.0 * t[3] - 3.0 * t[4] - 6.0 * t[5] - 3.0 * t[6] + 9.0 * t[8];
     o[8] = 4.0 * t[0] + 2.0 * t[1] + t[2] + 6.0 * t[3] + 3.0 * t[4] - 6.0 * t[5] - 3.0 * t[6] - 9.0 * t[8];
 #pragma unroll
-    for (int i = 0; i < 9; i++) g[i] = g[i] - o[i] / 36.0;
+    for (int i = 0; i < 9; i++) g[i] = g[i] - div_c<36>(o[i]);
 }
 
 // Collision_h_g_AMR (LBM_Subroutines.f90:116-189) for cell (i,j); h, g in registers
@@ -162,7 +183,7 @@ __device__ __forceinline__ void collide_cell(const double *sC, int i, int j, int
     double Fb = (ph.rhoh - Rho) * ph.gy;
     forcing(Gamma, dRho3, mu, e, Ux, Uy, gx, gy, Fb, hlp);
 #pragma unroll
-    for (int k = 0; k < 9; k++) geq[k] = P * wa(k) + (Gamma[k] - wa(k)) * Rho / 3.0 - 0.5 * hlp[k];
+    for (int k = 0; k < 9; k++) geq[k] = P * wa(k) + div_c<3>((Gamma[k] - wa(k)) * Rho) - 0.5 * hlp[k];
     ed_m(sC, i, j, dx, e);
     gradient_ed(e, gx, gy);
     forcing(Gamma, dRho3, mu, e, Ux, Uy, gx, gy, Fb, hlp);
@@ -186,13 +207,13 @@ __device__ __forceinline__ void macro_g_cell(const double *sC, int i, int j, int
 {
     const double dRho6 = ph.drho6;
     const double inv_dx = 1.0 / (double)dx;   // exact: dx is a power of two
-    double gx = ((sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)]) / 3.0 +
-                 (sC[cidx(i + 1, j + 1)] + sC[cidx(i + 1, j - 1)] - sC[cidx(i - 1, j + 1)] - sC[cidx(i - 1, j - 1)]) /
-                     12.0) *
+    double gx = (div_c<3>(sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)]) +
+                 div_c<12>(sC[cidx(i + 1, j + 1)] + sC[cidx(i + 1, j - 1)] - sC[cidx(i - 1, j + 1)] -
+                           sC[cidx(i - 1, j - 1)])) *
                 inv_dx;
-    double gy = ((sC[cidx(i, j + 1)] - sC[cidx(i, j - 1)]) / 3.0 +
-                 (sC[cidx(i + 1, j + 1)] + sC[cidx(i - 1, j + 1)] - sC[cidx(i + 1, j - 1)] - sC[cidx(i - 1, j - 1)]) /
-                     12.0) *
+    double gy = (div_c<3>(sC[cidx(i, j + 1)] - sC[cidx(i, j - 1)]) +
+                 div_c<12>(sC[cidx(i + 1, j + 1)] + sC[cidx(i - 1, j + 1)] - sC[cidx(i + 1, j - 1)] -
+                           sC[cidx(i - 1, j - 1)])) *
                 inv_dx;
     double mu = chem_pot(sC, i, j, dx, ph);
     double Rho = ph.rhol + sC[cidx(i, j)] * (ph.rhoh - ph.rhol);

@@ -1115,6 +1115,7 @@ static int create_mesh(const amrlbm_params *p, bool f32, amrlbm_mesh **out)
     m->ph.drho6 = (p->rhoh - p->rhol) / 6.0;
     m->ph.inv_tauh = 1.0 / p->tauh;
     m->ph.inv_taul = 1.0 / p->taul;
+    m->ph.inv_W = 1.0 / p->W;
     m->dm.f32 = f32 ? 1 : 0;
     PhysF &f = m->dm.phf;
     f.M0 = (float)p->M0;
@@ -2026,6 +2027,7 @@ struct OneBlock {
         ph.drho6 = (p->rhoh - p->rhol) / 6.0;
         ph.inv_tauh = 1.0 / p->tauh;
         ph.inv_taul = 1.0 / p->taul;
+        ph.inv_W = 1.0 / p->W;
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         HIPCHK(hipMalloc(&dm.pop[0], sizeof(double) * POPB));
         HIPCHK(hipMalloc(&dm.pop[1], sizeof(double) * POPB));
