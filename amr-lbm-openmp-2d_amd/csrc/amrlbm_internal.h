@@ -149,6 +149,8 @@ struct PHdr {
 struct Prog {
     const PHdr *hdr;          // one per leaf, leaf-list order
     const PNode *nodes;
+    int naux = 0;             // most aux cells of one leaf: a population's LDS stride is FS + naux
+    int nmax = 0;             // most recipes of one leaf
 };
 
 // kernel launchers (amrlbm_kernels.hip); all enqueue on stream s

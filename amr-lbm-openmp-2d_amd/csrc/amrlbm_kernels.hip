@@ -427,6 +427,10 @@ template <class T> __device__ __forceinline__ double c_shift() { return std::is_
 // from interior data of the previous kernel, the coarse neighbours' ghost
 // cells included, so one launch covers every level.
 // ---------------------------------------------------------------------------
+// byte offset of the recipes in the hg kernel's LDS, after 9 population
+// arrays of lcf cells of esz bytes (16-byte aligned)
+__host__ __device__ constexpr size_t hg_nodes_offset(int lcf, size_t esz) { return (9 * (size_t)lcf * esz + 15) & ~size_t(15); }
+
 // Ghost cell t of the ring-1 population block: sides L,R,B,T then the corners.
 __device__ __forceinline__ int ring1_cell(int t)
 {
@@ -450,6 +454,18 @@ __device__ __forceinline__ void wpat(int p, T w[3])
     w[0] = p == 0 ? T(3) : (p == 1 ? T(-1) : (p == 2 ? T(5) : T(-3)));
     w[1] = p < 2 ? T(6) : T(30);
     w[2] = p == 0 ? T(-1) : (p == 1 ? T(3) : (p == 2 ? T(-3) : T(5)));
+}
+
+// end of stage s of a ghost program. A select over the constant indices
+// keeps the header in registers; indexing end[] with a variable would place
+// the header in per-lane scratch memory, written and read back through L2.
+__device__ __forceinline__ int stage_end(const PHdr &h, int s)
+{
+    int e = 0;
+#pragma unroll
+    for (int t = 0; t < MAX_STAGES; t++)
+        if (s == t) e = h.end[t];
+    return e;
 }
 
 // node load as three 16-byte loads
@@ -529,19 +545,20 @@ __device__ __forceinline__ T eval_global(const PNode &nd, const T *__restrict__ 
     return type == NT_FINE ? o / T(64) : o / T(1024);
 }
 
-// population q of set `set` for one recipe; sF holds that set's 9 arrays.
+// population q of set `set` for one recipe; sF holds that set's 9 arrays,
+// lcf cells apart.
 // Bounce-back rows take the opposite population of the first row
 // (Bounce_Back_3/4, BC_Cell.f90:2-42).
 template <bool GLOBAL_ONLY, class T>
-__device__ __forceinline__ T eval_hg(const PNode &nd, int set, int q, const T *__restrict__ Fc, const T *sF)
+__device__ __forceinline__ T eval_hg(const PNode &nd, int set, int q, const T *__restrict__ Fc, const T *sF, int lcf)
 {
     const int type = nd.info & 15;
     const int c = set * 9 + q;
-    if (type < NT_BB_BOT) return GLOBAL_ONLY ? eval_global(nd, Fc, c * GS) : eval_node(nd, Fc, c * GS, sF, q * LC_F);
+    if (type < NT_BB_BOT) return GLOBAL_ONLY ? eval_global(nd, Fc, c * GS) : eval_node(nd, Fc, c * GS, sF, q * lcf);
     const int e = c_ex[q] * c_ey[q];
     const int r = nd.ref[1 + (type == NT_BB_TOP ? -e : e)];
     const int qo = c_opp[q];
-    return (GLOBAL_ONLY || r >= 0) ? Fc[r + (set * 9 + qo) * GS] : sF[-1 - r + qo * LC_F];
+    return (GLOBAL_ONLY || r >= 0) ? Fc[r + (set * 9 + qo) * GS] : sF[-1 - r + qo * lcf];
 }
 
 // One stage of an hg ghost program for one population set: recipes [b, e)
@@ -550,7 +567,7 @@ __device__ __forceinline__ T eval_hg(const PNode &nd, int set, int q, const T *_
 // population array; U items per thread have their operands in flight together.
 template <bool GLOBAL_ONLY, int U, class T>
 __device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, int set, const T *__restrict__ Fc, T *sF,
-                                         T *F, bool store, int tid)
+                                         int lcf, T *F, bool store, int tid)
 {
     const int ns = e - b;
     for (int base = 0; base < 9 * ns; base += 256 * U) {
@@ -560,7 +577,7 @@ __device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, int set,
             const int t = base + tid + 256 * r;
             if (t < 9 * ns) {
                 const int q = t / ns;
-                v[r] = eval_hg<GLOBAL_ONLY>(sN[b + t - q * ns], set, q, Fc, sF);
+                v[r] = eval_hg<GLOBAL_ONLY>(sN[b + t - q * ns], set, q, Fc, sF, lcf);
             }
         }
 #pragma unroll
@@ -569,7 +586,7 @@ __device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, int set,
             if (t < 9 * ns) {
                 const int q = t / ns;
                 const PNode &nd = sN[b + t - q * ns];
-                sF[q * LC_F + nd.dest] = v[r];
+                sF[q * lcf + nd.dest] = v[r];
                 if (store && nd.own >= 0) F[(set * 9 + q) * GS + nd.own] = v[r];
             }
         }
@@ -621,7 +638,7 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMeshT<T> m, int cur, const in
     }
     // this thread's first recipe; first-stage recipes read only interior
     // values in memory, so they are done before the first barrier
-    const int total = hd.nst > 0 ? hd.end[hd.nst - 1] : 0;
+    const int total = hd.nst > 0 ? stage_end(hd, hd.nst - 1) : 0;
     const int n1 = hd.nst > 0 ? hd.end[0] : 0;
     PNode nd0;
     T gv[9];
@@ -644,7 +661,7 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMeshT<T> m, int cur, const in
     if (fill) {
         int begin = 0;
         for (int s = 0; s < hd.nst; s++) {
-            const int end = hd.end[s];
+            const int end = stage_end(hd, s);
             if (s > 0 && tid >= begin && tid < end) {
                 const T v = eval_pref(nd0, gv, (const T *)sC, 0);
                 sC[nd0.dest] = v;
@@ -697,13 +714,14 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMeshT<T> m, int cur, const in
     }
 }
 
-// LDS layout of a population block: component c, cell (x,y) at c*LC_F + y*FW + x
-__device__ __forceinline__ int lidx(int c, int x, int y) { return c * LC_F + y * FW + x; }
+// LDS layout of a population block: component c, cell (x,y) at c*lcf + y*FW + x
+// (lcf = FS + the aux cells of the launch's largest ghost program)
+__device__ __forceinline__ int lidx(int c, int x, int y, int lcf) { return c * lcf + y * FW + x; }
 
 // Perfect_Shift_AMR / Streaming_LaxWendroff_AMR (LBM_Subroutines.f90:41-90)
 // of one cell from the LDS copy of the block into G, and C = sum(h) (:193-201)
 template <class T>
-__device__ __forceinline__ T stream_cell(const T *sF, T *__restrict__ G, int set, int i, int j, int dx)
+__device__ __forceinline__ T stream_cell(const T *sF, int lcf, T *__restrict__ G, int set, int i, int j, int dx)
 {
     const T cfl = T(1) / (T)dx;
     const T a = cfl / T(2) * (T(1) + cfl), b = T(1) - cfl * cfl, c = cfl / T(2) * (T(1) - cfl);
@@ -712,12 +730,12 @@ __device__ __forceinline__ T stream_cell(const T *sF, T *__restrict__ G, int set
     for (int q = 0; q < 9; q++) {
         T v;
         if (q == 0)
-            v = sF[lidx(q, i, j)];
+            v = sF[lidx(q, i, j, lcf)];
         else if (cfl > T(0.9))
-            v = sF[lidx(q, i - c_ex[q], j - c_ey[q])];
+            v = sF[lidx(q, i - c_ex[q], j - c_ey[q], lcf)];
         else
-            v = a * sF[lidx(q, i - c_ex[q], j - c_ey[q])] + b * sF[lidx(q, i, j)] -
-                c * sF[lidx(q, i + c_ex[q], j + c_ey[q])];
+            v = a * sF[lidx(q, i - c_ex[q], j - c_ey[q], lcf)] + b * sF[lidx(q, i, j, lcf)] -
+                c * sF[lidx(q, i + c_ex[q], j + c_ey[q], lcf)];
         G[fidx(set, q, i, j)] = v;
         s += v;
     }
@@ -755,21 +773,25 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMeshT<T> m, int cur, const i
     const BlockDev me = m.blk[slot];
     const T *__restrict__ Fc = m.pop[cur];
     T *F = m.pop[cur] + (size_t)slot * POPB;
-    __shared__ T sF[9 * LC_F];
-    __shared__ PNode sN[NODES_HG];
+    // LDS sized at launch (hg_lds_bytes): the 9 populations of the set, FS
+    // cells plus the program's aux cells each, then the block's recipes
+    extern __shared__ __align__(16) unsigned char hg_lds[];
+    const int lcf = FS + prog.naux;
+    T *sF = reinterpret_cast<T *>(hg_lds);
+    PNode *sN = reinterpret_cast<PNode *>(hg_lds + hg_nodes_offset(lcf, sizeof(T)));
     const int i = tid % NX + 1, j = tid / NX + 1;
-    const int nn = hd.nst > 0 ? hd.end[hd.nst - 1] : 0;
+    const int nn = hd.nst > 0 ? stage_end(hd, hd.nst - 1) : 0;
     {   // the block's recipes, staged in LDS
         const int4 *src = reinterpret_cast<const int4 *>(prog.nodes + hd.off);
         int4 *dst = reinterpret_cast<int4 *>(sN);
         for (int t = tid; t < 3 * nn; t += 256) dst[t] = src[t];
     }
 #pragma unroll
-    for (int q = 0; q < 9; q++) sF[lidx(q, i, j)] = F[fidx(set, q, i, j)];
+    for (int q = 0; q < 9; q++) sF[lidx(q, i, j, lcf)] = F[fidx(set, q, i, j)];
     if (!(flags & 1)) {
         for (int t = tid; t < 68 * 9; t += 256) {   // the ring is stored in ring1_cell order
             int q = t / 68, r = t - q * 68;
-            sF[q * LC_F + ring1_cell(r)] = F[(set * 9 + q) * GS + NX * NY + r];
+            sF[q * lcf + ring1_cell(r)] = F[(set * 9 + q) * GS + NX * NY + r];
         }
     }
     __syncthreads();
@@ -777,22 +799,22 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMeshT<T> m, int cur, const i
     // also read cells of earlier stages
     int begin = 0;
     for (int s = 0; s < hd.nst; s++) {
-        const int end = hd.end[s];
+        const int end = stage_end(hd, s);
         if (s == 0)
-            hg_stage<true, 4>(sN, begin, end, set, Fc, sF, F, (flags & 16) != 0, tid);
+            hg_stage<true, 4>(sN, begin, end, set, Fc, sF, lcf, F, (flags & 16) != 0, tid);
         else
-            hg_stage<false, 2>(sN, begin, end, set, Fc, sF, F, (flags & 16) != 0, tid);
+            hg_stage<false, 2>(sN, begin, end, set, Fc, sF, lcf, F, (flags & 16) != 0, tid);
         begin = end;
         __syncthreads();
     }
     if (!(flags & 2)) return;
     T *G = m.pop[cur ^ 1] + (size_t)slot * POPB;
-    const T csum = stream_cell((const T *)sF, G, set, i, j, me.dx);
+    const T csum = stream_cell((const T *)sF, lcf, G, set, i, j, me.dx);
     if ((flags & 4) && set == 0) m.C[(size_t)slot * CS + cidx(i, j)] = csum;
     if (flags & 8) {
         for (int t = tid; t < 68 * 9; t += 256) {
             int q = t / 68, r = t - q * 68;
-            G[(set * 9 + q) * GS + NX * NY + r] = sF[q * LC_F + ring1_cell(r)];
+            G[(set * 9 + q) * GS + NX * NY + r] = sF[q * lcf + ring1_cell(r)];
         }
     }
 }
@@ -1210,10 +1232,14 @@ void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Pro
     int flags = (prog.hdr ? 1 : 0) | (stream_after ? 2 : 0) | (macro_h ? 4 : 0) | (copy_ghosts ? 8 : 0) |
                 (prog.hdr && !stream_after ? 16 : 0);
     const dim3 grid(16 * ((n + 7) / 8));   // two sets per list entry, entries padded to a multiple of 8
+    // LDS for the launch's largest program only: fewer aux cells, more workgroups per CU
+    const int lcf = FS + prog.naux;
+    const size_t esz = m.f32 ? sizeof(float) : sizeof(double);
+    const size_t lds = hg_nodes_offset(lcf, esz) + (size_t)prog.nmax * sizeof(PNode);
     if (m.f32)
-        hipLaunchKernelGGL(k_prog_hg<float>, grid, dim3(256), 0, s, typed<float>(m), cur, list, n, prog, flags);
+        hipLaunchKernelGGL(k_prog_hg<float>, grid, dim3(256), lds, s, typed<float>(m), cur, list, n, prog, flags);
     else
-        hipLaunchKernelGGL(k_prog_hg<double>, grid, dim3(256), 0, s, typed<double>(m), cur, list, n, prog, flags);
+        hipLaunchKernelGGL(k_prog_hg<double>, grid, dim3(256), lds, s, typed<double>(m), cur, list, n, prog, flags);
 }
 
 void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream_t s)

@@ -425,6 +425,7 @@ struct ProgBuild {
 // Ghost program of one leaf for one fill kind, nodes grouped by stage.
 struct LeafProg {
     std::vector<PNode> nodes;
+    int naux = 0;
     int nst = 0;
     int end[MAX_STAGES] = {};
     std::string err;
@@ -474,6 +475,7 @@ void build_leaf_program(amrlbm_mesh *m, int kind, int n, LeafProg &out)
         return;
     }
     out.nst = nst;
+    out.naux = pb.naux;
     out.nodes.reserve(pb.nodes.size());
     for (int st = 1; st <= nst; st++) {
         for (size_t i = 0; i < pb.nodes.size(); i++)
@@ -496,7 +498,13 @@ int build_programs(amrlbm_mesh *m)
         if (!lp.err.empty()) return fail(AMRLBM_ERR_TOPOLOGY, lp.err);
     for (int kind = 0; kind < 3; kind++) {
         size_t total = 0;
-        for (int k = 0; k < nl; k++) total += progs[kind * nl + k].nodes.size();
+        int naux = 0, nmax = 0;
+        for (int k = 0; k < nl; k++) {
+            const LeafProg &lp = progs[kind * nl + k];
+            total += lp.nodes.size();
+            naux = std::max(naux, lp.naux);
+            nmax = std::max(nmax, (int)lp.nodes.size());
+        }
         // staged in pinned host memory, so the upload runs at full link speed
         const size_t need = sizeof(PHdr) * nl + sizeof(PNode) * total;
         if (need > m->stage_cap[kind]) {
@@ -533,6 +541,8 @@ int build_programs(amrlbm_mesh *m)
             HIPCHK(hipMemcpyAsync(m->d_pnode[kind], all, sizeof(PNode) * total, hipMemcpyHostToDevice, m->stream));
         m->prog[kind].hdr = m->d_phdr[kind];
         m->prog[kind].nodes = m->d_pnode[kind];
+        m->prog[kind].naux = naux;   // the hg kernel sizes its LDS by these
+        m->prog[kind].nmax = nmax;
     }
     HIPCHK(hipStreamSynchronize(m->stream));   // the pinned stages are reused by the next build
     return AMRLBM_OK;

@@ -122,6 +122,33 @@ def pmc_traffic(kernel_key, workload, precision=64):
         return None
 
 
+def roofline(key, kt, nleaf, w, precision):
+    """Roofline entry of step kernel `key` from its average launch time kt[key] (ms)."""
+    scale = 0.5 if precision == 32 else 1.0   # 4-byte values
+    b = ALG_BYTES[key] * scale * nleaf * 256
+    ach = b / (kt[key] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(key, w.name, precision),
+            "kernel": KERNEL_NAMES[key], "avg_launch_us": round(kt[key] * 1e3, 3),
+            "algorithmic_bytes_per_leaf_cell": ALG_BYTES[key] * scale, "leaf_blocks": nleaf,
+            "bytes_per_launch": round(b)}
+
+
+def kernels_only(amrlbm, w, calibrate_mb, precisions):
+    """The profiling run: the set-up mesh of workload w, then the three step
+    kernels launched back to back (amrlbm_time_kernels, the launches the bench
+    line's roofline times), with no whole steps; one JSON line per precision."""
+    for prec in precisions:
+        with amrlbm.Mesh(w, precision=prec) as m:
+            m.setup_rising_bubble()
+            kt = m.time_kernels(200)
+            nleaf = m.counts()["nleaf"]
+        if calibrate_mb:
+            calibration_copy(calibrate_mb << 20)
+        print(json.dumps({"mode": "kernels-only", "workload": w.name, "dtype": f"f{prec}", "leaf_blocks": nleaf,
+                          "roofline": {kk: roofline(kk, kt, nleaf, w, prec) for kk in kt}}), flush=True)
+
+
 def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64):
     """Time `steps` steps of workload w after `warmup`, then the step kernels
     back to back; returns the numbers of one bench entry."""
@@ -144,16 +171,8 @@ def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64):
     share = {kk: kt[kk] * LAUNCHES_PER_STEP.get(kk, 1.0 / k) for kk in kt}
     dom = max(share, key=share.get)
 
-    scale = 0.5 if precision == 32 else 1.0   # 4-byte values
-
     def roof(key):
-        b = ALG_BYTES[key] * scale * c1["nleaf"] * 256
-        ach = b / (kt[key] * 1e-3) / 1e9
-        return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(key, w.name, precision),
-                "kernel": KERNEL_NAMES[key], "avg_launch_us": round(kt[key] * 1e3, 3),
-                "algorithmic_bytes_per_leaf_cell": ALG_BYTES[key] * scale, "leaf_blocks": c1["nleaf"],
-                "bytes_per_launch": round(b)}
+        return roofline(key, kt, c1["nleaf"], w, precision)
     return {
         "value": w.fine_cells * steps / wall / 1e6, "wall": wall, "c0": c0, "c1": c1, "regrids": rstats,
         "roofline": roof(dom), "roofline_all": {kk: roof(kk) for kk in kt},
@@ -209,6 +228,8 @@ def main():
     ap.add_argument("--large-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (rocprof counter calibration)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="profiling run: time the step kernels on the set-up mesh only (no whole steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -218,6 +239,9 @@ def main():
     import amrlbm
     w = amrlbm.WORKLOADS[args.workload]
     k = args.regrid_every
+    if args.kernels_only:
+        kernels_only(amrlbm, w, args.calibrate_copy_mb, (64,) if args.no_fp32 else (64, 32))
+        return
     r = measure(amrlbm, w, args.steps, args.warmup, k, args.calibrate_copy_mb)
     c0, c1 = r["c0"], r["c1"]
     nleaf_mean = 0.5 * (c0["nleaf"] + c1["nleaf"])
