@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <string>
@@ -45,6 +46,39 @@ struct Block {
     int level = -1, dx = 0, I = 0, J = 0, parent = 0;
     int child[4] = {0, 0, 0, 0};
     bool has_child = false, isleaf = false, hasleaf = false, refin = false, deref = false;
+};
+
+// Ghost program of one leaf for one fill kind, nodes grouped by stage.
+struct LeafProg {
+    std::vector<PNode> nodes;
+    int naux = 0;
+    int nst = 0;
+    int end[MAX_STAGES] = {};
+    std::string err;
+};
+
+// what a program build reads of a block
+struct BlockSig {
+    int level, I, J, child0, flags;
+    bool operator==(const BlockSig &o) const
+    {
+        return level == o.level && I == o.I && J == o.J && child0 == o.child0 && flags == o.flags;
+    }
+};
+inline BlockSig block_sig(const Block &b)
+{
+    return {b.level, b.I, b.J, b.child[0], (b.isleaf ? 1 : 0) | (b.has_child ? 2 : 0)};
+}
+
+// The ghost programs of one block id and what they were built from: every
+// block the builds read (neighbours, their children, coarse neighbours'
+// neighbours), with its level, position, first child and leaf flag then. The
+// builds read nothing else of the topology, so while all of these are equal
+// the programs are too and are reused after a regrid.
+struct ProgCache {
+    LeafProg p[3];
+    std::vector<std::pair<int, BlockSig>> deps;
+    bool valid = false;
 };
 
 }  // namespace
@@ -98,6 +132,8 @@ struct amrlbm_mesh {
     void *h_stage[3] = {nullptr, nullptr, nullptr};   // pinned upload staging per kind
     size_t stage_cap[3] = {0, 0, 0};
     Prog prog[3] = {};
+    std::vector<ProgCache> pcache;   // by block id
+    long long progs_built = 0;       // leaves whose programs were (re)built
 
     // one HIP graph per (segment length, population-buffer parity) for the
     // current topology; a segment is the run of steps between two regrids
@@ -292,12 +328,33 @@ struct ProgBuild {
     std::vector<PNode> nodes;
     std::vector<int> nstage;                 // stage of each node
     std::vector<int> cell_stage;             // stage of the node that writes LDS cell c
-    // node memo: open addressing on (block, x, y); -2 marks a node being built
+    // node memo: open addressing on (block, x, y); -2 marks a node being built.
+    // The table is the host thread's, reused by every build on that thread: a
+    // slot belongs to this build iff its stamp is the build's.
     static constexpr int MEMO = 2048;
-    std::vector<long long> memo_key = std::vector<long long>(MEMO, -1);
-    std::vector<int> memo_val = std::vector<int>(MEMO, -1);
+    struct Memo {
+        std::vector<long long> key = std::vector<long long>(MEMO, -1);
+        std::vector<int> val = std::vector<int>(MEMO, -1);
+        std::vector<unsigned> stamp = std::vector<unsigned>(MEMO, 0);
+        unsigned now = 0;
+    };
+    Memo &memo = fresh_memo();
+    static Memo &fresh_memo()
+    {
+        thread_local Memo tm;
+        if (++tm.now == 0) {   // stamp wrap: clear once
+            std::fill(tm.stamp.begin(), tm.stamp.end(), 0u);
+            tm.now = 1;
+        }
+        return tm;
+    }
     int naux = 0;
     std::string err;
+    std::vector<int> *deps = nullptr;   // every block the build reads (see ProgCache)
+    void dep(int b)
+    {
+        if (deps && b > 0) deps->push_back(b);
+    }
 
     int own_cells() const { return kind == KIND_HG ? FS : CS; }
     int cell_index(int x, int y) const { return kind == KIND_HG ? y * FW + x : (y + 1) * CW + (x + 1); }
@@ -306,15 +363,17 @@ struct ProgBuild {
     {
         return kind == KIND_HG ? (b - 1) * POPB + pcell(x, y) : (b - 1) * CS + cell_index(x, y);
     }
+    bool memo_used(int h) const { return memo.stamp[h] == memo.now; }
     int memo_slot(long long key) const
     {
         unsigned h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> 40) & (MEMO - 1);
-        while (memo_key[h] != -1 && memo_key[h] != key) h = (h + 1) & (MEMO - 1);
+        while (memo_used(h) && memo.key[h] != key) h = (h + 1) & (MEMO - 1);
         return (int)h;
     }
     int ref(int b, int x, int y, int &st)
     {
         st = 0;
+        dep(b);
         if (interior(x, y)) return global_ref(b, x, y);
         int dest = node(b, x, y);
         if (dest < 0) return 0;
@@ -325,18 +384,22 @@ struct ProgBuild {
     {
         const long long key = ((long long)b << 16) | ((long long)(x + 8) << 8) | (long long)(y + 8);
         const int slot = memo_slot(key);
-        if (memo_key[slot] == key) {
-            if (memo_val[slot] == -2) {
+        if (memo_used(slot)) {
+            if (memo.val[slot] == -2) {
                 err = "cyclic ghost dependency";
                 return -1;
             }
-            return memo_val[slot];
+            return memo.val[slot];
         }
-        memo_key[slot] = key;
-        memo_val[slot] = -2;
+        memo.stamp[slot] = memo.now;
+        memo.key[slot] = key;
+        memo.val[slot] = -2;
         PNode nd{};
         int st = 0, s2 = 0;
+        dep(b);
         const Block &B = m->blk[b];
+        if (y <= 0) dep(nb_value(m, b, 3));
+        if (y >= NY + 1) dep(nb_value(m, b, 4));
         const bool bot = y <= 0 && nb_value(m, b, 3) < 0;
         const bool top = y >= NY + 1 && nb_value(m, b, 4) < 0;
         if (bot || top) {
@@ -358,6 +421,7 @@ struct ProgBuild {
             const int sx = x < 1 ? -1 : (x > NX ? 1 : 0), sy = y < 1 ? -1 : (y > NY ? 1 : 0);
             const int d = dtab[sx + 1][sy + 1];
             const int N1 = nb_value(m, b, d + 1);
+            dep(N1);
             if (d < 0 || N1 <= 0) {
                 err = "ghost cell (" + std::to_string(x) + "," + std::to_string(y) + ") of block " + std::to_string(b) +
                       " has no neighbour to fill it";
@@ -410,7 +474,7 @@ struct ProgBuild {
         cell_stage[dest] = st + 1;
         nodes.push_back(nd);
         nstage.push_back(st + 1);
-        memo_val[slot] = dest;
+        memo.val[slot] = dest;
         return dest;
     }
     int build()
@@ -422,22 +486,16 @@ struct ProgBuild {
     }
 };
 
-// Ghost program of one leaf for one fill kind, nodes grouped by stage.
-struct LeafProg {
-    std::vector<PNode> nodes;
-    int naux = 0;
-    int nst = 0;
-    int end[MAX_STAGES] = {};
-    std::string err;
-};
 
-void build_leaf_program(amrlbm_mesh *m, int kind, int n, LeafProg &out)
+void build_leaf_program(amrlbm_mesh *m, int kind, int n, LeafProg &out, std::vector<int> *deps = nullptr)
 {
     ProgBuild pb;
     pb.m = m;
     pb.kind = kind;
     pb.R = kind == KIND_C2 ? 2 : 1;
     pb.n = n;
+    pb.deps = deps;
+    pb.dep(n);
     if (pb.build()) {
         out.err = "ghost program of block " + std::to_string(n) + ": " + pb.err;
         return;
@@ -484,23 +542,70 @@ void build_leaf_program(amrlbm_mesh *m, int kind, int n, LeafProg &out)
     }
 }
 
-// Programs of every leaf for the three kinds; the builds only read the
-// topology, so they run on several host threads.
+bool cache_current(const amrlbm_mesh *m, int id)
+{
+    const ProgCache &pc = m->pcache[id];
+    if (!pc.valid) return false;
+    for (const auto &d : pc.deps)
+        if (!(block_sig(m->blk[d.first]) == d.second)) return false;
+    return true;
+}
+
+// Programs of every leaf for the three kinds. A leaf's programs are rebuilt
+// only when a block they were built from changed (ProgCache); the builds only
+// read the topology, so they run on several host threads. With
+// AMRLBM_VERIFY_PROGRAMS set in the environment every reused program is
+// checked against a fresh build.
 int build_programs(amrlbm_mesh *m)
 {
     HostTimer timer(m->host_ms_programs);
-    const int nl = (int)m->leaf.size(), ntask = 3 * nl;
-    std::vector<LeafProg> progs(ntask);
-    const int nthr = std::max(1, std::min(omp_get_max_threads(), ntask / 32));
+    const int nl = (int)m->leaf.size();
+    if (m->pcache.size() < m->blk.size()) m->pcache.resize(m->blk.size());
+    std::vector<int> dirty;
+    for (int k = 0; k < nl; k++)
+        if (!cache_current(m, m->disp[k])) dirty.push_back(m->disp[k]);
+    const int nd = (int)dirty.size(), ntask = 3 * nd;
+    std::vector<std::vector<int>> tdeps(ntask);
+    const int nthr = std::max(1, std::min(omp_get_max_threads(), std::max(ntask, 3 * nl) / 32));
 #pragma omp parallel for schedule(dynamic, 8) num_threads(nthr)
-    for (int t = 0; t < ntask; t++) build_leaf_program(m, t / nl, m->disp[t % nl], progs[t]);
-    for (const LeafProg &lp : progs)
-        if (!lp.err.empty()) return fail(AMRLBM_ERR_TOPOLOGY, lp.err);
+    for (int t = 0; t < ntask; t++) {
+        LeafProg &lp = m->pcache[dirty[t / 3]].p[t % 3];
+        lp = LeafProg{};
+        build_leaf_program(m, t % 3, dirty[t / 3], lp, &tdeps[t]);
+    }
+    for (int i = 0; i < nd; i++) {
+        ProgCache &pc = m->pcache[dirty[i]];
+        pc.valid = false;
+        for (int kind = 0; kind < 3; kind++)
+            if (!pc.p[kind].err.empty()) return fail(AMRLBM_ERR_TOPOLOGY, pc.p[kind].err);
+        std::vector<int> d;
+        for (int kind = 0; kind < 3; kind++) d.insert(d.end(), tdeps[3 * i + kind].begin(), tdeps[3 * i + kind].end());
+        std::sort(d.begin(), d.end());
+        d.erase(std::unique(d.begin(), d.end()), d.end());
+        pc.deps.clear();
+        for (int b : d) pc.deps.push_back({b, block_sig(m->blk[b])});
+        pc.valid = true;
+    }
+    m->progs_built += nd;
+    if (std::getenv("AMRLBM_VERIFY_PROGRAMS")) {
+        std::vector<LeafProg> fresh((size_t)3 * nl);
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nthr)
+        for (int t = 0; t < 3 * nl; t++) build_leaf_program(m, t % 3, m->disp[t / 3], fresh[t]);
+        for (int t = 0; t < 3 * nl; t++) {
+            const LeafProg &a = fresh[t], &b = m->pcache[m->disp[t / 3]].p[t % 3];
+            bool same = a.err.empty() && a.nst == b.nst && a.naux == b.naux && a.nodes.size() == b.nodes.size() &&
+                        std::equal(a.end, a.end + MAX_STAGES, b.end) &&
+                        std::memcmp(a.nodes.data(), b.nodes.data(), sizeof(PNode) * a.nodes.size()) == 0;
+            if (!same)
+                return fail(AMRLBM_ERR_TOPOLOGY, "reused ghost program of block " + std::to_string(m->disp[t / 3]) +
+                                                     " differs from a fresh build");
+        }
+    }
     for (int kind = 0; kind < 3; kind++) {
         size_t total = 0;
         int naux = 0, nmax = 0;
         for (int k = 0; k < nl; k++) {
-            const LeafProg &lp = progs[kind * nl + k];
+            const LeafProg &lp = m->pcache[m->disp[k]].p[kind];
             total += lp.nodes.size();
             naux = std::max(naux, lp.naux);
             nmax = std::max(nmax, (int)lp.nodes.size());
@@ -517,13 +622,18 @@ int build_programs(amrlbm_mesh *m)
         PHdr *hdr = reinterpret_cast<PHdr *>(all + total);
         size_t pos = 0;
         for (int k = 0; k < nl; k++) {
-            const LeafProg &lp = progs[kind * nl + k];
+            const LeafProg &lp = m->pcache[m->disp[k]].p[kind];
             PHdr &h = hdr[k];
             h.off = (int)pos;
             h.nst = lp.nst;
             for (int st = 0; st < MAX_STAGES; st++) h.end[st] = lp.end[st];
-            std::memcpy(all + pos, lp.nodes.data(), sizeof(PNode) * lp.nodes.size());
             pos += lp.nodes.size();
+        }
+        // the node copies (tens of MB on large meshes) on the build's threads
+#pragma omp parallel for schedule(static) num_threads(nthr)
+        for (int k = 0; k < nl; k++) {
+            const LeafProg &lp = m->pcache[m->disp[k]].p[kind];
+            std::memcpy(all + hdr[k].off, lp.nodes.data(), sizeof(PNode) * lp.nodes.size());
         }
         if ((size_t)nl > m->phdr_cap[kind]) {
             if (m->d_phdr[kind]) HIPCHK(hipFree(m->d_phdr[kind]));
