@@ -565,25 +565,29 @@ int build_programs(amrlbm_mesh *m)
     for (int k = 0; k < nl; k++)
         if (!cache_current(m, m->disp[k])) dirty.push_back(m->disp[k]);
     const int nd = (int)dirty.size(), ntask = 3 * nd;
-    std::vector<std::vector<int>> tdeps(ntask);
     const int nthr = std::max(1, std::min(omp_get_max_threads(), std::max(ntask, 3 * nl) / 32));
-#pragma omp parallel for schedule(dynamic, 8) num_threads(nthr)
-    for (int t = 0; t < ntask; t++) {
-        LeafProg &lp = m->pcache[dirty[t / 3]].p[t % 3];
-        lp = LeafProg{};
-        build_leaf_program(m, t % 3, dirty[t / 3], lp, &tdeps[t]);
+#pragma omp parallel num_threads(nthr)
+    {
+        std::vector<int> d;
+#pragma omp for schedule(dynamic, 4)
+        for (int i = 0; i < nd; i++) {   // one leaf's three programs, then the blocks they read
+            ProgCache &pc = m->pcache[dirty[i]];
+            pc.valid = false;
+            d.clear();
+            for (int kind = 0; kind < 3; kind++) {
+                pc.p[kind] = LeafProg{};
+                build_leaf_program(m, kind, dirty[i], pc.p[kind], &d);
+            }
+            std::sort(d.begin(), d.end());
+            d.erase(std::unique(d.begin(), d.end()), d.end());
+            pc.deps.clear();
+            for (int b : d) pc.deps.push_back({b, block_sig(m->blk[b])});
+        }
     }
     for (int i = 0; i < nd; i++) {
         ProgCache &pc = m->pcache[dirty[i]];
-        pc.valid = false;
         for (int kind = 0; kind < 3; kind++)
             if (!pc.p[kind].err.empty()) return fail(AMRLBM_ERR_TOPOLOGY, pc.p[kind].err);
-        std::vector<int> d;
-        for (int kind = 0; kind < 3; kind++) d.insert(d.end(), tdeps[3 * i + kind].begin(), tdeps[3 * i + kind].end());
-        std::sort(d.begin(), d.end());
-        d.erase(std::unique(d.begin(), d.end()), d.end());
-        pc.deps.clear();
-        for (int b : d) pc.deps.push_back({b, block_sig(m->blk[b])});
         pc.valid = true;
     }
     m->progs_built += nd;
