@@ -73,6 +73,7 @@ def lib():
             "amrlbm_setup_rising_bubble": [vp, dp, dp, dp],
             "amrlbm_setup_replicas": [vp, ip, dp, dp, dp],
             "amrlbm_setup_bubbles": [vp, ip, _DP, _DP, _DP],
+            "amrlbm_tecplot_cell_center": [vp, ctypes.c_char_p, dp],
             "amrlbm_get_counts": [vp, _IP],
             "amrlbm_get_topology": [vp, _IP, _IP],
             "amrlbm_set_topology": [vp, ip, _IP, ip, ip, _IP, ip],
@@ -231,6 +232,11 @@ class Mesh:
             _check(lib().amrlbm_setup_replicas(self.h, w.instances, R, x, y))
         else:
             _check(lib().amrlbm_setup_rising_bubble(self.h, R, x, y))
+
+    def write_tecplot(self, path, D=None):
+        """The reference's Tecplot file of the leaves (TecPlot_CellCenter);
+        D defaults to the workload's bubble diameter."""
+        _check(lib().amrlbm_tecplot_cell_center(self.h, os.fsencode(path), float(D if D is not None else self.w.D)))
 
     def setup_bubbles(self, R, x0, y0):
         """Device-resident set-up of several bubbles (amrlbm_setup_bubbles)."""

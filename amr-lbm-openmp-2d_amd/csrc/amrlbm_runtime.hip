@@ -1630,6 +1630,183 @@ int amrlbm_put_block(amrlbm_mesh *m, int id, const double *h, const double *g, c
     return put_block(m, id, h, g, C, P, Ux, Uy, true);
 }
 
+}  // extern "C"
+
+// ---- Tecplot output: Fortran edit descriptors as the reference's runtime
+// writes them (each right-justified in its width, asterisks when too wide) --
+namespace {
+
+std::string fit(const std::string &t, int w)
+{
+    if ((int)t.size() > w) return std::string(w, '*');
+    return std::string(w - t.size(), ' ') + t;
+}
+
+std::string fort_i(long long v, int w) { return fit(std::to_string(v), w); }
+
+// Fw.d: the binary value rounded to d decimals (ties to even), leading zero kept
+std::string fort_f(double x, int w, int d)
+{
+    char b[64];
+    std::snprintf(b, sizeof b, "%.*f", d, x);
+    return fit(b, w);
+}
+
+// Ew.d: 0.d1..dd E+ee, the exponent without the E when it needs three digits
+std::string fort_e(double x, int w, int d)
+{
+    std::string t = std::signbit(x) ? "-" : "";
+    if (x == 0.0) {
+        t += "0." + std::string(d, '0') + "E+00";
+        return fit(t, w);
+    }
+    char b[64];
+    std::snprintf(b, sizeof b, "%.*e", d - 1, std::fabs(x));   // d.ddddde+XX, d significant digits
+    const char *e = std::strchr(b, 'e');
+    std::string digits(b, e - b);
+    digits.erase(1, 1);   // the point
+    const int ex = std::atoi(e + 1) + 1;
+    char eb[16];
+    if (ex >= -99 && ex <= 99)
+        std::snprintf(eb, sizeof eb, "E%c%02d", ex < 0 ? '-' : '+', std::abs(ex));
+    else
+        std::snprintf(eb, sizeof eb, "%c%03d", ex < 0 ? '-' : '+', std::abs(ex));
+    t += "0." + digits + eb;
+    return fit(t, w);
+}
+
+// List-directed REAL(8): the fewest significant digits that read back to the
+// same value (of the equally short forms, the truncation of the value where
+// it reads back, else the rounded one), in F form without a leading zero when
+// the value rounded to one digit is 0.d E+e with 0 <= e <= 15, else d.ddE+ee.
+std::string fort_list_real(double x)
+{
+    std::string sign = std::signbit(x) ? "-" : "";
+    if (x == 0.0) return sign + "0.";
+    const double a = std::fabs(x);
+    char b[96];
+    int L = 1;
+    for (; L < 17; L++) {
+        std::snprintf(b, sizeof b, "%.*e", L - 1, a);
+        if (std::strtod(b, nullptr) == a) break;
+    }
+    std::snprintf(b, sizeof b, "%.*e", L - 1, a);   // rounded, L digits
+    char ex40[96];
+    std::snprintf(ex40, sizeof ex40, "%.40e", a);
+    const char *e40 = std::strchr(ex40, 'e');
+    std::string all(ex40, e40 - ex40);
+    all.erase(1, 1);
+    const std::string trunc = all.substr(0, L);
+    std::string tstr = trunc.substr(0, 1) + "." + trunc.substr(1) + e40;
+    std::string D;
+    int E10;
+    if (std::strtod(tstr.c_str(), nullptr) == a) {
+        D = trunc;
+        E10 = std::atoi(e40 + 1);
+    } else {
+        const char *e = std::strchr(b, 'e');
+        D = std::string(b, e - b);
+        D.erase(1, 1);
+        E10 = std::atoi(e + 1);
+    }
+    while (D.size() > 1 && D.back() == '0') D.pop_back();
+    char one[32];
+    std::snprintf(one, sizeof one, "%.0e", a);
+    const int expo1 = std::atoi(std::strchr(one, 'e') + 1) + 1;
+    std::string t;
+    if (expo1 >= 0 && expo1 <= 15) {
+        const int p = E10 + 1;
+        if (p <= 0)
+            t = "." + std::string(-p, '0') + D;
+        else if (p >= (int)D.size())
+            t = D + std::string(p - D.size(), '0') + ".";
+        else
+            t = D.substr(0, p) + "." + D.substr(p);
+    } else {
+        char eb[16];
+        std::snprintf(eb, sizeof eb, "E%c%02d", E10 < 0 ? '-' : '+', std::abs(E10));
+        t = D.substr(0, 1) + "." + D.substr(1) + eb;
+    }
+    return sign + t;
+}
+
+// records of `per` values each, as a format with a repeat count writes them
+template <class V, class F>
+void records(std::string &out, const std::vector<V> &v, int per, F fmt)
+{
+    for (size_t k = 0; k < v.size(); k++) {
+        out += fmt(v[k]);
+        if ((k + 1) % per == 0 || k + 1 == v.size()) out += '\n';
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int amrlbm_tecplot_cell_center(amrlbm_mesh *m, const char *path, double D)
+{
+    if (!m) return fail(AMRLBM_ERR_ARG, "null mesh");
+    if (!m->ready) return fail(AMRLBM_ERR_STATE, "mesh not set up");
+    if (!path || !(D > 0.0)) return fail(AMRLBM_ERR_ARG, "bad Tecplot arguments");
+    const size_t nb = (size_t)m->nblks;
+    std::vector<double> C(nb * CS), PU(nb * PUB);
+    const bool f32 = m->dm.f32;
+    int rc = download(m, m->dm.C, 0, nb * CS, C.data(), [f32](size_t) { return f32 ? 1.0 : 0.0; });
+    if (rc) return rc;
+    if ((rc = download(m, m->dm.PU, 0, nb * PUB, PU.data(), no_shift))) return rc;
+    // TecPlot_CellCenter (TecPlot_ASCII.f90:21-77): leaves in Array_of_Leaf_Blocks order
+    const long long nl = (long long)m->leaf.size();
+    const long long Nn = nl * (NX + 1) * (NY + 1), Ne = nl * NX * NY;
+    std::vector<float> X, Y;
+    std::vector<double> u, v, c, p;
+    std::vector<long long> conn;
+    X.reserve(Nn); Y.reserve(Nn); c.reserve(Nn);
+    u.reserve(Ne); v.reserve(Ne); p.reserve(Ne);
+    conn.reserve(4 * Ne);
+    const float L0 = (float)m->prm.L0;
+    const double sigma = m->ph.sigma;
+    long long N = 1;
+    for (int b : m->leaf) {
+        const Block &B = m->blk[b];
+        const int dx = B.dx;
+        const double *Cb = C.data() + (size_t)(b - 1) * CS, *PUb = PU.data() + (size_t)(b - 1) * PUB;
+        auto Cat = [Cb](int x, int y) { return Cb[(y + 1) * CW + (x + 1)]; };
+        for (int J = 1; J <= NY + 1; J++)
+            for (int I = 1; I <= NX + 1; I++) {
+                const int xn = B.I * dx * NX + dx * (I - 1), yn = B.J * dx * NY + dx * (J - 1);
+                X.push_back((float)xn / L0);   // default-kind REAL division
+                Y.push_back((float)yn / L0);
+                c.push_back(0.25 * (((Cat(I - 1, J - 1) + Cat(I, J - 1)) + Cat(I - 1, J)) + Cat(I, J)) - 0.5);
+                if (I <= NX && J <= NY) {
+                    const int k = (I - 1) + NX * (J - 1);
+                    u.push_back(PUb[PS + k]);
+                    v.push_back(PUb[2 * PS + k]);
+                    p.push_back(PUb[k] * 0.5 * D / sigma);
+                    conn.insert(conn.end(), {N, N + 1, N + 1 + NX + 1, N + NX + 1});
+                }
+                N++;
+            }
+    }
+    std::string out;
+    out.reserve((size_t)(Nn * 34 + Ne * 70 + 256));
+    out += " Variables = X, Y, Ux, Uy, C, P\n";
+    out += "Zone N =" + fort_i(Nn, 6) + ", E =" + fort_i(Ne, 6) + ", ZONETYPE=FEquadrilateral, DATAPACKING=Block\n";
+    out += " VARLOCATION=([3,4,6]=CELLCENTERED), T = \" " + fort_list_real(m->T * std::sqrt(m->ph.gy / D)) + " \"\n";
+    records(out, X, 9, [](float q) { return fort_f(q, 10, 3); });
+    records(out, Y, 9, [](float q) { return fort_f(q, 10, 3); });
+    records(out, u, 8, [](double q) { return fort_e(q, 14, 6); });
+    records(out, v, 8, [](double q) { return fort_e(q, 14, 6); });
+    records(out, c, 9, [](double q) { return fort_e(q, 14, 6); });
+    records(out, p, 8, [](double q) { return fort_e(q, 14, 6); });
+    records(out, conn, 8, [](long long q) { return fort_i(q, 7); });
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(AMRLBM_ERR_ARG, std::string("cannot open ") + path);
+    const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+    if (std::fclose(f) != 0 || !ok) return fail(AMRLBM_ERR_ARG, std::string("cannot write ") + path);
+    return AMRLBM_OK;
+}
+
 int amrlbm_get_block(amrlbm_mesh *m, int id, double *h, double *g, double *C, double *P, double *Ux, double *Uy)
 {
     return get_block(m, id, h, g, C, P, Ux, Uy, true);

@@ -101,6 +101,16 @@ int amrlbm_set_topology(amrlbm_mesh *m, int nblks, const int *blocks, int kk, in
  * file's nx, ny, L0 and level range must equal the mesh's. */
 int amrlbm_save_snapshot(amrlbm_mesh *m, const char *path);
 int amrlbm_load_snapshot(amrlbm_mesh *m, const char *path);
+/* TecPlot_CellCenter (TecPlot_ASCII.f90:1-86): the reference's ASCII
+ * FE-quadrilateral file of the leaves (Array_of_Leaf_Blocks order) at step T,
+ * written to path instead of Rising_Bubble_2D_NN.dat: node X, Y / L0, cell
+ * Ux, Uy, node C - 1/2 (mean of the four cells around the node, first ghost
+ * ring included), cell P D / (2 sigma), with the reference's edit descriptors.
+ * D is the bubble diameter (D2Q9_Mod's D = 2 R). The file is the reference's
+ * byte for byte, except that the time T sqrt(gy / D) in the third line can
+ * differ in its last digit where the Fortran runtime picks another of the
+ * equally short decimal forms of the same double. */
+int amrlbm_tecplot_cell_center(amrlbm_mesh *m, const char *path, double D);
 /* arrays of one live block (NULL pointers skip a field) */
 int amrlbm_put_block(amrlbm_mesh *m, int id, const double *h, const double *g, const double *C,
                      const double *P, const double *Ux, const double *Uy);

@@ -64,7 +64,7 @@ for cfg in "${configs[@]}"; do
   $FC $FFLAGS $MODF -c "$D/src/Parameters.f90" -o Parameters.o
   objs="D2Q9_Mod.o Parameters.o"
   for f in LBM_Subroutines Interpolations BC_Cell Neighbors Prolong Restrict Refine_Derefine \
-           Fill_Ghost_Cells_C Fill_Ghost_Cells_C2 Fill_Ghost_Cells_hg; do
+           Fill_Ghost_Cells_C Fill_Ghost_Cells_C2 Fill_Ghost_Cells_hg TecPlot_ASCII; do
     $FC $FFLAGS $MODF -c "$REF/$f.f90" -o $f.o
     objs="$objs $f.o"
   done

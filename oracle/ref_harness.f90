@@ -328,6 +328,11 @@ PROGRAM amrlbm_ref_harness
 	T = nsteps
 	CALL Dump_State( TRIM(outdir)//'/final.bin' )
 
+	! AMRLBM_TECPLOT set: the reference's Tecplot writer on the final state
+	! (TecPlot_ASCII.f90:1-86), Rising_Bubble_2D_00.dat in the working directory
+	CALL GET_ENVIRONMENT_VARIABLE( 'AMRLBM_TECPLOT', arg, k )
+	IF( k > 0 ) CALL TecPlot_CellCenter
+
 	OPEN( NEWUNIT=ut, FILE=TRIM(outdir)//'/timing.txt', STATUS='REPLACE' )
 	WRITE(ut,'(A,I6)') 'threads ', nthreads
 	WRITE(ut,'(A,I8)') 'steps ', nsteps

@@ -9,6 +9,9 @@ Outputs
                           layout of oracle/ref_harness.f90)
   tiny_init.bin           the tiny workload's state after set-up (full data)
   tiny_s1_p3.bin          the tiny workload after the hg ghost fill of step 1
+  (tecplot keys)          SHA-256 of the reference's Tecplot file (TecPlot_CellCenter,
+                          TecPlot_ASCII.f90) of the final state of the tiny, cfg1 and
+                          cfg2 runs
   cfg1_diag.txt           reference FP64 diagnostics, cfg1, 300 steps
   cfg2_diag.txt           reference FP64 diagnostics, cfg2, 40 steps, regrid every 10
 Sequences
@@ -56,8 +59,9 @@ def output_table(stdout):
 def run(cfg, d, args, env=None):
     e = dict(os.environ)
     e.update(env or {})
+    e["AMRLBM_TECPLOT"] = "1"   # the final state's Tecplot file, Rising_Bubble_2D_00.dat in d
     subprocess.run([os.path.join(REF, cfg, "ref_harness"), d] + [str(a) for a in args], check=True,
-                   env=e, stdout=subprocess.DEVNULL)
+                   env=e, stdout=subprocess.DEVNULL, cwd=d)
 
 
 def main():
@@ -72,7 +76,8 @@ def main():
         out["tiny"] = {"steps": 20, "regrid_every": 1, "phase_steps": [0, 1, 10],
                        "init": sha(os.path.join(d, "init.bin")), "final": sha(os.path.join(d, "final.bin")),
                        "phases": {f"s{t}_p{k}": sha(os.path.join(d, f"s{t:06d}_p{k}.bin"))
-                                  for t in (0, 1, 10) for k in range(9)}}
+                                  for t in (0, 1, 10) for k in range(9)},
+                       "tecplot": sha(os.path.join(d, "Rising_Bubble_2D_00.dat"))}
         shutil.copy(os.path.join(d, "init.bin"), os.path.join(HERE, "tiny_init.bin"))
         shutil.copy(os.path.join(d, "s000001_p3.bin"), os.path.join(HERE, "tiny_s1_p3.bin"))
 
@@ -80,14 +85,16 @@ def main():
         os.makedirs(d)
         run("cfg1", d, [300, 1, 50, 4, -1])
         out["cfg1"] = {"steps": 300, "regrid_every": 1, "init": sha(os.path.join(d, "init.bin")),
-                       "final": sha(os.path.join(d, "final.bin"))}
+                       "final": sha(os.path.join(d, "final.bin")),
+                       "tecplot": sha(os.path.join(d, "Rising_Bubble_2D_00.dat"))}
         shutil.copy(os.path.join(d, "diag.txt"), os.path.join(HERE, "cfg1_diag.txt"))
 
         d = os.path.join(tmp, "cfg2")
         os.makedirs(d)
         run("cfg2", d, [40, 10, 10, 8, -1])
         out["cfg2"] = {"steps": 40, "regrid_every": 10, "init": sha(os.path.join(d, "init.bin")),
-                       "final": sha(os.path.join(d, "final.bin"))}
+                       "final": sha(os.path.join(d, "final.bin")),
+                       "tecplot": sha(os.path.join(d, "Rising_Bubble_2D_00.dat"))}
         shutil.copy(os.path.join(d, "diag.txt"), os.path.join(HERE, "cfg2_diag.txt"))
 
         d = os.path.join(tmp, "walk")
