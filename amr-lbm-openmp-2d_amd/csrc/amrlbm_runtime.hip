@@ -135,14 +135,15 @@ struct amrlbm_mesh {
     std::vector<ProgCache> pcache;   // by block id
     long long progs_built = 0;       // leaves whose programs were (re)built
 
-    // one HIP graph per (segment length, population-buffer parity) for the
-    // current topology; a segment is the run of steps between two regrids
+    // one HIP graph per (segment length, population-buffer parity), updated
+    // to the current topology when used; a segment is the run of steps
+    // between two regrids
     struct GraphEntry {
         int len, parity;
         hipGraphExec_t exec;
+        int topo;   // topology version the executable was built or updated for
     };
     std::vector<GraphEntry> graphs;
-    int graph_topo = -1;
     int topo_version = 0;
 
     int nlev() const { return prm.max_levels + 1; }
@@ -1147,17 +1148,20 @@ void drop_graphs(amrlbm_mesh *m)
     m->graphs.clear();
 }
 
+// The executable graph of a segment of len steps from the current buffer
+// parity. After a topology change the segment has the same launches with new
+// grids and arguments, so the new capture updates the existing executable
+// (hipGraphExecUpdate) instead of instantiating another; an update the
+// runtime refuses falls back to instantiation.
 int segment_graph(amrlbm_mesh *m, int len, hipGraphExec_t *out)
 {
-    if (m->graph_topo != m->topo_version) {
-        drop_graphs(m);
-        m->graph_topo = m->topo_version;
-    }
+    amrlbm_mesh::GraphEntry *entry = nullptr;
     for (auto &g : m->graphs)
-        if (g.len == len && g.parity == m->cur) {
-            *out = g.exec;
-            return AMRLBM_OK;
-        }
+        if (g.len == len && g.parity == m->cur) entry = &g;
+    if (entry && entry->topo == m->topo_version) {
+        *out = entry->exec;
+        return AMRLBM_OK;
+    }
     HostTimer timer(m->host_ms_graphs);
     m->graphs_built++;
     hipGraph_t g;
@@ -1166,10 +1170,27 @@ int segment_graph(amrlbm_mesh *m, int len, hipGraphExec_t *out)
     hipError_t e = hipStreamEndCapture(m->stream, &g);
     if (rc) return rc;
     if (e != hipSuccess) return fail(AMRLBM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    if (entry) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(entry->exec, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+            HIPCHK(hipGraphDestroy(g));
+            entry->topo = m->topo_version;
+            *out = entry->exec;
+            return AMRLBM_OK;
+        }
+        (void)hipGetLastError();   // a refused update leaves an error code behind
+        HIPCHK(hipGraphExecDestroy(entry->exec));
+    }
     hipGraphExec_t ex;
     HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(g));
-    m->graphs.push_back({len, m->cur, ex});
+    if (entry) {
+        entry->exec = ex;
+        entry->topo = m->topo_version;
+    } else {
+        m->graphs.push_back({len, m->cur, ex, m->topo_version});
+    }
     *out = ex;
     return AMRLBM_OK;
 }

@@ -173,7 +173,8 @@ int amrlbm_last_step_ms(amrlbm_mesh *m, double *ms);
  * out[0] device regrid decisions taken by amrlbm_step, out[1] of them that changed the mesh,
  * out[2] segments launched ahead of such a change and taken back, host wall ms spent in
  * out[3] Refine_Derefine_Mesh (excluding table upload), out[4] table upload (including
- * out[5] ghost-program build), out[6] graph capture and instantiation; out[7] graphs built */
+ * out[5] ghost-program build), out[6] graph capture and instantiation or update; out[7] graphs
+ * captured (built, or updated after a topology change) */
 int amrlbm_regrid_stats(amrlbm_mesh *m, double out[8]);
 
 /* ---- learned regrid predictor (no counterpart in the reference) ----------
