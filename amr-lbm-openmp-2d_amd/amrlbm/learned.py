@@ -87,25 +87,40 @@ def generate(w, variants, steps, every=HORIZON, horizon=HORIZON, regrid_every=10
 def run(mesh, nsteps, weights=None, predict_every=HORIZON, judge_every=10, threshold=0.0):
     """Advance the mesh nsteps with the full regrid every predict_every steps
     (the plain criterion plus, with weights, the blocks the network flags for
-    refinement) and the plain criterion judging every judge_every steps."""
+    refinement) and the plain criterion judging every judge_every steps.
+
+    Two regrids never follow each other without a step between them.
+    Err_Interface reads C's ghost ring, which only the step fills, and a
+    block just derefined holds a ring of zeros (Restrict.f90:23-30), so a
+    second regrid at once would refine every such block again from the
+    spurious jump and prolong C from the zero ring into its children. The
+    judge is therefore not consulted at a multiple of predict_every, where
+    the full regrid comes next (in this call or at the start of the next),
+    and a call that starts at the step of the last regrid this function ran
+    on the mesh (a judge's, ending the previous call) skips its first full
+    regrid."""
     st = {"predicted_regrids": 0, "predicted_blocks": 0, "misses": 0, "checks": 0}
     t = 0
     while t < nsteps:
-        if t % predict_every == 0:
+        if t % predict_every == 0 and getattr(mesh, "_last_regrid_step", None) != mesh.counts()["T"]:
             ids = np.zeros(0, dtype=np.int32)
             if weights is not None:
                 x, all_ids = mesh.cnn_features()
                 ids = all_ids[cnn_forward(x, weights)[:, 0] > threshold]
             mesh.refine_derefine_predicted(ids)
+            mesh._last_regrid_step = mesh.counts()["T"]
             st["predicted_regrids"] += 1
             st["predicted_blocks"] += int(ids.size)
         n = min(judge_every, nsteps - t, predict_every - t % predict_every)
         mesh.step(n, 1 << 30)
         t += n
+        if t % predict_every == 0:
+            continue
         _, refine = mesh.regrid_check()
         st["checks"] += 1
         if refine:
             mesh.refine_derefine_mesh()
+            mesh._last_regrid_step = mesh.counts()["T"]
             st["misses"] += 1
     return st
 

@@ -128,6 +128,11 @@ int amrlbm_phase_field_lbm_h_g(amrlbm_mesh *m, double ms[3]);
 int amrlbm_macroscopic_values_h(amrlbm_mesh *m);      /* :328-335 -> LBM_Subroutines.f90:193-201 */
 int amrlbm_fill_ghost_cells_c(amrlbm_mesh *m);        /* Fill_Ghost_Cells_C.f90:2-40 */
 int amrlbm_macroscopic_values_g(amrlbm_mesh *m);      /* :346-354 -> LBM_Subroutines.f90:205-240 */
+/* Refine_Derefine_Mesh (Refine_Derefine.f90:2-14). As in the reference,
+ * Err_Interface reads C's ghost ring, which the step's C fill writes and a
+ * derefined block holds as zeros (Restrict.f90:23-30): a second regrid with
+ * no step between refines every block just derefined again. Callers step
+ * between regrids, as Main_Loop does. */
 int amrlbm_refine_derefine_mesh(amrlbm_mesh *m, double refin_tol, double deref_tol);
 /* The device's regrid decision at the current state without regridding
  * (no counterpart in the reference): out[0] = 1 iff Refine_Derefine_Mesh
