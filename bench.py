@@ -204,6 +204,57 @@ def learned_regrid(amrlbm, w, steps=2000):
     return out
 
 
+def multi_bubble(amrlbm, steps, k):
+    """The 32-bubble case (amrlbm.multibubble): FP64, FP32 and FP32 with the
+    learned regrid (full regrid every 50 steps, plain criterion judging every
+    10) from the same device set-up, and the FP32 runs' total mass and
+    per-bubble centroids against FP64's."""
+    import numpy as np
+    from amrlbm import learned
+    from amrlbm import multibubble as mb
+    w, bub = mb.WORKLOAD, mb.bubbles()
+    out = {"workload": f"{len(bub)} bubbles, diameters 24..96 cells (seeded), {w.fine_shape[0]}x{w.fine_shape[1]} "
+                       f"finest-equivalent, levels {w.min_levels}..{w.max_levels}, Eo={w.Bo}, W={w.W:g}, "
+                       f"density ratio 1000, viscosity ratio 100, set up on the device",
+           "steps": steps, "unit": "MLUPS (finest-equivalent)"}
+    runs = {}
+    for name, prec, use_net in (("fp64", 64, False), ("fp32", 32, False), ("fp32_learned", 32, True)):
+        if use_net and not os.path.exists(learned.WEIGHTS):
+            continue
+        with amrlbm.Mesh(w, precision=prec) as m:
+            mb.setup(m, bub)
+            m.synchronize()
+            t0 = time.perf_counter()
+            try:
+                st = learned.run(m, steps, np.load(learned.WEIGHTS)) if use_net else None
+                if not use_net:
+                    m.step(steps, k)
+                m.synchronize()
+            except amrlbm.AmrLbmError as exc:   # a state that turned non-finite is reported, not raised
+                out[name] = {"error": str(exc)}
+                continue
+            wall = time.perf_counter() - t0
+            runs[name] = {"diag": m.output(), "regions": mb.gas_regions(m)}
+            e = {"value": round(w.fine_cells * steps / wall / 1e6, 1), "ms_per_step": round(wall * 1e3 / steps, 4),
+                 "leaf_blocks": m.counts()["nleaf"], "regrid_every": 50 if use_net else k}
+            if st:
+                e.update({f: st[f] for f in ("predicted_regrids", "predicted_blocks", "misses", "checks")})
+            out[name] = e
+    if "fp64" not in runs:
+        return out
+    ref = runs["fp64"]
+    out["fp64"]["gas_regions"] = len(ref["regions"])
+    for name in ("fp32", "fp32_learned"):
+        if name in runs:
+            r = runs[name]
+            out[name]["error_vs_fp64"] = {
+                "mass_rel": abs(r["diag"][3] - ref["diag"][3]) / abs(ref["diag"][3]),
+                "gas_regions": len(r["regions"]),
+                "centroid_max_diff_cells": max(mb.match(ref["regions"], r["regions"]),
+                                               mb.match(r["regions"], ref["regions"]))}
+    return out
+
+
 def describe(w, k, precision=64):
     path = "plain FP64 path" if precision == 64 else "FP32 path (deviation storage)"
     what = (f"rising bubble {w.fine_shape[0]}x{w.fine_shape[1]} finest-equivalent, 16x16 blocks, "
@@ -228,6 +279,8 @@ def main():
     ap.add_argument("--large-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (rocprof counter calibration)")
+    ap.add_argument("--multi-steps", type=int, default=3000,
+                    help="steps of the 32-bubble case (0 to skip)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: time the step kernels on the set-up mesh only (no whole steps)")
     args = ap.parse_args()
@@ -294,6 +347,8 @@ def main():
                 "regrids": {f: rl["regrids"][f] for f in ("checks", "changes")},
             }
         out["large_working_set"] = entries
+    if args.multi_steps > 0:
+        out["multi_bubble"] = multi_bubble(amrlbm, args.multi_steps, k)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, k)
     print(json.dumps(out), flush=True)
