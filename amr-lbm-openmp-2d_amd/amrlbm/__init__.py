@@ -35,6 +35,22 @@ class Params(ctypes.Structure):
                 ("M0", "W", "sigma", "beta", "kappa", "taul", "tauh", "rhol", "rhoh", "gy")]
 
 
+class Instance(ctypes.Structure):
+    """amrlbm_instance: one instance of an ensemble mesh."""
+    _fields_ = [(n, ctypes.c_int) for n in ("L0", "min_levels", "max_levels")] + \
+               [(n, ctypes.c_double) for n in
+                ("M0", "W", "sigma", "beta", "kappa", "taul", "tauh", "rhol", "rhoh", "gy", "R", "x0", "y0")]
+
+
+def instance_from_workload(w):
+    d = Instance()
+    d.L0, d.min_levels, d.max_levels = w.L0, w.min_levels, w.max_levels
+    d.M0, d.W, d.sigma, d.beta, d.kappa = w.M0, w.W, w.sigma, w.beta, w.kappa
+    d.taul, d.tauh, d.rhol, d.rhoh, d.gy = w.taul, w.tauh, w.rhol, w.rhoh, w.gy
+    d.R, d.x0, d.y0 = w.radius, w.x0, w.y0
+    return d
+
+
 def params_from_workload(w):
     p = Params()
     p.nx, p.ny = w.nx, w.ny
@@ -73,6 +89,8 @@ def lib():
             "amrlbm_setup_rising_bubble": [vp, dp, dp, dp],
             "amrlbm_setup_replicas": [vp, ip, dp, dp, dp],
             "amrlbm_setup_bubbles": [vp, ip, _DP, _DP, _DP],
+            "amrlbm_setup_ensemble": [vp, ip, ctypes.POINTER(Instance)],
+            "amrlbm_output_instances": [vp, ip, _DP],
             "amrlbm_tecplot_cell_center": [vp, ctypes.c_char_p, dp],
             "amrlbm_get_counts": [vp, _IP],
             "amrlbm_get_topology": [vp, _IP, _IP],
@@ -225,6 +243,9 @@ class Mesh:
         """The reference's set-up; a workload of several instances gets one
         bubble per instance (amrlbm_setup_replicas)."""
         w = self.w
+        if hasattr(w, "members"):   # an ensemble: each instance its own bubble
+            self.setup_ensemble(w.members)
+            return
         R = w.radius if radius is None else radius
         x = w.x0 if x0 is None else x0
         y = w.y0 if y0 is None else y0
@@ -237,6 +258,22 @@ class Mesh:
         """The reference's Tecplot file of the leaves (TecPlot_CellCenter);
         D defaults to the workload's bubble diameter."""
         _check(lib().amrlbm_tecplot_cell_center(self.h, os.fsencode(path), float(D if D is not None else self.w.D)))
+
+    def setup_ensemble(self, members):
+        """Instances of uneven size, level range and fluid in one mesh
+        (amrlbm_setup_ensemble); members are one-domain Workloads."""
+        arr = (Instance * len(members))(*[instance_from_workload(w) for w in members])
+        _check(lib().amrlbm_setup_ensemble(self.h, len(members), arr))
+
+    def output_instances(self, allow_nonfinite=False):
+        """[n, 9] per-instance diagnostics of an ensemble mesh, each in its
+        instance's frame (amrlbm_output_instances)."""
+        n = self.w.instances
+        out = np.zeros((n, 9))
+        rc = lib().amrlbm_output_instances(self.h, n, _d(out))
+        if rc != 0 and not (allow_nonfinite and rc == 5):
+            _check(rc)
+        return out
 
     def setup_bubbles(self, R, x0, y0):
         """Device-resident set-up of several bubbles (amrlbm_setup_bubbles)."""

@@ -69,7 +69,7 @@ struct BlockDev {
     int isleaf;
     int child0;   // slot of the first child when refined, else -1
     int hasleaf;  // children exist and are all leaves (derefinement candidate)
-    int pad1;
+    int inst;     // instance of an ensemble mesh (0 in any other mesh)
 };
 
 // Neighbour descriptor: the resolved nghbr(k)%n plus what a ghost fill needs
@@ -95,6 +95,13 @@ struct DevMeshT {
     const int *skip;         // [0] != 0: the step kernels return at once (a
                              // segment launched ahead of a regrid that changes
                              // the mesh)
+    // an ensemble mesh (independent instances, each with its own fluid and
+    // level range): per-instance constants indexed by BlockDev::inst, and
+    // the roots per instance; null / 0 in any other mesh
+    const Phys *phs;         // [inst]
+    const PhysF *phsf;       // [inst]
+    const int *lv;           // [inst][2]: min_levels, max_levels
+    int tile_roots;
 };
 
 // The host's handle: the FP64 view, whose field pointers hold float arrays
@@ -116,6 +123,10 @@ inline DevMeshT<T> typed(const DevMesh &m)
     t.nbd = m.nbd;
     t.err_flags = m.err_flags;
     t.skip = m.skip;
+    t.phs = m.phs;
+    t.phsf = m.phsf;
+    t.lv = m.lv;
+    t.tile_roots = m.tile_roots;
     return t;
 }
 
@@ -180,8 +191,10 @@ void launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, in
 void launch_restrict(const DevMesh &m, int cur, const int *parents, int n, hipStream_t s);
 void launch_prolong(const DevMesh &m, int cur, const int *pairs, int n, const Phys &ph, hipStream_t s);
 void launch_init_fields(const DevMesh &m, int cur, const int *list, int n, const Phys &ph, double R, hipStream_t s,
-                        const double *bub = nullptr, int nb = 0);
+                        const double *bub = nullptr, int nb = 0, const double *inst_R = nullptr);
+// diagnostics of the leaves in list into out[9]; with ninst > 0 (an ensemble
+// mesh) also per instance into out_inst[ninst][9]
 void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, double *partial, double *out,
-                 hipStream_t s);
+                 hipStream_t s, int ninst = 0, double *out_inst = nullptr);
 
 }  // namespace amrlbm

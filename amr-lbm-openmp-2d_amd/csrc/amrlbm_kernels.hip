@@ -413,9 +413,11 @@ __device__ __forceinline__ void macro_g_cell(const float *sC, int i, int j, int 
 template <class T> struct PhysOf;
 template <> struct PhysOf<double> {
     static __device__ __forceinline__ const Phys &get(const Phys &p, const PhysF &) { return p; }
+    static __device__ __forceinline__ const Phys *table(const DevMeshT<double> &m) { return m.phs; }
 };
 template <> struct PhysOf<float> {
     static __device__ __forceinline__ const PhysF &get(const Phys &, const PhysF &p) { return p; }
+    static __device__ __forceinline__ const PhysF *table(const DevMeshT<float> &m) { return m.phsf; }
 };
 
 // storage shift of the FP32 path: C is stored as C - 1, h_q as h_q - w_q
@@ -600,12 +602,14 @@ __device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, int set,
 // with no regrid, the C fill that ends a step and the C2 fill that starts the
 // next compute the same first ghost ring from the same C, so U, P come out
 // identical). The collision and U,P read only this block's C. With
-// prog.hdr == null the ghosts are read from memory (no fill).
-template <class T, int OP>
+// prog.hdr == null the ghosts are read from memory (no fill). ENS: an
+// ensemble mesh, whose fluid constants are the block's instance's (m.phs,
+// m.phsf); else the kernel arguments'.
+template <class T, int OP, bool ENS>
 __global__ __launch_bounds__(256) void k_prog_c(DevMeshT<T> m, int cur, const int *__restrict__ list, Prog prog,
                                                 Phys ph64, PhysF ph32, double *err_slot)
 {
-    const auto &ph = PhysOf<T>::get(ph64, ph32);
+    const auto &ph_arg = PhysOf<T>::get(ph64, ph32);
     const int k = blockIdx.x;
     const int tid = threadIdx.x;
     const int skip = *m.skip;
@@ -617,6 +621,7 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMeshT<T> m, int cur, const in
     if (fill) hd = prog.hdr[k];
     if (skip) return;
     const BlockDev me = m.blk[slot];
+    const auto &ph = ENS ? PhysOf<T>::table(m)[me.inst] : ph_arg;
     T *C = m.C + (size_t)slot * CS;
     __shared__ T sC[LC_C];
     const int i = tid % NX + 1, j = tid / NX + 1;
@@ -887,8 +892,9 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
     for (int n = tid; n < nblks; n += 1024) {
         const BlockDev b = m.blk[n];
         if (b.level < 0) continue;
-        if (b.isleaf && (b.level < min_levels || (b.level < max_levels && err_slot[n] > refin_tol))) ref = 1;
-        if (!b.hasleaf || b.level < min_levels || b.child0 < 0) continue;
+        const int lo = m.lv ? m.lv[2 * b.inst] : min_levels, hi = m.lv ? m.lv[2 * b.inst + 1] : max_levels;
+        if (b.isleaf && (b.level < lo || (b.level < hi && err_slot[n] > refin_tol))) ref = 1;
+        if (!b.hasleaf || b.level < lo || b.child0 < 0) continue;
         bool d = true;
         for (int c = b.child0; c < b.child0 + 4; c++) {
             const double e = m.blk[c].isleaf ? err_slot[c] : 0.0;
@@ -998,8 +1004,9 @@ template <class T>
 __global__ __launch_bounds__(256) void k_prolong(DevMeshT<T> m, int cur, const int *__restrict__ pairs, Phys ph64,
                                                  PhysF ph32)
 {
-    const auto &ph = PhysOf<T>::get(ph64, ph32);
     const int ps = pairs[2 * blockIdx.x], c0 = pairs[2 * blockIdx.x + 1];
+    const auto *tab = PhysOf<T>::table(m);   // an ensemble mesh: the parent's instance's constants
+    const auto &ph = tab ? tab[m.blk[ps].inst] : PhysOf<T>::get(ph64, ph32);
     const int tid = threadIdx.x;
     const T *Fp = m.pop[cur] + (size_t)ps * POPB;
     const T *Cp = m.C + (size_t)ps * CS;
@@ -1063,10 +1070,12 @@ __global__ __launch_bounds__(256) void k_prolong(DevMeshT<T> m, int cur, const i
 // bubble, finest cells) C itself is set here first, on both ghost rings:
 // 0.5 + 0.5 tanh(2 d / W) with d the distance to the nearest bubble surface,
 // min_k(|x - x_k| - R_k), and R in P is the nearest bubble's; else C comes
-// from memory and R is the argument.
+// from memory and R is the argument, or with inst_R (an ensemble mesh)
+// inst_R[instance]. An ensemble mesh takes its instances' fluid constants.
 template <class T>
-__global__ __launch_bounds__(256) void k_init_fields(DevMeshT<T> m, int cur, const int *__restrict__ list, Phys ph,
-                                                     double R, const double *__restrict__ bub, int nb)
+__global__ __launch_bounds__(256) void k_init_fields(DevMeshT<T> m, int cur, const int *__restrict__ list,
+                                                     Phys ph_arg, double R, const double *__restrict__ bub, int nb,
+                                                     const double *__restrict__ inst_R)
 {
     const int slot = list[blockIdx.x];
     const int tid = threadIdx.x;
@@ -1074,6 +1083,8 @@ __global__ __launch_bounds__(256) void k_init_fields(DevMeshT<T> m, int cur, con
     __shared__ double sR[CS];
     T *C = m.C + (size_t)slot * CS;
     const BlockDev b = m.blk[slot];
+    const Phys &ph = m.phs ? m.phs[b.inst] : ph_arg;
+    if (inst_R) R = inst_R[b.inst];
     if (nb > 0) {
         for (int t = tid; t < CS; t += 256) {
             const int x = t % CW - 1, y = t / CW - 1;
@@ -1128,22 +1139,26 @@ __global__ __launch_bounds__(256) void k_init_fields(DevMeshT<T> m, int cur, con
 }
 
 // FP64 diagnostics, stage 1: per-leaf partials in a fixed reduction order
-// (min rho, max rho, max|Uy|, mass, area, x-moment, y-moment, Uy-moment, non-finite)
+// (min rho, max rho, max|Uy|, mass, area, x-moment, y-moment, Uy-moment,
+// non-finite). In an ensemble mesh the block's instance's constants and the
+// instance's own frame (y from its first root).
 template <class T>
-__global__ __launch_bounds__(256) void k_diag_partial(DevMeshT<T> m, const int *__restrict__ list, Phys ph,
+__global__ __launch_bounds__(256) void k_diag_partial(DevMeshT<T> m, const int *__restrict__ list, Phys ph_arg,
                                                       double *part)
 {
     const int slot = list[blockIdx.x];
     const int tid = threadIdx.x;
     const int i = tid % NX + 1, j = tid / NX + 1;
     const BlockDev b = m.blk[slot];
+    const Phys &ph = m.phs ? m.phs[b.inst] : ph_arg;
+    const int J = m.phs ? b.J - b.inst * (m.tile_roots << b.level) : b.J;
     double c = (double)m.C[(size_t)slot * CS + cidx(i, j)] + c_shift<T>();
     double uy = (double)m.PU[(size_t)slot * PUB + 2 * PS + pidx(i, j)];
     double rho = ph.rhol + c * (ph.rhoh - ph.rhol);
     double dx2 = (double)(b.dx * b.dx);
     double w = (1.0 - c) * dx2;
     double xc = (double)(b.I * b.dx * NX) + (double)b.dx * ((double)i - 0.5);
-    double yc = (double)(b.J * b.dx * NY) + (double)b.dx * ((double)j - 0.5);
+    double yc = (double)(J * b.dx * NY) + (double)b.dx * ((double)j - 0.5);
     __shared__ double red[9][256];
     red[0][tid] = rho;
     red[1][tid] = rho;
@@ -1195,23 +1210,54 @@ __global__ __launch_bounds__(256) void k_diag_final(const double *part, int n, d
     if (tid < 9) out[tid] = red[tid][0];
 }
 
+// stage 2 per instance of an ensemble mesh: workgroup q folds the partials of
+// instance q's leaves in leaf-list order (the same tree as k_diag_final)
+__global__ __launch_bounds__(256) void k_diag_final_inst(DevMeshT<double> m, const int *__restrict__ list,
+                                                         const double *part, int n, double *out)
+{
+    const int tid = threadIdx.x, q = blockIdx.x;
+    __shared__ double red[9][256];
+    double a[9] = {10.0, -10.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = tid; k < n; k += 256) {
+        if (m.blk[list[k]].inst != q) continue;
+        const double *p = part + (size_t)k * 9;
+        a[0] = fmin(a[0], p[0]);
+        a[1] = fmax(a[1], p[1]);
+        a[2] = fmax(a[2], p[2]);
+        for (int f = 3; f < 9; f++) a[f] += p[f];
+    }
+    for (int f = 0; f < 9; f++) red[f][tid] = a[f];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+            red[0][tid] = fmin(red[0][tid], red[0][tid + s]);
+            red[1][tid] = fmax(red[1][tid], red[1][tid + s]);
+            red[2][tid] = fmax(red[2][tid], red[2][tid + s]);
+            for (int f = 3; f < 9; f++) red[f][tid] = red[f][tid] + red[f][tid + s];
+        }
+        __syncthreads();
+    }
+    if (tid < 9) out[(size_t)q * 9 + tid] = red[tid][0];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, bool ENS>
 static void prog_c(const DevMesh &dm, int cur, const int *list, int n, int op, const Prog &prog, const Phys &ph,
                    hipStream_t s, double *err_slot)
 {
     const DevMeshT<T> m = typed<T>(dm);
     dim3 g(n), b(256);
     if (op == OP_MACRO_G_COLLIDE)
-        hipLaunchKernelGGL((k_prog_c<T, OP_MACRO_G_COLLIDE>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
+        hipLaunchKernelGGL((k_prog_c<T, OP_MACRO_G_COLLIDE, ENS>), g, b, 0, s, m, cur, list, prog, ph, dm.phf,
+                           err_slot);
     else if (op == OP_COLLIDE)
-        hipLaunchKernelGGL((k_prog_c<T, OP_COLLIDE>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
+        hipLaunchKernelGGL((k_prog_c<T, OP_COLLIDE, ENS>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
     else if (op == OP_MACRO_G)
-        hipLaunchKernelGGL((k_prog_c<T, OP_MACRO_G>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
+        hipLaunchKernelGGL((k_prog_c<T, OP_MACRO_G, ENS>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
     else
-        hipLaunchKernelGGL((k_prog_c<T, OP_NONE>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
+        hipLaunchKernelGGL((k_prog_c<T, OP_NONE, ENS>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
 }
 
 void launch_prog_c(const DevMesh &m, int cur, const int *list, int n, int kind, int op, const Prog &prog,
@@ -1219,10 +1265,13 @@ void launch_prog_c(const DevMesh &m, int cur, const int *list, int n, int kind, 
 {
     if (n <= 0) return;
     (void)kind;   // the program decides the ring; the op needs C with the rings it reads
+    const bool ens = m.phs != nullptr;
     if (m.f32)
-        prog_c<float>(m, cur, list, n, op, prog, ph, s, err_slot);
+        ens ? prog_c<float, true>(m, cur, list, n, op, prog, ph, s, err_slot)
+            : prog_c<float, false>(m, cur, list, n, op, prog, ph, s, err_slot);
     else
-        prog_c<double>(m, cur, list, n, op, prog, ph, s, err_slot);
+        ens ? prog_c<double, true>(m, cur, list, n, op, prog, ph, s, err_slot)
+            : prog_c<double, false>(m, cur, list, n, op, prog, ph, s, err_slot);
 }
 
 void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Prog &prog, bool stream_after,
@@ -1288,17 +1337,19 @@ void launch_prolong(const DevMesh &m, int cur, const int *pairs, int n, const Ph
 }
 
 void launch_init_fields(const DevMesh &m, int cur, const int *list, int n, const Phys &ph, double R, hipStream_t s,
-                        const double *bub, int nb)
+                        const double *bub, int nb, const double *inst_R)
 {
     if (n <= 0) return;
     if (m.f32)
-        hipLaunchKernelGGL(k_init_fields<float>, dim3(n), dim3(256), 0, s, typed<float>(m), cur, list, ph, R, bub, nb);
+        hipLaunchKernelGGL(k_init_fields<float>, dim3(n), dim3(256), 0, s, typed<float>(m), cur, list, ph, R, bub, nb,
+                           inst_R);
     else
         hipLaunchKernelGGL(k_init_fields<double>, dim3(n), dim3(256), 0, s, typed<double>(m), cur, list, ph, R, bub,
-                           nb);
+                           nb, inst_R);
 }
 
-void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, double *partial, double *out, hipStream_t s)
+void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, double *partial, double *out, hipStream_t s,
+                 int ninst, double *out_inst)
 {
     if (n <= 0) return;
     if (m.f32)
@@ -1306,6 +1357,9 @@ void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, doubl
     else
         hipLaunchKernelGGL(k_diag_partial<double>, dim3(n), dim3(256), 0, s, typed<double>(m), list, ph, partial);
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, partial, n, out);
+    if (ninst > 0 && out_inst)
+        hipLaunchKernelGGL(k_diag_final_inst, dim3(ninst), dim3(256), 0, s, typed<double>(m), list, partial, n,
+                           out_inst);
 }
 
 }  // namespace amrlbm

@@ -42,6 +42,12 @@ int fail(int code, const std::string &msg)
             return fail(AMRLBM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
     } while (0)
 
+#define NEED_READY(m)                                                       \
+    do {                                                                    \
+        if (!(m)) return fail(AMRLBM_ERR_ARG, "null mesh");                 \
+        if (!(m)->ready) return fail(AMRLBM_ERR_STATE, "mesh not set up"); \
+    } while (0)
+
 struct Block {
     int level = -1, dx = 0, I = 0, J = 0, parent = 0;
     int child[4] = {0, 0, 0, 0};
@@ -146,6 +152,18 @@ struct amrlbm_mesh {
     std::vector<GraphEntry> graphs;
     int topo_version = 0;
 
+    // an ensemble mesh (amrlbm_setup_ensemble): independent instances of two
+    // roots each, every one with its own root cell size, level range, fluid
+    // and bubble; ens is 0 in any other mesh
+    int ens = 0;
+    std::vector<amrlbm_instance> inst;
+    Phys *d_phs = nullptr;           // [ens] device tables the kernels index by BlockDev::inst
+    PhysF *d_phsf = nullptr;
+    int *d_lv = nullptr;             // [ens][2] min, max levels
+    double *d_iR = nullptr;          // [ens] bubble radius (initial P)
+    double *d_diag_inst = nullptr;   // [ens][9]
+    int ens_cap = 0;
+
     int nlev() const { return prm.max_levels + 1; }
     int width(int L) const { return 1 << L; }
     int height(int L) const { return prm.nroots << L; }
@@ -165,6 +183,18 @@ struct HostTimer {
         acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 };
+
+// instance of a block in an ensemble mesh (its roots' pair), else 0
+int inst_of(const amrlbm_mesh *m, const Block &b) { return m->ens ? (b.J >> b.level) / m->tile_roots : 0; }
+// the level range that governs a block: its instance's in an ensemble mesh
+int min_level_of(const amrlbm_mesh *m, const Block &b)
+{
+    return m->ens ? m->inst[inst_of(m, b)].min_levels : m->prm.min_levels;
+}
+int max_level_of(const amrlbm_mesh *m, const Block &b)
+{
+    return m->ens ? m->inst[inst_of(m, b)].max_levels : m->prm.max_levels;
+}
 
 // ---------------------------------------------------------------------------
 // topology
@@ -721,7 +751,7 @@ int upload_tables(amrlbm_mesh *m)
         d.isleaf = b.isleaf ? 1 : 0;
         d.child0 = (b.level >= 0 && !b.isleaf && b.has_child) ? b.child[0] - 1 : -1;
         d.hasleaf = (b.level >= 0 && b.hasleaf) ? 1 : 0;
-        d.pad1 = 0;
+        d.inst = b.level >= 0 ? inst_of(m, b) : 0;
         if (b.level >= 0)
             for (int k = 1; k <= 8; k++) {
                 int v = nb_value(m, n, k);
@@ -849,7 +879,10 @@ int phase_macro_g(amrlbm_mesh *m)
 // ---------------------------------------------------------------------------
 // regrid: Refine_Derefine_Mesh (Refine_Derefine.f90:2-14)
 // ---------------------------------------------------------------------------
-int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector<char> *extra = nullptr)
+// extra: blocks flagged for refinement besides the criterion; frozen: the
+// instances of an ensemble mesh left as they are (no flags)
+int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector<char> *extra = nullptr,
+           const std::vector<char> *frozen = nullptr)
 {
     int rc = upload_tables(m);
     if (rc) return rc;
@@ -863,15 +896,16 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
     std::vector<double> err(m->nblks + 1, 0.0);
     for (int k = 0; k < nleaf; k++) err[m->disp[k]] = m->h_pinned[k];
 
-    const amrlbm_params &p = m->prm;
+    auto is_frozen = [&](const Block &b) { return frozen && (*frozen)[inst_of(m, b)]; };
     // Flag_for_refinement (Refine_Derefine.f90:18-54)
     for (int n = 1; n <= m->nblks; n++) {
         Block &b = m->blk[n];
         b.refin = false;
-        if (b.isleaf) {
-            if (b.level < p.min_levels) b.refin = true;
-            else if (b.level < p.max_levels && err[n] > refin_tol) b.refin = true;
-            else if (extra && b.level < p.max_levels && (*extra)[n]) b.refin = true;   // predicted
+        if (b.isleaf && !is_frozen(b)) {
+            const int lo = min_level_of(m, b), hi = max_level_of(m, b);
+            if (b.level < lo) b.refin = true;
+            else if (b.level < hi && err[n] > refin_tol) b.refin = true;
+            else if (extra && b.level < hi && (*extra)[n]) b.refin = true;   // predicted
         }
     }
     // Enforce_1_Level_Jump / Check_Level_Jump (:58-93): least fixed point
@@ -893,7 +927,7 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
     for (int n = 1; n <= m->nblks; n++) {
         Block &b = m->blk[n];
         b.deref = false;
-        if (!b.hasleaf || b.level < p.min_levels) continue;
+        if (!b.hasleaf || b.level < min_level_of(m, b) || is_frozen(b)) continue;
         int c1 = b.child[0];
         bool any = false;
         for (int c = c1; c <= c1 + 3; c++) any = any || m->blk[c].refin;
@@ -1082,7 +1116,7 @@ void reset_topology(amrlbm_mesh *m)
 }
 
 // analytic phase field of Initialize_AMR_Rising_Bubble (AMR_Rising_2D_OMP.f90:174-185)
-void bubble_c(const amrlbm_mesh *m, const Block &b, double R, double X0, double Y0, double *C)
+void bubble_c(const amrlbm_mesh *m, const Block &b, double R, double X0, double Y0, double W, double *C)
 {
     const int dx = b.dx;
     const int J = b.J % m->tile_height(b.level);   // the instance's own frame
@@ -1091,7 +1125,7 @@ void bubble_c(const amrlbm_mesh *m, const Block &b, double R, double X0, double 
             double Xc = (double)(b.I * dx * NX) + (double)dx * ((double)i - 0.5);
             double Yc = (double)(J * dx * NY) + (double)dx * ((double)j - 0.5);
             double Ri = std::sqrt((Xc - X0) * (Xc - X0) + (Yc - Y0) * (Yc - Y0));
-            C[(j + 1) * CW + (i + 1)] = 0.5 + 0.5 * std::tanh(2.0 * (Ri - R) / m->ph.W);
+            C[(j + 1) * CW + (i + 1)] = 0.5 + 0.5 * std::tanh(2.0 * (Ri - R) / W);
         }
 }
 
@@ -1101,7 +1135,7 @@ int init_bubble(amrlbm_mesh *m, double R, double X0, double Y0)
     if (rc) return rc;
     const size_t nl = m->leaf.size();
     std::vector<double> cbuf(nl * CS);
-    for (size_t k = 0; k < nl; k++) bubble_c(m, m->blk[m->leaf[k]], R, X0, Y0, &cbuf[k * CS]);
+    for (size_t k = 0; k < nl; k++) bubble_c(m, m->blk[m->leaf[k]], R, X0, Y0, m->ph.W, &cbuf[k * CS]);
     const double cs = m->dm.f32 ? 1.0 : 0.0;
     for (size_t k = 0; k < nl; k++) {
         int rc2 = upload(m, m->dm.C, (size_t)(m->leaf[k] - 1) * CS, CS, &cbuf[k * CS], [cs](size_t) { return cs; });
@@ -1234,37 +1268,35 @@ extern "C" {
 
 const char *amrlbm_last_error(void) { return g_last_error.c_str(); }
 
-static int create_mesh(const amrlbm_params *p, bool f32, amrlbm_mesh **out)
+// the kernels' constants from the module variables (D2Q9_Mod.f90:10-16); the
+// quotients with the IEEE divisions the kernels would use
+static Phys phys_from(const amrlbm_params *p)
 {
-    if (!p || !out) return fail(AMRLBM_ERR_ARG, "null argument");
-    *out = nullptr;
-    if (p->nx != NX || p->ny != NY) return fail(AMRLBM_ERR_ARG, "this build supports 16 x 16 blocks only");
-    if (p->nroots < 1 || p->min_levels < 0 || p->max_levels < p->min_levels || p->max_levels > 14 ||
-        p->max_blocks < p->nroots || p->L0 != (NX << p->max_levels))
-        return fail(AMRLBM_ERR_ARG, "inconsistent mesh parameters (L0 must be nx * 2^max_levels)");
-    amrlbm_mesh *m = new amrlbm_mesh();
-    m->prm = *p;
-    m->tile_roots = p->nroots;
-    m->ph.M0 = p->M0;
-    m->ph.wc = 1.0 / (0.5 + p->M0 * 3.0);
-    m->ph.W = p->W;
-    m->ph.sigma = p->sigma;
-    m->ph.beta = p->beta;
-    m->ph.kappa = p->kappa;
-    m->ph.taul = p->taul;
-    m->ph.tauh = p->tauh;
-    m->ph.rhol = p->rhol;
-    m->ph.rhoh = p->rhoh;
-    m->ph.gy = p->gy;
-    m->ph.drho3 = (p->rhoh - p->rhol) / 3.0;
-    m->ph.drho6 = (p->rhoh - p->rhol) / 6.0;
-    m->ph.inv_tauh = 1.0 / p->tauh;
-    m->ph.inv_taul = 1.0 / p->taul;
-    m->ph.inv_W = 1.0 / p->W;
-    m->dm.f32 = f32 ? 1 : 0;
-    PhysF &f = m->dm.phf;
+    Phys ph{};
+    ph.M0 = p->M0;
+    ph.wc = 1.0 / (0.5 + p->M0 * 3.0);
+    ph.W = p->W;
+    ph.sigma = p->sigma;
+    ph.beta = p->beta;
+    ph.kappa = p->kappa;
+    ph.taul = p->taul;
+    ph.tauh = p->tauh;
+    ph.rhol = p->rhol;
+    ph.rhoh = p->rhoh;
+    ph.gy = p->gy;
+    ph.drho3 = (p->rhoh - p->rhol) / 3.0;
+    ph.drho6 = (p->rhoh - p->rhol) / 6.0;
+    ph.inv_tauh = 1.0 / p->tauh;
+    ph.inv_taul = 1.0 / p->taul;
+    ph.inv_W = 1.0 / p->W;
+    return ph;
+}
+
+static PhysF physf_from(const amrlbm_params *p)
+{
+    PhysF f{};
     f.M0 = (float)p->M0;
-    f.wc = (float)m->ph.wc;
+    f.wc = (float)(1.0 / (0.5 + p->M0 * 3.0));
     f.beta = (float)p->beta;
     f.kappa = (float)p->kappa;
     f.taul = (float)p->taul;
@@ -1278,6 +1310,23 @@ static int create_mesh(const amrlbm_params *p, bool f32, amrlbm_mesh **out)
     f.inv_tauh = (float)(1.0 / p->tauh);
     f.inv_taul = (float)(1.0 / p->taul);
     f.k12W = (float)(12.0 * p->M0 / p->W);
+    return f;
+}
+
+static int create_mesh(const amrlbm_params *p, bool f32, amrlbm_mesh **out)
+{
+    if (!p || !out) return fail(AMRLBM_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (p->nx != NX || p->ny != NY) return fail(AMRLBM_ERR_ARG, "this build supports 16 x 16 blocks only");
+    if (p->nroots < 1 || p->min_levels < 0 || p->max_levels < p->min_levels || p->max_levels > 14 ||
+        p->max_blocks < p->nroots || p->L0 != (NX << p->max_levels))
+        return fail(AMRLBM_ERR_ARG, "inconsistent mesh parameters (L0 must be nx * 2^max_levels)");
+    amrlbm_mesh *m = new amrlbm_mesh();
+    m->prm = *p;
+    m->tile_roots = p->nroots;
+    m->ph = phys_from(p);
+    m->dm.f32 = f32 ? 1 : 0;
+    m->dm.phf = physf_from(p);
     int rc = alloc_device(m);
     if (rc) {
         amrlbm_destroy(m);
@@ -1318,6 +1367,11 @@ int amrlbm_destroy(amrlbm_mesh *m)
     if (m->ev_check) (void)hipEventDestroy(m->ev_check);
     (void)hipFree(m->d_part);
     (void)hipFree(m->d_diag);
+    (void)hipFree(m->d_phs);
+    (void)hipFree(m->d_phsf);
+    (void)hipFree(m->d_lv);
+    (void)hipFree(m->d_iR);
+    (void)hipFree(m->d_diag_inst);
     if (m->h_pinned) (void)hipHostFree(m->h_pinned);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
     if (m->ev1) (void)hipEventDestroy(m->ev1);
@@ -1335,12 +1389,15 @@ int amrlbm_setup_rising_bubble(amrlbm_mesh *m, double R, double x0, double y0)
 
 // AMR_LBM_Rising_Bubble set-up (AMR_Rising_2D_OMP.f90:51-67) with init(m)
 // setting the fields of every leaf (Initialize_AMR_Rising_Bubble)
+static void clear_ensemble(amrlbm_mesh *m);
+
 template <class Init>
 static int setup_mesh(amrlbm_mesh *m, int ninst, Init init)
 {
     if (!m) return fail(AMRLBM_ERR_ARG, "null mesh");
     if (ninst < 1 || m->prm.nroots % ninst)
         return fail(AMRLBM_ERR_ARG, "the roots must split evenly into the instances");
+    clear_ensemble(m);
     m->tile_roots = m->prm.nroots / ninst;
     reset_topology(m);
     m->cur = 0;
@@ -1420,6 +1477,201 @@ int amrlbm_setup_bubbles(amrlbm_mesh *m, int n, const double *R, const double *x
     return rc;
 }
 
+}  // extern "C"
+
+// An ensemble mesh goes back to one fluid and level range (a set-up or
+// topology of another kind replaces it).
+static void clear_ensemble(amrlbm_mesh *m)
+{
+    m->ens = 0;
+    m->inst.clear();
+    m->dm.phs = nullptr;
+    m->dm.phsf = nullptr;
+    m->dm.lv = nullptr;
+    m->dm.tile_roots = 0;
+}
+
+// Fields of every leaf of an ensemble mesh (Initialize_AMR_Rising_Bubble,
+// AMR_Rising_2D_OMP.f90:142-231): C from each instance's bubble and W on the
+// host, as init_bubble, then U, P, h, g with the instance's constants.
+static int init_ensemble(amrlbm_mesh *m)
+{
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    const size_t nl = m->leaf.size();
+    std::vector<double> cbuf(nl * CS);
+    for (size_t k = 0; k < nl; k++) {
+        const Block &b = m->blk[m->leaf[k]];
+        const amrlbm_instance &q = m->inst[inst_of(m, b)];
+        bubble_c(m, b, q.R, q.x0, q.y0, q.W, &cbuf[k * CS]);
+    }
+    const double cs = m->dm.f32 ? 1.0 : 0.0;
+    for (size_t k = 0; k < nl; k++) {
+        rc = upload(m, m->dm.C, (size_t)(m->leaf[k] - 1) * CS, CS, &cbuf[k * CS], [cs](size_t) { return cs; });
+        if (rc) return rc;
+    }
+    launch_init_fields(m->dm, m->cur, m->d_lists + m->all_off, (int)nl, m->ph, 0.0, m->stream, nullptr, 0, m->d_iR);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return AMRLBM_OK;
+}
+
+// AMR_LBM_Rising_Bubble (AMR_Rising_2D_OMP.f90:51-67) for every instance at
+// once. Initial_Mesh refines, level by level and in id order, the blocks whose
+// instance's min_levels lies above their level; pass r of the init-regrid
+// loop regrids the instances with r < max_levels - min_levels (the others are
+// frozen: a pass beyond an instance's own count could derefine it) and sets
+// up the fields of every leaf again (a function of the mesh alone, so a frozen
+// instance's fields come out the same).
+static int setup_ensemble_mesh(amrlbm_mesh *m)
+{
+    const int n = m->ens;
+    reset_topology(m);
+    m->cur = 0;
+    for (int N = 1; N <= 2 * n; N++) {   // Initialize_Base_Block (AMR_Rising_2D_OMP.f90:79-115)
+        const amrlbm_instance &q = m->inst[(N - 1) / 2];
+        Block &b = m->blk[N];
+        b.level = 0;
+        b.dx = q.L0 / NX;
+        b.I = 0;
+        b.J = N - 1;
+        b.has_child = true;
+        for (int k = 0; k < 4; k++) b.child[k] = -1;
+        b.parent = -1;
+        b.isleaf = true;
+        m->nblks = N;
+        place(m, N);
+    }
+    const size_t S = (size_t)m->prm.max_blocks;
+    HIPCHK(hipMemsetAsync(m->dm.pop[0], 0, esz(m) * POPB * S, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.C, 0, esz(m) * CS * S, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.PU, 0, esz(m) * PUB * S, m->stream));
+    int top = 0, passes = 0;
+    for (const amrlbm_instance &q : m->inst) {
+        top = std::max(top, q.min_levels);
+        passes = std::max(passes, q.max_levels - q.min_levels);
+    }
+    for (int L = 1; L <= top; L++) {   // Initial_Mesh (AMR_Rising_2D_OMP.f90:493-513)
+        const int nend = m->nblks;
+        for (int id = 1; id <= nend; id++) {
+            const Block &b = m->blk[id];
+            if (b.level == L - 1 && b.isleaf && L <= min_level_of(m, b)) {
+                int rc = add_child_blocks(m, id, m->nblks + 1);
+                if (rc) return rc;
+            }
+        }
+    }
+    array_of_leaf_blocks(m);
+    determine_min_max_levels(m);
+    int rc = init_ensemble(m);
+    if (rc) return rc;
+    std::vector<char> frozen(n, 0);
+    for (int r = 0; r < passes; r++) {
+        for (int q = 0; q < n; q++) frozen[q] = r >= m->inst[q].max_levels - m->inst[q].min_levels;
+        determine_min_max_levels(m);
+        if ((rc = regrid(m, 0.002, 0.001, nullptr, &frozen))) return rc;
+        if ((rc = init_ensemble(m))) return rc;
+    }
+    m->T = 0;
+    determine_min_max_levels(m);
+    m->ready = true;
+    return check_err_flags(m);
+}
+
+extern "C" {
+
+int amrlbm_setup_ensemble(amrlbm_mesh *m, int n, const amrlbm_instance *inst)
+{
+    if (!m || n < 1 || !inst) return fail(AMRLBM_ERR_ARG, "bad ensemble arguments");
+    if (m->prm.nroots != 2 * n) return fail(AMRLBM_ERR_ARG, "an ensemble of n instances needs nroots = 2 n");
+    for (int q = 0; q < n; q++) {
+        const amrlbm_instance &d = inst[q];
+        if (d.min_levels < 0 || d.max_levels < d.min_levels || d.max_levels > m->prm.max_levels ||
+            d.L0 != (NX << d.max_levels))
+            return fail(AMRLBM_ERR_ARG, "instance " + std::to_string(q) +
+                                            ": bad level range (L0 = nx 2^max_levels <= the mesh's L0)");
+        if (!(d.R > 0.0) || !(d.W > 0.0) || !(d.tauh > 0.0) || !(d.taul > 0.0) || !(d.rhoh > d.rhol) ||
+            !(d.rhol > 0.0))
+            return fail(AMRLBM_ERR_ARG, "instance " + std::to_string(q) + ": bad fluid or bubble");
+    }
+    clear_ensemble(m);
+    if (m->ens_cap < n) {
+        (void)hipFree(m->d_phs);
+        (void)hipFree(m->d_phsf);
+        (void)hipFree(m->d_lv);
+        (void)hipFree(m->d_iR);
+        (void)hipFree(m->d_diag_inst);
+        m->d_phs = nullptr;
+        m->d_phsf = nullptr;
+        m->d_lv = nullptr;
+        m->d_iR = nullptr;
+        m->d_diag_inst = nullptr;
+        m->ens_cap = 0;
+        HIPCHK(hipMalloc(&m->d_phs, sizeof(Phys) * n));
+        HIPCHK(hipMalloc(&m->d_phsf, sizeof(PhysF) * n));
+        HIPCHK(hipMalloc(&m->d_lv, sizeof(int) * 2 * n));
+        HIPCHK(hipMalloc(&m->d_iR, sizeof(double) * n));
+        HIPCHK(hipMalloc(&m->d_diag_inst, sizeof(double) * 9 * n));
+        m->ens_cap = n;
+    }
+    std::vector<Phys> ph(n);
+    std::vector<PhysF> phf(n);
+    std::vector<int> lv(2 * n);
+    std::vector<double> R(n);
+    for (int q = 0; q < n; q++) {
+        const amrlbm_instance &d = inst[q];
+        amrlbm_params p = m->prm;   // the instance's constants as a one-domain mesh would hold them
+        p.M0 = d.M0; p.W = d.W; p.sigma = d.sigma; p.beta = d.beta; p.kappa = d.kappa;
+        p.taul = d.taul; p.tauh = d.tauh; p.rhol = d.rhol; p.rhoh = d.rhoh; p.gy = d.gy;
+        ph[q] = phys_from(&p);
+        phf[q] = physf_from(&p);
+        lv[2 * q] = d.min_levels;
+        lv[2 * q + 1] = d.max_levels;
+        R[q] = d.R;
+    }
+    HIPCHK(hipMemcpyAsync(m->d_phs, ph.data(), sizeof(Phys) * n, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->d_phsf, phf.data(), sizeof(PhysF) * n, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->d_lv, lv.data(), sizeof(int) * 2 * n, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->d_iR, R.data(), sizeof(double) * n, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    m->ens = n;
+    m->inst.assign(inst, inst + n);
+    m->tile_roots = 2;
+    m->dm.tile_roots = 2;
+    m->dm.phs = m->d_phs;
+    m->dm.phsf = m->d_phsf;
+    m->dm.lv = m->d_lv;
+    int rc = setup_ensemble_mesh(m);
+    if (rc) clear_ensemble(m);
+    return rc;
+}
+
+int amrlbm_output_instances(amrlbm_mesh *m, int n, double *out)
+{
+    NEED_READY(m);
+    if (!out || !m->ens || n != m->ens) return fail(AMRLBM_ERR_ARG, "not an ensemble mesh of n instances");
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    launch_diag(m->dm, m->d_lists + m->all_off, (int)m->leaf.size(), m->ph, m->d_part, m->d_diag, m->stream, n,
+                m->d_diag_inst);
+    HIPCHK(hipGetLastError());
+    std::vector<double> d(9 * (size_t)n);
+    HIPCHK(hipMemcpyAsync(d.data(), m->d_diag_inst, sizeof(double) * d.size(), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    double bad = 0.0;
+    for (int q = 0; q < n; q++) {
+        const double *a = &d[9 * (size_t)q];
+        double *o = out + 9 * (size_t)q;
+        for (int k = 0; k < 9; k++) o[k] = a[k];
+        o[5] = a[5] / a[4];
+        o[6] = a[6] / a[4];
+        o[7] = a[7] / a[4];
+        bad += a[8];
+    }
+    if (bad != 0.0) return fail(AMRLBM_ERR_NONFINITE, "non-finite phase field in " + std::to_string((long)bad) + " cells");
+    return AMRLBM_OK;
+}
+
 int amrlbm_get_counts(amrlbm_mesh *m, int counts[7])
 {
     if (!m || !counts) return fail(AMRLBM_ERR_ARG, "null argument");
@@ -1457,6 +1709,7 @@ static int set_topology(amrlbm_mesh *m, int nblks, const int *blocks, int kk, in
     if (!m || !blocks || nblks < 1 || nblks > m->prm.max_blocks || nfree < 0 || (nfree > 0 && !freeq))
         return fail(AMRLBM_ERR_ARG, "bad topology arguments");
     if (ninst < 1 || m->prm.nroots % ninst) return fail(AMRLBM_ERR_ARG, "the roots must split evenly into the instances");
+    clear_ensemble(m);
     m->tile_roots = m->prm.nroots / ninst;
     reset_topology(m);
     m->nblks = nblks;
@@ -1510,6 +1763,7 @@ int amrlbm_save_snapshot(amrlbm_mesh *m, const char *path)
 {
     if (!m || !path) return fail(AMRLBM_ERR_ARG, "null argument");
     if (!m->ready) return fail(AMRLBM_ERR_STATE, "mesh not set up");
+    if (m->ens) return fail(AMRLBM_ERR_ARG, "an ensemble mesh has no restart file (one fluid per file)");
     HIPCHK(hipStreamSynchronize(m->stream));
     std::vector<int> rows((size_t)m->nblks * 21), freeq(m->freeq.size() + 1);
     int rc = amrlbm_get_topology(m, rows.data(), freeq.data());
@@ -1770,6 +2024,7 @@ int amrlbm_tecplot_cell_center(amrlbm_mesh *m, const char *path, double D)
     if (!m) return fail(AMRLBM_ERR_ARG, "null mesh");
     if (!m->ready) return fail(AMRLBM_ERR_STATE, "mesh not set up");
     if (!path || !(D > 0.0)) return fail(AMRLBM_ERR_ARG, "bad Tecplot arguments");
+    if (m->ens) return fail(AMRLBM_ERR_ARG, "an ensemble mesh has no Tecplot file (one domain per file)");
     const size_t nb = (size_t)m->nblks;
     std::vector<double> C(nb * CS), PU(nb * PUB);
     const bool f32 = m->dm.f32;
@@ -1841,12 +2096,6 @@ int amrlbm_determine_min_max_levels(amrlbm_mesh *m, int *levelmin, int *levelmax
     if (levelmax) *levelmax = m->levelmax;
     return AMRLBM_OK;
 }
-
-#define NEED_READY(m)                                                       \
-    do {                                                                    \
-        if (!(m)) return fail(AMRLBM_ERR_ARG, "null mesh");                 \
-        if (!(m)->ready) return fail(AMRLBM_ERR_STATE, "mesh not set up"); \
-    } while (0)
 
 int amrlbm_fill_ghost_cells_c2(amrlbm_mesh *m)
 {

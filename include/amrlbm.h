@@ -78,6 +78,31 @@ int amrlbm_setup_replicas(amrlbm_mesh *m, int ninst, double R, double x0, double
  * that, with the host's tanh, for bit-exact parity). */
 int amrlbm_setup_bubbles(amrlbm_mesh *m, int n, const double *R, const double *x0, const double *y0);
 
+/* One rising-bubble instance of an ensemble: its domain of two roots stacked
+ * in y (L0 x 2 L0 finest cells, L0 = nx 2^max_levels, so its finest cells
+ * are 1), its level range, its fluid constants (as amrlbm_params) and its
+ * bubble in its own frame. */
+typedef struct {
+    int L0, min_levels, max_levels;
+    double M0, W, sigma, beta, kappa, taul, tauh, rhol, rhoh, gy;
+    double R, x0, y0;
+} amrlbm_instance;
+/* n independent rising-bubble domains of uneven size, level range and fluid
+ * in one mesh (nroots = 2 n; the mesh's max_levels at least every instance's,
+ * its max_blocks their total): instance q has roots 2q+1 and 2q+2 with
+ * bounce-back walls at its ends. Set-up as AMR_LBM_Rising_Bubble
+ * (AMR_Rising_2D_OMP.f90:51-67) per instance: roots of cell size L0/nx,
+ * Initial_Mesh to its min_levels, its phase field profile with its W (host,
+ * as amrlbm_setup_rising_bubble), and its own max_levels - min_levels passes
+ * of the init-regrid loop. Every launch of the step then covers the leaves
+ * of all instances; each kernel takes a block's fluid constants from its
+ * instance and each regrid its level range. Every instance evolves as it
+ * would alone. No counterpart in the reference (one domain per program). */
+int amrlbm_setup_ensemble(amrlbm_mesh *m, int n, const amrlbm_instance *inst);
+/* Output (AMR_Rising_2D_OMP.f90:456-489) of each instance of an ensemble
+ * mesh in its own frame, out[n][9] as amrlbm_output's. */
+int amrlbm_output_instances(amrlbm_mesh *m, int n, double *out);
+
 /* ---- state exchange with the caller's Blk(:) mirror ----------------------- */
 /* counts: Nblks, nLeaf, LevelMin, LevelMax, KK, free-list length, step T */
 int amrlbm_get_counts(amrlbm_mesh *m, int counts[7]);
