@@ -136,9 +136,20 @@ class Ensemble:
         raise AttributeError(name)
 
 
-def ensemble(n=64, seed=2024, block_cap=32768, name=None):
-    """The batched ensemble: n instances drawn from seed."""
-    return Ensemble(name or f"ens{n}_s{seed}", tuple(draw(n, seed)), block_cap)
+def base_blocks(members):
+    """Blocks of the instances' base meshes (two roots refined uniformly to
+    min_levels): 2 (4^(min+1) - 1) / 3 each."""
+    return sum(2 * (4 ** (w.min_levels + 1) - 1) // 3 for w in members)
+
+
+def ensemble(n=64, seed=2024, block_cap=None, name=None):
+    """The batched ensemble: n instances drawn from seed; by default room for
+    twice the base meshes' blocks (the interfaces' refinement and the free
+    list), in multiples of 4096."""
+    members = tuple(draw(n, seed))
+    if block_cap is None:
+        block_cap = -(-2 * base_blocks(members) // 4096) * 4096
+    return Ensemble(name or f"ens{n}_s{seed}", members, block_cap)
 
 
 def instance_of(level, J):

@@ -196,5 +196,8 @@ void launch_init_fields(const DevMesh &m, int cur, const int *list, int n, const
 // mesh) also per instance into out_inst[ninst][9]
 void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, double *partial, double *out,
                  hipStream_t s, int ninst = 0, double *out_inst = nullptr);
+// n copies of 16-byte words: copy k takes desc[3k+2] words from src + desc[3k]
+// to dst + desc[3k+1] (offsets in words)
+void launch_scatter_words(const void *src, const int *desc, int n, void *dst, hipStream_t s);
 
 }  // namespace amrlbm

@@ -1240,6 +1240,16 @@ __global__ __launch_bounds__(256) void k_diag_final_inst(DevMeshT<double> m, con
     if (tid < 9) out[(size_t)q * 9 + tid] = red[tid][0];
 }
 
+// Ghost-program upload: descriptor k copies desc[3k+2] 16-byte words from
+// src + desc[3k] to dst + desc[3k+1] (one workgroup each).
+__global__ __launch_bounds__(256) void k_scatter_words(const int4 *__restrict__ src, const int *__restrict__ desc,
+                                                       int4 *__restrict__ dst)
+{
+    const int k = blockIdx.x;
+    const int so = desc[3 * k], d0 = desc[3 * k + 1], n = desc[3 * k + 2];
+    for (int t = threadIdx.x; t < n; t += 256) dst[d0 + t] = src[so + t];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1360,6 +1370,13 @@ void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, doubl
     if (ninst > 0 && out_inst)
         hipLaunchKernelGGL(k_diag_final_inst, dim3(ninst), dim3(256), 0, s, typed<double>(m), list, partial, n,
                            out_inst);
+}
+
+void launch_scatter_words(const void *src, const int *desc, int n, void *dst, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scatter_words, dim3(n), dim3(256), 0, s, static_cast<const int4 *>(src), desc,
+                       static_cast<int4 *>(dst));
 }
 
 }  // namespace amrlbm

@@ -85,6 +85,7 @@ struct ProgCache {
     LeafProg p[3];
     std::vector<std::pair<int, BlockSig>> deps;
     bool valid = false;
+    bool on_device = false;   // p is in the block's slot of the device arenas
 };
 
 }  // namespace
@@ -132,11 +133,19 @@ struct amrlbm_mesh {
     double host_ms_regrid = 0.0, host_ms_tables = 0.0, host_ms_programs = 0.0, host_ms_graphs = 0.0;
 
     // ghost programs per fill kind (C1, C2, HG), leaf-list order
+    // the programs on the device: per kind, block id n's nodes at
+    // (n - 1) * kProgSlot[kind] of d_pnode (an arena of prog_slots slots),
+    // so a mesh change uploads only the leaves whose programs were rebuilt;
+    // per leaf (launch order) a header pointing there
     PHdr *d_phdr[3] = {nullptr, nullptr, nullptr};
     PNode *d_pnode[3] = {nullptr, nullptr, nullptr};
-    size_t phdr_cap[3] = {0, 0, 0}, pnode_cap[3] = {0, 0, 0};
+    size_t phdr_cap[3] = {0, 0, 0};
+    int prog_slots = 0;
     void *h_stage[3] = {nullptr, nullptr, nullptr};   // pinned upload staging per kind
     size_t stage_cap[3] = {0, 0, 0};
+    void *d_stage[3] = {nullptr, nullptr, nullptr};   // the same on the device (nodes, then copy descriptors)
+    size_t dstage_cap[3] = {0, 0, 0};
+    long long prog_nodes_uploaded = 0;
     Prog prog[3] = {};
     std::vector<ProgCache> pcache;   // by block id
     long long progs_built = 0;       // leaves whose programs were (re)built
@@ -573,6 +582,10 @@ void build_leaf_program(amrlbm_mesh *m, int kind, int n, LeafProg &out, std::vec
     }
 }
 
+// nodes per block slot in the device arenas, by kind (C1, C2, HG): the most
+// a program may hold (build_leaf_program refuses more)
+constexpr int kProgSlot[3] = {NODES_C1, NODES_C2, NODES_HG};
+
 bool cache_current(const amrlbm_mesh *m, int id)
 {
     const ProgCache &pc = m->pcache[id];
@@ -592,11 +605,14 @@ int build_programs(amrlbm_mesh *m)
     HostTimer timer(m->host_ms_programs);
     const int nl = (int)m->leaf.size();
     if (m->pcache.size() < m->blk.size()) m->pcache.resize(m->blk.size());
+    const int nthr = std::max(1, std::min(omp_get_max_threads(), 3 * nl / 32));
+    std::vector<char> stale(nl);
+#pragma omp parallel for schedule(static) num_threads(nthr)
+    for (int k = 0; k < nl; k++) stale[k] = !cache_current(m, m->disp[k]);
     std::vector<int> dirty;
     for (int k = 0; k < nl; k++)
-        if (!cache_current(m, m->disp[k])) dirty.push_back(m->disp[k]);
-    const int nd = (int)dirty.size(), ntask = 3 * nd;
-    const int nthr = std::max(1, std::min(omp_get_max_threads(), std::max(ntask, 3 * nl) / 32));
+        if (stale[k]) dirty.push_back(m->disp[k]);
+    const int nd = (int)dirty.size();
 #pragma omp parallel num_threads(nthr)
     {
         std::vector<int> d;
@@ -604,6 +620,7 @@ int build_programs(amrlbm_mesh *m)
         for (int i = 0; i < nd; i++) {   // one leaf's three programs, then the blocks they read
             ProgCache &pc = m->pcache[dirty[i]];
             pc.valid = false;
+            pc.on_device = false;
             d.clear();
             for (int kind = 0; kind < 3; kind++) {
                 pc.p[kind] = LeafProg{};
@@ -636,59 +653,89 @@ int build_programs(amrlbm_mesh *m)
                                                      " differs from a fresh build");
         }
     }
+    // the arenas hold a slot per block id up to nblks; grown (and refilled) as ids pass it
+    if (m->nblks > m->prog_slots) {
+        const int cap = std::min(m->prm.max_blocks, std::max(m->nblks + m->nblks / 2, 1024));
+        for (int kind = 0; kind < 3; kind++) {
+            if (m->d_pnode[kind]) HIPCHK(hipFree(m->d_pnode[kind]));
+            m->d_pnode[kind] = nullptr;
+            HIPCHK(hipMalloc(&m->d_pnode[kind], sizeof(PNode) * kProgSlot[kind] * (size_t)cap));
+        }
+        m->prog_slots = cap;
+        for (ProgCache &pc : m->pcache) pc.on_device = false;
+    }
+    std::vector<int> up;   // leaves whose programs are not in their slots yet
+    for (int k = 0; k < nl; k++)
+        if (!m->pcache[m->disp[k]].on_device) up.push_back(m->disp[k]);
+    const int nup = (int)up.size();
+    constexpr int W = (int)(sizeof(PNode) / 16);   // 16-byte words per node
     for (int kind = 0; kind < 3; kind++) {
-        size_t total = 0;
         int naux = 0, nmax = 0;
         for (int k = 0; k < nl; k++) {
             const LeafProg &lp = m->pcache[m->disp[k]].p[kind];
-            total += lp.nodes.size();
             naux = std::max(naux, lp.naux);
             nmax = std::max(nmax, (int)lp.nodes.size());
         }
-        // staged in pinned host memory, so the upload runs at full link speed
-        const size_t need = sizeof(PHdr) * nl + sizeof(PNode) * total;
+        // pinned stage: the nodes of the leaves in up, packed, then one copy
+        // descriptor (source word, destination word, words) per leaf, then
+        // the headers of every leaf
+        std::vector<size_t> at(nup + 1, 0);
+        for (int i = 0; i < nup; i++) at[i + 1] = at[i] + m->pcache[up[i]].p[kind].nodes.size();
+        const size_t total = at[nup];
+        const size_t desc_off = sizeof(PNode) * total, hdr_off = desc_off + sizeof(int) * 3 * (size_t)nup;
+        const size_t need = hdr_off + sizeof(PHdr) * nl;
         if (need > m->stage_cap[kind]) {
             if (m->h_stage[kind]) HIPCHK(hipHostFree(m->h_stage[kind]));
             m->h_stage[kind] = nullptr;
             m->stage_cap[kind] = need + need / 4 + 4096;
             HIPCHK(hipHostMalloc(&m->h_stage[kind], m->stage_cap[kind], hipHostMallocDefault));
         }
-        PNode *all = reinterpret_cast<PNode *>(m->h_stage[kind]);
-        PHdr *hdr = reinterpret_cast<PHdr *>(all + total);
-        size_t pos = 0;
+        char *stage = static_cast<char *>(m->h_stage[kind]);
+        PNode *nodes = reinterpret_cast<PNode *>(stage);
+        int *desc = reinterpret_cast<int *>(stage + desc_off);
+        PHdr *hdr = reinterpret_cast<PHdr *>(stage + hdr_off);
+#pragma omp parallel for schedule(static) num_threads(nthr)
+        for (int i = 0; i < nup; i++) {
+            const LeafProg &lp = m->pcache[up[i]].p[kind];
+            std::memcpy(nodes + at[i], lp.nodes.data(), sizeof(PNode) * lp.nodes.size());
+            desc[3 * i] = (int)at[i] * W;
+            desc[3 * i + 1] = (up[i] - 1) * kProgSlot[kind] * W;
+            desc[3 * i + 2] = (int)lp.nodes.size() * W;
+        }
         for (int k = 0; k < nl; k++) {
             const LeafProg &lp = m->pcache[m->disp[k]].p[kind];
             PHdr &h = hdr[k];
-            h.off = (int)pos;
+            h.off = (m->disp[k] - 1) * kProgSlot[kind];
             h.nst = lp.nst;
             for (int st = 0; st < MAX_STAGES; st++) h.end[st] = lp.end[st];
-            pos += lp.nodes.size();
-        }
-        // the node copies (tens of MB on large meshes) on the build's threads
-#pragma omp parallel for schedule(static) num_threads(nthr)
-        for (int k = 0; k < nl; k++) {
-            const LeafProg &lp = m->pcache[m->disp[k]].p[kind];
-            std::memcpy(all + hdr[k].off, lp.nodes.data(), sizeof(PNode) * lp.nodes.size());
         }
         if ((size_t)nl > m->phdr_cap[kind]) {
             if (m->d_phdr[kind]) HIPCHK(hipFree(m->d_phdr[kind]));
-            m->phdr_cap[kind] = nl + 64;
+            m->d_phdr[kind] = nullptr;
+            m->phdr_cap[kind] = nl + nl / 4 + 64;
             HIPCHK(hipMalloc(&m->d_phdr[kind], sizeof(PHdr) * m->phdr_cap[kind]));
         }
-        if (total > m->pnode_cap[kind]) {
-            if (m->d_pnode[kind]) HIPCHK(hipFree(m->d_pnode[kind]));
-            m->pnode_cap[kind] = total + total / 4 + 256;
-            HIPCHK(hipMalloc(&m->d_pnode[kind], sizeof(PNode) * m->pnode_cap[kind]));
+        if (nup > 0) {
+            if (hdr_off > m->dstage_cap[kind]) {
+                if (m->d_stage[kind]) HIPCHK(hipFree(m->d_stage[kind]));
+                m->d_stage[kind] = nullptr;
+                m->dstage_cap[kind] = hdr_off + hdr_off / 4 + 4096;
+                HIPCHK(hipMalloc(&m->d_stage[kind], m->dstage_cap[kind]));
+            }
+            HIPCHK(hipMemcpyAsync(m->d_stage[kind], stage, hdr_off, hipMemcpyHostToDevice, m->stream));
+            launch_scatter_words(m->d_stage[kind], reinterpret_cast<const int *>(static_cast<char *>(m->d_stage[kind]) + desc_off),
+                                 nup, m->d_pnode[kind], m->stream);
+            HIPCHK(hipGetLastError());
         }
         if (nl > 0)
             HIPCHK(hipMemcpyAsync(m->d_phdr[kind], hdr, sizeof(PHdr) * nl, hipMemcpyHostToDevice, m->stream));
-        if (total > 0)
-            HIPCHK(hipMemcpyAsync(m->d_pnode[kind], all, sizeof(PNode) * total, hipMemcpyHostToDevice, m->stream));
         m->prog[kind].hdr = m->d_phdr[kind];
         m->prog[kind].nodes = m->d_pnode[kind];
         m->prog[kind].naux = naux;   // the hg kernel sizes its LDS by these
         m->prog[kind].nmax = nmax;
+        m->prog_nodes_uploaded += (long long)total;
     }
+    for (int id : up) m->pcache[id].on_device = true;
     HIPCHK(hipStreamSynchronize(m->stream));   // the pinned stages are reused by the next build
     return AMRLBM_OK;
 }
@@ -710,18 +757,24 @@ int ensure_int_buffer(int **buf, int *cap, int need)
 // XCD has its own L2, so the leaves go along a Morton curve (finest-cell
 // origin of the block), the curve is cut into 8 runs of neighbouring blocks,
 // and run c takes the positions k = c mod 8: a block and most of its
-// neighbours then run on one XCD in every launch.
+// neighbours then run on one XCD in every launch. In an ensemble mesh the
+// curve runs instance by instance, each in its own frame (instances differ
+// in cell size, so their absolute origins overlap).
 void dispatch_order(amrlbm_mesh *m)
 {
     const int n = (int)m->leaf.size();
     std::vector<std::pair<unsigned long long, int>> key(n);
+#pragma omp parallel for schedule(static) if (n > 4096)
     for (int k = 0; k < n; k++) {
         const Block &b = m->blk[m->leaf[k]];
-        const unsigned x = (unsigned)(b.I * b.dx * NX), y = (unsigned)(b.J * b.dx * NY);
+        const int q = inst_of(m, b), J = b.J - q * (m->tile_roots << b.level);
+        const unsigned x = (unsigned)(b.I * b.dx * NX), y = (unsigned)(J * b.dx * NY);
+        // an ensemble's frames lie below 2^19 (L0 <= nx 2^14, two roots), leaving bits 40.. for the instance
+        const int nbits = m->ens ? 20 : 31;
         unsigned long long z = 0;
-        for (int bit = 0; bit < 31; bit++)
+        for (int bit = 0; bit < nbits; bit++)
             z |= (unsigned long long)((x >> bit) & 1u) << (2 * bit) | (unsigned long long)((y >> bit) & 1u) << (2 * bit + 1);
-        key[k] = {z, m->leaf[k]};
+        key[k] = {(unsigned long long)q << 40 | z, m->leaf[k]};
     }
     std::sort(key.begin(), key.end());
     const int q = n / 8, r = n % 8;   // runs 0..r-1 hold q + 1 blocks, the others q
@@ -741,6 +794,8 @@ int upload_tables(amrlbm_mesh *m)
     const int nb = m->nblks;
     std::vector<BlockDev> bd(nb);
     std::vector<NbDesc> nbt((size_t)nb * 8, NbDesc{0, -1, 0, -1});
+    int bad_block = 0, bad_k = 0;   // a block without neighbour k (2:1 balance violated)
+#pragma omp parallel for schedule(static) if (nb > 4096)
     for (int n = 1; n <= nb; n++) {
         const Block &b = m->blk[n];
         BlockDev &d = bd[n - 1];
@@ -755,8 +810,14 @@ int upload_tables(amrlbm_mesh *m)
         if (b.level >= 0)
             for (int k = 1; k <= 8; k++) {
                 int v = nb_value(m, n, k);
-                if (v == 0) return fail(AMRLBM_ERR_TOPOLOGY, "block " + std::to_string(n) + " has no neighbour " +
-                                                                 std::to_string(k) + " (2:1 balance violated)");
+                if (v == 0) {
+#pragma omp critical(amrlbm_bad_nb)
+                    if (!bad_block || n < bad_block) {
+                        bad_block = n;
+                        bad_k = k;
+                    }
+                    continue;
+                }
                 NbDesc &e = nbt[(size_t)(n - 1) * 8 + k - 1];
                 e.id = v;
                 if (v > 0) {
@@ -767,6 +828,9 @@ int upload_tables(amrlbm_mesh *m)
                 }
             }
     }
+    if (bad_block)
+        return fail(AMRLBM_ERR_TOPOLOGY, "block " + std::to_string(bad_block) + " has no neighbour " +
+                                             std::to_string(bad_k) + " (2:1 balance violated)");
     // leaves grouped by level, each group ascending; then all leaves
     const int NL = m->nlev();
     m->lvl_off.assign(NL + 1, 0);
@@ -1355,6 +1419,7 @@ int amrlbm_destroy(amrlbm_mesh *m)
     for (int k = 0; k < 3; k++) {
         (void)hipFree(m->d_phdr[k]);
         (void)hipFree(m->d_pnode[k]);
+        (void)hipFree(m->d_stage[k]);
         if (m->h_stage[k]) (void)hipHostFree(m->h_stage[k]);
     }
     (void)hipFree(m->d_err_flags);
