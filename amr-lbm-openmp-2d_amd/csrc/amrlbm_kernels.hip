@@ -280,6 +280,11 @@ __device__ __forceinline__ void ed_m(const float *sC, int i, int j, float inv_dx
 
 // Gamma - w and heq - w (Equilibrium_h_g_AMR, LBM_Subroutines.f90:2-21):
 // heq - w = C~ Gamma + (Gamma - w) + C (12 (1 - C) / W) M0 w e.n
+// The nine sum to C~ exactly in real arithmetic. The binary lattice weights
+// sum to 1 + 7.5e-9, so summed term by term they would move C~ by that much
+// times C~ (-1 in the gas) every collision: a steady drift of the gas's phase
+// mass. heq_0 - w_0 is therefore C~ minus the other eight, which leaves only
+// rounding.
 __device__ __forceinline__ void equilibrium(float ct, float U, float V, float ni, float nj, const PhysF &ph,
                                             float gmw[9], float heqd[9])
 {
@@ -294,6 +299,10 @@ __device__ __forceinline__ void equilibrium(float ct, float U, float V, float ni
         gmw[i] = waf(i) * (eU * (3.0f + 4.5f * eU) - 1.5f * U2);
         heqd[i] = ct * (waf(i) + gmw[i]) + gmw[i] + C * tmp * waf(i) * eN;
     }
+    float rest = 0.0f;
+#pragma unroll
+    for (int i = 1; i < 9; i++) rest += heqd[i];
+    heqd[0] = ct - rest;
 }
 
 __device__ __forceinline__ void forcing(const float gmw[9], float drho3, float mu, const float e[9], float Ux,
