@@ -32,6 +32,13 @@ large_working_set repeats the measurement on 16 independent copies of the
 same domain in one mesh (about 4,600 leaf blocks, 0.5 GB of FP64 leaf state,
 past the 256 MB Infinity Cache), FP64 and FP32; second measurements, not the
 headline.
+
+multi_bubble runs the 32-bubble case (2048 x 4096, levels 3..7) in FP64,
+FP32 and FP32 with the learned regrid; ensemble runs 64 rising-bubble
+instances of uneven size, level range and fluid drawn from a seed, in one
+mesh (every launch over the leaves of all instances), FP32 and FP64, with
+each instance's FP32 error against FP64. Both report finest-equivalent MLUPS
+summed over the whole mesh.
 """
 import argparse
 import json
@@ -255,6 +262,54 @@ def multi_bubble(amrlbm, steps, k):
     return out
 
 
+def ensemble_run(amrlbm, n, seed, steps, k):
+    """The batched ensemble (amrlbm.ensemble): n instances drawn from seed in
+    one mesh, FP32 and FP64 over the same steps from the same set-up, and
+    each instance's FP32 mass, rise velocity and centroid against FP64."""
+    import numpy as np
+    from amrlbm.ensemble import ensemble
+    e = ensemble(n, seed)
+    out = {"workload": f"{n} rising-bubble instances drawn with seed {seed} (SplitMix64): domains L0 x 2 L0 with "
+                       f"L0 in {{256, 512, 1024}}, 2-4 levels, D 16..64, Eo 1..100, density ratio 10..1000, "
+                       f"viscosity ratio 1..100, W=5; one mesh, every launch over the leaves of all instances",
+           "instances": n, "fine_cells": e.fine_cells, "steps": steps, "regrid_every": k,
+           "unit": "MLUPS (finest-equivalent, all instances)"}
+    diags = {}
+    for prec in (32, 64):
+        with amrlbm.Mesh(e, precision=prec) as m:
+            try:
+                m.setup_rising_bubble()
+                m.synchronize()
+                c0 = m.counts()
+                s0 = m.regrid_stats()
+                t0 = time.perf_counter()
+                m.step(steps, k)
+                m.synchronize()
+                wall = time.perf_counter() - t0
+                s1 = m.regrid_stats()
+                diags[prec] = m.output_instances()
+            except amrlbm.AmrLbmError as exc:   # reported, not raised
+                out[f"fp{prec}"] = {"error": str(exc)}
+                continue
+            changes = s1["changes"] - s0["changes"]
+            host = sum(s1["host_ms"][f] - s0["host_ms"][f] for f in ("regrid", "graphs"))
+            out[f"fp{prec}"] = {
+                "value": round(e.fine_cells * steps / wall / 1e6, 1), "ms_per_step": round(wall * 1e3 / steps, 4),
+                "leaf_blocks": [c0["nleaf"], m.counts()["nleaf"]], "mesh_changes": changes,
+                "host_ms_per_change": round(host / max(changes, 1), 2)}
+    if 32 in diags and 64 in diags:
+        d32, d64 = diags[32], diags[64]
+        mass = np.abs(d32[:, 3] - d64[:, 3]) / np.abs(d64[:, 3])
+        rise = np.abs(d32[:, 7] - d64[:, 7]) / np.abs(d64[:, 7])
+        cen = np.maximum(np.abs(d32[:, 5] - d64[:, 5]), np.abs(d32[:, 6] - d64[:, 6]))
+        out["fp32_error_vs_fp64"] = {
+            "mass_rel_max": float(mass.max()), "mass_rel_median": float(np.median(mass)),
+            "rise_velocity_rel_max": float(rise.max()), "rise_velocity_rel_median": float(np.median(rise)),
+            "centroid_abs_max_cells": float(cen.max()),
+            "nonfinite_instances": int(np.sum(d32[:, 8] != 0) + np.sum(d64[:, 8] != 0))}
+    return out
+
+
 def describe(w, k, precision=64):
     path = "plain FP64 path" if precision == 64 else "FP32 path (deviation storage)"
     what = (f"rising bubble {w.fine_shape[0]}x{w.fine_shape[1]} finest-equivalent, 16x16 blocks, "
@@ -281,6 +336,10 @@ def main():
                     help="also run one device-to-device copy of this size (rocprof counter calibration)")
     ap.add_argument("--multi-steps", type=int, default=3000,
                     help="steps of the 32-bubble case (0 to skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=10000,
+                    help="steps of the 64-instance ensemble (0 to skip)")
+    ap.add_argument("--ensemble-size", type=int, default=64)
+    ap.add_argument("--ensemble-seed", type=int, default=2024)
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: time the step kernels on the set-up mesh only (no whole steps)")
     args = ap.parse_args()
@@ -349,6 +408,8 @@ def main():
         out["large_working_set"] = entries
     if args.multi_steps > 0:
         out["multi_bubble"] = multi_bubble(amrlbm, args.multi_steps, k)
+    if args.ensemble_steps > 0:
+        out["ensemble"] = ensemble_run(amrlbm, args.ensemble_size, args.ensemble_seed, args.ensemble_steps, k)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, k)
     print(json.dumps(out), flush=True)
