@@ -114,6 +114,9 @@ def lib():
             "amrlbm_last_step_ms": [vp, _DP],
             "amrlbm_regrid_stats": [vp, _DP],
             "amrlbm_time_kernels": [vp, ip, _DP],
+            "amrlbm_set_tuning": [vp, _IP],
+            "amrlbm_get_tuning": [vp, _IP],
+            "amrlbm_autotune": [vp, ip, ip, _DP],
             "amrlbm_save_snapshot": [vp, ctypes.c_char_p],
             "amrlbm_cnn_forward": [ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
             "amrlbm_cnn_gradient": [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_float), ctypes.c_float, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
@@ -412,6 +415,22 @@ class Mesh:
                 "host_ms": {"regrid": round(out[3], 3), "tables": round(out[4], 3),
                             "programs": round(out[5], 3), "graphs": round(out[6], 3)},
                 "graphs_built": int(out[7])}
+
+    def set_tuning(self, order, depth):
+        """Launch variant (amrlbm_set_tuning): leaf order 0-2, hg operand depth 0-2."""
+        _check(lib().amrlbm_set_tuning(self.h, _i(np.array([order, depth], dtype=np.int32))))
+
+    def get_tuning(self):
+        k = np.zeros(2, dtype=np.int32)
+        _check(lib().amrlbm_get_tuning(self.h, _i(k)))
+        return int(k[0]), int(k[1])
+
+    def autotune(self, reps=10, regrid_every=10):
+        """Time the step graph of the 9 launch variants on the current state
+        and keep the fastest (amrlbm_autotune): [(order, depth, us per step)]."""
+        t = np.zeros(27)
+        _check(lib().amrlbm_autotune(self.h, int(reps), int(regrid_every), _d(t)))
+        return [(int(t[3 * v]), int(t[3 * v + 1]), float(t[3 * v + 2])) for v in range(9)]
 
     def output(self, allow_nonfinite=False):
         """rho_min, rho_max, max|Uy|, mass, bubble area, centroid x, y, rise velocity, non-finite."""

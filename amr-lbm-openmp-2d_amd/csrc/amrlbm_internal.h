@@ -109,6 +109,7 @@ struct DevMeshT {
 struct DevMesh : DevMeshT<double> {
     int f32 = 0;
     PhysF phf{};
+    int hg_depth = 1;   // hg fill operands in flight per thread: 0: 2/1, 1: 4/2, 2: 8/4 (first / later stages)
 };
 
 template <class T>

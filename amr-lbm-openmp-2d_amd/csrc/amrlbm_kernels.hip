@@ -768,8 +768,10 @@ __device__ __forceinline__ T stream_cell(const T *sF, int lcf, T *__restrict__ G
 // 8 carry the ghost ring into the new buffer (as the in-place reference stream),
 // 16 store the filled ghost ring in memory (the phase-by-phase fill; a fill
 // followed by the stream in the same kernel keeps it in LDS only: no kernel
-// reads a population ghost cell from memory, and the next fill rewrites it)
-template <class T>
+// reads a population ghost cell from memory, and the next fill rewrites it).
+// U0 / U1: recipes per thread whose operands are in flight together in the
+// first stage / later stages (a launch variant; the results do not depend on it).
+template <class T, int U0, int U1>
 __global__ __launch_bounds__(256) void k_prog_hg(DevMeshT<T> m, int cur, const int *__restrict__ list, int n,
                                                  Prog prog, int flags)
 {
@@ -815,9 +817,9 @@ __global__ __launch_bounds__(256) void k_prog_hg(DevMeshT<T> m, int cur, const i
     for (int s = 0; s < hd.nst; s++) {
         const int end = stage_end(hd, s);
         if (s == 0)
-            hg_stage<true, 4>(sN, begin, end, set, Fc, sF, lcf, F, (flags & 16) != 0, tid);
+            hg_stage<true, U0>(sN, begin, end, set, Fc, sF, lcf, F, (flags & 16) != 0, tid);
         else
-            hg_stage<false, 2>(sN, begin, end, set, Fc, sF, lcf, F, (flags & 16) != 0, tid);
+            hg_stage<false, U1>(sN, begin, end, set, Fc, sF, lcf, F, (flags & 16) != 0, tid);
         begin = end;
         __syncthreads();
     }
@@ -1293,6 +1295,19 @@ void launch_prog_c(const DevMesh &m, int cur, const int *list, int n, int kind, 
             : prog_c<double, false>(m, cur, list, n, op, prog, ph, s, err_slot);
 }
 
+// the hg kernel of operand depth d: 2/1, 4/2 or 8/4 recipes in flight per thread
+template <class T>
+static void hg_launch(int d, dim3 grid, size_t lds, hipStream_t s, const DevMeshT<T> &m, int cur, const int *list, int n,
+                      const Prog &prog, int flags)
+{
+    if (d == 0)
+        hipLaunchKernelGGL((k_prog_hg<T, 2, 1>), grid, dim3(256), lds, s, m, cur, list, n, prog, flags);
+    else if (d == 2)
+        hipLaunchKernelGGL((k_prog_hg<T, 8, 4>), grid, dim3(256), lds, s, m, cur, list, n, prog, flags);
+    else
+        hipLaunchKernelGGL((k_prog_hg<T, 4, 2>), grid, dim3(256), lds, s, m, cur, list, n, prog, flags);
+}
+
 void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Prog &prog, bool stream_after,
                     bool macro_h, bool copy_ghosts, hipStream_t s)
 {
@@ -1305,9 +1320,9 @@ void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Pro
     const size_t esz = m.f32 ? sizeof(float) : sizeof(double);
     const size_t lds = hg_nodes_offset(lcf, esz) + (size_t)prog.nmax * sizeof(PNode);
     if (m.f32)
-        hipLaunchKernelGGL(k_prog_hg<float>, grid, dim3(256), lds, s, typed<float>(m), cur, list, n, prog, flags);
+        hg_launch<float>(m.hg_depth, grid, lds, s, typed<float>(m), cur, list, n, prog, flags);
     else
-        hipLaunchKernelGGL(k_prog_hg<double>, grid, dim3(256), lds, s, typed<double>(m), cur, list, n, prog, flags);
+        hg_launch<double>(m.hg_depth, grid, lds, s, typed<double>(m), cur, list, n, prog, flags);
 }
 
 void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream_t s)

@@ -166,6 +166,7 @@ struct amrlbm_mesh {
     // and bubble; ens is 0 in any other mesh
     int ens = 0;
     std::vector<amrlbm_instance> inst;
+    int order = 0;   // leaf launch order (amrlbm_set_tuning): 0 Morton in 8 XCD runs, 1 Morton, 2 ascending id
     Phys *d_phs = nullptr;           // [ens] device tables the kernels index by BlockDev::inst
     PhysF *d_phsf = nullptr;
     int *d_lv = nullptr;             // [ens][2] min, max levels
@@ -776,7 +777,16 @@ void dispatch_order(amrlbm_mesh *m)
             z |= (unsigned long long)((x >> bit) & 1u) << (2 * bit) | (unsigned long long)((y >> bit) & 1u) << (2 * bit + 1);
         key[k] = {(unsigned long long)q << 40 | z, m->leaf[k]};
     }
+    if (m->order == 2) {   // ascending id (Array_of_Leaf_Blocks)
+        m->disp = m->leaf;
+        return;
+    }
     std::sort(key.begin(), key.end());
+    if (m->order == 1) {   // the Morton curve as it runs
+        m->disp.resize(n);
+        for (int k = 0; k < n; k++) m->disp[k] = key[k].second;
+        return;
+    }
     const int q = n / 8, r = n % 8;   // runs 0..r-1 hold q + 1 blocks, the others q
     m->disp.assign(n, 0);
     int start = 0;
@@ -2440,6 +2450,101 @@ int amrlbm_time_kernels(amrlbm_mesh *m, int reps, double out[3])
     HIPCHK(hipStreamSynchronize(m->stream));
     HIPCHK(hipFree(save));
     return check_err_flags(m);
+}
+
+int amrlbm_set_tuning(amrlbm_mesh *m, const int knobs[2])
+{
+    if (!m || !knobs) return fail(AMRLBM_ERR_ARG, "null argument");
+    if (knobs[0] < 0 || knobs[0] > 2 || knobs[1] < 0 || knobs[1] > 2) return fail(AMRLBM_ERR_ARG, "no such variant");
+    if (knobs[0] == m->order && knobs[1] == m->dm.hg_depth) return AMRLBM_OK;
+    HIPCHK(hipStreamSynchronize(m->stream));
+    m->order = knobs[0];
+    m->dm.hg_depth = knobs[1];
+    drop_graphs(m);   // captured with the previous order and kernels
+    m->tables_dirty = true;
+    m->topo_version++;
+    return m->ready ? upload_tables(m) : AMRLBM_OK;
+}
+
+int amrlbm_get_tuning(amrlbm_mesh *m, int knobs[2])
+{
+    if (!m || !knobs) return fail(AMRLBM_ERR_ARG, "null argument");
+    knobs[0] = m->order;
+    knobs[1] = m->dm.hg_depth;
+    return AMRLBM_OK;
+}
+
+}  // extern "C"
+
+// Average time of one step of the segment graph of len steps (the step
+// kernels as amrlbm_step launches them, no regrid), over reps launches from
+// the current state (len even, so every launch starts from the same buffer);
+// the state is restored afterwards.
+static int time_segment(amrlbm_mesh *m, int len, int reps, double *us_per_step)
+{
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    determine_min_max_levels(m);
+    const size_t nb = (size_t)m->nblks;
+    const size_t e = esz(m), npop = e * POPB * nb, nc = e * CS * nb, npu = e * PUB * nb;
+    char *save = nullptr;
+    HIPCHK(hipMalloc(&save, npop + nc + npu));
+    HIPCHK(hipMemcpyAsync(save, m->dm.pop[m->cur], npop, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(save + npop, m->dm.C, nc, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(save + npop + nc, m->dm.PU, npu, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int), m->stream));   // no skip word between regrids
+    hipGraphExec_t ex;
+    rc = segment_graph(m, len, &ex);
+    if (rc) {
+        (void)hipFree(save);
+        return rc;
+    }
+    HIPCHK(hipGraphLaunch(ex, m->stream));   // warm
+    HIPCHK(hipEventRecord(m->ev0, m->stream));
+    for (int r = 0; r < reps; r++) HIPCHK(hipGraphLaunch(ex, m->stream));
+    HIPCHK(hipEventRecord(m->ev1, m->stream));
+    HIPCHK(hipEventSynchronize(m->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    *us_per_step = 1e3 * ms / ((double)reps * len);
+    HIPCHK(hipMemcpyAsync(m->dm.pop[m->cur], save, npop, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->dm.C, save + npop, nc, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->dm.PU, save + npop + nc, npu, hipMemcpyDeviceToDevice, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipFree(save));
+    return check_err_flags(m);
+}
+
+extern "C" {
+
+int amrlbm_autotune(amrlbm_mesh *m, int reps, int regrid_every, double table[27])
+{
+    NEED_READY(m);
+    if (reps < 1 || regrid_every < 1 || !table) return fail(AMRLBM_ERR_ARG, "bad autotune arguments");
+    const int len = regrid_every % 2 ? 2 * regrid_every : regrid_every;
+    int best[2] = {m->order, m->dm.hg_depth};
+    double best_us = 1e300;
+    int v = 0;
+    for (int order = 0; order < 3; order++)
+        for (int depth = 0; depth < 3; depth++, v++) {
+            const int knobs[2] = {order, depth};
+            int rc = amrlbm_set_tuning(m, knobs);
+            if (rc) return rc;
+            double us = 0.0;
+            if ((rc = time_segment(m, len, reps, &us))) return rc;
+            table[3 * v] = order;
+            table[3 * v + 1] = depth;
+            table[3 * v + 2] = us;
+            if (us < best_us) {
+                best_us = us;
+                best[0] = order;
+                best[1] = depth;
+            }
+        }
+    int rc = amrlbm_set_tuning(m, best);
+    if (rc) return rc;
+    hipGraphExec_t ex;   // the chosen variant's step graph, captured now rather than in the caller's next steps
+    return segment_graph(m, len, &ex);
 }
 
 // ---- learned regrid predictor --------------------------------------------

@@ -156,12 +156,28 @@ def kernels_only(amrlbm, w, calibrate_mb, precisions):
                           "roofline": {kk: roofline(kk, kt, nleaf, w, prec) for kk in kt}}), flush=True)
 
 
+TUNE = {"on": True}
+
+
+def tuning_entry(table, chosen):
+    """The autotuner's table: the chosen launch variant and its time per step
+    (the step graph between two regrids) against the default variant's (order
+    0, depth 1)."""
+    t = {(o, d): us for o, d, us in table}
+    return {"chosen": {"order": chosen[0], "hg_depth": chosen[1]},
+            "us_per_step": round(t[chosen], 3), "default_us_per_step": round(t[(0, 1)], 3),
+            "gain_vs_default": round(t[(0, 1)] / t[chosen], 4),
+            "variants_us": {f"{o}/{d}": round(us, 3) for (o, d), us in sorted(t.items())}}
+
+
 def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64):
-    """Time `steps` steps of workload w after `warmup`, then the step kernels
-    back to back; returns the numbers of one bench entry."""
+    """Time `steps` steps of workload w after `warmup` (and, untimed, the
+    autotuner's choice of launch variant), then the step kernels back to
+    back; returns the numbers of one bench entry."""
     m = amrlbm.Mesh(w, precision=precision)
     m.setup_rising_bubble()
     m.step(warmup, k)
+    tuning = tuning_entry(m.autotune(20, k), m.get_tuning()) if TUNE["on"] else None
     m.synchronize()
     c0 = m.counts()
     t0 = time.perf_counter()
@@ -185,7 +201,7 @@ def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64):
         "roofline": roof(dom), "roofline_all": {kk: roof(kk) for kk in kt},
         "kernel_us": {kk: round(v * 1e3, 3) for kk, v in kt.items()},
         "step_kernel_us": sum(kt[kk] * LAUNCHES_PER_STEP.get(kk, 1.0 / k) for kk in kt) * 1e3,
-        "diag": diag,
+        "diag": diag, "tuning": tuning,
     }
 
 
@@ -279,6 +295,7 @@ def ensemble_run(amrlbm, n, seed, steps, k):
         with amrlbm.Mesh(e, precision=prec) as m:
             try:
                 m.setup_rising_bubble()
+                tuning = tuning_entry(m.autotune(3, k), m.get_tuning()) if TUNE["on"] else None
                 m.synchronize()
                 c0 = m.counts()
                 s0 = m.regrid_stats()
@@ -296,7 +313,7 @@ def ensemble_run(amrlbm, n, seed, steps, k):
             out[f"fp{prec}"] = {
                 "value": round(e.fine_cells * steps / wall / 1e6, 1), "ms_per_step": round(wall * 1e3 / steps, 4),
                 "leaf_blocks": [c0["nleaf"], m.counts()["nleaf"]], "mesh_changes": changes,
-                "host_ms_per_change": round(host / max(changes, 1), 2)}
+                "host_ms_per_change": round(host / max(changes, 1), 2), "tuning": tuning}
     if 32 in diags and 64 in diags:
         d32, d64 = diags[32], diags[64]
         mass = np.abs(d32[:, 3] - d64[:, 3]) / np.abs(d64[:, 3])
@@ -340,6 +357,8 @@ def main():
                     help="steps of the 64-instance ensemble (0 to skip)")
     ap.add_argument("--ensemble-size", type=int, default=64)
     ap.add_argument("--ensemble-seed", type=int, default=2024)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch variant (no autotuner before the timed runs)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: time the step kernels on the set-up mesh only (no whole steps)")
     args = ap.parse_args()
@@ -349,6 +368,7 @@ def main():
         sys.exit(2)
 
     import amrlbm
+    TUNE["on"] = not args.no_autotune
     w = amrlbm.WORKLOADS[args.workload]
     k = args.regrid_every
     if args.kernels_only:
@@ -370,6 +390,7 @@ def main():
         "kernel_avg_us": r["kernel_us"],
         "kernel_us_per_step": round(r["step_kernel_us"], 3),
         "regrids": r["regrids"],
+        "tuning": r["tuning"],
         "diagnostics": {"mass": diag[3], "rise_velocity": diag[7], "centroid_y": diag[6]},
     }
     if not args.no_fp32:
@@ -379,7 +400,7 @@ def main():
             "workload": describe(w, k, 32), "value": round(r32["value"], 1), "unit": "MLUPS (finest-equivalent)",
             "speedup_vs_fp64": round(r32["value"] / r["value"], 3),
             "ms_per_step": round(r32["wall"] * 1e3 / args.steps, 4), "dtype": "f32",
-            "roofline": r32["roofline"], "kernel_avg_us": r32["kernel_us"],
+            "roofline": r32["roofline"], "kernel_avg_us": r32["kernel_us"], "tuning": r32["tuning"],
             "error_vs_fp64": {
                 "after_steps": args.warmup + args.steps,
                 "mass_rel": abs(d32[3] - diag[3]) / abs(diag[3]),
@@ -404,6 +425,7 @@ def main():
                 "roofline": {kk: {f: v[f] for f in ("kernel", "achieved", "frac", "avg_launch_us", "traffic")}
                              for kk, v in rl["roofline_all"].items()},
                 "regrids": {f: rl["regrids"][f] for f in ("checks", "changes")},
+                "tuning": rl["tuning"],
             }
         out["large_working_set"] = entries
     if args.multi_steps > 0:

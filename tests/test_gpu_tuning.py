@@ -1,0 +1,74 @@
+"""Launch variants (amrlbm_set_tuning) and the autotuner (amrlbm_autotune).
+The leaves' launch order and the number of hg fill operands in flight per
+thread change no arithmetic, so every variant must leave the state bit for
+bit the same. The autotuner times the step graph of all nine on the live
+mesh, keeps the fastest, and leaves the state as it found it."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def amrlbm():
+    import amrlbm as mod
+    mod.lib()
+    return mod
+
+
+def _state(m):
+    rows, free = m.topology()
+    arrays = {n: m.get_block(n) for n in range(1, rows.shape[0] + 1) if rows[n - 1, 0] >= 0 and rows[n - 1, 5]}
+    return rows, free, arrays
+
+
+def _same(a, b):
+    ra, fa, xa = a
+    rb, fb, xb = b
+    assert np.array_equal(ra, rb) and fa == fb and xa.keys() == xb.keys()
+    for n in xa:
+        for f in ("C", "P", "Ux", "Uy"):
+            assert np.array_equal(xa[n][f], xb[n][f]), (n, f)
+        for f in ("h", "g"):   # interiors (the fused stream leaves the previous ghost ring)
+            assert np.array_equal(xa[n][f][:, 1:-1, 1:-1], xb[n][f][:, 1:-1, 1:-1]), (n, f)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_variants_are_bitwise_equal(amrlbm, workloads, precision):
+    w = workloads["cfg2"]
+    variants = [(o, d) for o in range(3) for d in range(3)] if precision == 64 else [(0, 0), (1, 1), (2, 2)]
+    ref = None
+    for order, depth in variants:
+        print(f"[tuning] FP{precision} cfg2 variant order {order} depth {depth}: 2000 steps", flush=True)
+        with amrlbm.Mesh(w, precision=precision) as m:
+            m.setup_rising_bubble()
+            m.set_tuning(order, depth)
+            assert m.get_tuning() == (order, depth)
+            m.step(2000, 10)
+            st = _state(m)
+            assert m.regrid_stats()["changes"] >= 2   # across mesh changes
+        if ref is None:
+            ref = st
+        else:
+            _same(st, ref)
+
+
+def test_autotune_keeps_the_fastest_and_the_state(amrlbm, workloads):
+    w = workloads["cfg2"]
+    with amrlbm.Mesh(w) as a, amrlbm.Mesh(w) as b:
+        a.setup_rising_bubble()
+        b.setup_rising_bubble()
+        a.step(100, 10)
+        b.step(100, 10)
+        before = a.output()
+        table = a.autotune(reps=10)
+        for order, depth, us in table:
+            print(f"[tuning] cfg2 order {order} depth {depth}: {us:.2f} us per step")
+        assert sorted((o, d) for o, d, _ in table) == [(o, d) for o in range(3) for d in range(3)]
+        assert all(us > 0 for _, _, us in table)
+        best = min(table, key=lambda r: r[2])
+        assert a.get_tuning() == best[:2]
+        assert np.array_equal(a.output(), before)
+        a.step(200, 10)
+        b.step(200, 10)
+        _same(_state(a), _state(b))
