@@ -205,16 +205,18 @@ def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64):
     }
 
 
-def learned_regrid(amrlbm, w, steps=2000):
-    """The learned regrid (committed network) on a held-out bubble of the same
-    geometry, FP32, against the same cadence without the network and the plain
-    regrid every 10 steps."""
+def learned_regrid(amrlbm, w, steps=2000, variant=None):
+    """The learned regrid (committed network) on a held-out bubble (of the
+    same geometry by default), FP32, against the same cadence without the
+    network and the plain regrid every 10 steps; missed refinements are the
+    judge's checks that found the interface on an under-refined block."""
     import numpy as np
     from amrlbm import learned
     if not os.path.exists(learned.WEIGHTS):
         return None
-    ev = learned.evaluate(w, np.load(learned.WEIGHTS), steps)
-    out = {"workload": f"held-out bubble R={learned.EVAL_VARIANT[0]} at {learned.EVAL_VARIANT[1:]}, {steps} steps, "
+    var = variant or learned.EVAL_VARIANT
+    ev = learned.evaluate(w, np.load(learned.WEIGHTS), steps, variant_=var)
+    out = {"workload": f"{w.name} geometry, bubble R={var[0]} at {tuple(var[1:])}, {steps} steps, "
                        f"full regrid every {learned.HORIZON} steps, plain judge every 10",
            "precision": "f32"}
     for k, v in ev.items():
@@ -222,6 +224,7 @@ def learned_regrid(amrlbm, w, steps=2000):
     base = ev["plain_every_10"]["diag"]
     out["learned"]["mass_rel_vs_plain"] = abs(ev["learned"]["diag"][3] - base[3]) / abs(base[3])
     out["learned"]["rise_velocity_rel_vs_plain"] = abs(ev["learned"]["diag"][7] - base[7]) / abs(base[7])
+    out["learned"]["missed_refinement_rate"] = ev["learned"]["misses"] / max(ev["learned"]["checks"], 1)
     out["gain_mlups_vs_no_network"] = round(ev["learned"]["mlups"] / ev["no_network"]["mlups"], 3)
     out["gain_mlups_vs_plain_every_10"] = round(ev["learned"]["mlups"] / ev["plain_every_10"]["mlups"], 3)
     return out
@@ -408,6 +411,9 @@ def main():
                 "centroid_abs_cells": max(abs(d32[5] - diag[5]), abs(d32[6] - diag[6])),
                 "same_mesh": r32["c1"] == r["c1"]},
             "learned_regrid": learned_regrid(amrlbm, w),
+            # the network trained on the 1024x2048 geometry, deployed on the 128x256 bubble (D=32 at (64, 64))
+            "learned_regrid_cfg1": learned_regrid(amrlbm, amrlbm.WORKLOADS["cfg1"],
+                                                  variant=(16.0, 64.0, 64.0)),
         }
     if args.large_workload:
         wl = amrlbm.WORKLOADS[args.large_workload]
