@@ -19,6 +19,7 @@
 #include <deque>
 #include <string>
 #include <omp.h>
+#include <parallel/algorithm>
 #include <vector>
 
 #include "amrlbm_internal.h"
@@ -148,6 +149,11 @@ struct amrlbm_mesh {
     long long prog_nodes_uploaded = 0;
     Prog prog[3] = {};
     std::vector<ProgCache> pcache;   // by block id
+    // per kind and block id: the header of its cached program (offset into its
+    // slot), its aux cells and recipes, so a mesh change gathers the headers of
+    // every leaf from compact arrays
+    std::vector<PHdr> hdr_id[3];
+    std::vector<int> naux_id[3], nn_id[3];
     long long progs_built = 0;       // leaves whose programs were (re)built
 
     // one HIP graph per (segment length, population-buffer parity), updated
@@ -182,6 +188,13 @@ struct amrlbm_mesh {
 };
 
 namespace {
+
+// device memory freed when its scope ends (error returns included)
+struct DeviceBuffer {
+    void *p = nullptr;
+    ~DeviceBuffer() { (void)hipFree(p); }
+    char *bytes() const { return static_cast<char *>(p); }
+};
 
 // adds the wall time of its scope to acc (ms)
 struct HostTimer {
@@ -633,10 +646,25 @@ int build_programs(amrlbm_mesh *m)
             for (int b : d) pc.deps.push_back({b, block_sig(m->blk[b])});
         }
     }
+    for (int kind = 0; kind < 3; kind++)
+        if (m->hdr_id[kind].size() < m->pcache.size()) {
+            m->hdr_id[kind].resize(m->pcache.size());
+            m->naux_id[kind].resize(m->pcache.size(), 0);
+            m->nn_id[kind].resize(m->pcache.size(), 0);
+        }
     for (int i = 0; i < nd; i++) {
-        ProgCache &pc = m->pcache[dirty[i]];
-        for (int kind = 0; kind < 3; kind++)
-            if (!pc.p[kind].err.empty()) return fail(AMRLBM_ERR_TOPOLOGY, pc.p[kind].err);
+        const int id = dirty[i];
+        ProgCache &pc = m->pcache[id];
+        for (int kind = 0; kind < 3; kind++) {
+            const LeafProg &lp = pc.p[kind];
+            if (!lp.err.empty()) return fail(AMRLBM_ERR_TOPOLOGY, lp.err);
+            PHdr &h = m->hdr_id[kind][id];
+            h.off = (id - 1) * kProgSlot[kind];
+            h.nst = lp.nst;
+            for (int st = 0; st < MAX_STAGES; st++) h.end[st] = lp.end[st];
+            m->naux_id[kind][id] = lp.naux;
+            m->nn_id[kind][id] = (int)lp.nodes.size();
+        }
         pc.valid = true;
     }
     m->progs_built += nd;
@@ -672,10 +700,11 @@ int build_programs(amrlbm_mesh *m)
     constexpr int W = (int)(sizeof(PNode) / 16);   // 16-byte words per node
     for (int kind = 0; kind < 3; kind++) {
         int naux = 0, nmax = 0;
+        const int *na = m->naux_id[kind].data(), *nn = m->nn_id[kind].data();
+#pragma omp parallel for schedule(static) num_threads(nthr) reduction(max : naux, nmax)
         for (int k = 0; k < nl; k++) {
-            const LeafProg &lp = m->pcache[m->disp[k]].p[kind];
-            naux = std::max(naux, lp.naux);
-            nmax = std::max(nmax, (int)lp.nodes.size());
+            naux = std::max(naux, na[m->disp[k]]);
+            nmax = std::max(nmax, nn[m->disp[k]]);
         }
         // pinned stage: the nodes of the leaves in up, packed, then one copy
         // descriptor (source word, destination word, words) per leaf, then
@@ -703,13 +732,9 @@ int build_programs(amrlbm_mesh *m)
             desc[3 * i + 1] = (up[i] - 1) * kProgSlot[kind] * W;
             desc[3 * i + 2] = (int)lp.nodes.size() * W;
         }
-        for (int k = 0; k < nl; k++) {
-            const LeafProg &lp = m->pcache[m->disp[k]].p[kind];
-            PHdr &h = hdr[k];
-            h.off = (m->disp[k] - 1) * kProgSlot[kind];
-            h.nst = lp.nst;
-            for (int st = 0; st < MAX_STAGES; st++) h.end[st] = lp.end[st];
-        }
+        const PHdr *hid = m->hdr_id[kind].data();
+#pragma omp parallel for schedule(static) num_threads(nthr)
+        for (int k = 0; k < nl; k++) hdr[k] = hid[m->disp[k]];
         if ((size_t)nl > m->phdr_cap[kind]) {
             if (m->d_phdr[kind]) HIPCHK(hipFree(m->d_phdr[kind]));
             m->d_phdr[kind] = nullptr;
@@ -781,7 +806,10 @@ void dispatch_order(amrlbm_mesh *m)
         m->disp = m->leaf;
         return;
     }
-    std::sort(key.begin(), key.end());
+    if (n > 4096)
+        __gnu_parallel::sort(key.begin(), key.end());
+    else
+        std::sort(key.begin(), key.end());
     if (m->order == 1) {   // the Morton curve as it runs
         m->disp.resize(n);
         for (int k = 0; k < n; k++) m->disp[k] = key[k].second;
@@ -1537,10 +1565,11 @@ int amrlbm_setup_bubbles(amrlbm_mesh *m, int n, const double *R, const double *x
         bub[3 * k + 1] = y0[k];
         bub[3 * k + 2] = R[k];
     }
-    double *d_bub = nullptr;
-    HIPCHK(hipMalloc(&d_bub, sizeof(double) * bub.size()));
-    HIPCHK(hipMemcpy(d_bub, bub.data(), sizeof(double) * bub.size(), hipMemcpyHostToDevice));
-    int rc = setup_mesh(m, 1, [&](amrlbm_mesh *mm) {
+    DeviceBuffer buf;
+    HIPCHK(hipMalloc(&buf.p, sizeof(double) * bub.size()));
+    const double *d_bub = static_cast<const double *>(buf.p);
+    HIPCHK(hipMemcpy(buf.p, bub.data(), sizeof(double) * bub.size(), hipMemcpyHostToDevice));
+    return setup_mesh(m, 1, [&](amrlbm_mesh *mm) {
         int r2 = upload_tables(mm);
         if (r2) return r2;
         launch_init_fields(mm->dm, mm->cur, mm->d_lists + mm->all_off, nleaves(mm), mm->ph, 0.0, mm->stream, d_bub, n);
@@ -1548,8 +1577,6 @@ int amrlbm_setup_bubbles(amrlbm_mesh *m, int n, const double *R, const double *x
         HIPCHK(hipStreamSynchronize(mm->stream));
         return AMRLBM_OK;
     });
-    (void)hipFree(d_bub);
-    return rc;
 }
 
 }  // extern "C"
@@ -2418,9 +2445,10 @@ int amrlbm_time_kernels(amrlbm_mesh *m, int reps, double out[3])
     const int n = nleaves(m);
     const size_t nb = (size_t)m->nblks;
     // the launches below overwrite populations, C and U,P: keep a copy
-    char *save = nullptr;
+    DeviceBuffer buf;
     const size_t e = esz(m), npop = e * POPB * nb, nc = e * CS * nb, npu = e * PUB * nb;
-    HIPCHK(hipMalloc(&save, npop + nc + npu));
+    HIPCHK(hipMalloc(&buf.p, npop + nc + npu));
+    char *save = buf.bytes();
     HIPCHK(hipMemcpyAsync(save, m->dm.pop[m->cur], npop, hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(hipMemcpyAsync(save + npop, m->dm.C, nc, hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(hipMemcpyAsync(save + npop + nc, m->dm.PU, npu, hipMemcpyDeviceToDevice, m->stream));
@@ -2448,7 +2476,6 @@ int amrlbm_time_kernels(amrlbm_mesh *m, int reps, double out[3])
     HIPCHK(hipMemcpyAsync(m->dm.C, save + npop, nc, hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(hipMemcpyAsync(m->dm.PU, save + npop + nc, npu, hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
-    HIPCHK(hipFree(save));
     return check_err_flags(m);
 }
 
@@ -2487,18 +2514,15 @@ static int time_segment(amrlbm_mesh *m, int len, int reps, double *us_per_step)
     determine_min_max_levels(m);
     const size_t nb = (size_t)m->nblks;
     const size_t e = esz(m), npop = e * POPB * nb, nc = e * CS * nb, npu = e * PUB * nb;
-    char *save = nullptr;
-    HIPCHK(hipMalloc(&save, npop + nc + npu));
+    DeviceBuffer buf;
+    HIPCHK(hipMalloc(&buf.p, npop + nc + npu));
+    char *save = buf.bytes();
     HIPCHK(hipMemcpyAsync(save, m->dm.pop[m->cur], npop, hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(hipMemcpyAsync(save + npop, m->dm.C, nc, hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(hipMemcpyAsync(save + npop + nc, m->dm.PU, npu, hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int), m->stream));   // no skip word between regrids
     hipGraphExec_t ex;
-    rc = segment_graph(m, len, &ex);
-    if (rc) {
-        (void)hipFree(save);
-        return rc;
-    }
+    if ((rc = segment_graph(m, len, &ex))) return rc;
     HIPCHK(hipGraphLaunch(ex, m->stream));   // warm
     HIPCHK(hipEventRecord(m->ev0, m->stream));
     for (int r = 0; r < reps; r++) HIPCHK(hipGraphLaunch(ex, m->stream));
@@ -2511,7 +2535,6 @@ static int time_segment(amrlbm_mesh *m, int len, int reps, double *us_per_step)
     HIPCHK(hipMemcpyAsync(m->dm.C, save + npop, nc, hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(hipMemcpyAsync(m->dm.PU, save + npop + nc, npu, hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
-    HIPCHK(hipFree(save));
     return check_err_flags(m);
 }
 
