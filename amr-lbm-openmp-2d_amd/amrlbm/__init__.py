@@ -417,7 +417,7 @@ class Mesh:
                 "graphs_built": int(out[7])}
 
     def set_tuning(self, order, depth):
-        """Launch variant (amrlbm_set_tuning): leaf order 0-2, hg operand depth 0-2."""
+        """Launch variant (amrlbm_set_tuning): leaf order 0-3, hg operand depth 0-2."""
         _check(lib().amrlbm_set_tuning(self.h, _i(np.array([order, depth], dtype=np.int32))))
 
     def get_tuning(self):
@@ -426,11 +426,11 @@ class Mesh:
         return int(k[0]), int(k[1])
 
     def autotune(self, reps=10, regrid_every=10):
-        """Time the step graph of the 9 launch variants on the current state
+        """Time the step graph of the 12 launch variants on the current state
         and keep the fastest (amrlbm_autotune): [(order, depth, us per step)]."""
-        t = np.zeros(27)
+        t = np.zeros(36)
         _check(lib().amrlbm_autotune(self.h, int(reps), int(regrid_every), _d(t)))
-        return [(int(t[3 * v]), int(t[3 * v + 1]), float(t[3 * v + 2])) for v in range(9)]
+        return [(int(t[3 * v]), int(t[3 * v + 1]), float(t[3 * v + 2])) for v in range(12)]
 
     def output(self, allow_nonfinite=False):
         """rho_min, rho_max, max|Uy|, mass, bubble area, centroid x, y, rise velocity, non-finite."""

@@ -154,6 +154,7 @@ struct amrlbm_mesh {
     // every leaf from compact arrays
     std::vector<PHdr> hdr_id[3];
     std::vector<int> naux_id[3], nn_id[3];
+    std::vector<int> cost_id;   // operand loads of the C2 and hg programs (launch order 3)
     long long progs_built = 0;       // leaves whose programs were (re)built
 
     // one HIP graph per (segment length, population-buffer parity), updated
@@ -172,7 +173,8 @@ struct amrlbm_mesh {
     // and bubble; ens is 0 in any other mesh
     int ens = 0;
     std::vector<amrlbm_instance> inst;
-    int order = 0;   // leaf launch order (amrlbm_set_tuning): 0 Morton in 8 XCD runs, 1 Morton, 2 ascending id
+    int order = 0;   // leaf launch order (amrlbm_set_tuning): 0 Morton in 8 XCD runs, 1 Morton, 2 ascending id,
+                     // 3 as 0 with each run's heaviest ghost programs first
     Phys *d_phs = nullptr;           // [ens] device tables the kernels index by BlockDev::inst
     PhysF *d_phsf = nullptr;
     int *d_lv = nullptr;             // [ens][2] min, max levels
@@ -614,18 +616,32 @@ bool cache_current(const amrlbm_mesh *m, int id)
 // read the topology, so they run on several host threads. With
 // AMRLBM_VERIFY_PROGRAMS set in the environment every reused program is
 // checked against a fresh build.
-int build_programs(amrlbm_mesh *m)
+// the host threads a program build or upload over nl leaves uses
+int prog_threads(int nl) { return std::max(1, std::min(omp_get_max_threads(), 3 * nl / 32)); }
+
+// operand loads of a program: the cost of a block's ghost fill
+int program_operands(const LeafProg &lp)
+{
+    int ops = 0;
+    for (const PNode &nd : lp.nodes) {
+        const int type = nd.info & 15;
+        ops += type == NT_COPY ? 1 : (type >= NT_BB_BOT ? 3 : 9);
+    }
+    return ops;
+}
+
+int rebuild_programs(amrlbm_mesh *m)
 {
     HostTimer timer(m->host_ms_programs);
     const int nl = (int)m->leaf.size();
     if (m->pcache.size() < m->blk.size()) m->pcache.resize(m->blk.size());
-    const int nthr = std::max(1, std::min(omp_get_max_threads(), 3 * nl / 32));
+    const int nthr = prog_threads(nl);
     std::vector<char> stale(nl);
 #pragma omp parallel for schedule(static) num_threads(nthr)
-    for (int k = 0; k < nl; k++) stale[k] = !cache_current(m, m->disp[k]);
+    for (int k = 0; k < nl; k++) stale[k] = !cache_current(m, m->leaf[k]);
     std::vector<int> dirty;
     for (int k = 0; k < nl; k++)
-        if (stale[k]) dirty.push_back(m->disp[k]);
+        if (stale[k]) dirty.push_back(m->leaf[k]);
     const int nd = (int)dirty.size();
 #pragma omp parallel num_threads(nthr)
     {
@@ -652,6 +668,7 @@ int build_programs(amrlbm_mesh *m)
             m->naux_id[kind].resize(m->pcache.size(), 0);
             m->nn_id[kind].resize(m->pcache.size(), 0);
         }
+    if (m->cost_id.size() < m->pcache.size()) m->cost_id.resize(m->pcache.size(), 0);
     for (int i = 0; i < nd; i++) {
         const int id = dirty[i];
         ProgCache &pc = m->pcache[id];
@@ -665,23 +682,34 @@ int build_programs(amrlbm_mesh *m)
             m->naux_id[kind][id] = lp.naux;
             m->nn_id[kind][id] = (int)lp.nodes.size();
         }
+        m->cost_id[id] = program_operands(pc.p[KIND_C2]) + program_operands(pc.p[KIND_HG]);
         pc.valid = true;
     }
     m->progs_built += nd;
     if (std::getenv("AMRLBM_VERIFY_PROGRAMS")) {
         std::vector<LeafProg> fresh((size_t)3 * nl);
 #pragma omp parallel for schedule(dynamic, 8) num_threads(nthr)
-        for (int t = 0; t < 3 * nl; t++) build_leaf_program(m, t % 3, m->disp[t / 3], fresh[t]);
+        for (int t = 0; t < 3 * nl; t++) build_leaf_program(m, t % 3, m->leaf[t / 3], fresh[t]);
         for (int t = 0; t < 3 * nl; t++) {
-            const LeafProg &a = fresh[t], &b = m->pcache[m->disp[t / 3]].p[t % 3];
+            const LeafProg &a = fresh[t], &b = m->pcache[m->leaf[t / 3]].p[t % 3];
             bool same = a.err.empty() && a.nst == b.nst && a.naux == b.naux && a.nodes.size() == b.nodes.size() &&
                         std::equal(a.end, a.end + MAX_STAGES, b.end) &&
                         std::memcmp(a.nodes.data(), b.nodes.data(), sizeof(PNode) * a.nodes.size()) == 0;
             if (!same)
-                return fail(AMRLBM_ERR_TOPOLOGY, "reused ghost program of block " + std::to_string(m->disp[t / 3]) +
+                return fail(AMRLBM_ERR_TOPOLOGY, "reused ghost program of block " + std::to_string(m->leaf[t / 3]) +
                                                      " differs from a fresh build");
         }
     }
+    return AMRLBM_OK;
+}
+
+// The programs of every leaf on the device: the rebuilt ones into their
+// slots, the headers in launch order (m->disp).
+int upload_programs(amrlbm_mesh *m)
+{
+    HostTimer timer(m->host_ms_programs);
+    const int nl = (int)m->leaf.size();
+    const int nthr = prog_threads(nl);
     // the arenas hold a slot per block id up to nblks; grown (and refilled) as ids pass it
     if (m->nblks > m->prog_slots) {
         const int cap = std::min(m->prm.max_blocks, std::max(m->nblks + m->nblks / 2, 1024));
@@ -820,6 +848,11 @@ void dispatch_order(amrlbm_mesh *m)
     int start = 0;
     for (int c = 0; c < 8; c++) {
         const int len = q + (c < r ? 1 : 0);
+        if (m->order == 3)   // the run's blocks with the most ghost operand loads first
+            std::stable_sort(key.begin() + start, key.begin() + start + len,
+                             [m](const std::pair<unsigned long long, int> &a, const std::pair<unsigned long long, int> &b) {
+                                 return m->cost_id[a.second] > m->cost_id[b.second];
+                             });
         for (int i = 0; i < len; i++) m->disp[8 * i + c] = key[start + i].second;
         start += len;
     }
@@ -869,6 +902,10 @@ int upload_tables(amrlbm_mesh *m)
     if (bad_block)
         return fail(AMRLBM_ERR_TOPOLOGY, "block " + std::to_string(bad_block) + " has no neighbour " +
                                              std::to_string(bad_k) + " (2:1 balance violated)");
+    if (!m->leaf.empty()) {   // the programs first: launch order 3 weighs them
+        int rc0 = rebuild_programs(m);
+        if (rc0) return rc0;
+    }
     // leaves grouped by level, each group ascending; then all leaves
     const int NL = m->nlev();
     m->lvl_off.assign(NL + 1, 0);
@@ -893,7 +930,7 @@ int upload_tables(amrlbm_mesh *m)
     m->dm.blk = m->d_blk;
     m->dm.nbd = m->d_nbd;
     if (!m->leaf.empty()) {
-        int rc2 = build_programs(m);
+        int rc2 = upload_programs(m);
         if (rc2) return rc2;
     }
     m->tables_dirty = false;
@@ -2482,7 +2519,7 @@ int amrlbm_time_kernels(amrlbm_mesh *m, int reps, double out[3])
 int amrlbm_set_tuning(amrlbm_mesh *m, const int knobs[2])
 {
     if (!m || !knobs) return fail(AMRLBM_ERR_ARG, "null argument");
-    if (knobs[0] < 0 || knobs[0] > 2 || knobs[1] < 0 || knobs[1] > 2) return fail(AMRLBM_ERR_ARG, "no such variant");
+    if (knobs[0] < 0 || knobs[0] > 3 || knobs[1] < 0 || knobs[1] > 2) return fail(AMRLBM_ERR_ARG, "no such variant");
     if (knobs[0] == m->order && knobs[1] == m->dm.hg_depth) return AMRLBM_OK;
     HIPCHK(hipStreamSynchronize(m->stream));
     m->order = knobs[0];
@@ -2540,7 +2577,7 @@ static int time_segment(amrlbm_mesh *m, int len, int reps, double *us_per_step)
 
 extern "C" {
 
-int amrlbm_autotune(amrlbm_mesh *m, int reps, int regrid_every, double table[27])
+int amrlbm_autotune(amrlbm_mesh *m, int reps, int regrid_every, double table[36])
 {
     NEED_READY(m);
     if (reps < 1 || regrid_every < 1 || !table) return fail(AMRLBM_ERR_ARG, "bad autotune arguments");
@@ -2548,7 +2585,7 @@ int amrlbm_autotune(amrlbm_mesh *m, int reps, int regrid_every, double table[27]
     int best[2] = {m->order, m->dm.hg_depth};
     double best_us = 1e300;
     int v = 0;
-    for (int order = 0; order < 3; order++)
+    for (int order = 0; order < 4; order++)
         for (int depth = 0; depth < 3; depth++, v++) {
             const int knobs[2] = {order, depth};
             int rc = amrlbm_set_tuning(m, knobs);

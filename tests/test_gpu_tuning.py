@@ -1,7 +1,7 @@
 """Launch variants (amrlbm_set_tuning) and the autotuner (amrlbm_autotune).
 The leaves' launch order and the number of hg fill operands in flight per
 thread change no arithmetic, so every variant must leave the state bit for
-bit the same. The autotuner times the step graph of all nine on the live
+bit the same. The autotuner times the step graph of all twelve on the live
 mesh, keeps the fastest, and leaves the state as it found it."""
 import numpy as np
 import pytest
@@ -36,7 +36,7 @@ def _same(a, b):
 @pytest.mark.parametrize("precision", [64, 32])
 def test_variants_are_bitwise_equal(amrlbm, workloads, precision):
     w = workloads["cfg2"]
-    variants = [(o, d) for o in range(3) for d in range(3)] if precision == 64 else [(0, 0), (1, 1), (2, 2)]
+    variants = [(o, d) for o in range(4) for d in range(3)] if precision == 64 else [(0, 0), (1, 1), (2, 2), (3, 1)]
     ref = None
     for order, depth in variants:
         print(f"[tuning] FP{precision} cfg2 variant order {order} depth {depth}: 2000 steps", flush=True)
@@ -60,15 +60,16 @@ def test_autotune_keeps_the_fastest_and_the_state(amrlbm, workloads):
         b.setup_rising_bubble()
         a.step(100, 10)
         b.step(100, 10)
-        before = a.output()
+        before = _state(a)
         table = a.autotune(reps=10)
         for order, depth, us in table:
             print(f"[tuning] cfg2 order {order} depth {depth}: {us:.2f} us per step")
-        assert sorted((o, d) for o, d, _ in table) == [(o, d) for o in range(3) for d in range(3)]
+        assert sorted((o, d) for o, d, _ in table) == [(o, d) for o in range(4) for d in range(3)]
         assert all(us > 0 for _, _, us in table)
         best = min(table, key=lambda r: r[2])
         assert a.get_tuning() == best[:2]
-        assert np.array_equal(a.output(), before)
+        # the fields as they were (the diagnostics sum leaves in launch order, so they may differ in the last bit)
+        _same(_state(a), before)
         a.step(200, 10)
         b.step(200, 10)
         _same(_state(a), _state(b))
