@@ -141,18 +141,24 @@ def roofline(key, kt, nleaf, w, precision):
             "bytes_per_launch": round(b)}
 
 
-def kernels_only(amrlbm, w, calibrate_mb, precisions):
-    """The profiling run: the set-up mesh of workload w, then the three step
-    kernels launched back to back (amrlbm_time_kernels, the launches the bench
-    line's roofline times), with no whole steps; one JSON line per precision."""
+def kernels_only(amrlbm, w, calibrate_mb, precisions, variant=None):
+    """The profiling run: the set-up mesh of workload w in launch variant
+    (order, depth) if given (the autotuner's choice in the default run; the
+    tuner itself launches step graphs, which this run avoids), then the three
+    step kernels launched back to back (amrlbm_time_kernels, the launches the
+    bench line's roofline times), with no whole steps; one JSON line per
+    precision."""
     for prec in precisions:
         with amrlbm.Mesh(w, precision=prec) as m:
             m.setup_rising_bubble()
+            if variant:
+                m.set_tuning(*variant)
             kt = m.time_kernels(200)
             nleaf = m.counts()["nleaf"]
         if calibrate_mb:
             calibration_copy(calibrate_mb << 20)
         print(json.dumps({"mode": "kernels-only", "workload": w.name, "dtype": f"f{prec}", "leaf_blocks": nleaf,
+                          "variant": list(variant) if variant else [0, 1],
                           "roofline": {kk: roofline(kk, kt, nleaf, w, prec) for kk in kt}}), flush=True)
 
 
@@ -362,6 +368,8 @@ def main():
     ap.add_argument("--ensemble-seed", type=int, default=2024)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch variant (no autotuner before the timed runs)")
+    ap.add_argument("--variant", default="",
+                    help="launch variant 'order,depth' for --kernels-only (amrlbm_set_tuning)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: time the step kernels on the set-up mesh only (no whole steps)")
     args = ap.parse_args()
@@ -375,7 +383,8 @@ def main():
     w = amrlbm.WORKLOADS[args.workload]
     k = args.regrid_every
     if args.kernels_only:
-        kernels_only(amrlbm, w, args.calibrate_copy_mb, (64,) if args.no_fp32 else (64, 32))
+        variant = tuple(int(v) for v in args.variant.split(",")) if args.variant else None
+        kernels_only(amrlbm, w, args.calibrate_copy_mb, (64,) if args.no_fp32 else (64, 32), variant)
         return
     r = measure(amrlbm, w, args.steps, args.warmup, k, args.calibrate_copy_mb)
     c0, c1 = r["c0"], r["c1"]
