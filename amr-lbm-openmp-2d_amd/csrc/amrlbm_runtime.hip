@@ -616,8 +616,9 @@ bool cache_current(const amrlbm_mesh *m, int id)
 // read the topology, so they run on several host threads. With
 // AMRLBM_VERIFY_PROGRAMS set in the environment every reused program is
 // checked against a fresh build.
-// the host threads a program build or upload over nl leaves uses
-int prog_threads(int nl) { return std::max(1, std::min(omp_get_max_threads(), 3 * nl / 32)); }
+// the host threads for a pass over nl leaves: one below 4096 leaves, where
+// waking a team costs more than the pass
+int prog_threads(int nl) { return nl < 4096 ? 1 : std::min(omp_get_max_threads(), nl / 512); }
 
 // operand loads of a program: the cost of a block's ghost fill
 int program_operands(const LeafProg &lp)
@@ -643,7 +644,9 @@ int rebuild_programs(amrlbm_mesh *m)
     for (int k = 0; k < nl; k++)
         if (stale[k]) dirty.push_back(m->leaf[k]);
     const int nd = (int)dirty.size();
-#pragma omp parallel num_threads(nthr)
+    // about 80 us per leaf: threads for any handful of dirty leaves
+    const int nbuild = std::max(1, std::min(omp_get_max_threads(), nd / 2));
+#pragma omp parallel num_threads(nbuild)
     {
         std::vector<int> d;
 #pragma omp for schedule(dynamic, 4)
@@ -1138,7 +1141,8 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
         launch_restrict(m->dm, m->cur, m->d_pairs, (int)rpairs.size() / 2, m->stream);
         launch_prolong(m->dm, m->cur, m->d_pairs + rpairs.size(), (int)ppairs.size() / 2, m->ph, m->stream);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(m->stream));
+        // no wait here: the table upload below is ordered after these kernels on the stream, and it waits
+        // before pairs goes out of scope
     }
     if (rpairs.empty() && ppairs.empty()) return AMRLBM_OK;   // flags only: tables and graphs stay valid
     array_of_leaf_blocks(m);

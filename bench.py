@@ -348,7 +348,8 @@ def describe(w, k, precision=64):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=20000,
+                    help="timed steps (the 1024x2048 configuration's full run length)")
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--regrid-every", type=int, default=10)
