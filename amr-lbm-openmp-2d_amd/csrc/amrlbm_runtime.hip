@@ -127,6 +127,8 @@ struct amrlbm_mesh {
     int all_off = 0;
     bool tables_dirty = true;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_check = nullptr;
+    hipEvent_t ev_tab = nullptr;   // after the last copies out of h_tab
+    bool tab_recorded = false;     // ev_tab has been recorded (an unrecorded event must not be waited on)
     double last_step_ms = 0.0;
     int regrid_checks = 0, regrid_changes = 0, segments_skipped = 0, graphs_built = 0;
     // host wall time (ms): Refine_Derefine_Mesh, table upload, ghost-program
@@ -144,6 +146,8 @@ struct amrlbm_mesh {
     int prog_slots = 0;
     void *h_stage[3] = {nullptr, nullptr, nullptr};   // pinned upload staging per kind
     size_t stage_cap[3] = {0, 0, 0};
+    void *h_tab = nullptr;   // pinned staging of the block and neighbour tables and the leaf lists
+    size_t tab_cap = 0;
     void *d_stage[3] = {nullptr, nullptr, nullptr};   // the same on the device (nodes, then copy descriptors)
     size_t dstage_cap[3] = {0, 0, 0};
     long long prog_nodes_uploaded = 0;
@@ -407,9 +411,9 @@ struct ProgBuild {
     int naux = 0;
     std::string err;
     std::vector<int> *deps = nullptr;   // every block the build reads (see ProgCache)
-    void dep(int b)
+    void dep(int b)   // runs of one block (most refs of a node) are kept once; the caller sorts and dedupes
     {
-        if (deps && b > 0) deps->push_back(b);
+        if (deps && b > 0 && (deps->empty() || deps->back() != b)) deps->push_back(b);
     }
 
     int own_cells() const { return kind == KIND_HG ? FS : CS; }
@@ -637,19 +641,25 @@ int rebuild_programs(amrlbm_mesh *m)
     const int nl = (int)m->leaf.size();
     if (m->pcache.size() < m->blk.size()) m->pcache.resize(m->blk.size());
     const int nthr = prog_threads(nl);
+    // one team for both passes: the check of every leaf's cache (about 0.2 us
+    // per leaf), then the builds of the stale ones (tens of us per leaf, one
+    // leaf at a time so that a few dirty leaves spread over all threads)
+    const int team = nl >= 128 ? std::min(omp_get_max_threads(), 16) : 1;
     std::vector<char> stale(nl);
-#pragma omp parallel for schedule(static) num_threads(nthr)
-    for (int k = 0; k < nl; k++) stale[k] = !cache_current(m, m->leaf[k]);
     std::vector<int> dirty;
-    for (int k = 0; k < nl; k++)
-        if (stale[k]) dirty.push_back(m->leaf[k]);
-    const int nd = (int)dirty.size();
-    // about 80 us per leaf: threads for any handful of dirty leaves
-    const int nbuild = std::max(1, std::min(omp_get_max_threads(), nd / 2));
-#pragma omp parallel num_threads(nbuild)
+    int nd = 0;
+#pragma omp parallel num_threads(team)
     {
+#pragma omp for schedule(static)
+        for (int k = 0; k < nl; k++) stale[k] = !cache_current(m, m->leaf[k]);
+#pragma omp single
+        {
+            for (int k = 0; k < nl; k++)
+                if (stale[k]) dirty.push_back(m->leaf[k]);
+            nd = (int)dirty.size();
+        }
         std::vector<int> d;
-#pragma omp for schedule(dynamic, 4)
+#pragma omp for schedule(dynamic, 1)
         for (int i = 0; i < nd; i++) {   // one leaf's three programs, then the blocks they read
             ProgCache &pc = m->pcache[dirty[i]];
             pc.valid = false;
@@ -755,7 +765,8 @@ int upload_programs(amrlbm_mesh *m)
         PNode *nodes = reinterpret_cast<PNode *>(stage);
         int *desc = reinterpret_cast<int *>(stage + desc_off);
         PHdr *hdr = reinterpret_cast<PHdr *>(stage + hdr_off);
-#pragma omp parallel for schedule(static) num_threads(nthr)
+        // a few hundred KB after a regrid: copied by the team the builds just used
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nup >= 8 ? std::min(omp_get_max_threads(), 8) : 1)
         for (int i = 0; i < nup; i++) {
             const LeafProg &lp = m->pcache[up[i]].p[kind];
             std::memcpy(nodes + at[i], lp.nodes.data(), sizeof(PNode) * lp.nodes.size());
@@ -866,13 +877,29 @@ int upload_tables(amrlbm_mesh *m)
     if (!m->tables_dirty) return AMRLBM_OK;
     HostTimer timer(m->host_ms_tables);
     const int nb = m->nblks;
-    std::vector<BlockDev> bd(nb);
-    std::vector<NbDesc> nbt((size_t)nb * 8, NbDesc{0, -1, 0, -1});
+    // the tables and lists are staged in pinned memory, so that their copies
+    // run behind the host; the previous call's copies out of it have finished
+    // before this one writes it (ev_tab; normally long done, as the call ends
+    // with a wait on the stream)
+    if (m->tab_recorded) HIPCHK(hipEventSynchronize(m->ev_tab));
+    const size_t nlists = (size_t)m->leaf.size() * 2;
+    const size_t tab_need = sizeof(BlockDev) * nb + sizeof(NbDesc) * 8 * (size_t)nb + sizeof(int) * nlists + 64;
+    if (tab_need > m->tab_cap) {
+        if (m->h_tab) HIPCHK(hipHostFree(m->h_tab));
+        m->h_tab = nullptr;
+        m->tab_cap = tab_need + tab_need / 2 + 4096;
+        HIPCHK(hipHostMalloc(&m->h_tab, m->tab_cap, hipHostMallocDefault));
+    }
+    BlockDev *bd = static_cast<BlockDev *>(m->h_tab);
+    NbDesc *nbt = reinterpret_cast<NbDesc *>(bd + nb);
+    int *lists = reinterpret_cast<int *>(nbt + (size_t)nb * 8);
     int bad_block = 0, bad_k = 0;   // a block without neighbour k (2:1 balance violated)
 #pragma omp parallel for schedule(static) if (nb > 4096)
     for (int n = 1; n <= nb; n++) {
         const Block &b = m->blk[n];
         BlockDev &d = bd[n - 1];
+        d = BlockDev{};
+        for (int k = 0; k < 8; k++) nbt[(size_t)(n - 1) * 8 + k] = NbDesc{0, -1, 0, -1};
         d.level = b.level;
         d.dx = b.dx;
         d.I = b.I;
@@ -915,26 +942,29 @@ int upload_tables(amrlbm_mesh *m)
     m->lvl_cnt.assign(NL, 0);
     for (int n : m->leaf) m->lvl_cnt[m->blk[n].level]++;
     for (int L = 0; L < NL; L++) m->lvl_off[L + 1] = m->lvl_off[L] + m->lvl_cnt[L];
-    std::vector<int> lists(m->lvl_off[NL] + m->leaf.size());
+    const size_t nlist = (size_t)m->lvl_off[NL] + m->leaf.size();
     std::vector<int> fillpos(m->lvl_off.begin(), m->lvl_off.end() - 1);
     for (int n : m->leaf) lists[fillpos[m->blk[n].level]++] = n - 1;
     m->all_off = m->lvl_off[NL];
     dispatch_order(m);
     for (size_t k = 0; k < m->disp.size(); k++) lists[m->all_off + k] = m->disp[k] - 1;
-    int rc = ensure_int_buffer(&m->d_lists, &m->lists_cap, (int)lists.size());
+    int rc = ensure_int_buffer(&m->d_lists, &m->lists_cap, (int)nlist);
     if (rc) return rc;
     if (nb > 0) {
-        HIPCHK(hipMemcpyAsync(m->d_blk, bd.data(), sizeof(BlockDev) * nb, hipMemcpyHostToDevice, m->stream));
-        HIPCHK(hipMemcpyAsync(m->d_nbd, nbt.data(), sizeof(NbDesc) * nbt.size(), hipMemcpyHostToDevice, m->stream));
+        HIPCHK(hipMemcpyAsync(m->d_blk, bd, sizeof(BlockDev) * nb, hipMemcpyHostToDevice, m->stream));
+        HIPCHK(hipMemcpyAsync(m->d_nbd, nbt, sizeof(NbDesc) * 8 * (size_t)nb, hipMemcpyHostToDevice, m->stream));
     }
-    if (!lists.empty())
-        HIPCHK(hipMemcpyAsync(m->d_lists, lists.data(), sizeof(int) * lists.size(), hipMemcpyHostToDevice, m->stream));
-    HIPCHK(hipStreamSynchronize(m->stream));   // host vectors go out of scope
+    if (nlist > 0)
+        HIPCHK(hipMemcpyAsync(m->d_lists, lists, sizeof(int) * nlist, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipEventRecord(m->ev_tab, m->stream));
+    m->tab_recorded = true;
     m->dm.blk = m->d_blk;
     m->dm.nbd = m->d_nbd;
     if (!m->leaf.empty()) {
-        int rc2 = upload_programs(m);
+        int rc2 = upload_programs(m);   // ends with a wait on the stream
         if (rc2) return rc2;
+    } else {
+        HIPCHK(hipStreamSynchronize(m->stream));
     }
     m->tables_dirty = false;
     return AMRLBM_OK;
@@ -1231,6 +1261,7 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int) * 2, m->stream));
     HIPCHK(hipHostMalloc(&m->h_changed, sizeof(int) * 2, hipHostMallocDefault));
     HIPCHK(hipEventCreateWithFlags(&m->ev_check, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&m->ev_tab, hipEventDisableTiming));
     HIPCHK(hipMalloc(&m->d_part, sizeof(double) * 9 * S));
     HIPCHK(hipMalloc(&m->d_diag, sizeof(double) * 16));
     HIPCHK(hipHostMalloc(&m->h_pinned, sizeof(double) * std::max<size_t>(S, 16), hipHostMallocDefault));
@@ -1501,6 +1532,7 @@ int amrlbm_destroy(amrlbm_mesh *m)
         (void)hipFree(m->d_stage[k]);
         if (m->h_stage[k]) (void)hipHostFree(m->h_stage[k]);
     }
+    if (m->h_tab) (void)hipHostFree(m->h_tab);
     (void)hipFree(m->d_err_flags);
     (void)hipFree(m->d_lists);
     (void)hipFree(m->d_pairs);
@@ -1509,6 +1541,7 @@ int amrlbm_destroy(amrlbm_mesh *m)
     (void)hipFree(m->d_changed);
     if (m->h_changed) (void)hipHostFree(m->h_changed);
     if (m->ev_check) (void)hipEventDestroy(m->ev_check);
+    if (m->ev_tab) (void)hipEventDestroy(m->ev_tab);
     (void)hipFree(m->d_part);
     (void)hipFree(m->d_diag);
     (void)hipFree(m->d_phs);
