@@ -641,10 +641,13 @@ int rebuild_programs(amrlbm_mesh *m)
     const int nl = (int)m->leaf.size();
     if (m->pcache.size() < m->blk.size()) m->pcache.resize(m->blk.size());
     const int nthr = prog_threads(nl);
-    // one team for both passes: the check of every leaf's cache (about 0.2 us
-    // per leaf), then the builds of the stale ones (tens of us per leaf, one
-    // leaf at a time so that a few dirty leaves spread over all threads)
-    const int team = nl >= 128 ? std::min(omp_get_max_threads(), 16) : 1;
+    // one team for both passes: the check of every leaf's cache (about 0.03 us
+    // per leaf), then the builds of the stale ones (about 20 us per leaf, one
+    // leaf at a time so that a few dirty leaves spread over the team). Eight
+    // threads: 20,000 steps of the 1024x2048 rising bubble took 523 ms with 8, 546 ms with 16
+    // (the idle team spins between regrids and takes host time from the step
+    // loop) and 568 ms with 1.
+    const int team = nl >= 128 ? std::min(omp_get_max_threads(), 8) : 1;
     std::vector<char> stale(nl);
     std::vector<int> dirty;
     int nd = 0;
