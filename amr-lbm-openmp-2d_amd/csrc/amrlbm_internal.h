@@ -199,6 +199,12 @@ void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, doubl
                  hipStream_t s, int ninst = 0, double *out_inst = nullptr);
 // n copies of 16-byte words: copy k takes desc[3k+2] words from src + desc[3k]
 // to dst + desc[3k+1] (offsets in words)
-void launch_scatter_words(const void *src, const int *desc, int n, void *dst, hipStream_t s);
+// a segment of an upload batch: nwords 4-byte words from word src_word of the
+// batch to the device address dst
+struct ScatterSeg {
+    unsigned long long dst;
+    int src_word, nwords;
+};
+void launch_scatter_segments(const void *src, const ScatterSeg *desc, int n, hipStream_t s);
 
 }  // namespace amrlbm

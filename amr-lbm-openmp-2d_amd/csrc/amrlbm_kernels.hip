@@ -1253,12 +1253,15 @@ __global__ __launch_bounds__(256) void k_diag_final_inst(DevMeshT<double> m, con
 
 // Ghost-program upload: descriptor k copies desc[3k+2] 16-byte words from
 // src + desc[3k] to dst + desc[3k+1] (one workgroup each).
-__global__ __launch_bounds__(256) void k_scatter_words(const int4 *__restrict__ src, const int *__restrict__ desc,
-                                                       int4 *__restrict__ dst)
+// One upload batch: segment k (one workgroup) copies desc[k].nwords 4-byte
+// words from src + desc[k].src_word to the device address desc[k].dst.
+__global__ __launch_bounds__(256) void k_scatter_segments(const int *__restrict__ src,
+                                                          const ScatterSeg *__restrict__ desc)
 {
-    const int k = blockIdx.x;
-    const int so = desc[3 * k], d0 = desc[3 * k + 1], n = desc[3 * k + 2];
-    for (int t = threadIdx.x; t < n; t += 256) dst[d0 + t] = src[so + t];
+    const ScatterSeg d = desc[blockIdx.x];
+    int *dst = reinterpret_cast<int *>(d.dst);
+    const int *from = src + d.src_word;
+    for (int t = threadIdx.x; t < d.nwords; t += 256) dst[t] = from[t];
 }
 
 // ---------------------------------------------------------------------------
@@ -1396,11 +1399,10 @@ void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, doubl
                            out_inst);
 }
 
-void launch_scatter_words(const void *src, const int *desc, int n, void *dst, hipStream_t s)
+void launch_scatter_segments(const void *src, const ScatterSeg *desc, int n, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_scatter_words, dim3(n), dim3(256), 0, s, static_cast<const int4 *>(src), desc,
-                       static_cast<int4 *>(dst));
+    hipLaunchKernelGGL(k_scatter_segments, dim3(n), dim3(256), 0, s, static_cast<const int *>(src), desc);
 }
 
 }  // namespace amrlbm

@@ -127,7 +127,7 @@ struct amrlbm_mesh {
     int all_off = 0;
     bool tables_dirty = true;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_check = nullptr;
-    hipEvent_t ev_tab = nullptr;   // after the last copies out of h_tab
+    hipEvent_t ev_tab = nullptr;   // after the last batch scatter (and the work before it)
     bool tab_recorded = false;     // ev_tab has been recorded (an unrecorded event must not be waited on)
     double last_step_ms = 0.0;
     int regrid_checks = 0, regrid_changes = 0, segments_skipped = 0, graphs_built = 0;
@@ -144,12 +144,17 @@ struct amrlbm_mesh {
     PNode *d_pnode[3] = {nullptr, nullptr, nullptr};
     size_t phdr_cap[3] = {0, 0, 0};
     int prog_slots = 0;
-    void *h_stage[3] = {nullptr, nullptr, nullptr};   // pinned upload staging per kind
-    size_t stage_cap[3] = {0, 0, 0};
-    void *h_tab = nullptr;   // pinned staging of the block and neighbour tables and the leaf lists
-    size_t tab_cap = 0;
-    void *d_stage[3] = {nullptr, nullptr, nullptr};   // the same on the device (nodes, then copy descriptors)
-    size_t dstage_cap[3] = {0, 0, 0};
+    // The upload batch of a table upload: the block and neighbour tables, the
+    // leaf lists, the rebuilt ghost programs and the program headers, packed
+    // into one pinned buffer, sent with one copy and put in place by one
+    // scatter launch (segments listed in segs)
+    char *h_up = nullptr;
+    size_t up_cap = 0, up_used = 0;
+    void *d_up = nullptr;
+    size_t dup_cap = 0;
+    std::vector<ScatterSeg> segs;
+    int *h_pairs = nullptr;   // pinned restrict / prolong work lists
+    size_t hpairs_cap = 0;
     long long prog_nodes_uploaded = 0;
     Prog prog[3] = {};
     std::vector<ProgCache> pcache;   // by block id
@@ -721,6 +726,71 @@ int rebuild_programs(amrlbm_mesh *m)
 
 // The programs of every leaf on the device: the rebuilt ones into their
 // slots, the headers in launch order (m->disp).
+// The pinned batch buffer is written only once the previous batch's scatter
+// (and the restrict / prolong launches queued before it) has run.
+int batch_begin(amrlbm_mesh *m)
+{
+    if (m->tab_recorded) HIPCHK(hipEventSynchronize(m->ev_tab));
+    m->up_used = 0;
+    m->segs.clear();
+    return AMRLBM_OK;
+}
+
+// bytes of batch space, 16-byte aligned; *at = its offset. A pointer into
+// the buffer is valid until the next batch_alloc (the buffer may grow).
+int batch_alloc(amrlbm_mesh *m, size_t bytes, size_t *at)
+{
+    const size_t off = (m->up_used + 15) & ~(size_t)15;
+    if (off + bytes > m->up_cap) {
+        const size_t cap = (off + bytes) + (off + bytes) / 2 + 65536;
+        char *p = nullptr;
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&p), cap, hipHostMallocDefault));
+        if (m->up_used) std::memcpy(p, m->h_up, m->up_used);
+        if (m->h_up) HIPCHK(hipHostFree(m->h_up));
+        m->h_up = p;
+        m->up_cap = cap;
+    }
+    m->up_used = off + bytes;
+    *at = off;
+    return AMRLBM_OK;
+}
+
+// batch bytes [at, at + bytes) go to device address dst, in segments of at
+// most 8 KB (one workgroup each), so that a large table spreads over the GPU
+void batch_seg(amrlbm_mesh *m, void *dst, size_t at, size_t bytes)
+{
+    constexpr size_t kSeg = 8192;
+    char *d = static_cast<char *>(dst);
+    for (size_t o = 0; o < bytes; o += kSeg) {
+        const size_t n = std::min(kSeg, bytes - o);
+        m->segs.push_back({(unsigned long long)(uintptr_t)(d + o), (int)((at + o) / 4), (int)(n / 4)});
+    }
+}
+
+// one copy of the batch and one scatter launch; no wait (batch_begin waits)
+int batch_flush(amrlbm_mesh *m)
+{
+    if (m->segs.empty()) return AMRLBM_OK;
+    size_t at = 0;
+    const size_t ns = m->segs.size();
+    int rc = batch_alloc(m, sizeof(ScatterSeg) * ns, &at);
+    if (rc) return rc;
+    std::memcpy(m->h_up + at, m->segs.data(), sizeof(ScatterSeg) * ns);
+    if (m->up_used > m->dup_cap) {
+        if (m->d_up) HIPCHK(hipFree(m->d_up));
+        m->d_up = nullptr;
+        m->dup_cap = m->up_used + m->up_used / 2 + 65536;
+        HIPCHK(hipMalloc(&m->d_up, m->dup_cap));
+    }
+    HIPCHK(hipMemcpyAsync(m->d_up, m->h_up, m->up_used, hipMemcpyHostToDevice, m->stream));
+    launch_scatter_segments(m->d_up, reinterpret_cast<const ScatterSeg *>(static_cast<char *>(m->d_up) + at), (int)ns,
+                            m->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(m->ev_tab, m->stream));
+    m->tab_recorded = true;
+    return AMRLBM_OK;
+}
+
 int upload_programs(amrlbm_mesh *m)
 {
     HostTimer timer(m->host_ms_programs);
@@ -741,7 +811,6 @@ int upload_programs(amrlbm_mesh *m)
     for (int k = 0; k < nl; k++)
         if (!m->pcache[m->disp[k]].on_device) up.push_back(m->disp[k]);
     const int nup = (int)up.size();
-    constexpr int W = (int)(sizeof(PNode) / 16);   // 16-byte words per node
     for (int kind = 0; kind < 3; kind++) {
         int naux = 0, nmax = 0;
         const int *na = m->naux_id[kind].data(), *nn = m->nn_id[kind].data();
@@ -750,64 +819,44 @@ int upload_programs(amrlbm_mesh *m)
             naux = std::max(naux, na[m->disp[k]]);
             nmax = std::max(nmax, nn[m->disp[k]]);
         }
-        // pinned stage: the nodes of the leaves in up, packed, then one copy
-        // descriptor (source word, destination word, words) per leaf, then
-        // the headers of every leaf
-        std::vector<size_t> at(nup + 1, 0);
-        for (int i = 0; i < nup; i++) at[i + 1] = at[i] + m->pcache[up[i]].p[kind].nodes.size();
-        const size_t total = at[nup];
-        const size_t desc_off = sizeof(PNode) * total, hdr_off = desc_off + sizeof(int) * 3 * (size_t)nup;
-        const size_t need = hdr_off + sizeof(PHdr) * nl;
-        if (need > m->stage_cap[kind]) {
-            if (m->h_stage[kind]) HIPCHK(hipHostFree(m->h_stage[kind]));
-            m->h_stage[kind] = nullptr;
-            m->stage_cap[kind] = need + need / 4 + 4096;
-            HIPCHK(hipHostMalloc(&m->h_stage[kind], m->stage_cap[kind], hipHostMallocDefault));
-        }
-        char *stage = static_cast<char *>(m->h_stage[kind]);
-        PNode *nodes = reinterpret_cast<PNode *>(stage);
-        int *desc = reinterpret_cast<int *>(stage + desc_off);
-        PHdr *hdr = reinterpret_cast<PHdr *>(stage + hdr_off);
-        // a few hundred KB after a regrid: copied by the team the builds just used
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nup >= 8 ? std::min(omp_get_max_threads(), 8) : 1)
-        for (int i = 0; i < nup; i++) {
-            const LeafProg &lp = m->pcache[up[i]].p[kind];
-            std::memcpy(nodes + at[i], lp.nodes.data(), sizeof(PNode) * lp.nodes.size());
-            desc[3 * i] = (int)at[i] * W;
-            desc[3 * i + 1] = (up[i] - 1) * kProgSlot[kind] * W;
-            desc[3 * i + 2] = (int)lp.nodes.size() * W;
-        }
-        const PHdr *hid = m->hdr_id[kind].data();
-#pragma omp parallel for schedule(static) num_threads(nthr)
-        for (int k = 0; k < nl; k++) hdr[k] = hid[m->disp[k]];
         if ((size_t)nl > m->phdr_cap[kind]) {
             if (m->d_phdr[kind]) HIPCHK(hipFree(m->d_phdr[kind]));
             m->d_phdr[kind] = nullptr;
             m->phdr_cap[kind] = nl + nl / 4 + 64;
             HIPCHK(hipMalloc(&m->d_phdr[kind], sizeof(PHdr) * m->phdr_cap[kind]));
         }
-        if (nup > 0) {
-            if (hdr_off > m->dstage_cap[kind]) {
-                if (m->d_stage[kind]) HIPCHK(hipFree(m->d_stage[kind]));
-                m->d_stage[kind] = nullptr;
-                m->dstage_cap[kind] = hdr_off + hdr_off / 4 + 4096;
-                HIPCHK(hipMalloc(&m->d_stage[kind], m->dstage_cap[kind]));
-            }
-            HIPCHK(hipMemcpyAsync(m->d_stage[kind], stage, hdr_off, hipMemcpyHostToDevice, m->stream));
-            launch_scatter_words(m->d_stage[kind], reinterpret_cast<const int *>(static_cast<char *>(m->d_stage[kind]) + desc_off),
-                                 nup, m->d_pnode[kind], m->stream);
-            HIPCHK(hipGetLastError());
+        // the nodes of the leaves in up, packed, each a segment to its block slot
+        std::vector<size_t> at(nup + 1, 0);
+        for (int i = 0; i < nup; i++) at[i + 1] = at[i] + m->pcache[up[i]].p[kind].nodes.size();
+        const size_t total = at[nup];
+        size_t base = 0;
+        int rc = batch_alloc(m, sizeof(PNode) * total, &base);
+        if (rc) return rc;
+        PNode *nodes = reinterpret_cast<PNode *>(m->h_up + base);
+        // a few hundred KB after a regrid: copied by the team the builds just used
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nup >= 8 ? std::min(omp_get_max_threads(), 8) : 1)
+        for (int i = 0; i < nup; i++) {
+            const LeafProg &lp = m->pcache[up[i]].p[kind];
+            std::memcpy(nodes + at[i], lp.nodes.data(), sizeof(PNode) * lp.nodes.size());
         }
-        if (nl > 0)
-            HIPCHK(hipMemcpyAsync(m->d_phdr[kind], hdr, sizeof(PHdr) * nl, hipMemcpyHostToDevice, m->stream));
+        for (int i = 0; i < nup; i++)
+            batch_seg(m, m->d_pnode[kind] + (size_t)(up[i] - 1) * kProgSlot[kind], base + sizeof(PNode) * at[i],
+                      sizeof(PNode) * (at[i + 1] - at[i]));
+        // the headers of every leaf in launch order
+        rc = batch_alloc(m, sizeof(PHdr) * nl, &base);
+        if (rc) return rc;
+        PHdr *hdr = reinterpret_cast<PHdr *>(m->h_up + base);
+        const PHdr *hid = m->hdr_id[kind].data();
+#pragma omp parallel for schedule(static) num_threads(nthr)
+        for (int k = 0; k < nl; k++) hdr[k] = hid[m->disp[k]];
+        batch_seg(m, m->d_phdr[kind], base, sizeof(PHdr) * nl);
         m->prog[kind].hdr = m->d_phdr[kind];
         m->prog[kind].nodes = m->d_pnode[kind];
         m->prog[kind].naux = naux;   // the hg kernel sizes its LDS by these
         m->prog[kind].nmax = nmax;
         m->prog_nodes_uploaded += (long long)total;
     }
-    for (int id : up) m->pcache[id].on_device = true;
-    HIPCHK(hipStreamSynchronize(m->stream));   // the pinned stages are reused by the next build
+    for (int id : up) m->pcache[id].on_device = true;   // in stream order: the batch scatter precedes any use
     return AMRLBM_OK;
 }
 
@@ -880,22 +929,14 @@ int upload_tables(amrlbm_mesh *m)
     if (!m->tables_dirty) return AMRLBM_OK;
     HostTimer timer(m->host_ms_tables);
     const int nb = m->nblks;
-    // the tables and lists are staged in pinned memory, so that their copies
-    // run behind the host; the previous call's copies out of it have finished
-    // before this one writes it (ev_tab; normally long done, as the call ends
-    // with a wait on the stream)
-    if (m->tab_recorded) HIPCHK(hipEventSynchronize(m->ev_tab));
-    const size_t nlists = (size_t)m->leaf.size() * 2;
-    const size_t tab_need = sizeof(BlockDev) * nb + sizeof(NbDesc) * 8 * (size_t)nb + sizeof(int) * nlists + 64;
-    if (tab_need > m->tab_cap) {
-        if (m->h_tab) HIPCHK(hipHostFree(m->h_tab));
-        m->h_tab = nullptr;
-        m->tab_cap = tab_need + tab_need / 2 + 4096;
-        HIPCHK(hipHostMalloc(&m->h_tab, m->tab_cap, hipHostMallocDefault));
-    }
-    BlockDev *bd = static_cast<BlockDev *>(m->h_tab);
+    // tables, lists and programs go out in one upload batch
+    int rcb = batch_begin(m);
+    if (rcb) return rcb;
+    size_t tab_at = 0;
+    rcb = batch_alloc(m, sizeof(BlockDev) * nb + sizeof(NbDesc) * 8 * (size_t)nb, &tab_at);
+    if (rcb) return rcb;
+    BlockDev *bd = reinterpret_cast<BlockDev *>(m->h_up + tab_at);
     NbDesc *nbt = reinterpret_cast<NbDesc *>(bd + nb);
-    int *lists = reinterpret_cast<int *>(nbt + (size_t)nb * 8);
     int bad_block = 0, bad_k = 0;   // a block without neighbour k (2:1 balance violated)
 #pragma omp parallel for schedule(static) if (nb > 4096)
     for (int n = 1; n <= nb; n++) {
@@ -935,6 +976,8 @@ int upload_tables(amrlbm_mesh *m)
     if (bad_block)
         return fail(AMRLBM_ERR_TOPOLOGY, "block " + std::to_string(bad_block) + " has no neighbour " +
                                              std::to_string(bad_k) + " (2:1 balance violated)");
+    batch_seg(m, m->d_blk, tab_at, sizeof(BlockDev) * nb);
+    batch_seg(m, m->d_nbd, tab_at + sizeof(BlockDev) * nb, sizeof(NbDesc) * 8 * (size_t)nb);
     if (!m->leaf.empty()) {   // the programs first: launch order 3 weighs them
         int rc0 = rebuild_programs(m);
         if (rc0) return rc0;
@@ -946,6 +989,10 @@ int upload_tables(amrlbm_mesh *m)
     for (int n : m->leaf) m->lvl_cnt[m->blk[n].level]++;
     for (int L = 0; L < NL; L++) m->lvl_off[L + 1] = m->lvl_off[L] + m->lvl_cnt[L];
     const size_t nlist = (size_t)m->lvl_off[NL] + m->leaf.size();
+    size_t lists_at = 0;
+    rcb = batch_alloc(m, sizeof(int) * nlist, &lists_at);
+    if (rcb) return rcb;
+    int *lists = reinterpret_cast<int *>(m->h_up + lists_at);
     std::vector<int> fillpos(m->lvl_off.begin(), m->lvl_off.end() - 1);
     for (int n : m->leaf) lists[fillpos[m->blk[n].level]++] = n - 1;
     m->all_off = m->lvl_off[NL];
@@ -953,22 +1000,15 @@ int upload_tables(amrlbm_mesh *m)
     for (size_t k = 0; k < m->disp.size(); k++) lists[m->all_off + k] = m->disp[k] - 1;
     int rc = ensure_int_buffer(&m->d_lists, &m->lists_cap, (int)nlist);
     if (rc) return rc;
-    if (nb > 0) {
-        HIPCHK(hipMemcpyAsync(m->d_blk, bd, sizeof(BlockDev) * nb, hipMemcpyHostToDevice, m->stream));
-        HIPCHK(hipMemcpyAsync(m->d_nbd, nbt, sizeof(NbDesc) * 8 * (size_t)nb, hipMemcpyHostToDevice, m->stream));
-    }
-    if (nlist > 0)
-        HIPCHK(hipMemcpyAsync(m->d_lists, lists, sizeof(int) * nlist, hipMemcpyHostToDevice, m->stream));
-    HIPCHK(hipEventRecord(m->ev_tab, m->stream));
-    m->tab_recorded = true;
+    batch_seg(m, m->d_lists, lists_at, sizeof(int) * nlist);
     m->dm.blk = m->d_blk;
     m->dm.nbd = m->d_nbd;
     if (!m->leaf.empty()) {
-        int rc2 = upload_programs(m);   // ends with a wait on the stream
+        int rc2 = upload_programs(m);
         if (rc2) return rc2;
-    } else {
-        HIPCHK(hipStreamSynchronize(m->stream));
     }
+    rc = batch_flush(m);
+    if (rc) return rc;
     m->tables_dirty = false;
     return AMRLBM_OK;
 }
@@ -1168,14 +1208,23 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
     if (!pairs.empty()) {
         rc = ensure_int_buffer(&m->d_pairs, &m->pairs_cap, (int)pairs.size());
         if (rc) return rc;
-        HIPCHK(hipMemcpyAsync(m->d_pairs, pairs.data(), sizeof(int) * pairs.size(), hipMemcpyHostToDevice, m->stream));
+        // pinned, reused once the last batch (queued after the previous
+        // regrid's restrict and prolong) has run
+        if (m->tab_recorded) HIPCHK(hipEventSynchronize(m->ev_tab));
+        if (pairs.size() > m->hpairs_cap) {
+            if (m->h_pairs) HIPCHK(hipHostFree(m->h_pairs));
+            m->h_pairs = nullptr;
+            m->hpairs_cap = pairs.size() + pairs.size() / 2 + 1024;
+            HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&m->h_pairs), sizeof(int) * m->hpairs_cap, hipHostMallocDefault));
+        }
+        std::memcpy(m->h_pairs, pairs.data(), sizeof(int) * pairs.size());
+        HIPCHK(hipMemcpyAsync(m->d_pairs, m->h_pairs, sizeof(int) * pairs.size(), hipMemcpyHostToDevice, m->stream));
         // the kernels read the pre-regrid block table (parent dx, children);
         // it is replaced only after them
         launch_restrict(m->dm, m->cur, m->d_pairs, (int)rpairs.size() / 2, m->stream);
         launch_prolong(m->dm, m->cur, m->d_pairs + rpairs.size(), (int)ppairs.size() / 2, m->ph, m->stream);
         HIPCHK(hipGetLastError());
-        // no wait here: the table upload below is ordered after these kernels on the stream, and it waits
-        // before pairs goes out of scope
+        // no wait here: the table upload below is ordered after these kernels on the stream
     }
     if (rpairs.empty() && ppairs.empty()) return AMRLBM_OK;   // flags only: tables and graphs stay valid
     array_of_leaf_blocks(m);
@@ -1532,10 +1581,10 @@ int amrlbm_destroy(amrlbm_mesh *m)
     for (int k = 0; k < 3; k++) {
         (void)hipFree(m->d_phdr[k]);
         (void)hipFree(m->d_pnode[k]);
-        (void)hipFree(m->d_stage[k]);
-        if (m->h_stage[k]) (void)hipHostFree(m->h_stage[k]);
     }
-    if (m->h_tab) (void)hipHostFree(m->h_tab);
+    if (m->h_up) (void)hipHostFree(m->h_up);
+    (void)hipFree(m->d_up);
+    if (m->h_pairs) (void)hipHostFree(m->h_pairs);
     (void)hipFree(m->d_err_flags);
     (void)hipFree(m->d_lists);
     (void)hipFree(m->d_pairs);
