@@ -129,6 +129,8 @@ struct amrlbm_mesh {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_check = nullptr;
     hipEvent_t ev_tab = nullptr;   // after the last batch scatter (and the work before it)
     bool tab_recorded = false;     // ev_tab has been recorded (an unrecorded event must not be waited on)
+    hipEvent_t ev_pairs = nullptr; // after the last copy out of h_pairs
+    bool pairs_recorded = false;
     double last_step_ms = 0.0;
     int regrid_checks = 0, regrid_changes = 0, segments_skipped = 0, graphs_built = 0;
     // host wall time (ms): Refine_Derefine_Mesh, table upload, ghost-program
@@ -1208,9 +1210,8 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
     if (!pairs.empty()) {
         rc = ensure_int_buffer(&m->d_pairs, &m->pairs_cap, (int)pairs.size());
         if (rc) return rc;
-        // pinned, reused once the last batch (queued after the previous
-        // regrid's restrict and prolong) has run
-        if (m->tab_recorded) HIPCHK(hipEventSynchronize(m->ev_tab));
+        // pinned, rewritten once the previous copy out of it has run
+        if (m->pairs_recorded) HIPCHK(hipEventSynchronize(m->ev_pairs));
         if (pairs.size() > m->hpairs_cap) {
             if (m->h_pairs) HIPCHK(hipHostFree(m->h_pairs));
             m->h_pairs = nullptr;
@@ -1219,6 +1220,8 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
         }
         std::memcpy(m->h_pairs, pairs.data(), sizeof(int) * pairs.size());
         HIPCHK(hipMemcpyAsync(m->d_pairs, m->h_pairs, sizeof(int) * pairs.size(), hipMemcpyHostToDevice, m->stream));
+        HIPCHK(hipEventRecord(m->ev_pairs, m->stream));
+        m->pairs_recorded = true;
         // the kernels read the pre-regrid block table (parent dx, children);
         // it is replaced only after them
         launch_restrict(m->dm, m->cur, m->d_pairs, (int)rpairs.size() / 2, m->stream);
@@ -1314,6 +1317,7 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipHostMalloc(&m->h_changed, sizeof(int) * 2, hipHostMallocDefault));
     HIPCHK(hipEventCreateWithFlags(&m->ev_check, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&m->ev_tab, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&m->ev_pairs, hipEventDisableTiming));
     HIPCHK(hipMalloc(&m->d_part, sizeof(double) * 9 * S));
     HIPCHK(hipMalloc(&m->d_diag, sizeof(double) * 16));
     HIPCHK(hipHostMalloc(&m->h_pinned, sizeof(double) * std::max<size_t>(S, 16), hipHostMallocDefault));
@@ -1594,6 +1598,7 @@ int amrlbm_destroy(amrlbm_mesh *m)
     if (m->h_changed) (void)hipHostFree(m->h_changed);
     if (m->ev_check) (void)hipEventDestroy(m->ev_check);
     if (m->ev_tab) (void)hipEventDestroy(m->ev_tab);
+    if (m->ev_pairs) (void)hipEventDestroy(m->ev_pairs);
     (void)hipFree(m->d_part);
     (void)hipFree(m->d_diag);
     (void)hipFree(m->d_phs);
