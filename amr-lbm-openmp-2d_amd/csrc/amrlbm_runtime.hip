@@ -940,7 +940,8 @@ int upload_tables(amrlbm_mesh *m)
     BlockDev *bd = reinterpret_cast<BlockDev *>(m->h_up + tab_at);
     NbDesc *nbt = reinterpret_cast<NbDesc *>(bd + nb);
     int bad_block = 0, bad_k = 0;   // a block without neighbour k (2:1 balance violated)
-#pragma omp parallel for schedule(static) if (nb > 4096)
+    // the same team size as the program rebuild that follows, so that it finds the threads awake
+#pragma omp parallel for schedule(static) num_threads(nb >= 256 ? std::min(omp_get_max_threads(), 8) : 1)
     for (int n = 1; n <= nb; n++) {
         const Block &b = m->blk[n];
         BlockDev &d = bd[n - 1];
