@@ -46,7 +46,7 @@ def instance_from_workload(w):
     d = Instance()
     d.L0, d.min_levels, d.max_levels = w.L0, w.min_levels, w.max_levels
     d.M0, d.W, d.sigma, d.beta, d.kappa = w.M0, w.W, w.sigma, w.beta, w.kappa
-    d.taul, d.tauh, d.rhol, d.rhoh, d.gy = w.taul, w.tauh, w.rhol, w.rhoh, w.gy
+    d.taul, d.tauh, d.rhol, d.rhoh, d.gy = w.taul, w.tauh, w.rhol, w.rhoh, w.gy_run
     d.R, d.x0, d.y0 = w.radius, w.x0, w.y0
     return d
 
@@ -57,7 +57,7 @@ def params_from_workload(w):
     p.L0, p.nroots, p.min_levels, p.max_levels = w.L0, w.nroots, w.min_levels, w.max_levels
     p.max_blocks = w.max_blocks
     p.M0, p.W, p.sigma, p.beta, p.kappa = w.M0, w.W, w.sigma, w.beta, w.kappa
-    p.taul, p.tauh, p.rhol, p.rhoh, p.gy = w.taul, w.tauh, w.rhol, w.rhoh, w.gy
+    p.taul, p.tauh, p.rhol, p.rhoh, p.gy = w.taul, w.tauh, w.rhol, w.rhoh, w.gy_run
     return p
 
 
@@ -93,6 +93,7 @@ def lib():
             "amrlbm_output_instances": [vp, ip, _DP],
             "amrlbm_tecplot_cell_center": [vp, ctypes.c_char_p, dp],
             "amrlbm_get_counts": [vp, _IP],
+            "amrlbm_set_step": [vp, ip],
             "amrlbm_get_topology": [vp, _IP, _IP],
             "amrlbm_set_topology": [vp, ip, _IP, ip, ip, _IP, ip],
             "amrlbm_put_block": [vp, ip, _DP, _DP, _DP, _DP, _DP, _DP],
@@ -288,6 +289,11 @@ class Mesh:
         _check(lib().amrlbm_get_counts(self.h, _i(c)))
         keys = ("nblks", "nleaf", "levelmin", "levelmax", "kk", "nfree", "T")
         return dict(zip(keys, (int(v) for v in c)))
+
+    def set_step(self, T):
+        """The step counter T (amrlbm_set_step): the phase entry points leave it
+        to the caller, as the reference's subroutines leave it to Main_Loop."""
+        _check(lib().amrlbm_set_step(self.h, int(T)))
 
     def topology(self):
         """(blocks[nblks,21], free list) in the reference's terms."""

@@ -131,7 +131,7 @@ class Ensemble:
         return sum(m.fine_cells for m in self.members)
 
     def __getattr__(self, name):   # M0, W, sigma, ...: the first instance's
-        if name in ("M0", "W", "sigma", "beta", "kappa", "taul", "tauh", "rhol", "rhoh", "gy"):
+        if name in ("M0", "W", "sigma", "beta", "kappa", "taul", "tauh", "rhol", "rhoh", "gy", "gy_run"):
             return getattr(self.members[0], name)
         raise AttributeError(name)
 

@@ -32,6 +32,9 @@ class Workload:
     refin_tol: float = 0.002
     deref_tol: float = 0.001
     instances: int = 1      # independent copies stacked in y (amrlbm_setup_replicas)
+    gy_step: float = 0.0    # when set, the gravity of the time steps (the set-up's sigma, tau and
+                            # pressure keep the ones gy gives): a faster-rising bubble whose
+                            # interface crosses blocks within a few thousand steps
     block_cap: int = 0      # MaxBlks when set; else the reference's formula
 
     @property
@@ -53,6 +56,10 @@ class Workload:
     @property
     def fine_cells(self):
         return self.fine_shape[0] * self.fine_shape[1]
+
+    @property
+    def gy_run(self):          # the gravity the time steps use (D2Q9_Mod's Gy)
+        return self.gy_step or self.gy
 
     @property
     def D(self):
@@ -97,6 +104,11 @@ WORKLOADS = {
     # of leaf state): the step kernels at a working set beyond the 256 MB Infinity Cache
     "cfg2x16": Workload("cfg2x16", L0=1024, min_levels=3, radius=64.0, x0=512.0, y0=512.0, nroots=32,
                         instances=16, block_cap=16384),
+    # the tiny and 128 x 256 bubbles with 20x and 10x the gravity in the time steps: the mesh
+    # changes within 2,400 steps (refinement, derefinement, reuse of freed ids); parity cases
+    # across mesh changes
+    "tiny_rise": Workload("tiny_rise", L0=64, min_levels=1, radius=10.0, x0=32.0, y0=32.0, gy_step=1.0e-4),
+    "cfg1_rise": Workload("cfg1_rise", L0=128, min_levels=1, radius=16.0, x0=64.0, y0=64.0, gy_step=5.0e-5),
     # 2 copies of cfg1: the replicated-instance parity case
     "cfg1x2": Workload("cfg1x2", L0=128, min_levels=1, radius=16.0, x0=64.0, y0=64.0, nroots=4, instances=2),
 }

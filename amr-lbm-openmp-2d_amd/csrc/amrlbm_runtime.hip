@@ -53,6 +53,13 @@ struct Block {
     int level = -1, dx = 0, I = 0, J = 0, parent = 0;
     int child[4] = {0, 0, 0, 0};
     bool has_child = false, isleaf = false, hasleaf = false, refin = false, deref = false;
+    // the reference's Blk(n)%h etc. are ALLOCATED: from Add_Child_Blocks
+    // (AMR_Rising_2D_OMP.f90:565-570) until Prolong frees a refined block's
+    // (Prolong.f90:26), or Restrict allocates a derefined one's
+    // (Restrict.f90:25-27) until Remove_Children frees its children's
+    // (Refine_Derefine.f90:242-243). Initial_Mesh's inner blocks keep theirs,
+    // zero. Restart files carry arrays for exactly these blocks.
+    bool alloc = false;
 };
 
 // Ghost program of one leaf for one fill kind, nodes grouped by stage.
@@ -312,6 +319,7 @@ int add_child_blocks(amrlbm_mesh *m, int par, int freeblk)
             b.hasleaf = false;
             b.refin = false;
             b.deref = false;
+            b.alloc = true;
             place(m, N);
         }
     m->nblks = std::max(m->nblks, N);
@@ -327,6 +335,7 @@ void remove_children(amrlbm_mesh *m, int par)
         m->blk[c].level = -1;
         m->blk[c].isleaf = false;
         m->blk[c].has_child = false;
+        m->blk[c].alloc = false;
     }
     Block &P = m->blk[par];
     for (int k = 0; k < 4; k++) P.child[k] = par;
@@ -1181,6 +1190,7 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
         if (m->blk[n].deref) {
             rpairs.push_back(n - 1);
             rpairs.push_back(m->blk[n].child[0] - 1);
+            m->blk[n].alloc = true;
             remove_children(m, n);
         }
     // Refine (:262-294): ids from the free-list head or Nblks+1
@@ -1201,6 +1211,7 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
                 rc = add_child_blocks(m, n, c1);
                 if (rc) return rc;
             }
+            m->blk[n].alloc = false;
             ppairs.push_back(n - 1);
             ppairs.push_back(c1 - 1);
         }
@@ -1654,6 +1665,7 @@ static int setup_mesh(amrlbm_mesh *m, int ninst, Init init)
     }
     // Initial_Mesh (AMR_Rising_2D_OMP.f90:493-513); device slots start zeroed
     HIPCHK(hipMemsetAsync(m->dm.pop[0], 0, esz(m) * POPB * p.max_blocks, m->stream));
+    HIPCHK(hipMemsetAsync(m->dm.pop[1], 0, esz(m) * POPB * p.max_blocks, m->stream));
     HIPCHK(hipMemsetAsync(m->dm.C, 0, esz(m) * CS * p.max_blocks, m->stream));
     HIPCHK(hipMemsetAsync(m->dm.PU, 0, esz(m) * PUB * p.max_blocks, m->stream));
     for (int L = 1; L <= m->levelmin; L++) {
@@ -1919,6 +1931,13 @@ int amrlbm_get_counts(amrlbm_mesh *m, int counts[7])
     return AMRLBM_OK;
 }
 
+int amrlbm_set_step(amrlbm_mesh *m, int T)
+{
+    if (!m || T < 0) return fail(AMRLBM_ERR_ARG, "bad step counter");
+    m->T = T;
+    return AMRLBM_OK;
+}
+
 int amrlbm_get_topology(amrlbm_mesh *m, int *blocks, int *freeq)
 {
     if (!m || !blocks) return fail(AMRLBM_ERR_ARG, "null argument");
@@ -1947,26 +1966,64 @@ static int set_topology(amrlbm_mesh *m, int nblks, const int *blocks, int kk, in
     m->tile_roots = m->prm.nroots / ninst;
     reset_topology(m);
     m->nblks = nblks;
+    // a rejected table leaves no mesh behind (every later call reports it)
+    auto bad = [&](int n, const std::string &what) {
+        reset_topology(m);
+        m->ready = false;
+        return fail(AMRLBM_ERR_TOPOLOGY, "block " + std::to_string(n) + ": " + what);
+    };
     for (int n = 1; n <= nblks; n++) {
         const int *r = blocks + (size_t)(n - 1) * 21;
         Block &b = m->blk[n];
         b.level = r[0]; b.dx = r[1]; b.I = r[2]; b.J = r[3]; b.parent = r[4];
         b.isleaf = r[5]; b.hasleaf = r[6]; b.refin = r[7]; b.deref = r[8];
         b.has_child = b.level >= 0;
+        b.alloc = b.level >= 0 && b.isleaf;   // load_snapshot sets it from the file
         for (int k = 0; k < 4; k++) b.child[k] = r[9 + k];
-        if (b.level > m->prm.max_levels) return fail(AMRLBM_ERR_ARG, "level above max_levels");
+        if (b.level > m->prm.max_levels) return bad(n, "level above max_levels");
         if (b.level >= 0) {
             if (b.I < 0 || b.I >= m->width(b.level) || b.J < 0 || b.J >= m->height(b.level))
-                return fail(AMRLBM_ERR_ARG, "block position outside the domain");
+                return bad(n, "block position outside the domain");
+            if (b.dx != (m->prm.L0 / NX) >> b.level) return bad(n, "dx is not L0 / nx / 2^Level");
             place(m, n);
         }
+    }
+    for (int n = 1; n <= nblks; n++) {
+        const Block &b = m->blk[n];
+        if (b.level < -1) return bad(n, "level below -1");
+        if (b.level < 0) continue;
+        // a root's Parent is -1 (AMR_Rising_2D_OMP.f90:95)
+        if (b.level == 0 ? b.parent != -1 : (b.parent < 1 || b.parent > nblks))
+            return bad(n, "parent id not -1 for a root or in 1..Nblks below");
+        if (b.level > 0) {
+            const Block &p = m->blk[b.parent];
+            if (p.level != b.level - 1 || p.isleaf || n < p.child[0] || n > p.child[0] + 3)
+                return bad(n, "parent " + std::to_string(b.parent) + " does not list it as a child");
+        }
+        if (!b.isleaf) {
+            if (b.child[0] < 1 || b.child[0] > nblks - 3) return bad(n, "first child id outside 1..Nblks-3");
+            for (int k = 0; k < 4; k++) {
+                const Block &c = m->blk[b.child[0] + k];
+                if (b.child[k] != b.child[0] + k || c.level != b.level + 1 || c.parent != n)
+                    return bad(n, "children are not 4 contiguous blocks one level finer");
+                if (b.hasleaf && !c.isleaf) return bad(n, "hasLeaf set with a refined child");
+            }
+        } else if (b.hasleaf) {
+            return bad(n, "hasLeaf set on a leaf");
+        }
+    }
+    if (kk != 4 * nfree) return bad(0, "KK is not 4 times the free-list length");
+    for (int k = 0; k < nfree; k++) {
+        const int f = freeq[k];
+        if (f < 1 || f > nblks - 3) return bad(f, "free-list id outside 1..Nblks-3");
+        for (int c = f; c < f + 4; c++)
+            if (m->blk[c].level != -1) return bad(c, "on the free list but in use");
     }
     for (int n = 1; n <= nblks; n++) {
         if (m->blk[n].level < 0) continue;
         for (int k = 1; k <= 8; k++)
             if (nb_value(m, n, k) != blocks[(size_t)(n - 1) * 21 + 12 + k])
-                return fail(AMRLBM_ERR_TOPOLOGY, "neighbour " + std::to_string(k) + " of block " + std::to_string(n) +
-                                                     " disagrees with the quadtree");
+                return bad(n, "neighbour " + std::to_string(k) + " disagrees with the quadtree");
     }
     for (int k = 0; k < nfree; k++) m->freeq.push_back(freeq[k]);
     m->kk = kk;
@@ -2017,7 +2074,7 @@ int amrlbm_save_snapshot(amrlbm_mesh *m, const char *path)
         int iv[22];
         std::memcpy(iv, &rows[(size_t)(n - 1) * 21], sizeof(int) * 21);
         const Block &b = m->blk[n];
-        iv[21] = (b.level >= 0 && b.isleaf) ? 1 : 0;
+        iv[21] = (b.level >= 0 && (b.isleaf || b.alloc)) ? 1 : 0;
         ok = std::fwrite(iv, sizeof(int), 22, fp) == 22;
         if (!ok || !iv[21]) continue;
         double *d = a.data();
@@ -2067,6 +2124,7 @@ int amrlbm_load_snapshot(amrlbm_mesh *m, const char *path)
     std::fclose(fp);
     int rc = set_topology(m, nblks, rows.data(), hdr[12], nfree, freeq.data(), hdr[9], ninst);
     if (rc) return rc;
+    for (int n : leafs) m->blk[n].alloc = true;
     const bool raw = file_dev && m->dm.f32;   // same storage: no conversion at all
     for (size_t k = 0; k < leafs.size(); k++) {
         double *d = &arrays[k * na];
@@ -2440,16 +2498,18 @@ int amrlbm_refine_derefine_predicted(amrlbm_mesh *m, const int *ids, int n)
     return regrid(m, 0.002, 0.001, &extra);
 }
 
-int amrlbm_step(amrlbm_mesh *m, int nsteps, int regrid_every)
+}  // extern "C"
+
+// The steps of amrlbm_step. Stream order: segment r, regrid check r, segment
+// r+1, ... The host reads check r while segment r+1 runs. When check r says the
+// mesh changes, segment r+1 has skipped all its work: the host takes its steps
+// back, runs Refine_Derefine_Mesh on the state of check r and goes on from
+// there. *ahead is set while a segment launched behind a pending check has
+// advanced T and the buffer parity (so an error return can take it back).
+static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahead_len, int *ahead_cur)
 {
-    NEED_READY(m);
-    if (nsteps < 0 || regrid_every < 1) return fail(AMRLBM_ERR_ARG, "bad step arguments");
     int rc = AMRLBM_OK;
     HIPCHK(hipEventRecord(m->ev0, m->stream));
-    // Stream order: segment r, regrid check r, segment r+1, ... The host reads
-    // check r while segment r+1 runs. When check r says the mesh changes,
-    // segment r+1 has skipped all its work: the host takes its steps back,
-    // runs Refine_Derefine_Mesh on the state of check r and goes on from there.
     int done = 0;
     bool pending = false;
     while (done < nsteps || pending) {
@@ -2468,6 +2528,10 @@ int amrlbm_step(amrlbm_mesh *m, int nsteps, int regrid_every)
             m->T += len;
             done += len;
             launched = true;
+            if (pending) {
+                *ahead_len = len;
+                *ahead_cur = cur0;
+            }
         }
         if (pending) {
             pending = false;
@@ -2480,11 +2544,16 @@ int amrlbm_step(amrlbm_mesh *m, int nsteps, int regrid_every)
                     done -= len;
                     m->segments_skipped++;
                 }
-                HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int), m->stream));
+                *ahead_len = 0;
                 m->regrid_changes++;
+                // the skip word stays set while the host regrids (no step
+                // kernel runs meanwhile); it is cleared below, or by the
+                // caller on an error return
                 if ((rc = regrid(m, 0.002, 0.001))) return rc;
+                HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int), m->stream));
                 continue;
             }
+            *ahead_len = 0;
             clear_regrid_flags(m);
         }
         if (launched && m->T % regrid_every == 0) {
@@ -2499,6 +2568,31 @@ int amrlbm_step(amrlbm_mesh *m, int nsteps, int regrid_every)
     HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
     m->last_step_ms = ms;
     return check_err_flags(m);
+}
+
+extern "C" {
+
+int amrlbm_step(amrlbm_mesh *m, int nsteps, int regrid_every)
+{
+    NEED_READY(m);
+    if (nsteps < 0 || regrid_every < 1) return fail(AMRLBM_ERR_ARG, "bad step arguments");
+    int ahead_len = 0, ahead_cur = 0;
+    const int rc = step_segments(m, nsteps, regrid_every, &ahead_len, &ahead_cur);
+    if (rc) {
+        // an error return leaves no device decision behind: with the stream
+        // drained, the skip word every step kernel reads is cleared, so later
+        // calls do their work, and a segment launched behind a decision that
+        // said "changed" (it skipped its work) is taken back
+        (void)hipStreamSynchronize(m->stream);
+        if (ahead_len && *m->h_changed) {
+            m->T -= ahead_len;
+            m->cur = ahead_cur;
+        }
+        (void)hipMemsetAsync(m->d_changed, 0, sizeof(int) * 2, m->stream);
+        (void)hipStreamSynchronize(m->stream);
+        (void)hipGetLastError();
+    }
+    return rc;
 }
 
 int amrlbm_profile_steps(amrlbm_mesh *m, int nsteps, int regrid_every, double stats[12])

@@ -106,6 +106,11 @@ int amrlbm_output_instances(amrlbm_mesh *m, int n, double *out);
 /* ---- state exchange with the caller's Blk(:) mirror ----------------------- */
 /* counts: Nblks, nLeaf, LevelMin, LevelMax, KK, free-list length, step T */
 int amrlbm_get_counts(amrlbm_mesh *m, int counts[7]);
+/* the step counter T (module variable, Parameters.f90:6; Main_Loop's DO T,
+ * AMR_Rising_2D_OMP.f90:302): amrlbm_step advances it, the phase entry points
+ * leave it to the caller, as the reference's subroutines leave it to the loop.
+ * The regrid cadence of amrlbm_step and the snapshot header read it. */
+int amrlbm_set_step(amrlbm_mesh *m, int T);
 /* topology of blocks 1..Nblks, 21 ints each: Level, dx, I, J, Parent, IsLeaf,
  * hasLeaf, Refin, Deref, child(1:4), nghbr(1:8)%n (Parameters.f90:26-35);
  * freeq receives the free-list ids head first (Parameters.f90:41-45) */

@@ -92,7 +92,7 @@ def make_params(w):
     p.L0, p.nroots, p.min_levels, p.max_levels = w.L0, w.nroots, w.min_levels, w.max_levels
     p.max_blocks = w.max_blocks
     p.M0, p.W, p.sigma, p.beta, p.kappa = w.M0, w.W, w.sigma, w.beta, w.kappa
-    p.taul, p.tauh, p.rhol, p.rhoh, p.gy = w.taul, w.tauh, w.rhol, w.rhoh, w.gy
+    p.taul, p.tauh, p.rhol, p.rhoh, p.gy = w.taul, w.tauh, w.rhol, w.rhoh, w.gy_run
     return p
 
 

@@ -37,9 +37,10 @@ def mesh_state(mesh):
     return rows, free, c["kk"], arrays
 
 
-def compare_state(mesh, snap, pop_ghosts=True, c_ring2=True, tol=1e-12):
+def compare_state(mesh, snap, pop_ghosts=True, c_ring2=True, tol=1e-12, bit_exact=True):
     """Compare a library mesh with a Snapshot: topology exactly, arrays to a
-    relative tolerance |a-b| <= tol * max(|b|, 1). Returns (max_rel, nbitdiff)."""
+    relative tolerance |a-b| <= tol * max(|b|, 1) and, with bit_exact (the FP64
+    path's claim), every value bit for bit. Returns (max_rel, nbitdiff)."""
     rows, free, kk, arrays = mesh_state(mesh)
     ref_rows = snapshot_rows(snap)
     assert rows.shape == ref_rows.shape, f"Nblks {rows.shape[0]} != {ref_rows.shape[0]}"
@@ -60,8 +61,10 @@ def compare_state(mesh, snap, pop_ghosts=True, c_ring2=True, tol=1e-12):
             m = float(d.max())
             assert m <= tol, f"block {n} field {f}: max rel diff {m:.3e}"
             worst = max(worst, m)
-            nbit += int(np.count_nonzero(np.ascontiguousarray(x).view(np.int64) !=
-                                         np.ascontiguousarray(y).view(np.int64)))
+            nb = int(np.count_nonzero(np.ascontiguousarray(x).view(np.int64) !=
+                                      np.ascontiguousarray(y).view(np.int64)))
+            assert not (bit_exact and nb), f"block {n} field {f}: {nb} values differ in some bit"
+            nbit += nb
     return worst, nbit
 
 

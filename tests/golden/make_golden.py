@@ -20,6 +20,12 @@ Sequences
   cfg2   40 steps, regrid every 10 (init + final)
   walk   cfg1 set-up, then 14 moves of the analytic bubble by (3, 5) cells with a
          Refine_Derefine_Mesh after each move
+  rise   runs whose mesh changes while stepping (refinement, derefinement and
+         reuse of freed ids), snapshots after every phase of each step whose
+         regrid changes the mesh:
+           tiny_rise  tiny set-up, gravity 1e-4 in the steps, 2400 steps, regrid every step
+           cfg1_rise  cfg1 set-up, gravity 5e-5 in the steps, 2200 steps, regrid every step
+           cfg2_regrid cfg2 as it is, 1100 steps, regrid every 10
   dropin the reference's complete program (oracle/build_dropin.sh: 16x16 blocks,
          L0=64, three roots, Tf=12800 steps, regrid every step): its Output table
          (dropin_output.txt) and the SHA-256 of its 21 Tecplot files
@@ -64,6 +70,32 @@ def run(cfg, d, args, env=None):
                    env=e, stdout=subprocess.DEVNULL, cwd=d)
 
 
+# (sequence, workload of amrlbm.workloads, reference build, gravity override
+# of the time steps, steps, regrid interval, steps whose regrid changes the
+# mesh); the set-up's parameters are the build's own (the harness reads
+# AMRLBM_GY_OVERRIDE after the set-up)
+RISE = [
+    ("tiny_rise", "tiny_rise", "tiny", "1e-4", 2400, 1, [1077, 1517, 2347]),
+    ("cfg1_rise", "cfg1_rise", "cfg1", "5e-5", 2200, 1, [813, 1740, 1924, 2161]),
+    ("cfg2_regrid", "cfg2", "cfg2", None, 1100, 10, [709, 1079]),
+]
+
+
+def rise_sequences(tmp):
+    out = {}
+    for name, workload, cfg, gy, steps, k, change_steps in RISE:
+        d = os.path.join(tmp, name)
+        os.makedirs(d)
+        env = {"AMRLBM_GY_OVERRIDE": gy} if gy else {}
+        run(cfg, d, [steps, k, 0, 8] + change_steps, env=env)
+        out[name] = {"workload": workload, "steps": steps, "regrid_every": k, "gy_step": float(gy) if gy else None,
+                     "change_steps": change_steps,
+                     "init": sha(os.path.join(d, "init.bin")), "final": sha(os.path.join(d, "final.bin")),
+                     "phases": {f"s{t}_p{p}": sha(os.path.join(d, f"s{t:06d}_p{p}.bin"))
+                                for t in change_steps for p in range(9)}}
+    return out
+
+
 def main():
     for cfg in ("tiny", "cfg1", "cfg2"):
         if not os.path.exists(os.path.join(REF, cfg, "ref_harness")):
@@ -102,6 +134,7 @@ def main():
         run("cfg1", d, [0, 1, 1, 1, -1], env={"AMRLBM_WALK": "14 3.0 5.0"})
         out["walk"] = {"workload": "cfg1", "moves": 14, "step": [3.0, 5.0],
                        "snapshots": [sha(os.path.join(d, f"walk{k:03d}.bin")) for k in range(1, 15)]}
+        out["rise"] = rise_sequences(tmp)
         exe = os.path.join(REF, "dropin", "ref_program")
         if os.path.exists(exe):
             d = os.path.join(tmp, "dropin")

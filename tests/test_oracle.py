@@ -64,6 +64,75 @@ def test_runs_match_reference(oracle_mod, workloads, name):
     assert snap_hash(o) == g["final"]
 
 
+def replay_with_phase_hashes(oracle_mod, w, g, check=True):
+    """Run sequence g of the reference build on the oracle: the set-up, whole
+    steps, and phase by phase at each step whose regrid changes the mesh,
+    comparing the snapshot hashes there. Returns the oracle mesh at the end
+    and the (Nblks, nLeaf, KK) before and after each such regrid."""
+    o = oracle_mod.Oracle(w)
+    o.setup()
+    if check:
+        assert snap_hash(o) == g["init"]
+    k = g["regrid_every"]
+    moves = []
+    for t in range(g["steps"]):
+        if t not in g["change_steps"]:
+            o.step(k)
+            continue
+        o.set_step(t)
+        o.phase("levels")
+        if check:
+            assert snap_hash(o) == g["phases"][f"s{t}_p0"], f"step {t} start"
+        for p, ph in enumerate(PHASES, start=1):
+            o.phase(ph)
+            if check:
+                assert snap_hash(o) == g["phases"][f"s{t}_p{p}"], f"step {t} after {ph}"
+        before = o.snapshot()
+        if (t + 1) % k == 0:
+            o.regrid()
+        after = o.snapshot()
+        if check:
+            assert snap_hash(o) == g["phases"][f"s{t}_p8"], f"step {t} after regrid"
+        moves.append(((before.nblks, before.nleaf, before.kk), (after.nblks, after.nleaf, after.kk),
+                      before.leaves != after.leaves))
+        o.set_step(t + 1)
+    o.phase("levels")
+    if check:
+        assert snap_hash(o) == g["final"]
+    return o, moves
+
+
+@pytest.mark.parametrize("name", ["tiny_rise", "cfg1_rise", "cfg2_regrid"])
+def test_mesh_changing_runs_match_reference(oracle_mod, workloads, name):
+    """Runs whose mesh changes while stepping: every phase of each step whose
+    regrid changes the mesh (refinement, derefinement, reuse of freed ids,
+    prolong and restrict of stepped fields, Err_Interface from the C fill),
+    ghosts included, and the final state, bit for bit."""
+    g = HASHES["rise"][name]
+    w = workloads[g["workload"]]
+    assert (w.gy_step or None) == g["gy_step"]
+    _, moves = replay_with_phase_hashes(oracle_mod, w, g)
+    assert all(changed for _, _, changed in moves), moves
+    print(name, moves)
+
+
+def test_mesh_changing_runs_cover_refine_derefine_and_reuse(oracle_mod, workloads):
+    """The golden mesh-changing runs hold at least one refinement, one
+    derefinement and one refinement that takes freed ids from the FIFO."""
+    kinds = set()
+    for name in ("tiny_rise", "cfg1_rise"):
+        g = HASHES["rise"][name]
+        _, moves = replay_with_phase_hashes(oracle_mod, workloads[g["workload"]], g, check=False)
+        for (nb0, nl0, kk0), (nb1, nl1, kk1), _ in moves:
+            if nl1 > nl0 and kk1 < kk0:
+                kinds.add("reuse")
+            elif nl1 > nl0:
+                kinds.add("refine")
+            elif nl1 < nl0:
+                kinds.add("derefine")
+    assert kinds == {"refine", "derefine", "reuse"}, kinds
+
+
 def test_regrid_walk_matches_reference(oracle_mod, workloads):
     """Refine, derefine and free-list reuse on every call, prolong/restrict data."""
     g = HASHES["walk"]
