@@ -422,21 +422,23 @@ class Mesh:
                             "programs": round(out[5], 3), "graphs": round(out[6], 3)},
                 "graphs_built": int(out[7])}
 
-    def set_tuning(self, order, depth):
-        """Launch variant (amrlbm_set_tuning): leaf order 0-3, hg operand depth 0-2."""
-        _check(lib().amrlbm_set_tuning(self.h, _i(np.array([order, depth], dtype=np.int32))))
+    def set_tuning(self, order, depth, execution=0):
+        """Launch variant (amrlbm_set_tuning): leaf order 0-3, hg operand depth
+        0-2, execution 0 (step graph) or 1 (persistent segment kernel)."""
+        _check(lib().amrlbm_set_tuning(self.h, _i(np.array([order, depth, execution], dtype=np.int32))))
 
     def get_tuning(self):
-        k = np.zeros(2, dtype=np.int32)
+        k = np.zeros(3, dtype=np.int32)
         _check(lib().amrlbm_get_tuning(self.h, _i(k)))
-        return int(k[0]), int(k[1])
+        return int(k[0]), int(k[1]), int(k[2])
 
     def autotune(self, reps=10, regrid_every=10):
-        """Time the step graph of the 12 launch variants on the current state
-        and keep the fastest (amrlbm_autotune): [(order, depth, us per step)]."""
-        t = np.zeros(36)
+        """Time the 24 launch variants on the current state and keep the
+        fastest (amrlbm_autotune): [(order, depth, execution, us per step)],
+        us -1 where the segment kernel does not fit."""
+        t = np.zeros(96)
         _check(lib().amrlbm_autotune(self.h, int(reps), int(regrid_every), _d(t)))
-        return [(int(t[3 * v]), int(t[3 * v + 1]), float(t[3 * v + 2])) for v in range(12)]
+        return [(int(t[4 * v]), int(t[4 * v + 1]), int(t[4 * v + 2]), float(t[4 * v + 3])) for v in range(24)]
 
     def output(self, allow_nonfinite=False):
         """rho_min, rho_max, max|Uy|, mass, bubble area, centroid x, y, rise velocity, non-finite."""

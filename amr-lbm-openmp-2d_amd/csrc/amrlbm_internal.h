@@ -207,4 +207,17 @@ struct ScatterSeg {
 };
 void launch_scatter_segments(const void *src, const ScatterSeg *desc, int n, hipStream_t s);
 
+// The persistent segment kernel: len whole steps of every leaf in list (one
+// workgroup each; the caller checks with segment_capacity that all fit on the
+// GPU at once) in one launch, with the same results as the two-kernel step
+// graph of the segment. sync: the grid barrier's words (kSegSyncWords ints,
+// zeroed by the launch except word 15, the fault word, set when a barrier
+// wait timed out). Returns the launch's status.
+constexpr int kSegSyncWords = 16 + 16 * 8;
+hipError_t launch_segment(const DevMesh &m, int cur, const int *list, int n, const Prog &c2, const Prog &hg,
+                          const Prog &c1, const Phys &ph, int len, double *err_slot, unsigned *sync, hipStream_t s);
+// dynamic LDS of one segment workgroup, and how many the GPU holds at once
+size_t segment_lds_bytes(const DevMesh &m, const Prog &hg);
+int segment_capacity(const DevMesh &m, const Prog &hg);
+
 }  // namespace amrlbm

@@ -491,15 +491,33 @@ __device__ __forceinline__ PNode load_node(const PNode *p)
     return n;
 }
 
+// A load of a value another workgroup wrote in the same launch (COH: the
+// persistent segment kernel), as an agent-scope atomic load: it reads the
+// device-coherent value past a stale copy in this XCD's L2 without an L2
+// invalidation. Otherwise (values of an earlier kernel) a plain load.
+template <bool COH, class T>
+__device__ __forceinline__ T ldg(const T *p)
+{
+    if (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return *p;
+}
+// a store other workgroups read in the same launch: agent-scope atomic,
+// written through this XCD's L2, so that no L2 write-back is needed
+template <class T>
+__device__ __forceinline__ void stg_coh(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // the recipe's operands that live in global memory, fetched before any stage
 // barrier (they are interior values left by the previous kernel)
-template <class T>
+template <bool COH = false, class T>
 __device__ __forceinline__ void prefetch_node(const PNode &nd, const T *__restrict__ G, int gs, T gv[9])
 {
     const int type = nd.info & 15;
     const int nref = type == NT_COPY ? 1 : (type >= NT_BB_BOT ? 0 : 9);
 #pragma unroll
-    for (int r = 0; r < 9; r++) gv[r] = (r < nref && nd.ref[r] >= 0) ? G[nd.ref[r] + gs] : T(0);
+    for (int r = 0; r < 9; r++) gv[r] = (r < nref && nd.ref[r] >= 0) ? ldg<COH>(G + nd.ref[r] + gs) : T(0);
 }
 
 // one recipe from prefetched global operands gv and LDS L (+ shift ls)
@@ -522,11 +540,11 @@ __device__ __forceinline__ T eval_pref(const PNode &nd, const T gv[9], const T *
 
 // one recipe: copy, or the tensor product (pass along y, then x) divided by
 // 64 (from finer) or 1024 (from coarser)
-template <class T>
+template <bool COH = false, class T>
 __device__ __forceinline__ T eval_node(const PNode &nd, const T *__restrict__ G, int gs, const T *L, int ls)
 {
     const int type = nd.info & 15;
-#define PV(r) ((r) >= 0 ? G[(r) + gs] : L[-1 - (r) + ls])
+#define PV(r) ((r) >= 0 ? ldg<COH>(G + (r) + gs) : L[-1 - (r) + ls])
     if (type == NT_COPY) return PV(nd.ref[0]);
     T wx[3], wy[3];
     wpat((nd.info >> 4) & 3, wx);
@@ -540,18 +558,19 @@ __device__ __forceinline__ T eval_node(const PNode &nd, const T *__restrict__ G,
 }
 
 // a first-stage recipe: every operand is an interior value in memory
-template <class T>
+template <bool COH = false, class T>
 __device__ __forceinline__ T eval_global(const PNode &nd, const T *__restrict__ G, int gs)
 {
     const int type = nd.info & 15;
-    if (type == NT_COPY) return G[nd.ref[0] + gs];
+    if (type == NT_COPY) return ldg<COH>(G + nd.ref[0] + gs);
     T wx[3], wy[3];
     wpat((nd.info >> 4) & 3, wx);
     wpat((nd.info >> 6) & 3, wy);
     T t[3];
 #pragma unroll
     for (int a = 0; a < 3; a++)
-        t[a] = wy[0] * G[nd.ref[3 * a] + gs] + wy[1] * G[nd.ref[3 * a + 1] + gs] + wy[2] * G[nd.ref[3 * a + 2] + gs];
+        t[a] = wy[0] * ldg<COH>(G + nd.ref[3 * a] + gs) + wy[1] * ldg<COH>(G + nd.ref[3 * a + 1] + gs) +
+               wy[2] * ldg<COH>(G + nd.ref[3 * a + 2] + gs);
     T o = wx[0] * t[0] + wx[1] * t[1] + wx[2] * t[2];
     return type == NT_FINE ? o / T(64) : o / T(1024);
 }
@@ -560,16 +579,17 @@ __device__ __forceinline__ T eval_global(const PNode &nd, const T *__restrict__ 
 // lcf cells apart.
 // Bounce-back rows take the opposite population of the first row
 // (Bounce_Back_3/4, BC_Cell.f90:2-42).
-template <bool GLOBAL_ONLY, class T>
+template <bool GLOBAL_ONLY, bool COH = false, class T>
 __device__ __forceinline__ T eval_hg(const PNode &nd, int set, int q, const T *__restrict__ Fc, const T *sF, int lcf)
 {
     const int type = nd.info & 15;
     const int c = set * 9 + q;
-    if (type < NT_BB_BOT) return GLOBAL_ONLY ? eval_global(nd, Fc, c * GS) : eval_node(nd, Fc, c * GS, sF, q * lcf);
+    if (type < NT_BB_BOT)
+        return GLOBAL_ONLY ? eval_global<COH>(nd, Fc, c * GS) : eval_node<COH>(nd, Fc, c * GS, sF, q * lcf);
     const int e = c_ex[q] * c_ey[q];
     const int r = nd.ref[1 + (type == NT_BB_TOP ? -e : e)];
     const int qo = c_opp[q];
-    return (GLOBAL_ONLY || r >= 0) ? Fc[r + (set * 9 + qo) * GS] : sF[-1 - r + qo * lcf];
+    return (GLOBAL_ONLY || r >= 0) ? ldg<COH>(Fc + r + (set * 9 + qo) * GS) : sF[-1 - r + qo * lcf];
 }
 
 // One stage of an hg ghost program for one population set: recipes [b, e)
@@ -1265,6 +1285,328 @@ __global__ __launch_bounds__(256) void k_scatter_segments(const int *__restrict_
 }
 
 // ---------------------------------------------------------------------------
+// The persistent segment kernel: every step of a regrid segment in one
+// launch. Workgroup k owns leaf list[k] for the whole segment and keeps its
+// populations in registers from one step to the next; only what other blocks
+// read goes through memory: the post-collision populations (for their hg
+// fills) and C (for their C2 fills). Per step:
+//   C2 fill (ghost program) -> [U, P] + collision -> post-collision h, g to the
+//   exchange buffer -> grid barrier -> hg fill (ghost program, both sets) ->
+//   stream in LDS, C = sum h -> C to memory -> grid barrier
+// and at the end of the segment the C fill, U, P and Err_Interface, as the
+// two-kernel step graph's last launch, with the streamed populations stored
+// where that graph leaves them. The arithmetic is the same cell functions on
+// the same operands in the same order, so every value is bit for bit the
+// graph's. Needs every workgroup resident at once (a cooperative launch,
+// which the runtime refuses rather than under-schedules).
+// ---------------------------------------------------------------------------
+
+// Grid barrier with a bounded wait, two levels deep so that no memory word
+// takes more than one atomic per workgroup group: workgroup k arrives at the
+// counter of group k % 8 (its XCD under round-robin dispatch); the last of a
+// group arrives at the top counter; the last of those publishes the barrier's
+// ordinal to the 8 group flags, which the workgroups poll. sync[0..7] group
+// counters, sync[8] top counter, sync[16 + 16 g] group g's flag (each on its
+// own 64-byte line), sync[15] the fault word. A wait longer than
+// kBarrierSpins polls, or a fault word set by another workgroup, sets the
+// fault word and returns false: the caller exits, and so do the other
+// workgroups at their next barrier.
+// Visibility: every value one workgroup writes for another in the segment
+// kernel is written with an agent-scope atomic store (through the XCD's L2 to
+// the device-coherent level) and read with an agent-scope atomic load, and
+// each wave waits for its stores to be acknowledged (s_waitcnt) before the
+// workgroup arrives. So no L2 write-back or invalidation is needed: an
+// agent-scope release / acquire per workgroup, which writes back and
+// invalidates the XCD's whole L2, measured 70 us per barrier at 300
+// workgroups.
+constexpr unsigned kBarrierSpins = 1u << 22;
+
+__device__ __forceinline__ bool grid_barrier(unsigned *sync, unsigned epoch)
+{
+    __shared__ int s_ok;
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's stores acknowledged
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned nwg = gridDim.x, grp = blockIdx.x & 7u;
+        const unsigned gsize = (nwg - grp + 7u) >> 3, ngrp = nwg < 8u ? nwg : 8u;
+        int *fault = reinterpret_cast<int *>(sync + 15);
+        unsigned *flag = sync + 16 + 16 * grp;
+        const unsigned g_old = __hip_atomic_fetch_add(sync + grp, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (g_old + 1u == gsize * epoch) {
+            const unsigned t_old = __hip_atomic_fetch_add(sync + 8, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (t_old + 1u == ngrp * epoch)
+                for (unsigned f = 0; f < 8u; f++)
+                    __hip_atomic_store(sync + 16 + 16 * f, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        int ok = 1;
+        for (unsigned spins = 0; __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch; spins++) {
+            if ((spins & 255u) == 255u && __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = 0;
+                break;
+            }
+            if (spins >= kBarrierSpins) {
+                __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// One stage of the hg ghost program for both population sets (recipes [b, e)
+// in sN, items (set, population, recipe) with the recipe fastest); sF holds
+// the 18 population arrays, lcf cells apart; X is the exchange buffer.
+template <bool GLOBAL_ONLY, int U, class T>
+__device__ __forceinline__ void hg_stage_both(const PNode *sN, int b, int e, const T *__restrict__ X, T *sF, int lcf,
+                                              int tid)
+{
+    const int ns = e - b, nitem = 18 * ns;
+    for (int base = 0; base < nitem; base += 256 * U) {
+        T v[U];
+#pragma unroll
+        for (int r = 0; r < U; r++) {
+            const int t = base + tid + 256 * r;
+            if (t < nitem) {
+                const int c = t / ns, set = c >= 9 ? 1 : 0;
+                v[r] = eval_hg<GLOBAL_ONLY, true>(sN[b + t - c * ns], set, c - 9 * set, X, sF + 9 * set * lcf, lcf);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < U; r++) {
+            const int t = base + tid + 256 * r;
+            if (t < nitem) {
+                const int c = t / ns;
+                sF[c * lcf + sN[b + t - c * ns].dest] = v[r];
+            }
+        }
+    }
+}
+
+// Perfect_Shift_AMR / Streaming_LaxWendroff_AMR (LBM_Subroutines.f90:41-90) of
+// one cell of one set from its LDS copy into registers (stream_cell's
+// arithmetic)
+template <class T>
+__device__ __forceinline__ void stream_cell_regs(const T *sF, int lcf, int i, int j, int dx, T f[9])
+{
+    const T cfl = T(1) / (T)dx;
+    const T a = cfl / T(2) * (T(1) + cfl), b = T(1) - cfl * cfl, c = cfl / T(2) * (T(1) - cfl);
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        if (q == 0)
+            f[q] = sF[lidx(q, i, j, lcf)];
+        else if (cfl > T(0.9))
+            f[q] = sF[lidx(q, i - c_ex[q], j - c_ey[q], lcf)];
+        else
+            f[q] = a * sF[lidx(q, i - c_ex[q], j - c_ey[q], lcf)] + b * sF[lidx(q, i, j, lcf)] -
+                   c * sF[lidx(q, i + c_ex[q], j + c_ey[q], lcf)];
+    }
+}
+
+// A C fill (Fill_Ghost_Cells_C / _C2 by ghost program, as k_prog_c) of the
+// block into sC, its own ghost values also into memory. nd0: this thread's
+// first recipe (kept from one step to the next), cin: the block's C at this
+// thread's cell.
+template <class T>
+__device__ __forceinline__ void fill_c_block(const PHdr &hd, const PNode &nd0, const PNode *__restrict__ nodes,
+                                             const T *__restrict__ Cg, T *C, T *sC, T cin, int i, int j, int tid)
+{
+    const int total = hd.nst > 0 ? stage_end(hd, hd.nst - 1) : 0;
+    const int n1 = hd.nst > 0 ? hd.end[0] : 0;
+    T gv[9];
+    if (tid < total) prefetch_node<true>(nd0, Cg, 0, gv);
+    if (tid < n1) {
+        const T v = eval_pref(nd0, gv, (const T *)sC, 0);
+        sC[nd0.dest] = v;
+        if (nd0.dest < CS) C[nd0.dest] = v;
+    }
+    sC[cidx(i, j)] = cin;
+    __syncthreads();
+    int begin = 0;
+    for (int s = 0; s < hd.nst; s++) {
+        const int end = stage_end(hd, s);
+        if (s > 0 && tid >= begin && tid < end) {
+            const T v = eval_pref(nd0, gv, (const T *)sC, 0);
+            sC[nd0.dest] = v;
+            if (nd0.dest < CS) C[nd0.dest] = v;
+        }
+        for (int t = tid + 256; t < total; t += 256) {
+            if (t < begin || t >= end) continue;
+            const PNode nd = load_node(nodes + hd.off + t);
+            const T v = eval_node<true>(nd, Cg, 0, (const T *)sC, 0);
+            sC[nd.dest] = v;
+            if (nd.dest < CS) C[nd.dest] = v;
+        }
+        begin = end;
+        __syncthreads();
+    }
+}
+
+// the grid barrier's counters and flags to zero before a segment kernel (its
+// fault word, sync[15], stays until the host has read it)
+__global__ __launch_bounds__(64) void k_sync_reset(unsigned *sync)
+{
+    for (int t = threadIdx.x; t < kSegSyncWords; t += 64)
+        if (t != 15) sync[t] = 0u;
+}
+
+// byte offset of the staged hg recipes in the segment kernel's LDS, after 18
+// population arrays of lcf cells
+__host__ __device__ constexpr size_t seg_nodes_offset(int lcf, size_t esz) { return (18 * (size_t)lcf * esz + 15) & ~size_t(15); }
+
+template <class T, int U0, int U1, bool ENS>
+__global__ __launch_bounds__(256, 2) void k_segment(DevMeshT<T> m, int cur, const int *__restrict__ list, Prog pc2,
+                                                 Prog phg, Prog pc1, Phys ph64, PhysF ph32, int len,
+                                                 double *err_slot, unsigned *sync)
+{
+    const auto &ph_arg = PhysOf<T>::get(ph64, ph32);
+    const int k = blockIdx.x, tid0 = threadIdx.x, tid = tid0;
+    const int skip = *m.skip;
+    const int slot = list[k];
+    const PHdr h2 = pc2.hdr[k], hh = phg.hdr[k], h1 = pc1.hdr[k];
+    if (skip) return;   // uniform: every workgroup reads the same word before any barrier
+    const BlockDev me = m.blk[slot];
+    const auto &ph = ENS ? PhysOf<T>::table(m)[me.inst] : ph_arg;
+    const int i0 = tid % NX + 1, j0 = tid / NX + 1, i = i0, j = j0;
+    const T *__restrict__ Cg = m.C;
+    T *C = m.C + (size_t)slot * CS;
+    T *PU = m.PU + (size_t)slot * PUB;
+    const T *F0 = m.pop[cur] + (size_t)slot * POPB;
+    const T *__restrict__ X = m.pop[cur ^ 1];   // exchange buffer: post-collision populations
+    T *Xb = m.pop[cur ^ 1] + (size_t)slot * POPB;
+
+    extern __shared__ __align__(16) unsigned char seg_lds[];
+    const int lcf = FS + phg.naux;
+    T *sF = reinterpret_cast<T *>(seg_lds);
+    PNode *sN = reinterpret_cast<PNode *>(seg_lds + seg_nodes_offset(lcf, sizeof(T)));
+    __shared__ T sC[LC_C];
+
+    // the block's hg recipes, staged once for the whole segment
+    const int nn = hh.nst > 0 ? stage_end(hh, hh.nst - 1) : 0;
+    {
+        const int4 *src = reinterpret_cast<const int4 *>(phg.nodes + hh.off);
+        int4 *dst = reinterpret_cast<int4 *>(sN);
+        for (int t = tid; t < 3 * nn; t += 256) dst[t] = src[t];
+    }
+    // the block's populations between steps: the own cells of sF (the
+    // stream's result), so that only the collision holds them in registers
+    const int n2 = h2.nst > 0 ? stage_end(h2, h2.nst - 1) : 0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        sF[lidx(q, i, j, lcf)] = F0[fidx(0, q, i, j)];
+        sF[lidx(9 + q, i, j, lcf)] = F0[fidx(1, q, i, j)];
+    }
+    T csum = C[cidx(i, j)];
+    unsigned epoch = 0;
+    for (int s = 0; s < len; s++) {
+        // the cell, its size and the thread taken afresh each step (an empty
+        // asm the compiler cannot see through), so that the stencils'
+        // addresses and the constants derived from them are not hoisted out
+        // of the loop and held in registers for the whole segment
+        int i = i0, j = j0, dx = me.dx, tid = tid0;
+        asm volatile("" : "+v"(i), "+v"(j), "+v"(dx), "+v"(tid));
+        // Fill_Ghost_Cells_C2, then U, P of the previous step (none before the
+        // first: U, P are current in memory) and Collision_h_g_AMR
+        PNode nd2;
+        if (tid < n2) nd2 = load_node(pc2.nodes + h2.off + tid);
+        fill_c_block(h2, nd2, pc2.nodes, Cg, C, sC, csum, i, j, tid);
+        {
+            T h[9], g[9], P, Ux, Uy;
+#pragma unroll
+            for (int q = 0; q < 9; q++) {
+                h[q] = sF[lidx(q, i, j, lcf)];
+                g[q] = sF[lidx(9 + q, i, j, lcf)];
+            }
+            if (s > 0) {
+                macro_g_cell((const T *)sC, i, j, dx, g, ph, P, Ux, Uy);
+            } else {
+                P = PU[pidx(i, j)];
+                Ux = PU[PS + pidx(i, j)];
+                Uy = PU[2 * PS + pidx(i, j)];
+            }
+            collide_cell((const T *)sC, i, j, dx, P, Ux, Uy, ph, h, g);
+#pragma unroll
+            for (int q = 0; q < 9; q++) {
+                stg_coh(Xb + fidx(0, q, i, j), h[q]);
+                stg_coh(Xb + fidx(1, q, i, j), g[q]);
+                sF[lidx(q, i, j, lcf)] = h[q];
+                sF[lidx(9 + q, i, j, lcf)] = g[q];
+            }
+        }
+        if (!grid_barrier(sync, ++epoch)) return;
+        // Fill_Ghost_Cells_hg of both sets from the neighbours' post-collision
+        // values, then the stream and C = sum h (Macroscopic_Values_h_AMR)
+        int begin = 0;
+        for (int st = 0; st < hh.nst; st++) {
+            const int end = stage_end(hh, st);
+            if (st == 0)
+                hg_stage_both<true, U0>(sN, begin, end, X, sF, lcf, tid);
+            else
+                hg_stage_both<false, U1>(sN, begin, end, X, sF, lcf, tid);
+            begin = end;
+            __syncthreads();
+        }
+        {
+            T f[9];
+            stream_cell_regs((const T *)sF, lcf, i, j, dx, f);
+            csum = T(0);
+#pragma unroll
+            for (int q = 0; q < 9; q++) csum += f[q];
+            T f2[9];
+            stream_cell_regs((const T *)(sF + 9 * lcf), lcf, i, j, dx, f2);
+            __syncthreads();   // every cell's stream has read its neighbours
+#pragma unroll
+            for (int q = 0; q < 9; q++) {
+                sF[lidx(q, i, j, lcf)] = f[q];
+                sF[lidx(9 + q, i, j, lcf)] = f2[q];
+            }
+        }
+        stg_coh(C + cidx(i, j), csum);
+        if (!grid_barrier(sync, ++epoch)) return;
+    }
+    // Fill_Ghost_Cells_C, Macroscopic_Values_g_AMR and Err_Interface
+    // (Refine_Derefine.f90:327-352) for a regrid decision after the segment
+    PNode nd1;
+    if (tid < (h1.nst > 0 ? stage_end(h1, h1.nst - 1) : 0)) nd1 = load_node(pc1.nodes + h1.off + tid);
+    fill_c_block(h1, nd1, pc1.nodes, Cg, C, sC, csum, i, j, tid);
+    {
+        const T a = sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)];
+        const T b = sC[cidx(i, j + 1)] - sC[cidx(i, j - 1)];
+        T dc = a * a + b * b;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const T x = __shfl_xor(dc, o);
+            if (x > dc) dc = x;
+        }
+        __shared__ T wmax[4];
+        if ((tid & 63) == 0) wmax[tid >> 6] = dc;
+        __syncthreads();
+        if (tid == 0) {
+            T e = wmax[0];
+            for (int w = 1; w < 4; w++)
+                if (wmax[w] > e) e = wmax[w];
+            err_slot[slot] = (double)(sqrt(e) / (T(2) * (T)me.dx));
+        }
+    }
+    T g[9], P, Ux, Uy;
+#pragma unroll
+    for (int q = 0; q < 9; q++) g[q] = sF[lidx(9 + q, i, j, lcf)];
+    macro_g_cell((const T *)sC, i, j, me.dx, g, ph, P, Ux, Uy);
+    PU[pidx(i, j)] = P;
+    PU[PS + pidx(i, j)] = Ux;
+    PU[2 * PS + pidx(i, j)] = Uy;
+    T *Fend = m.pop[cur ^ (len & 1)] + (size_t)slot * POPB;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        Fend[fidx(0, q, i, j)] = sF[lidx(q, i, j, lcf)];
+        Fend[fidx(1, q, i, j)] = g[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <class T, bool ENS>
@@ -1403,6 +1745,72 @@ void launch_scatter_segments(const void *src, const ScatterSeg *desc, int n, hip
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_scatter_segments, dim3(n), dim3(256), 0, s, static_cast<const int *>(src), desc);
+}
+
+// ---- persistent segment kernel -------------------------------------------
+size_t segment_lds_bytes(const DevMesh &m, const Prog &hg)
+{
+    const size_t esz = m.f32 ? sizeof(float) : sizeof(double);
+    return seg_nodes_offset(FS + hg.naux, esz) + (size_t)hg.nmax * sizeof(PNode);
+}
+
+template <class T, bool ENS>
+static const void *segment_kernel(int depth)
+{
+    if (depth == 0) return reinterpret_cast<const void *>(&k_segment<T, 2, 1, ENS>);
+    if (depth == 2) return reinterpret_cast<const void *>(&k_segment<T, 8, 4, ENS>);
+    return reinterpret_cast<const void *>(&k_segment<T, 4, 2, ENS>);
+}
+
+static const void *segment_kernel_of(const DevMesh &m)
+{
+    const bool ens = m.phs != nullptr;
+    if (m.f32) return ens ? segment_kernel<float, true>(m.hg_depth) : segment_kernel<float, false>(m.hg_depth);
+    return ens ? segment_kernel<double, true>(m.hg_depth) : segment_kernel<double, false>(m.hg_depth);
+}
+
+int segment_capacity(const DevMesh &m, const Prog &hg)
+{
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    const size_t lds = segment_lds_bytes(m, hg);
+    // dynamic LDS beyond the default 64 KB per workgroup is asked for explicitly
+    if (hipFuncSetAttribute(segment_kernel_of(m), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, segment_kernel_of(m), 256, lds) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return per_cu * ncu;
+}
+
+hipError_t launch_segment(const DevMesh &dm, int cur, const int *list, int n, const Prog &c2, const Prog &hg,
+                          const Prog &c1, const Phys &ph, int len, double *err_slot, unsigned *sync, hipStream_t s)
+{
+    const size_t lds = segment_lds_bytes(dm, hg);
+    const PhysF phf = dm.phf;
+    void *args[11];
+    DevMeshT<double> m64 = typed<double>(dm);
+    DevMeshT<float> m32 = typed<float>(dm);
+    args[0] = dm.f32 ? static_cast<void *>(&m32) : static_cast<void *>(&m64);
+    args[1] = &cur;
+    args[2] = &list;
+    args[3] = const_cast<Prog *>(&c2);
+    args[4] = const_cast<Prog *>(&hg);
+    args[5] = const_cast<Prog *>(&c1);
+    args[6] = const_cast<Phys *>(&ph);
+    args[7] = const_cast<PhysF *>(&phf);
+    args[8] = &len;
+    args[9] = &err_slot;
+    args[10] = &sync;
+    // a plain launch: the host has checked that n workgroups fit on the GPU
+    // at once, and each barrier's wait is bounded (a cooperative launch, which
+    // guarantees residency, cost 100-140 us per segment here)
+    hipLaunchKernelGGL(k_sync_reset, dim3(1), dim3(64), 0, s, sync);
+    return hipLaunchKernel(segment_kernel_of(dm), dim3(n), dim3(256), args, lds, s);
 }
 
 }  // namespace amrlbm

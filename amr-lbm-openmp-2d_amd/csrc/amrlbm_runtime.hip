@@ -115,7 +115,8 @@ struct amrlbm_mesh {
     DevMesh dm{};
     BlockDev *d_blk = nullptr;
     NbDesc *d_nbd = nullptr;
-    int *d_err_flags = nullptr;
+    int *d_err_flags = nullptr;      // [0] a fill met a level jump
+    unsigned *d_sync = nullptr;      // the segment kernel's grid barrier words (fault word at 15)
     int *d_lists = nullptr;          // leaves grouped by level, then all leaves (ascending)
     int lists_cap = 0;
     int *d_pairs = nullptr;          // restrict / prolong work lists
@@ -191,6 +192,10 @@ struct amrlbm_mesh {
     // and bubble; ens is 0 in any other mesh
     int ens = 0;
     std::vector<amrlbm_instance> inst;
+    int exec = 0;    // how a segment runs (amrlbm_set_tuning): 0 a HIP graph of two kernels per step, 1 the
+                     // persistent segment kernel (when every leaf's workgroup fits on the GPU at once)
+    size_t seg_lds = 0;   // the segment kernel's LDS per workgroup at the last capacity query, and
+    int seg_cap = 0;      // how many such workgroups the GPU holds at once
     int order = 0;   // leaf launch order (amrlbm_set_tuning): 0 Morton in 8 XCD runs, 1 Morton, 2 ascending id,
                      // 3 as 0 with each run's heaviest ghost programs first
     Phys *d_phs = nullptr;           // [ens] device tables the kernels index by BlockDev::inst
@@ -1030,10 +1035,16 @@ int nleaves(const amrlbm_mesh *m) { return (int)m->leaf.size(); }
 
 int check_err_flags(amrlbm_mesh *m)
 {
-    int f = 0;
-    HIPCHK(hipMemcpyAsync(&f, m->d_err_flags, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    int f[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&f[0], m->d_err_flags, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipMemcpyAsync(&f[1], m->d_sync + 15, sizeof(int), hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
-    if (f) {
+    if (f[1]) {
+        HIPCHK(hipMemsetAsync(m->d_sync, 0, sizeof(unsigned) * kSegSyncWords, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+        return fail(AMRLBM_ERR_HIP, "a grid barrier of the segment kernel timed out (state undefined)");
+    }
+    if (f[0]) {
         HIPCHK(hipMemsetAsync(m->d_err_flags, 0, sizeof(int), m->stream));
         return fail(AMRLBM_ERR_TOPOLOGY, "ghost fill met a neighbour more than one level apart");
     }
@@ -1322,6 +1333,8 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipMalloc(&m->d_nbd, sizeof(NbDesc) * 8 * S));
     HIPCHK(hipMalloc(&m->d_err_flags, sizeof(int)));
     HIPCHK(hipMemsetAsync(m->d_err_flags, 0, sizeof(int), m->stream));
+    HIPCHK(hipMalloc(&m->d_sync, sizeof(unsigned) * kSegSyncWords));
+    HIPCHK(hipMemsetAsync(m->d_sync, 0, sizeof(unsigned) * kSegSyncWords, m->stream));
     HIPCHK(hipMalloc(&m->d_err, sizeof(double) * S));
     HIPCHK(hipMalloc(&m->d_err_slot, sizeof(double) * S));
     HIPCHK(hipMalloc(&m->d_changed, sizeof(int) * 2));
@@ -1471,6 +1484,45 @@ int segment_graph(amrlbm_mesh *m, int len, hipGraphExec_t *out)
     return AMRLBM_OK;
 }
 
+// Whether segments run as the persistent segment kernel: chosen (exec 1) and
+// every leaf's workgroup fits on the GPU at once with the LDS the current
+// ghost programs need.
+bool use_segment_kernel(amrlbm_mesh *m)
+{
+    if (m->exec != 1) return false;
+    const size_t lds = segment_lds_bytes(m->dm, m->prog[KIND_HG]);
+    if (lds != m->seg_lds) {
+        m->seg_lds = lds;
+        m->seg_cap = segment_capacity(m->dm, m->prog[KIND_HG]);
+    }
+    return nleaves(m) <= m->seg_cap;
+}
+
+// One segment of len steps as one launch of the persistent segment kernel
+// (the same state afterwards as the segment's step graph).
+int launch_segment_kernel(amrlbm_mesh *m, int len)
+{
+    hipError_t e = launch_segment(m->dm, m->cur, all_leaves(m), nleaves(m), m->prog[KIND_C2], m->prog[KIND_HG],
+                                  m->prog[KIND_C1], m->ph, len, m->d_err_slot, m->d_sync, m->stream);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(AMRLBM_ERR_HIP, std::string("segment kernel launch: ") + hipGetErrorString(e));
+    }
+    return AMRLBM_OK;
+}
+
+// The segment of len steps from the current state, as the tuning says: the
+// persistent segment kernel, or the segment's HIP graph.
+int launch_steps(amrlbm_mesh *m, int len)
+{
+    if (use_segment_kernel(m)) return launch_segment_kernel(m, len);
+    hipGraphExec_t ex;
+    int rc = segment_graph(m, len, &ex);
+    if (rc) return rc;
+    HIPCHK(hipGraphLaunch(ex, m->stream));
+    return AMRLBM_OK;
+}
+
 // Enqueue the device regrid decision at the current step, behind a segment
 // whose last kernel left Err_Interface of every leaf in d_err_slot:
 // k_regrid_check writes 1 to d_changed iff the host regrid would change the
@@ -1602,6 +1654,7 @@ int amrlbm_destroy(amrlbm_mesh *m)
     (void)hipFree(m->d_up);
     if (m->h_pairs) (void)hipHostFree(m->h_pairs);
     (void)hipFree(m->d_err_flags);
+    (void)hipFree(m->d_sync);
     (void)hipFree(m->d_lists);
     (void)hipFree(m->d_pairs);
     (void)hipFree(m->d_err);
@@ -2521,9 +2574,7 @@ static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahea
                 determine_min_max_levels(m);
                 if ((rc = upload_tables(m))) return rc;
             }
-            hipGraphExec_t ex;
-            if ((rc = segment_graph(m, len, &ex))) return rc;
-            HIPCHK(hipGraphLaunch(ex, m->stream));
+            if ((rc = launch_steps(m, len))) return rc;
             m->cur ^= (len & 1);
             m->T += len;
             done += len;
@@ -2705,12 +2756,15 @@ int amrlbm_time_kernels(amrlbm_mesh *m, int reps, double out[3])
     return check_err_flags(m);
 }
 
-int amrlbm_set_tuning(amrlbm_mesh *m, const int knobs[2])
+int amrlbm_set_tuning(amrlbm_mesh *m, const int knobs[3])
 {
     if (!m || !knobs) return fail(AMRLBM_ERR_ARG, "null argument");
-    if (knobs[0] < 0 || knobs[0] > 3 || knobs[1] < 0 || knobs[1] > 2) return fail(AMRLBM_ERR_ARG, "no such variant");
-    if (knobs[0] == m->order && knobs[1] == m->dm.hg_depth) return AMRLBM_OK;
+    if (knobs[0] < 0 || knobs[0] > 3 || knobs[1] < 0 || knobs[1] > 2 || knobs[2] < 0 || knobs[2] > 1)
+        return fail(AMRLBM_ERR_ARG, "no such variant");
+    if (knobs[0] == m->order && knobs[1] == m->dm.hg_depth && knobs[2] == m->exec) return AMRLBM_OK;
     HIPCHK(hipStreamSynchronize(m->stream));
+    m->exec = knobs[2];
+    m->seg_lds = 0;   // capacity queried again (another kernel instantiation)
     m->order = knobs[0];
     m->dm.hg_depth = knobs[1];
     drop_graphs(m);   // captured with the previous order and kernels
@@ -2719,11 +2773,12 @@ int amrlbm_set_tuning(amrlbm_mesh *m, const int knobs[2])
     return m->ready ? upload_tables(m) : AMRLBM_OK;
 }
 
-int amrlbm_get_tuning(amrlbm_mesh *m, int knobs[2])
+int amrlbm_get_tuning(amrlbm_mesh *m, int knobs[3])
 {
     if (!m || !knobs) return fail(AMRLBM_ERR_ARG, "null argument");
     knobs[0] = m->order;
     knobs[1] = m->dm.hg_depth;
+    knobs[2] = m->exec;
     return AMRLBM_OK;
 }
 
@@ -2747,11 +2802,10 @@ static int time_segment(amrlbm_mesh *m, int len, int reps, double *us_per_step)
     HIPCHK(hipMemcpyAsync(save + npop, m->dm.C, nc, hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(hipMemcpyAsync(save + npop + nc, m->dm.PU, npu, hipMemcpyDeviceToDevice, m->stream));
     HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int), m->stream));   // no skip word between regrids
-    hipGraphExec_t ex;
-    if ((rc = segment_graph(m, len, &ex))) return rc;
-    HIPCHK(hipGraphLaunch(ex, m->stream));   // warm
+    if ((rc = launch_steps(m, len))) return rc;   // warm
     HIPCHK(hipEventRecord(m->ev0, m->stream));
-    for (int r = 0; r < reps; r++) HIPCHK(hipGraphLaunch(ex, m->stream));
+    for (int r = 0; r < reps && !rc; r++) rc = launch_steps(m, len);
+    if (rc) return rc;
     HIPCHK(hipEventRecord(m->ev1, m->stream));
     HIPCHK(hipEventSynchronize(m->ev1));
     float ms = 0.f;
@@ -2766,32 +2820,36 @@ static int time_segment(amrlbm_mesh *m, int len, int reps, double *us_per_step)
 
 extern "C" {
 
-int amrlbm_autotune(amrlbm_mesh *m, int reps, int regrid_every, double table[36])
+int amrlbm_autotune(amrlbm_mesh *m, int reps, int regrid_every, double table[96])
 {
     NEED_READY(m);
     if (reps < 1 || regrid_every < 1 || !table) return fail(AMRLBM_ERR_ARG, "bad autotune arguments");
     const int len = regrid_every % 2 ? 2 * regrid_every : regrid_every;
-    int best[2] = {m->order, m->dm.hg_depth};
+    int best[3] = {m->order, m->dm.hg_depth, m->exec};
     double best_us = 1e300;
     int v = 0;
-    for (int order = 0; order < 4; order++)
-        for (int depth = 0; depth < 3; depth++, v++) {
-            const int knobs[2] = {order, depth};
-            int rc = amrlbm_set_tuning(m, knobs);
-            if (rc) return rc;
-            double us = 0.0;
-            if ((rc = time_segment(m, len, reps, &us))) return rc;
-            table[3 * v] = order;
-            table[3 * v + 1] = depth;
-            table[3 * v + 2] = us;
-            if (us < best_us) {
-                best_us = us;
-                best[0] = order;
-                best[1] = depth;
+    for (int exec = 0; exec < 2; exec++)
+        for (int order = 0; order < 4; order++)
+            for (int depth = 0; depth < 3; depth++, v++) {
+                const int knobs[3] = {order, depth, exec};
+                int rc = amrlbm_set_tuning(m, knobs);
+                if (rc) return rc;
+                double us = -1.0;   // the segment kernel does not fit: not a variant here
+                if ((exec == 0 || use_segment_kernel(m)) && (rc = time_segment(m, len, reps, &us))) return rc;
+                table[4 * v] = order;
+                table[4 * v + 1] = depth;
+                table[4 * v + 2] = exec;
+                table[4 * v + 3] = us;
+                if (us > 0.0 && us < best_us) {
+                    best_us = us;
+                    best[0] = order;
+                    best[1] = depth;
+                    best[2] = exec;
+                }
             }
-        }
     int rc = amrlbm_set_tuning(m, best);
     if (rc) return rc;
+    if (m->exec) return AMRLBM_OK;
     hipGraphExec_t ex;   // the chosen variant's step graph, captured now rather than in the caller's next steps
     return segment_graph(m, len, &ex);
 }

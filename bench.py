@@ -167,13 +167,14 @@ TUNE = {"on": True}
 
 def tuning_entry(table, chosen):
     """The autotuner's table: the chosen launch variant and its time per step
-    (the step graph between two regrids) against the default variant's (order
-    0, depth 1)."""
-    t = {(o, d): us for o, d, us in table}
-    return {"chosen": {"order": chosen[0], "hg_depth": chosen[1]},
-            "us_per_step": round(t[chosen], 3), "default_us_per_step": round(t[(0, 1)], 3),
-            "gain_vs_default": round(t[(0, 1)] / t[chosen], 4),
-            "variants_us": {f"{o}/{d}": round(us, 3) for (o, d), us in sorted(t.items())}}
+    (the steps between two regrids) against the default variant's (order 0,
+    depth 1, step graph)."""
+    t = {(o, d, e): us for o, d, e, us in table if us > 0}
+    return {"chosen": {"order": chosen[0], "hg_depth": chosen[1],
+                       "execution": ["step graph", "segment kernel"][chosen[2]]},
+            "us_per_step": round(t[chosen], 3), "default_us_per_step": round(t[(0, 1, 0)], 3),
+            "gain_vs_default": round(t[(0, 1, 0)] / t[chosen], 4),
+            "variants_us": {f"{o}/{d}/{e}": round(us, 3) for (o, d, e), us in sorted(t.items())}}
 
 
 def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64):

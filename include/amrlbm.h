@@ -198,16 +198,20 @@ int amrlbm_time_kernels(amrlbm_mesh *m, int reps, double out[3]);
  * Morton curve cut into 8 runs, run c on XCD c; 1: the Morton curve as it
  * runs; 2: ascending id; 3: as 0, each run's blocks with the most ghost
  * operand loads first), knobs[1] = hg fill operands in flight per thread
- * (0: 2 and 1, 1: 4 and 2, 2: 8 and 4 recipes in the first and later stages).
- * A new mesh starts at {0, 1}. */
-int amrlbm_set_tuning(amrlbm_mesh *m, const int knobs[2]);
-int amrlbm_get_tuning(amrlbm_mesh *m, int knobs[2]);
-/* The autotuner: for each of the 12 variants, times reps launches of the
- * HIP graph amrlbm_step runs between two regrids (regrid_every steps, or
- * twice that when odd; no regrid) from the mesh's current state, which it
- * then restores, and keeps the fastest variant (its graph captured);
- * table[3 v .. 3 v + 2] = order, depth, microseconds per step. */
-int amrlbm_autotune(amrlbm_mesh *m, int reps, int regrid_every, double table[36]);
+ * (0: 2 and 1, 1: 4 and 2, 2: 8 and 4 recipes in the first and later stages),
+ * knobs[2] = how the steps between two regrids run (0: a HIP graph of two
+ * kernels per step over every leaf; 1: one persistent segment kernel per
+ * regrid interval, one workgroup per leaf resident for all its steps, two
+ * grid barriers per step, when every leaf's workgroup fits on the GPU at
+ * once, else as 0). A new mesh starts at {0, 1, 0}. */
+int amrlbm_set_tuning(amrlbm_mesh *m, const int knobs[3]);
+int amrlbm_get_tuning(amrlbm_mesh *m, int knobs[3]);
+/* The autotuner: for each of the 24 variants, times reps runs of the steps
+ * amrlbm_step runs between two regrids (regrid_every steps, or twice that
+ * when odd; no regrid) from the mesh's current state, which it then
+ * restores, and keeps the fastest; table[4 v .. 4 v + 3] = order, depth,
+ * execution, microseconds per step (-1: the segment kernel does not fit). */
+int amrlbm_autotune(amrlbm_mesh *m, int reps, int regrid_every, double table[96]);
 
 /* FP64 diagnostics over the leaf interiors (Output, AMR_Rising_2D_OMP.f90:456-489):
  * out[0..8] = rho_min, rho_max, max|Uy|, mass (sum rho dx^2), bubble area

@@ -62,17 +62,21 @@ def step_by_phases(m, t, k, g=None):
     m.set_step(t + 1)
 
 
+@pytest.mark.parametrize("execution", [0, 1])
 @pytest.mark.parametrize("name", ["tiny_rise", "cfg1_rise", "cfg2_regrid"])
-def test_fused_steps_across_mesh_changes_equal_reference(amrlbm, workloads, name):
+def test_fused_steps_across_mesh_changes_equal_reference(amrlbm, workloads, name, execution):
     """amrlbm_step (fused kernels, regrid decided on the device, segments taken
-    back when the mesh changes) over the whole run, then its last step phase by
-    phase: the final state's SHA-256 equals the reference build's."""
+    back when the mesh changes; the step graph or the persistent segment
+    kernel) over the whole run, then its last step phase by phase: the final
+    state's SHA-256 equals the reference build's."""
     g = RISE[name]
     w = workloads[g["workload"]]
     n, k = g["steps"], g["regrid_every"]
-    print(f"[rise] {name}: {n} steps, regrid every {k}, mesh changes at {g['change_steps']}", flush=True)
+    print(f"[rise] {name}: {n} steps, regrid every {k}, execution {execution}, "
+          f"mesh changes at {g['change_steps']}", flush=True)
     with amrlbm.Mesh(w) as m:
         m.setup_rising_bubble()
+        m.set_tuning(0, 1, execution)
         assert snap_hash(m) == g["init"]
         m.step(n - 1, k)
         st = m.regrid_stats()

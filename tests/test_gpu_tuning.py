@@ -1,8 +1,9 @@
 """Launch variants (amrlbm_set_tuning) and the autotuner (amrlbm_autotune).
-The leaves' launch order and the number of hg fill operands in flight per
-thread change no arithmetic, so every variant must leave the state bit for
-bit the same. The autotuner times the step graph of all twelve on the live
-mesh, keeps the fastest, and leaves the state as it found it."""
+The leaves' launch order, the number of hg fill operands in flight per
+thread and the execution (step graph or persistent segment kernel) change no
+arithmetic, so every variant must leave the state bit for bit the same. The
+autotuner times all 24 on the live mesh, keeps the fastest, and leaves the
+state as it found it."""
 import numpy as np
 import pytest
 
@@ -36,14 +37,15 @@ def _same(a, b):
 @pytest.mark.parametrize("precision", [64, 32])
 def test_variants_are_bitwise_equal(amrlbm, workloads, precision):
     w = workloads["cfg2"]
-    variants = [(o, d) for o in range(4) for d in range(3)] if precision == 64 else [(0, 0), (1, 1), (2, 2), (3, 1)]
+    variants = [(o, d, e) for e in range(2) for o in range(4) for d in range(3)] if precision == 64 else \
+        [(0, 0, 0), (1, 1, 0), (2, 2, 0), (3, 1, 0), (0, 1, 1), (3, 0, 1), (2, 2, 1)]
     ref = None
-    for order, depth in variants:
-        print(f"[tuning] FP{precision} cfg2 variant order {order} depth {depth}: 2000 steps", flush=True)
+    for order, depth, ex in variants:
+        print(f"[tuning] FP{precision} cfg2 variant order {order} depth {depth} execution {ex}: 2000 steps", flush=True)
         with amrlbm.Mesh(w, precision=precision) as m:
             m.setup_rising_bubble()
-            m.set_tuning(order, depth)
-            assert m.get_tuning() == (order, depth)
+            m.set_tuning(order, depth, ex)
+            assert m.get_tuning() == (order, depth, ex)
             m.step(2000, 10)
             st = _state(m)
             assert m.regrid_stats()["changes"] >= 2   # across mesh changes
@@ -62,12 +64,13 @@ def test_autotune_keeps_the_fastest_and_the_state(amrlbm, workloads):
         b.step(100, 10)
         before = _state(a)
         table = a.autotune(reps=10)
-        for order, depth, us in table:
-            print(f"[tuning] cfg2 order {order} depth {depth}: {us:.2f} us per step")
-        assert sorted((o, d) for o, d, _ in table) == [(o, d) for o in range(4) for d in range(3)]
-        assert all(us > 0 for _, _, us in table)
-        best = min(table, key=lambda r: r[2])
-        assert a.get_tuning() == best[:2]
+        for order, depth, ex, us in table:
+            print(f"[tuning] cfg2 order {order} depth {depth} execution {ex}: {us:.2f} us per step")
+        assert sorted((o, d, e) for o, d, e, _ in table) == [(o, d, e) for o in range(4) for d in range(3)
+                                                              for e in range(2)]
+        assert all(us > 0 for _, _, e, us in table if e == 0)
+        best = min((r for r in table if r[3] > 0), key=lambda r: r[3])
+        assert a.get_tuning() == best[:3]
         # the fields as they were (the diagnostics sum leaves in launch order, so they may differ in the last bit)
         _same(_state(a), before)
         a.step(200, 10)
