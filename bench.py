@@ -177,20 +177,34 @@ def tuning_entry(table, chosen):
             "variants_us": {f"{o}/{d}/{e}": round(us, 3) for (o, d, e), us in sorted(t.items())}}
 
 
-def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64):
+def windows_for(steps):
+    """Timed windows of `steps` steps each: enough that about 20,000 steps are
+    timed in all (one window at the configuration's 20,000), so that a short
+    window's launch and synchronisation latency is measured many times over
+    and reported as a median with its spread."""
+    return max(1, min(1000, 20000 // max(steps, 1)))
+
+
+def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64, radius=None, windows=None):
     """Time `steps` steps of workload w after `warmup` (and, untimed, the
-    autotuner's choice of launch variant), then the step kernels back to
-    back; returns the numbers of one bench entry."""
+    autotuner's choice of launch variant): windows_for(steps) consecutive
+    windows of exactly `steps` steps, each between two device
+    synchronisations; then the step kernels back to back. Returns the numbers
+    of one bench entry (wall = the median window)."""
     m = amrlbm.Mesh(w, precision=precision)
-    m.setup_rising_bubble()
+    m.setup_rising_bubble(radius=radius)
     m.step(warmup, k)
     tuning = tuning_entry(m.autotune(20, k), m.get_tuning()) if TUNE["on"] else None
     m.synchronize()
     c0 = m.counts()
-    t0 = time.perf_counter()
-    m.step(steps, k)
-    m.synchronize()
-    wall = time.perf_counter() - t0
+    walls = []
+    for _ in range(windows or windows_for(steps)):
+        t0 = time.perf_counter()
+        m.step(steps, k)
+        m.synchronize()
+        walls.append(time.perf_counter() - t0)
+    walls.sort()
+    wall = walls[len(walls) // 2]
     c1 = m.counts()
     rstats = m.regrid_stats()
     kt = m.time_kernels(200)
@@ -203,8 +217,14 @@ def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64):
 
     def roof(key):
         return roofline(key, kt, c1["nleaf"], w, precision)
+    pct = lambda q: walls[min(len(walls) - 1, int(q * len(walls)))]
+    timing = {"windows": len(walls), "steps_per_window": steps, "median_ms": round(wall * 1e3, 4),
+              "min_ms": round(walls[0] * 1e3, 4), "p10_ms": round(pct(0.1) * 1e3, 4),
+              "p90_ms": round(pct(0.9) * 1e3, 4), "max_ms": round(walls[-1] * 1e3, 4),
+              "steps_timed_in_all": steps * len(walls)}
     return {
         "value": w.fine_cells * steps / wall / 1e6, "wall": wall, "c0": c0, "c1": c1, "regrids": rstats,
+        "timing": timing,
         "roofline": roof(dom), "roofline_all": {kk: roof(kk) for kk in kt},
         "kernel_us": {kk: round(v * 1e3, 3) for kk, v in kt.items()},
         "step_kernel_us": sum(kt[kk] * LAUNCHES_PER_STEP.get(kk, 1.0 / k) for kk in kt) * 1e3,
@@ -340,7 +360,10 @@ def ensemble_run(amrlbm, n, seed, steps, k):
 def describe(w, k, precision=64):
     path = "plain FP64 path" if precision == 64 else "FP32 path (deviation storage)"
     what = (f"rising bubble {w.fine_shape[0]}x{w.fine_shape[1]} finest-equivalent, 16x16 blocks, "
-            f"levels {w.min_levels}..{w.max_levels}, D={int(w.D)}, regrid every {k}, {path}")
+            f"levels {w.min_levels}..{w.max_levels}, D={int(w.D)}, regrid every {k}, {path}; "
+            f"fluid: the reference's own set (Bo={w.Bo}, Ar={w.Ar:g}, g={w.gy:g}, W={w.W:g}, M0={w.M0:g}, "
+            f"density ratio {w.rhoh / w.rhol:g}, viscosity ratio {w.visc_ratio:g}), not the configuration's "
+            f"W=4, sigma=0.01, tau_h=0.8, which turn non-finite in the reference")
     if w.instances > 1:
         what = f"{w.instances} independent copies of the {w.L0}x{w.nroots // w.instances * w.L0} " + what
     return what
@@ -396,6 +419,8 @@ def main():
         "metric": METRIC, "value": round(r["value"], 1), "unit": "MLUPS (finest-equivalent)",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(r["wall"] * 1e3 / args.steps, 4), "higher_is_better": True,
+        "timing": dict(r["timing"], note="value and ms_per_step are the median window; every window is exactly "
+                                         "--steps steps between two device synchronisations"),
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: analytic tanh rising-bubble initial state, deterministic",
         "config": {"workload": describe(w, k), "leaf_blocks": [c0["nleaf"], c1["nleaf"]],
@@ -410,17 +435,25 @@ def main():
     if not args.no_fp32:
         r32 = measure(amrlbm, w, args.steps, args.warmup, k, precision=32)
         d32 = r32["diag"]
+        # the flow's own sensitivity: FP64 from a set-up whose bubble radius is 1e-7 larger
+        rp = measure(amrlbm, w, args.steps, args.warmup, k, precision=64, radius=w.radius * (1.0 + 1e-7))
+        dp = rp["diag"]
         out["fp32"] = {
             "workload": describe(w, k, 32), "value": round(r32["value"], 1), "unit": "MLUPS (finest-equivalent)",
             "speedup_vs_fp64": round(r32["value"] / r["value"], 3),
-            "ms_per_step": round(r32["wall"] * 1e3 / args.steps, 4), "dtype": "f32",
+            "ms_per_step": round(r32["wall"] * 1e3 / args.steps, 4), "dtype": "f32", "timing": r32["timing"],
             "roofline": r32["roofline"], "kernel_avg_us": r32["kernel_us"], "tuning": r32["tuning"],
             "error_vs_fp64": {
-                "after_steps": args.warmup + args.steps,
+                "after_steps": args.warmup + args.steps * r32["timing"]["windows"],
                 "mass_rel": abs(d32[3] - diag[3]) / abs(diag[3]),
                 "rise_velocity_rel": abs(d32[7] - diag[7]) / abs(diag[7]),
                 "centroid_abs_cells": max(abs(d32[5] - diag[5]), abs(d32[6] - diag[6])),
-                "same_mesh": r32["c1"] == r["c1"]},
+                "same_mesh": r32["c1"] == r["c1"],
+                "fp64_radius_perturbed_1e-7_vs_fp64": {
+                    "mass_rel": abs(dp[3] - diag[3]) / abs(diag[3]),
+                    "rise_velocity_rel": abs(dp[7] - diag[7]) / abs(diag[7]),
+                    "centroid_abs_cells": max(abs(dp[5] - diag[5]), abs(dp[6] - diag[6]))},
+                "targets": {"mass_rel": 1e-6, "rise_velocity_rel": 5e-3}},
             "learned_regrid": learned_regrid(amrlbm, w),
             # the network trained on the 1024x2048 geometry, deployed on the 128x256 bubble (D=32 at (64, 64))
             "learned_regrid_cfg1": learned_regrid(amrlbm, amrlbm.WORKLOADS["cfg1"],
@@ -431,7 +464,7 @@ def main():
         warm = max(args.warmup // 5, 10)
         entries = {}
         for prec in ((64,) if args.no_fp32 else (64, 32)):
-            rl = measure(amrlbm, wl, args.large_steps, warm, k, precision=prec)
+            rl = measure(amrlbm, wl, args.large_steps, warm, k, precision=prec, windows=1)
             ll = rl["c1"]["nleaf"]
             entries[f"fp{prec}"] = {
                 "workload": describe(wl, k, prec), "steps": args.large_steps, "warmup": warm,

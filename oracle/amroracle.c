@@ -19,7 +19,7 @@
  * order that runs from the interface outward on the normal axis and upward on
  * the tangential axis, interpolates along y first and x second, and divides at
  * the end. The snapshots written by the reference harness pin this restatement
- * direction by direction (tests/test_oracle_vs_reference.py).
+ * direction by direction (tests/test_oracle.py).
  */
 #include "amroracle.h"
 #include <math.h>
