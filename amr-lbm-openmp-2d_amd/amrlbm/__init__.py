@@ -111,6 +111,7 @@ def lib():
             "amrlbm_step": [vp, ip, ip],
             "amrlbm_profile_steps": [vp, ip, ip, _DP],
             "amrlbm_output": [vp, _DP],
+            "amrlbm_output_noalloc": [vp, _DP],
             "amrlbm_synchronize": [vp],
             "amrlbm_last_step_ms": [vp, _DP],
             "amrlbm_regrid_stats": [vp, _DP],
@@ -444,6 +445,15 @@ class Mesh:
         """rho_min, rho_max, max|Uy|, mass, bubble area, centroid x, y, rise velocity, non-finite."""
         out = (ctypes.c_double * 9)()
         rc = lib().amrlbm_output(self.h, out)
+        if rc != 0 and not (allow_nonfinite and rc == 5):
+            _check(rc)
+        return np.array(out[:])
+
+    def output_noalloc(self, allow_nonfinite=False):
+        """output() through amrlbm_output_noalloc (partials in the library's
+        static device scratch; no allocation)."""
+        out = (ctypes.c_double * 9)()
+        rc = lib().amrlbm_output_noalloc(self.h, out)
         if rc != 0 and not (allow_nonfinite and rc == 5):
             _check(rc)
         return np.array(out[:])

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <string>
 #include <omp.h>
 #include <parallel/algorithm>
@@ -129,6 +130,7 @@ struct amrlbm_mesh {
     double *d_part = nullptr;        // diagnostics partials
     double *d_diag = nullptr;
     double *h_pinned = nullptr;      // staging (err, diagnostics)
+    double *scratch = nullptr;       // this device's copy of the file-scope scratch array
     int cur = 0;
     std::vector<int> lvl_off;        // offsets into d_lists per level (levels 0..max_levels), + all-leaves
     std::vector<int> lvl_cnt;
@@ -213,6 +215,40 @@ struct amrlbm_mesh {
 };
 
 namespace {
+
+// ---------------------------------------------------------------------------
+// Device scratch of amrlbm_output_noalloc: the per-leaf partials (9 doubles
+// per leaf, kScratchLeaves leaves at most) and the 9 totals of the FP64
+// diagnostics, in a file-scope array, so that the entry point allocates
+// nothing. Each GPU holds its own copy until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress gives on that
+// device (the array's name is a host-side stand-in, not a device address).
+// ---------------------------------------------------------------------------
+constexpr int kScratchLeaves = 32768;
+constexpr int kScratchTotals = 9 * kScratchLeaves;
+static __device__ __align__(256) double scratch[kScratchTotals + 16];
+
+struct ScratchTable {
+    std::mutex mu;
+    double *ptr[64] = {};   // per device id
+};
+ScratchTable g_scratch;
+
+// the scratch array's address on the current device, looked up once per device
+int scratch_on_current_device(double **out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(AMRLBM_ERR_HIP, "device id out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch.mu);
+    if (!g_scratch.ptr[dev]) {
+        double *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(reinterpret_cast<void **>(&p), HIP_SYMBOL(scratch)));
+        g_scratch.ptr[dev] = p;
+    }
+    *out = g_scratch.ptr[dev];
+    return AMRLBM_OK;
+}
 
 // device memory freed when its scope ends (error returns included)
 struct DeviceBuffer {
@@ -1622,6 +1658,7 @@ static int create_mesh(const amrlbm_params *p, bool f32, amrlbm_mesh **out)
     m->dm.f32 = f32 ? 1 : 0;
     m->dm.phf = physf_from(p);
     int rc = alloc_device(m);
+    if (!rc) rc = scratch_on_current_device(&m->scratch);
     if (rc) {
         amrlbm_destroy(m);
         return rc;
@@ -3005,6 +3042,29 @@ int amrlbm_output(amrlbm_mesh *m, double out[9])
     HIPCHK(hipMemcpyAsync(m->h_pinned, m->d_diag, sizeof(double) * 9, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
     double *d = m->h_pinned;
+    for (int k = 0; k < 9; k++) out[k] = d[k];
+    out[5] = d[5] / d[4];
+    out[6] = d[6] / d[4];
+    out[7] = d[7] / d[4];
+    if (d[8] != 0.0) return fail(AMRLBM_ERR_NONFINITE, "non-finite phase field in " + std::to_string((long)d[8]) + " cells");
+    return AMRLBM_OK;
+}
+
+int amrlbm_output_noalloc(amrlbm_mesh *m, double out[9])
+{
+    NEED_READY(m);
+    if (!out) return fail(AMRLBM_ERR_ARG, "null argument");
+    if (m->tables_dirty) return fail(AMRLBM_ERR_STATE, "device tables not current (step or regrid first)");
+    const int n = (int)m->leaf.size();
+    if (n > kScratchLeaves)
+        return fail(AMRLBM_ERR_CAPACITY, std::to_string(n) + " leaves: amrlbm_output_noalloc takes at most " +
+                                             std::to_string(kScratchLeaves));
+    double *part = m->scratch, *tot = m->scratch + kScratchTotals;
+    launch_diag(m->dm, m->d_lists + m->all_off, n, m->ph, part, tot, m->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(m->h_pinned, tot, sizeof(double) * 9, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    const double *d = m->h_pinned;
     for (int k = 0; k < 9; k++) out[k] = d[k];
     out[5] = d[5] / d[4];
     out[6] = d[6] / d[4];

@@ -218,6 +218,16 @@ int amrlbm_autotune(amrlbm_mesh *m, int reps, int regrid_every, double table[96]
  * (sum (1-C) dx^2), bubble centroid x, y, bubble rise velocity, non-finite count.
  * Returns AMRLBM_ERR_NONFINITE when the count is non-zero (outputs still set). */
 int amrlbm_output(amrlbm_mesh *m, double out[9]);
+/* amrlbm_output without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The per-leaf
+ * partials live in a file-scope device array of the library (one per GPU,
+ * held until the process ends) of at most 32768 leaves; a larger mesh gets
+ * AMRLBM_ERR_CAPACITY. Same results as amrlbm_output, bit for bit. Calls on
+ * meshes of one GPU share that array: calls that overlap on one GPU race.
+ * The mesh's device tables must be current (any step, phase or regrid leaves
+ * them so), else AMRLBM_ERR_STATE. */
+int amrlbm_output_noalloc(amrlbm_mesh *m, double out[9]);
 
 /* block until all queued device work of the mesh is done */
 int amrlbm_synchronize(amrlbm_mesh *m);
