@@ -177,13 +177,29 @@ void launch_cnn(const float *X, const int *idx, int n, const float *w, float *lo
 void launch_cnn_adam(float *w, float *m, float *v, float *g, float lr, int t, hipStream_t s);
 void launch_cnn_features(const DevMesh &m, const int *list, int n, float *X, hipStream_t s);
 
+// Control of one launch of the fused step's kernels: where the collision
+// writes, and the device judge. The fused step collides out of place (the
+// state before the step stays whole until the stream writes the next one), so
+// a step the judge stops leaves the state of the step before it.
+struct StepCtl {
+    int out = -1;            // population buffer the collision writes (-1: in place, as the phase calls)
+    int *judge = nullptr;    // the first ordinal (state after that many steps of the segment) at which the
+                             // plain refinement criterion fired; none: INT_MAX. Null: no judge
+    int ord = 0;             // this launch's state ordinal: the collision of step s and the stream of
+                             // step s - 1 see the state after s steps
+    int judge_on = 0;        // this launch tests the criterion on its state
+    int min_level = 0, max_level = 0;   // the mesh's level range (an ensemble's per instance from m.lv)
+    double refin_tol = 0.0;
+};
 void launch_prog_c(const DevMesh &m, int cur, const int *list, int n, int kind, int op, const Prog &prog,
                    const Phys &ph, hipStream_t s,
-                   double *err_slot = nullptr);
+                   double *err_slot = nullptr, const StepCtl &ctl = StepCtl{});
+// one int to v (a fill kernel, so that a captured graph holds no memset node)
+void launch_fill_int(int *p, int v, hipStream_t s);
 // hg fill from a ghost program (or ghosts from memory with prog.hdr == null),
 // then optionally the stream (+ C = sum h, + ghost ring carried over)
 void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Prog &prog, bool stream_after,
-                    bool macro_h, bool copy_ghosts, hipStream_t s);
+                    bool macro_h, bool copy_ghosts, hipStream_t s, const int *judge = nullptr, int ord = 0);
 void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream_t s);
 void launch_err(const DevMesh &m, const int *list, int n, double *err, double *err_slot, hipStream_t s);
 void launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, int min_levels, int max_levels,

@@ -636,12 +636,13 @@ __device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, int set,
 // m.phsf); else the kernel arguments'.
 template <class T, int OP, bool ENS>
 __global__ __launch_bounds__(256) void k_prog_c(DevMeshT<T> m, int cur, const int *__restrict__ list, Prog prog,
-                                                Phys ph64, PhysF ph32, double *err_slot)
+                                                Phys ph64, PhysF ph32, double *err_slot, StepCtl ctl)
 {
     const auto &ph_arg = PhysOf<T>::get(ph64, ph32);
     const int k = blockIdx.x;
     const int tid = threadIdx.x;
-    const int skip = *m.skip;
+    // a judged state needing refinement before this launch's: the segment stops there
+    const int skip = *m.skip || (ctl.judge && *ctl.judge < ctl.ord);
     const int slot = list[k];
     const bool fill = prog.hdr != nullptr;
     PHdr hd;
@@ -655,6 +656,7 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMeshT<T> m, int cur, const in
     __shared__ T sC[LC_C];
     const int i = tid % NX + 1, j = tid / NX + 1;
     T *F = m.pop[cur] + (size_t)slot * POPB;
+    T *Fo = ctl.out >= 0 ? m.pop[ctl.out] + (size_t)slot * POPB : F;   // the collision's output
     T *PU = m.PU + (size_t)slot * PUB;
     T h[9], g[9], P = T(0), Ux = T(0), Uy = T(0);
     if (OP == OP_COLLIDE || OP == OP_MACRO_G_COLLIDE) {
@@ -712,7 +714,7 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMeshT<T> m, int cur, const in
             __syncthreads();
         }
     }
-    if (err_slot) {
+    if (err_slot || ctl.judge_on) {
         // Err_Interface (Refine_Derefine.f90:327-352) from the filled ring
         const T a = sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)];
         const T b = sC[cidx(i, j + 1)] - sC[cidx(i, j - 1)];
@@ -729,7 +731,13 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMeshT<T> m, int cur, const in
             T e = wmax[0];
             for (int w = 1; w < 4; w++)
                 if (wmax[w] > e) e = wmax[w];
-            err_slot[slot] = (double)(sqrt(e) / (T(2) * (T)me.dx));
+            const double err = (double)(sqrt(e) / (T(2) * (T)me.dx));
+            if (err_slot) err_slot[slot] = err;
+            if (ctl.judge_on) {
+                // the judge: Flag_for_refinement's criterion (Refine_Derefine.f90:18-54) on this leaf
+                const int lo = ENS ? m.lv[2 * me.inst] : ctl.min_level, hi = ENS ? m.lv[2 * me.inst + 1] : ctl.max_level;
+                if (me.level < lo || (me.level < hi && err > ctl.refin_tol)) atomicMin(ctl.judge, ctl.ord);
+            }
         }
     }
     if (OP == OP_MACRO_G_COLLIDE || OP == OP_MACRO_G) {
@@ -742,8 +750,8 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMeshT<T> m, int cur, const in
         collide_cell(sC, i, j, me.dx, P, Ux, Uy, ph, h, g);
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            F[fidx(0, q, i, j)] = h[q];
-            F[fidx(1, q, i, j)] = g[q];
+            Fo[fidx(0, q, i, j)] = h[q];
+            Fo[fidx(1, q, i, j)] = g[q];
         }
     }
 }
@@ -793,13 +801,14 @@ __device__ __forceinline__ T stream_cell(const T *sF, int lcf, T *__restrict__ G
 // first stage / later stages (a launch variant; the results do not depend on it).
 template <class T, int U0, int U1>
 __global__ __launch_bounds__(256) void k_prog_hg(DevMeshT<T> m, int cur, const int *__restrict__ list, int n,
-                                                 Prog prog, int flags)
+                                                 Prog prog, int flags, const int *judge, int ord)
 {
     const int w = blockIdx.x;
     const int k = 8 * (w >> 4) + (w & 7), set = (w >> 3) & 1;
     if (k >= n) return;
     const int tid = threadIdx.x;
-    const int skip = *m.skip;
+    // the stream of step ord - 1 runs unless the judge stopped the segment at or before its state
+    const int skip = *m.skip || (judge && *judge < ord);
     const int slot = list[k];
     PHdr hd;
     hd.off = 0;
@@ -1598,7 +1607,7 @@ __global__ __launch_bounds__(256, 2) void k_segment(DevMeshT<T> m, int cur, cons
     PU[pidx(i, j)] = P;
     PU[PS + pidx(i, j)] = Ux;
     PU[2 * PS + pidx(i, j)] = Uy;
-    T *Fend = m.pop[cur ^ (len & 1)] + (size_t)slot * POPB;
+    T *Fend = m.pop[cur] + (size_t)slot * POPB;
 #pragma unroll
     for (int q = 0; q < 9; q++) {
         Fend[fidx(0, q, i, j)] = sF[lidx(q, i, j, lcf)];
@@ -1611,50 +1620,54 @@ __global__ __launch_bounds__(256, 2) void k_segment(DevMeshT<T> m, int cur, cons
 // ---------------------------------------------------------------------------
 template <class T, bool ENS>
 static void prog_c(const DevMesh &dm, int cur, const int *list, int n, int op, const Prog &prog, const Phys &ph,
-                   hipStream_t s, double *err_slot)
+                   hipStream_t s, double *err_slot, const StepCtl &ctl)
 {
     const DevMeshT<T> m = typed<T>(dm);
     dim3 g(n), b(256);
     if (op == OP_MACRO_G_COLLIDE)
         hipLaunchKernelGGL((k_prog_c<T, OP_MACRO_G_COLLIDE, ENS>), g, b, 0, s, m, cur, list, prog, ph, dm.phf,
-                           err_slot);
+                           err_slot, ctl);
     else if (op == OP_COLLIDE)
-        hipLaunchKernelGGL((k_prog_c<T, OP_COLLIDE, ENS>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
+        hipLaunchKernelGGL((k_prog_c<T, OP_COLLIDE, ENS>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot, ctl);
     else if (op == OP_MACRO_G)
-        hipLaunchKernelGGL((k_prog_c<T, OP_MACRO_G, ENS>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
+        hipLaunchKernelGGL((k_prog_c<T, OP_MACRO_G, ENS>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot, ctl);
     else
-        hipLaunchKernelGGL((k_prog_c<T, OP_NONE, ENS>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot);
+        hipLaunchKernelGGL((k_prog_c<T, OP_NONE, ENS>), g, b, 0, s, m, cur, list, prog, ph, dm.phf, err_slot, ctl);
 }
 
 void launch_prog_c(const DevMesh &m, int cur, const int *list, int n, int kind, int op, const Prog &prog,
-                   const Phys &ph, hipStream_t s, double *err_slot)
+                   const Phys &ph, hipStream_t s, double *err_slot, const StepCtl &ctl)
 {
     if (n <= 0) return;
     (void)kind;   // the program decides the ring; the op needs C with the rings it reads
     const bool ens = m.phs != nullptr;
     if (m.f32)
-        ens ? prog_c<float, true>(m, cur, list, n, op, prog, ph, s, err_slot)
-            : prog_c<float, false>(m, cur, list, n, op, prog, ph, s, err_slot);
+        ens ? prog_c<float, true>(m, cur, list, n, op, prog, ph, s, err_slot, ctl)
+            : prog_c<float, false>(m, cur, list, n, op, prog, ph, s, err_slot, ctl);
     else
-        ens ? prog_c<double, true>(m, cur, list, n, op, prog, ph, s, err_slot)
-            : prog_c<double, false>(m, cur, list, n, op, prog, ph, s, err_slot);
+        ens ? prog_c<double, true>(m, cur, list, n, op, prog, ph, s, err_slot, ctl)
+            : prog_c<double, false>(m, cur, list, n, op, prog, ph, s, err_slot, ctl);
 }
+
+__global__ void k_fill_int(int *p, int v) { *p = v; }
+
+void launch_fill_int(int *p, int v, hipStream_t s) { hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(1), 0, s, p, v); }
 
 // the hg kernel of operand depth d: 2/1, 4/2 or 8/4 recipes in flight per thread
 template <class T>
 static void hg_launch(int d, dim3 grid, size_t lds, hipStream_t s, const DevMeshT<T> &m, int cur, const int *list, int n,
-                      const Prog &prog, int flags)
+                      const Prog &prog, int flags, const int *judge, int ord)
 {
     if (d == 0)
-        hipLaunchKernelGGL((k_prog_hg<T, 2, 1>), grid, dim3(256), lds, s, m, cur, list, n, prog, flags);
+        hipLaunchKernelGGL((k_prog_hg<T, 2, 1>), grid, dim3(256), lds, s, m, cur, list, n, prog, flags, judge, ord);
     else if (d == 2)
-        hipLaunchKernelGGL((k_prog_hg<T, 8, 4>), grid, dim3(256), lds, s, m, cur, list, n, prog, flags);
+        hipLaunchKernelGGL((k_prog_hg<T, 8, 4>), grid, dim3(256), lds, s, m, cur, list, n, prog, flags, judge, ord);
     else
-        hipLaunchKernelGGL((k_prog_hg<T, 4, 2>), grid, dim3(256), lds, s, m, cur, list, n, prog, flags);
+        hipLaunchKernelGGL((k_prog_hg<T, 4, 2>), grid, dim3(256), lds, s, m, cur, list, n, prog, flags, judge, ord);
 }
 
 void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Prog &prog, bool stream_after,
-                    bool macro_h, bool copy_ghosts, hipStream_t s)
+                    bool macro_h, bool copy_ghosts, hipStream_t s, const int *judge, int ord)
 {
     if (n <= 0) return;
     int flags = (prog.hdr ? 1 : 0) | (stream_after ? 2 : 0) | (macro_h ? 4 : 0) | (copy_ghosts ? 8 : 0) |
@@ -1665,9 +1678,9 @@ void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Pro
     const size_t esz = m.f32 ? sizeof(float) : sizeof(double);
     const size_t lds = hg_nodes_offset(lcf, esz) + (size_t)prog.nmax * sizeof(PNode);
     if (m.f32)
-        hg_launch<float>(m.hg_depth, grid, lds, s, typed<float>(m), cur, list, n, prog, flags);
+        hg_launch<float>(m.hg_depth, grid, lds, s, typed<float>(m), cur, list, n, prog, flags, judge, ord);
     else
-        hg_launch<double>(m.hg_depth, grid, lds, s, typed<double>(m), cur, list, n, prog, flags);
+        hg_launch<double>(m.hg_depth, grid, lds, s, typed<double>(m), cur, list, n, prog, flags, judge, ord);
 }
 
 void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream_t s)

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -118,6 +119,7 @@ struct amrlbm_mesh {
     NbDesc *d_nbd = nullptr;
     int *d_err_flags = nullptr;      // [0] a fill met a level jump
     unsigned *d_sync = nullptr;      // the segment kernel's grid barrier words (fault word at 15)
+    int *d_judge = nullptr;          // the device judge's stop ordinal (enqueue_segment with judge)
     int *d_lists = nullptr;          // leaves grouped by level, then all leaves (ascending)
     int lists_cap = 0;
     int *d_pairs = nullptr;          // restrict / prolong work lists
@@ -183,6 +185,7 @@ struct amrlbm_mesh {
     // between two regrids
     struct GraphEntry {
         int len, parity;
+        bool judge;
         hipGraphExec_t exec;
         int topo;   // topology version the executable was built or updated for
     };
@@ -1370,6 +1373,7 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipMalloc(&m->d_err_flags, sizeof(int)));
     HIPCHK(hipMemsetAsync(m->d_err_flags, 0, sizeof(int), m->stream));
     HIPCHK(hipMalloc(&m->d_sync, sizeof(unsigned) * kSegSyncWords));
+    HIPCHK(hipMalloc(&m->d_judge, sizeof(int)));
     HIPCHK(hipMemsetAsync(m->d_sync, 0, sizeof(unsigned) * kSegSyncWords, m->stream));
     HIPCHK(hipMalloc(&m->d_err, sizeof(double) * S));
     HIPCHK(hipMalloc(&m->d_err_slot, sizeof(double) * S));
@@ -1444,26 +1448,48 @@ int init_bubble(amrlbm_mesh *m, double R, double X0, double Y0)
 //   step 1      : [C2 fill + collide]          [hg fill + stream + C = sum h]
 //   steps 2..len: [C2 fill + U,P + collide]    [hg fill + stream + C = sum h]
 //   end         : [C fill + U,P]
+// The collision writes the other population buffer and the stream writes
+// back, so the state stays in buffer m->cur from step to step, and a step
+// that does not run leaves it whole. With judge, every state after the
+// first is tested against the plain refinement criterion in its collision
+// kernel (the end kernel tests the last): the first state found needing
+// refinement stops the segment there (d_judge = its ordinal, the later
+// launches return at once), so the regrid can follow at once.
 // after(kind, cells) is called after every launch (kind 0: C2 kernel,
 // 1: hg kernel, 2: C kernel).
 template <class After>
-int enqueue_segment(amrlbm_mesh *m, int cur, int len, After after)
+int enqueue_segment(amrlbm_mesh *m, int len, bool judge, After after)
 {
     const int n = nleaves(m);
     const int *lst = all_leaves(m);
+    const int cur = m->cur;
+    StepCtl ctl;
+    ctl.out = cur ^ 1;
+    if (judge) {
+        ctl.judge = m->d_judge;
+        ctl.min_level = m->prm.min_levels;
+        ctl.max_level = m->prm.max_levels;
+        ctl.refin_tol = 0.002;
+        launch_fill_int(m->d_judge, INT_MAX, m->stream);
+    }
     for (int s = 0; s < len; s++) {
+        ctl.ord = s;
+        ctl.judge_on = judge && s > 0;   // the state at a segment's start was decided on before it
         launch_prog_c(m->dm, cur, lst, n, KIND_C2, s == 0 ? OP_COLLIDE : OP_MACRO_G_COLLIDE, m->prog[KIND_C2], m->ph,
-                      m->stream);
+                      m->stream, nullptr, ctl);
         int rc = after(0, n * PS);
         if (rc) return rc;
-        launch_prog_hg(m->dm, cur, lst, n, m->prog[KIND_HG], true, true, false, m->stream);
+        launch_prog_hg(m->dm, cur ^ 1, lst, n, m->prog[KIND_HG], true, true, false, m->stream, ctl.judge, s + 1);
         rc = after(1, n * PS);
         if (rc) return rc;
-        cur ^= 1;
     }
     // Err_Interface of every leaf comes out of the last kernel, for a regrid
     // decision that may follow the segment
-    launch_prog_c(m->dm, cur, lst, n, KIND_C1, OP_MACRO_G, m->prog[KIND_C1], m->ph, m->stream, m->d_err_slot);
+    StepCtl end = ctl;
+    end.out = -1;
+    end.ord = len;
+    end.judge_on = judge;
+    launch_prog_c(m->dm, cur, lst, n, KIND_C1, OP_MACRO_G, m->prog[KIND_C1], m->ph, m->stream, m->d_err_slot, end);
     return after(2, n * PS);
 }
 
@@ -1478,11 +1504,11 @@ void drop_graphs(amrlbm_mesh *m)
 // grids and arguments, so the new capture updates the existing executable
 // (hipGraphExecUpdate) instead of instantiating another; an update the
 // runtime refuses falls back to instantiation.
-int segment_graph(amrlbm_mesh *m, int len, hipGraphExec_t *out)
+int segment_graph(amrlbm_mesh *m, int len, hipGraphExec_t *out, bool judge = false)
 {
     amrlbm_mesh::GraphEntry *entry = nullptr;
     for (auto &g : m->graphs)
-        if (g.len == len && g.parity == m->cur) entry = &g;
+        if (g.len == len && g.parity == m->cur && g.judge == judge) entry = &g;
     if (entry && entry->topo == m->topo_version) {
         *out = entry->exec;
         return AMRLBM_OK;
@@ -1491,7 +1517,7 @@ int segment_graph(amrlbm_mesh *m, int len, hipGraphExec_t *out)
     m->graphs_built++;
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_segment(m, m->cur, len, [](int, int) { return AMRLBM_OK; });
+    int rc = enqueue_segment(m, len, judge, [](int, int) { return AMRLBM_OK; });
     hipError_t e = hipStreamEndCapture(m->stream, &g);
     if (rc) return rc;
     if (e != hipSuccess) return fail(AMRLBM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -1514,7 +1540,7 @@ int segment_graph(amrlbm_mesh *m, int len, hipGraphExec_t *out)
         entry->exec = ex;
         entry->topo = m->topo_version;
     } else {
-        m->graphs.push_back({len, m->cur, ex, m->topo_version});
+        m->graphs.push_back({len, m->cur, judge, ex, m->topo_version});
     }
     *out = ex;
     return AMRLBM_OK;
@@ -1548,12 +1574,13 @@ int launch_segment_kernel(amrlbm_mesh *m, int len)
 }
 
 // The segment of len steps from the current state, as the tuning says: the
-// persistent segment kernel, or the segment's HIP graph.
-int launch_steps(amrlbm_mesh *m, int len)
+// persistent segment kernel, or the segment's HIP graph (always the graph
+// with the judge, which the segment kernel does not carry).
+int launch_steps(amrlbm_mesh *m, int len, bool judge = false)
 {
-    if (use_segment_kernel(m)) return launch_segment_kernel(m, len);
+    if (!judge && use_segment_kernel(m)) return launch_segment_kernel(m, len);
     hipGraphExec_t ex;
-    int rc = segment_graph(m, len, &ex);
+    int rc = segment_graph(m, len, &ex, judge);
     if (rc) return rc;
     HIPCHK(hipGraphLaunch(ex, m->stream));
     return AMRLBM_OK;
@@ -1692,6 +1719,7 @@ int amrlbm_destroy(amrlbm_mesh *m)
     if (m->h_pairs) (void)hipHostFree(m->h_pairs);
     (void)hipFree(m->d_err_flags);
     (void)hipFree(m->d_sync);
+    (void)hipFree(m->d_judge);
     (void)hipFree(m->d_lists);
     (void)hipFree(m->d_pairs);
     (void)hipFree(m->d_err);
@@ -2594,9 +2622,9 @@ int amrlbm_refine_derefine_predicted(amrlbm_mesh *m, const int *ids, int n)
 // r+1, ... The host reads check r while segment r+1 runs. When check r says the
 // mesh changes, segment r+1 has skipped all its work: the host takes its steps
 // back, runs Refine_Derefine_Mesh on the state of check r and goes on from
-// there. *ahead is set while a segment launched behind a pending check has
-// advanced T and the buffer parity (so an error return can take it back).
-static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahead_len, int *ahead_cur)
+// there. *ahead_len is set while a segment launched behind a pending check
+// has advanced T (so an error return can take it back).
+static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahead_len)
 {
     int rc = AMRLBM_OK;
     HIPCHK(hipEventRecord(m->ev0, m->stream));
@@ -2604,7 +2632,7 @@ static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahea
     bool pending = false;
     while (done < nsteps || pending) {
         bool launched = false;
-        int len = 0, cur0 = m->cur;
+        int len = 0;
         if (done < nsteps) {
             len = segment_length(m, nsteps - done, regrid_every);
             if (!pending) {
@@ -2612,14 +2640,10 @@ static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahea
                 if ((rc = upload_tables(m))) return rc;
             }
             if ((rc = launch_steps(m, len))) return rc;
-            m->cur ^= (len & 1);
             m->T += len;
             done += len;
             launched = true;
-            if (pending) {
-                *ahead_len = len;
-                *ahead_cur = cur0;
-            }
+            if (pending) *ahead_len = len;
         }
         if (pending) {
             pending = false;
@@ -2627,7 +2651,6 @@ static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahea
             if (*m->h_changed) {
                 HIPCHK(hipStreamSynchronize(m->stream));
                 if (launched) {
-                    m->cur = cur0;
                     m->T -= len;
                     done -= len;
                     m->segments_skipped++;
@@ -2664,18 +2687,15 @@ int amrlbm_step(amrlbm_mesh *m, int nsteps, int regrid_every)
 {
     NEED_READY(m);
     if (nsteps < 0 || regrid_every < 1) return fail(AMRLBM_ERR_ARG, "bad step arguments");
-    int ahead_len = 0, ahead_cur = 0;
-    const int rc = step_segments(m, nsteps, regrid_every, &ahead_len, &ahead_cur);
+    int ahead_len = 0;
+    const int rc = step_segments(m, nsteps, regrid_every, &ahead_len);
     if (rc) {
         // an error return leaves no device decision behind: with the stream
         // drained, the skip word every step kernel reads is cleared, so later
         // calls do their work, and a segment launched behind a decision that
         // said "changed" (it skipped its work) is taken back
         (void)hipStreamSynchronize(m->stream);
-        if (ahead_len && *m->h_changed) {
-            m->T -= ahead_len;
-            m->cur = ahead_cur;
-        }
+        if (ahead_len && *m->h_changed) m->T -= ahead_len;
         (void)hipMemsetAsync(m->d_changed, 0, sizeof(int) * 2, m->stream);
         (void)hipStreamSynchronize(m->stream);
         (void)hipGetLastError();
@@ -2721,10 +2741,9 @@ int amrlbm_profile_steps(amrlbm_mesh *m, int nsteps, int regrid_every, double st
         determine_min_max_levels(m);
         if ((rc = upload_tables(m))) break;
         if ((rc = mark(-1, 0))) break;
-        if ((rc = enqueue_segment(m, m->cur, len, mark))) break;
+        if ((rc = enqueue_segment(m, len, false, mark))) break;
         HIPCHK(hipGetLastError());
         if ((rc = flush())) break;
-        m->cur ^= (len & 1);
         m->T += len;
         done += len;
         if (m->T % regrid_every == 0) {
@@ -2769,10 +2788,13 @@ int amrlbm_time_kernels(amrlbm_mesh *m, int reps, double out[3])
     const int *lst = all_leaves(m);
     for (int kind = 0; kind < 3 && !rc; kind++) {
         auto launch = [&]() {
+            StepCtl ctl;
+            ctl.out = m->cur ^ 1;   // as in the step: the collision writes the other buffer, the stream writes back
             if (kind == 0)
-                launch_prog_c(m->dm, m->cur, lst, n, KIND_C2, OP_MACRO_G_COLLIDE, m->prog[KIND_C2], m->ph, m->stream);
+                launch_prog_c(m->dm, m->cur, lst, n, KIND_C2, OP_MACRO_G_COLLIDE, m->prog[KIND_C2], m->ph, m->stream,
+                              nullptr, ctl);
             else if (kind == 1)
-                launch_prog_hg(m->dm, m->cur, lst, n, m->prog[KIND_HG], true, true, false, m->stream);
+                launch_prog_hg(m->dm, m->cur ^ 1, lst, n, m->prog[KIND_HG], true, true, false, m->stream);
             else
                 launch_prog_c(m->dm, m->cur, lst, n, KIND_C1, OP_MACRO_G, m->prog[KIND_C1], m->ph, m->stream);
         };
@@ -2823,7 +2845,7 @@ int amrlbm_get_tuning(amrlbm_mesh *m, int knobs[3])
 
 // Average time of one step of the segment graph of len steps (the step
 // kernels as amrlbm_step launches them, no regrid), over reps launches from
-// the current state (len even, so every launch starts from the same buffer);
+// the current state;
 // the state is restored afterwards.
 static int time_segment(amrlbm_mesh *m, int len, int reps, double *us_per_step)
 {
