@@ -125,6 +125,8 @@ def lib():
             "amrlbm_cnn_train": [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_float), ip, ip, ctypes.c_float, ctypes.c_float, ctypes.c_uint, ctypes.POINTER(ctypes.c_float)],
             "amrlbm_cnn_features": [vp, ctypes.POINTER(ctypes.c_float), _IP],
             "amrlbm_regrid_check": [vp, _IP],
+            "amrlbm_set_regrid_network": [vp, ctypes.POINTER(ctypes.c_float), dp],
+            "amrlbm_step_learned": [vp, ip, ip, ctypes.POINTER(ctypes.c_longlong)],
             "amrlbm_refine_derefine_predicted": [vp, _IP, ip],
             "amrlbm_load_snapshot": [vp, ctypes.c_char_p],
             "amrlbm_collision_block": [PP, ip, _DP, _DP, _DP, _DP, _DP, _DP],
@@ -396,6 +398,22 @@ class Mesh:
         """Refine_Derefine_Mesh with the given blocks flagged for refinement too."""
         ids = np.ascontiguousarray(ids, dtype=np.int32)
         _check(lib().amrlbm_refine_derefine_predicted(self.h, _i(ids), int(ids.size)))
+
+    def set_regrid_network(self, weights, threshold=0.0):
+        """The learned predictor amrlbm_step_learned uses (None: none)."""
+        if weights is None:
+            _check(lib().amrlbm_set_regrid_network(self.h, None, 0.0))
+            return
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        assert w.size == CNN_NPARAM
+        _check(lib().amrlbm_set_regrid_network(self.h, _f(w), float(threshold)))
+
+    def step_learned(self, nsteps, predict_every=50):
+        """nsteps steps under the learned regrid's cadence (amrlbm_step_learned):
+        {full_regrids, predicted_blocks, misses, judged}."""
+        st = (ctypes.c_longlong * 4)()
+        _check(lib().amrlbm_step_learned(self.h, int(nsteps), int(predict_every), st))
+        return {"full_regrids": st[0], "predicted_blocks": st[1], "misses": st[2], "judged": st[3]}
 
     def cnn_features(self):
         """Network input of every leaf: (x [nLeaf, 3, 18, 18] float32, block ids)."""

@@ -3,13 +3,27 @@ and velocity predicts whether the block's region will be refined (or
 derefined) HORIZON steps ahead, so that blocks the interface is about to
 enter are refined before it arrives and the full regrid can run every
 HORIZON steps. The plain gradient criterion (Refine_Derefine.f90:18-54,
-:327-352) stays the judge: it is evaluated on the device every `judge_every`
-steps, and any interface cell on an under-refined block triggers an
-immediate plain regrid (a miss).
+:327-352) stays the judge: the library tests it on the device at every
+step's state, and the interface on an under-refined block triggers an
+immediate plain regrid (a miss). amrlbm_step_learned runs that cadence with
+every decision on the device; the host works only when the mesh changes.
 
 The network, its training and inference are the library's HIP kernels
-(amrlbm_cnn_*); this module builds the training set from the solver's own
-runs, trains, and runs the learned regrid.
+(amrlbm_cnn_*, amrlbm_set_regrid_network); this module builds the training
+set from the solver's own runs (the generator below), trains, and evaluates.
+
+Training generator: trajectories of the 1024x2048 configuration's recipe
+(two roots stacked in y, 16x16 blocks, four levels, one rising bubble,
+regrid every 10 steps) at a quarter of its size (256 x 512, levels 1..4),
+with the bubble's diameter uniform in [24, 40] finest cells and its centre
+uniform in [R + 8, L0 - R - 8] in x (the set-up's profile does not wrap
+around the periodic x boundary) and in [L0/4, L0/2] in y, drawn from a seed
+with SplitMix64,
+and the gravity of the time steps uniform between the reference's and
+GEN_GY_MAX (eight times it), so that slow interfaces and interfaces that
+cross into new blocks within the 400 steps of a trajectory are both in the
+set. The trajectories run 64 at a time in one ensemble
+mesh; an instance that turns non-finite is left out of the set.
 
     python -m amrlbm.learned train    # writes regrid_cnn_weights.npy beside this file
 """
@@ -21,10 +35,21 @@ import time
 import numpy as np
 
 from . import Mesh, cnn_forward, cnn_init, cnn_train
-from .workloads import WORKLOADS
+from .ensemble import Ensemble, Stream
+from .workloads import WORKLOADS, Workload
 
 HORIZON = 50
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "regrid_cnn_weights.npy")
+# the decision threshold on logit 0 chosen on held-out trajectories, with their scores
+THRESHOLD_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "regrid_cnn_threshold.json")
+
+
+def _threshold():
+    """The committed network's decision threshold (0 without the file)."""
+    import json
+    if not os.path.exists(THRESHOLD_FILE):
+        return 0.0
+    return float(json.load(open(THRESHOLD_FILE))["threshold"])
 
 # bubble set-ups of the cfg2 geometry used for training (radius, centre) and
 # the one held out for evaluation
@@ -84,65 +109,41 @@ def generate(w, variants, steps, every=HORIZON, horizon=HORIZON, regrid_every=10
     return np.concatenate(X), np.concatenate(Y)
 
 
-def run(mesh, nsteps, weights=None, predict_every=HORIZON, judge_every=10, threshold=0.0):
-    """Advance the mesh nsteps with the full regrid every predict_every steps
-    (the plain criterion plus, with weights, the blocks the network flags for
-    refinement) and the plain criterion judging every judge_every steps.
-
-    Two regrids never follow each other without a step between them.
-    Err_Interface reads C's ghost ring, which only the step fills, and a
-    block just derefined holds a ring of zeros (Restrict.f90:23-30), so a
-    second regrid at once would refine every such block again from the
-    spurious jump and prolong C from the zero ring into its children. The
-    judge is therefore not consulted at a multiple of predict_every, where
-    the full regrid comes next (in this call or at the start of the next),
-    and a call that starts at the step of the last regrid this function ran
-    on the mesh (a judge's, ending the previous call) skips its first full
-    regrid."""
-    st = {"predicted_regrids": 0, "predicted_blocks": 0, "misses": 0, "checks": 0}
-    t = 0
-    while t < nsteps:
-        if t % predict_every == 0 and getattr(mesh, "_last_regrid_step", None) != mesh.counts()["T"]:
-            ids = np.zeros(0, dtype=np.int32)
-            if weights is not None:
-                x, all_ids = mesh.cnn_features()
-                ids = all_ids[cnn_forward(x, weights)[:, 0] > threshold]
-            mesh.refine_derefine_predicted(ids)
-            mesh._last_regrid_step = mesh.counts()["T"]
-            st["predicted_regrids"] += 1
-            st["predicted_blocks"] += int(ids.size)
-        n = min(judge_every, nsteps - t, predict_every - t % predict_every)
-        mesh.step(n, 1 << 30)
-        t += n
-        if t % predict_every == 0:
-            continue
-        _, refine = mesh.regrid_check()
-        st["checks"] += 1
-        if refine:
-            mesh.refine_derefine_mesh()
-            mesh._last_regrid_step = mesh.counts()["T"]
-            st["misses"] += 1
-    return st
+def run(mesh, nsteps, weights=None, predict_every=HORIZON, threshold=None):
+    """Advance the mesh nsteps under the learned regrid's cadence
+    (amrlbm_step_learned): the full regrid every predict_every steps with the
+    blocks the network flags (weights; none: the plain criterion alone) and
+    the plain criterion judging every step on the device."""
+    mesh.set_regrid_network(weights, _threshold() if threshold is None else threshold)
+    st = mesh.step_learned(nsteps, predict_every)
+    return {"predicted_regrids": st["full_regrids"], "predicted_blocks": st["predicted_blocks"],
+            "misses": st["misses"], "checks": st["judged"]}
 
 
-def evaluate(w, weights, steps, precision=32, variant_=EVAL_VARIANT):
+def evaluate(w, weights, steps, precision=32, variant_=None, predict_every=HORIZON):
     """The learned regrid against the same cadence without the network, and
-    against the plain regrid every 10 steps, on a held-out bubble."""
-    wv = variant(w, *variant_)
+    against the plain regrid every 10 steps, on workload w (a bubble variant
+    of it with variant_ = (radius, x0, y0))."""
+    wv = variant(w, *variant_) if variant_ else w
     out = {}
     for name, wts in (("learned", weights), ("no_network", None)):
         with Mesh(wv, precision=precision) as m:
             m.setup_rising_bubble()
+            m.step(1, 1 << 30)   # C's ghost ring and U, P of a stepped state, as every regrid sees them
+            m.synchronize()
             t0 = time.perf_counter()
-            st = run(m, steps, wts)
+            st = run(m, steps, wts, predict_every)
             m.synchronize()
             st["wall_s"] = time.perf_counter() - t0
             st["mlups"] = wv.fine_cells * steps / st["wall_s"] / 1e6
             st["leaves"] = m.counts()["nleaf"]
+            st["missed_refinement_rate"] = st["misses"] / max(st["checks"], 1)
             st["diag"] = m.output().tolist()
             out[name] = st
     with Mesh(wv, precision=precision) as m:
         m.setup_rising_bubble()
+        m.step(1, 1 << 30)
+        m.synchronize()
         t0 = time.perf_counter()
         m.step(steps, 10)
         m.synchronize()
@@ -153,24 +154,125 @@ def evaluate(w, weights, steps, precision=32, variant_=EVAL_VARIANT):
     return out
 
 
-def train(steps=3000, epochs=40, seed=1, pos_weight=None):
-    w = WORKLOADS["cfg2"]
-    X, Y = generate(w, TRAIN_VARIANTS, steps)
+# ---- training set from the generator -------------------------------------
+GEN_L0, GEN_MIN_LEVELS, GEN_GY_MAX, GEN_STEPS = 256, 1, 4.0e-5, 400
+GEN_SNAPS = range(100, GEN_STEPS - HORIZON + 1, 10)   # inputs at these steps, labels HORIZON later
+
+
+def generator_instances(n, seed):
+    """n trajectories of the generator (module docstring) drawn from seed."""
+    s = Stream(seed)
+    out = []
+    for q in range(n):
+        R = 0.5 * (24.0 + 16.0 * s.uniform())
+        x0 = R + 8.0 + (GEN_L0 - 2.0 * R - 16.0) * s.uniform()
+        y0 = GEN_L0 * (0.25 + 0.25 * s.uniform())
+        gy = 5.0e-6 + (GEN_GY_MAX - 5.0e-6) * s.uniform()
+        out.append(Workload(f"g{q}", L0=GEN_L0, min_levels=GEN_MIN_LEVELS, radius=R, x0=x0, y0=y0, gy_step=gy))
+    return out
+
+
+def generator_ensemble(n, seed):
+    members = tuple(generator_instances(n, seed))
+    return Ensemble(f"gen{n}_s{seed}", members, block_cap=-(-n * 400 // 4096) * 4096)
+
+
+def generate_ensembles(count, seed0, per=64, keep_near=0.25, keep_far=0.01, precision=32, log=None):
+    """Inputs and targets of count trajectories (per at a time in one
+    ensemble mesh, seeds seed0, seed0 + 1, ...): at each of GEN_SNAPS steps
+    the input of every leaf, labelled by the topology HORIZON steps later.
+    Kept, from the instances that stay finite: every positive, keep_near of
+    the negatives whose phase field varies across the block (max - min of C
+    > 0.01: the interface or near it), and keep_far of the others (uniform
+    bulk liquid or gas)."""
+    X, Y = [], []
+    rng = np.random.default_rng(seed0)
+    done = 0
+    e = 0
+    while done < count:
+        ens = generator_ensemble(min(per, count - done), seed0 + e)
+        with Mesh(ens, precision=precision) as m:
+            m.setup_rising_bubble()
+            snaps = {}
+            t = 0
+            for t_next in sorted(set(GEN_SNAPS) | {t + HORIZON for t in GEN_SNAPS}):
+                m.step(t_next - t, 10)
+                t = t_next
+                x, ids = m.cnn_features()
+                rows, _ = m.topology()
+                snaps[t] = (x, ids, rows)
+            finite = m.output_instances(allow_nonfinite=True)[:, 8] == 0
+        bad = int(np.sum(~finite))
+        for t in GEN_SNAPS:
+            x, ids, rows = snaps[t]
+            y = labels(rows, ids, snaps[t + HORIZON][2])
+            lev, J = rows[ids - 1, 0], rows[ids - 1, 3]
+            inst = (J >> lev) // 2
+            c = x[:, 0]
+            near = (c.max(axis=(1, 2)) - c.min(axis=(1, 2))) > 0.01
+            u = rng.random(len(ids))
+            keep = ((y[:, 0] > 0.5) | (near & (u < keep_near)) | (u < keep_far)) & finite[inst]
+            X.append(x[keep])
+            Y.append(y[keep])
+        done += len(ens.members)
+        e += 1
+        if log:
+            log(f"[generate] {done}/{count} trajectories, {sum(len(v) for v in Y)} samples, "
+                f"non-finite instances {bad}")
+    return np.concatenate(X), np.concatenate(Y)
+
+
+def scores(x, y, weights, threshold=0.0):
+    """Refinement recall and precision of the network on (x, y)."""
+    pred = cnn_forward(x, weights)[:, 0] > threshold
+    pos = y[:, 0] > 0.5
+    tp = int(np.sum(pred & pos))
+    return {"samples": int(len(y)), "positives": int(pos.sum()), "threshold": float(threshold),
+            "refine_recall": tp / max(int(pos.sum()), 1), "refine_precision": tp / max(int(pred.sum()), 1)}
+
+
+def best_threshold(x, y, weights):
+    """The threshold on logit 0 with the best F1 score of the refinement flag on (x, y)."""
+    z = cnn_forward(x, weights)[:, 0]
+    pos = y[:, 0] > 0.5
+    best, best_f1 = 0.0, -1.0
+    for t in np.quantile(z, np.linspace(0.5, 0.9999, 400)):
+        pred = z > t
+        tp = float(np.sum(pred & pos))
+        f1 = 2.0 * tp / max(float(pred.sum() + pos.sum()), 1.0)
+        if f1 > best_f1:
+            best, best_f1 = float(t), f1
+    return best
+
+
+TRAIN_TRAJECTORIES, TRAIN_SEED, HELDOUT_SEED = 5000, 1000, 9000
+
+
+def train(trajectories=TRAIN_TRAJECTORIES, epochs=20, seed=1, pos_weight=10.0, log=print):
+    X, Y = generate_ensembles(trajectories, TRAIN_SEED, log=log)
     pos = Y.mean(axis=0)
-    pw = pos_weight if pos_weight is not None else float(min(50.0, (1 - pos[0]) / max(pos[0], 1e-4)))
+    pw = float(pos_weight)
     weights, hist = cnn_train(X, Y, cnn_init(seed), epochs=epochs, batch=256, lr=2e-3, pos_weight=pw, seed=seed)
-    pred = cnn_forward(X, weights) > 0
-    tp = int(np.sum(pred[:, 0] & (Y[:, 0] > 0.5)))
-    info = {"samples": int(X.shape[0]), "positive_rate": pos.tolist(), "pos_weight": pw,
-            "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
-            "refine_recall": tp / max(int(np.sum(Y[:, 0] > 0.5)), 1),
-            "refine_precision": tp / max(int(np.sum(pred[:, 0])), 1)}
+    info = {"trajectories": trajectories, "positive_rate": pos.tolist(), "pos_weight": pw,
+            "loss": [float(v) for v in hist], "train": scores(X, Y, weights)}
+    del X, Y
+    # the threshold from one held-out draw, its scores on another
+    Xv, Yv = generate_ensembles(256, HELDOUT_SEED, log=log)
+    thr = best_threshold(Xv, Yv, weights)
+    info["threshold"] = thr
+    info["validation"] = scores(Xv, Yv, weights, thr)
+    del Xv, Yv
+    Xh, Yh = generate_ensembles(256, HELDOUT_SEED + 1000, log=log)
+    info["held_out"] = scores(Xh, Yh, weights, thr)
+    info["held_out_threshold_0"] = scores(Xh, Yh, weights, 0.0)
     return weights, info
 
 
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "train":
+        import json
         wts, info = train()
         np.save(WEIGHTS, wts)
-        print(info)
-        print(evaluate(WORKLOADS["cfg2"], wts, 2000))
+        json.dump({"threshold": info["threshold"], "held_out": info["held_out"]}, open(THRESHOLD_FILE, "w"), indent=1)
+        print(info, flush=True)
+        print(evaluate(WORKLOADS["cfg1"], wts, 2000), flush=True)

@@ -56,7 +56,9 @@ def gas_regions(mesh):
     rows, _ = mesh.topology()
     lx, ly = mesh.w.fine_shape
     nx = mesh.w.nx
-    c = x[:, 0, 1:nx + 1, 1:nx + 1].astype(np.float64)          # [leaf, y, x]
+    # the network input holds the signed distance d / 16 = (W / 4) ln(C / (1 - C)) / 16: C back from it
+    d = x[:, 0, 1:nx + 1, 1:nx + 1].astype(np.float64) * 16.0   # [leaf, y, x]
+    c = 1.0 / (1.0 + np.exp(-4.0 * d / mesh.w.W))
     gas = c < 0.5
     raster = np.zeros((ly, lx), dtype=bool)
     origin = []

@@ -294,22 +294,31 @@ __global__ void k_cnn_adam(float *__restrict__ w, float *__restrict__ m, float *
     g[i] = 0.f;
 }
 
-// Network input of every leaf in list (n entries): C (absolute) on x, y in
-// 0..17, Ux and Uy times kUScale on the interior, zero on the ring
+// Network input of every leaf in list (n entries): the signed distance to
+// the interface that the phase field's tanh profile C = 1/2 + 1/2 tanh(2 d /
+// W) gives, d = (W / 4) ln(C / (1 - C)) cells of the block's
+// level, clamped to +-20 and divided by 16, on x, y in 0..17 (a field that
+// BF16 resolves, where C itself reads 1 for every liquid cell within a few
+// widths of the interface); Ux and Uy times kUScale on the interior, zero on
+// the ring. W is the mesh's, or in an ensemble mesh the block's instance's.
 template <class T>
 __global__ __launch_bounds__(256) void k_cnn_features(DevMeshT<T> m, const int *__restrict__ list, int n,
-                                                      float *__restrict__ X)
+                                                      float *__restrict__ X, double W)
 {
     const int k = blockIdx.x, tid = threadIdx.x;
     if (k >= n) return;
     const int slot = list[k];
     const T *C = m.C + (size_t)slot * CS;
     const T *PU = m.PU + (size_t)slot * PUB;
-    const float shift = std::is_same<T, float>::value ? 1.f : 0.f;
+    const bool dev = std::is_same<T, float>::value;   // FP32 storage: C - 1
+    if (m.phs) W = m.phs[m.blk[slot].inst].W;
     float *x = X + (size_t)k * 3 * 18 * 18;
     for (int t = tid; t < 18 * 18; t += 256) {
         const int yy = t / 18, xx = t - 18 * yy;
-        x[t] = (float)C[(yy + 1) * CW + (xx + 1)] + shift;
+        const double v = (double)C[(yy + 1) * CW + (xx + 1)];
+        const double c = fmax(dev ? v + 1.0 : v, 1e-12), omc = fmax(dev ? -v : 1.0 - v, 1e-12);
+        const double d = 0.25 * W * (log(c) - log(omc));
+        x[t] = (float)(fmin(fmax(d, -20.0), 20.0) * (1.0 / 16.0));
         const bool in = xx >= 1 && xx <= NX && yy >= 1 && yy <= NY;
         x[18 * 18 + t] = in ? (float)PU[PS + (xx - 1) + NX * (yy - 1)] * kUScale : 0.f;
         x[2 * 18 * 18 + t] = in ? (float)PU[2 * PS + (xx - 1) + NX * (yy - 1)] * kUScale : 0.f;
@@ -330,13 +339,13 @@ void launch_cnn_adam(float *w, float *m, float *v, float *g, float lr, int t, hi
                        1e-8f, c1, c2);
 }
 
-void launch_cnn_features(const DevMesh &m, const int *list, int n, float *X, hipStream_t s)
+void launch_cnn_features(const DevMesh &m, const int *list, int n, float *X, double W, hipStream_t s)
 {
     if (n <= 0) return;
     if (m.f32)
-        hipLaunchKernelGGL(k_cnn_features<float>, dim3(n), dim3(256), 0, s, typed<float>(m), list, n, X);
+        hipLaunchKernelGGL(k_cnn_features<float>, dim3(n), dim3(256), 0, s, typed<float>(m), list, n, X, W);
     else
-        hipLaunchKernelGGL(k_cnn_features<double>, dim3(n), dim3(256), 0, s, typed<double>(m), list, n, X);
+        hipLaunchKernelGGL(k_cnn_features<double>, dim3(n), dim3(256), 0, s, typed<double>(m), list, n, X, W);
 }
 
 }  // namespace amrlbm

@@ -175,7 +175,7 @@ constexpr float kUScale = 10.0f;   // network input: velocity times this
 void launch_cnn(const float *X, const int *idx, int n, const float *w, float *logits, const float *Y,
                 float pos_weight, float *grad, float *loss, hipStream_t s);
 void launch_cnn_adam(float *w, float *m, float *v, float *g, float lr, int t, hipStream_t s);
-void launch_cnn_features(const DevMesh &m, const int *list, int n, float *X, hipStream_t s);
+void launch_cnn_features(const DevMesh &m, const int *list, int n, float *X, double W, hipStream_t s);
 
 // Control of one launch of the fused step's kernels: where the collision
 // writes, and the device judge. The fused step collides out of place (the
@@ -204,7 +204,11 @@ void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream
 void launch_err(const DevMesh &m, const int *list, int n, double *err, double *err_slot, hipStream_t s);
 void launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, int min_levels, int max_levels,
                          int levelmin, int levelmax, double refin_tol, double deref_tol, int *changed,
-                         hipStream_t s);
+                         hipStream_t s, const int *pred = nullptr);
+// pred[list[k]] = logits[2 k] > threshold and err_slot[list[k]] > gate (the
+// learned predictor's refinement flags by block slot)
+void launch_pred_flags(const float *logits, const int *list, int n, float threshold, const double *err_slot,
+                       double gate, int *pred, hipStream_t s);
 void launch_restrict(const DevMesh &m, int cur, const int *parents, int n, hipStream_t s);
 void launch_prolong(const DevMesh &m, int cur, const int *pairs, int n, const Phys &ph, hipStream_t s);
 void launch_init_fields(const DevMesh &m, int cur, const int *list, int n, const Phys &ph, double R, hipStream_t s,

@@ -919,7 +919,8 @@ __global__ __launch_bounds__(256) void k_err(DevMeshT<T> m, const int *__restric
 constexpr int RC_CAND = 512;
 __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int nblks, const double *__restrict__ err_slot,
                                                       int min_levels, int max_levels, int levelmin, int levelmax,
-                                                      double refin_tol, double deref_tol, int *changed)
+                                                      double refin_tol, double deref_tol, int *changed,
+                                                      const int *__restrict__ pred)
 {
     extern __shared__ unsigned dmask[];
     __shared__ int cand[RC_CAND], clev[RC_CAND], watch[RC_CAND * 16];
@@ -933,7 +934,9 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
         const BlockDev b = m.blk[n];
         if (b.level < 0) continue;
         const int lo = m.lv ? m.lv[2 * b.inst] : min_levels, hi = m.lv ? m.lv[2 * b.inst + 1] : max_levels;
-        if (b.isleaf && (b.level < lo || (b.level < hi && err_slot[n] > refin_tol))) ref = 1;
+        // the plain criterion, and with pred the blocks the learned predictor
+        // flags (Refine_Derefine_Mesh with extra flags: the same closure follows)
+        if (b.isleaf && (b.level < lo || (b.level < hi && (err_slot[n] > refin_tol || (pred && pred[n]))))) ref = 1;
         if (!b.hasleaf || b.level < lo || b.child0 < 0) continue;
         bool d = true;
         for (int c = b.child0; c < b.child0 + 4; c++) {
@@ -1703,11 +1706,33 @@ void launch_err(const DevMesh &m, const int *list, int n, double *err, double *e
 
 void launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, int min_levels, int max_levels,
                          int levelmin, int levelmax, double refin_tol, double deref_tol, int *changed,
-                         hipStream_t s)
+                         hipStream_t s, const int *pred)
 {
     const size_t lds = sizeof(unsigned) * (size_t)((nblks + 31) / 32 + 1);
     hipLaunchKernelGGL(k_regrid_check, dim3(1), dim3(1024), lds, s, typed<double>(m), nblks, err_slot, min_levels,
-                       max_levels, levelmin, levelmax, refin_tol, deref_tol, changed);
+                       max_levels, levelmin, levelmax, refin_tol, deref_tol, changed, pred);
+}
+
+// the learned predictor's refinement flag per block slot: logit 0 of leaf k
+// (launch order) above the threshold, on a block whose Err_Interface already
+// passes gate (the interface's profile reaches into it)
+__global__ __launch_bounds__(256) void k_pred_flags(const float *__restrict__ logits, const int *__restrict__ list,
+                                                    int n, float threshold, const double *__restrict__ err_slot,
+                                                    double gate, int *pred)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) {
+        const int slot = list[k];
+        pred[slot] = (logits[2 * k] > threshold && err_slot[slot] > gate) ? 1 : 0;
+    }
+}
+
+void launch_pred_flags(const float *logits, const int *list, int n, float threshold, const double *err_slot,
+                       double gate, int *pred, hipStream_t s)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_pred_flags, dim3((n + 255) / 256), dim3(256), 0, s, logits, list, n, threshold, err_slot,
+                           gate, pred);
 }
 
 void launch_restrict(const DevMesh &m, int cur, const int *pairs, int n, hipStream_t s)

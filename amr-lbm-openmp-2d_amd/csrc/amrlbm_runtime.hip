@@ -120,6 +120,14 @@ struct amrlbm_mesh {
     int *d_err_flags = nullptr;      // [0] a fill met a level jump
     unsigned *d_sync = nullptr;      // the segment kernel's grid barrier words (fault word at 15)
     int *d_judge = nullptr;          // the device judge's stop ordinal (enqueue_segment with judge)
+    int *h_judge = nullptr;          // pinned copy of it
+    // the learned regrid predictor of amrlbm_step_learned: its weights, the
+    // network inputs and logits of every leaf, its refinement flags by slot
+    float *d_net_w = nullptr, *d_net_x = nullptr, *d_net_logit = nullptr;
+    int *d_pred = nullptr;
+    int net_cap = 0;
+    bool has_net = false;
+    float net_threshold = 0.0f;
     int *d_lists = nullptr;          // leaves grouped by level, then all leaves (ascending)
     int lists_cap = 0;
     int *d_pairs = nullptr;          // restrict / prolong work lists
@@ -1374,6 +1382,7 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipMemsetAsync(m->d_err_flags, 0, sizeof(int), m->stream));
     HIPCHK(hipMalloc(&m->d_sync, sizeof(unsigned) * kSegSyncWords));
     HIPCHK(hipMalloc(&m->d_judge, sizeof(int)));
+    HIPCHK(hipHostMalloc(&m->h_judge, sizeof(int), hipHostMallocDefault));
     HIPCHK(hipMemsetAsync(m->d_sync, 0, sizeof(unsigned) * kSegSyncWords, m->stream));
     HIPCHK(hipMalloc(&m->d_err, sizeof(double) * S));
     HIPCHK(hipMalloc(&m->d_err_slot, sizeof(double) * S));
@@ -1720,6 +1729,11 @@ int amrlbm_destroy(amrlbm_mesh *m)
     (void)hipFree(m->d_err_flags);
     (void)hipFree(m->d_sync);
     (void)hipFree(m->d_judge);
+    if (m->h_judge) (void)hipHostFree(m->h_judge);
+    (void)hipFree(m->d_net_w);
+    (void)hipFree(m->d_net_x);
+    (void)hipFree(m->d_net_logit);
+    (void)hipFree(m->d_pred);
     (void)hipFree(m->d_lists);
     (void)hipFree(m->d_pairs);
     (void)hipFree(m->d_err);
@@ -2681,7 +2695,141 @@ static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahea
     return check_err_flags(m);
 }
 
+// The learned predictor's refinement flags of every leaf at the current state
+// (network inputs, forward pass and flags by slot, on the device).
+static int predict_refinement(amrlbm_mesh *m)
+{
+    const int n = nleaves(m);
+    if (n > m->net_cap) {
+        const int cap = n + n / 2 + 64;
+        (void)hipFree(m->d_net_x);
+        (void)hipFree(m->d_net_logit);
+        m->d_net_x = m->d_net_logit = nullptr;
+        HIPCHK(hipMalloc(&m->d_net_x, sizeof(float) * 3 * 18 * 18 * (size_t)cap));
+        HIPCHK(hipMalloc(&m->d_net_logit, sizeof(float) * 2 * (size_t)cap));
+        m->net_cap = cap;
+    }
+    launch_cnn_features(m->dm, all_leaves(m), n, m->d_net_x, m->ph.W, m->stream);
+    launch_cnn(m->d_net_x, nullptr, n, m->d_net_w, m->d_net_logit, nullptr, 0.f, nullptr, nullptr, m->stream);
+    // a prediction counts where the interface's profile already reaches the
+    // block (Err_Interface above a quarter of the refinement tolerance), so
+    // that refined children of a predicted block, which no training state
+    // held, do not call for the next level in turn
+    launch_pred_flags(m->d_net_logit, all_leaves(m), n, m->net_threshold, m->d_err_slot, 0.25 * 0.002, m->d_pred,
+                      m->stream);
+    HIPCHK(hipGetLastError());
+    return AMRLBM_OK;
+}
+
+// The learned regrid's cadence (no counterpart in the reference): segments up
+// to the next multiple of predict_every with the plain criterion judging every
+// state on the device; a state it finds needing refinement is regridded at
+// once (a miss). At each multiple of predict_every, the regrid decision takes
+// the plain criterion plus the network's flags (when a network is set) and
+// runs Refine_Derefine_Mesh with them when the mesh changes.
+static int step_learned(amrlbm_mesh *m, int nsteps, int predict_every, long long st[4])
+{
+    int rc = AMRLBM_OK, done = 0;
+    while (done < nsteps) {
+        determine_min_max_levels(m);
+        if ((rc = upload_tables(m))) return rc;
+        const int len = std::min(nsteps - done, predict_every - m->T % predict_every);
+        if ((rc = launch_steps(m, len, true))) return rc;
+        HIPCHK(hipMemcpyAsync(m->h_judge, m->d_judge, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+        const int J = *m->h_judge;
+        const int ran = J <= len ? J : len;
+        m->T += ran;
+        done += ran;
+        st[3] += ran;
+        if (J <= len) st[2]++;   // the judge found the interface on an under-refined block
+        if (m->T % predict_every != 0) {
+            if (J <= len) {
+                determine_min_max_levels(m);
+                m->regrid_changes++;
+                if ((rc = regrid(m, 0.002, 0.001))) return rc;
+            }
+            continue;
+        }
+        // the full regrid: the plain criterion and the predictions
+        determine_min_max_levels(m);
+        st[0]++;
+        const int n = nleaves(m);
+        if (m->has_net && (rc = predict_refinement(m))) return rc;
+        launch_regrid_check(m->dm, m->nblks, m->d_err_slot, m->prm.min_levels, m->prm.max_levels, m->levelmin,
+                            m->levelmax, 0.002, 0.001, m->d_changed, m->stream, m->has_net ? m->d_pred : nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(m->h_changed, m->d_changed, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+        const int changed = *m->h_changed;
+        HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int) * 2, m->stream));
+        if (!changed) {
+            clear_regrid_flags(m);
+            continue;
+        }
+        std::vector<char> extra;
+        if (m->has_net) {   // the device's flags, as the decision used them
+            std::vector<int> pred((size_t)m->nblks);
+            HIPCHK(hipMemcpyAsync(pred.data(), m->d_pred, sizeof(int) * m->nblks, hipMemcpyDeviceToHost, m->stream));
+            HIPCHK(hipStreamSynchronize(m->stream));
+            extra.assign(m->nblks + 1, 0);
+            for (int k = 0; k < n; k++) {
+                const int id = m->disp[k];
+                const Block &b = m->blk[id];
+                if (pred[id - 1] && b.level < max_level_of(m, b)) {
+                    extra[id] = 1;
+                    st[1]++;
+                }
+            }
+        }
+        m->regrid_changes++;
+        if ((rc = regrid(m, 0.002, 0.001, m->has_net ? &extra : nullptr))) return rc;
+    }
+    determine_min_max_levels(m);
+    return check_err_flags(m);
+}
+
 extern "C" {
+
+int amrlbm_set_regrid_network(amrlbm_mesh *m, const float *weights, double threshold)
+{
+    if (!m) return fail(AMRLBM_ERR_ARG, "null mesh");
+    if (!weights) {
+        m->has_net = false;
+        return AMRLBM_OK;
+    }
+    if (!m->d_net_w) {
+        HIPCHK(hipMalloc(&m->d_net_w, sizeof(float) * CNN_NPARAM));
+        HIPCHK(hipMalloc(&m->d_pred, sizeof(int) * (size_t)m->prm.max_blocks));
+        HIPCHK(hipMemsetAsync(m->d_pred, 0, sizeof(int) * (size_t)m->prm.max_blocks, m->stream));
+    }
+    HIPCHK(hipMemcpyAsync(m->d_net_w, weights, sizeof(float) * CNN_NPARAM, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    m->has_net = true;
+    m->net_threshold = (float)threshold;
+    return AMRLBM_OK;
+}
+
+int amrlbm_step_learned(amrlbm_mesh *m, int nsteps, int predict_every, long long stats[4])
+{
+    NEED_READY(m);
+    if (nsteps < 0 || predict_every < 1 || !stats) return fail(AMRLBM_ERR_ARG, "bad learned-step arguments");
+    HIPCHK(hipEventRecord(m->ev0, m->stream));
+    const int rc = step_learned(m, nsteps, predict_every, stats);
+    if (rc) {
+        (void)hipStreamSynchronize(m->stream);
+        (void)hipMemsetAsync(m->d_changed, 0, sizeof(int) * 2, m->stream);
+        (void)hipStreamSynchronize(m->stream);
+        (void)hipGetLastError();
+        return rc;
+    }
+    HIPCHK(hipEventRecord(m->ev1, m->stream));
+    HIPCHK(hipEventSynchronize(m->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    m->last_step_ms = ms;
+    return AMRLBM_OK;
+}
 
 int amrlbm_step(amrlbm_mesh *m, int nsteps, int regrid_every)
 {
@@ -3045,7 +3193,7 @@ int amrlbm_cnn_features(amrlbm_mesh *m, float *x, int *ids)
     const int n = nleaves(m);
     float *d = nullptr;
     HIPCHK(hipMalloc(&d, sizeof(float) * 3 * 18 * 18 * std::max(n, 1)));
-    launch_cnn_features(m->dm, all_leaves(m), n, d, m->stream);
+    launch_cnn_features(m->dm, all_leaves(m), n, d, m->ph.W, m->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(x, d, sizeof(float) * 3 * 18 * 18 * n, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));

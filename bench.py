@@ -23,10 +23,14 @@ Ghost-cell, neighbour-table and regrid traffic is overhead, not counted.
 
 fp32 repeats the measurement with the FP32 production form (fields stored in
 FP32 as deviations from the liquid at rest, FP32 collision; bytes per leaf
-cell halved) and states its error against the FP64 run after the same steps.
-fp32.learned_regrid runs the learned regrid (the committed network,
-amrlbm/regrid_cnn_weights.npy) on a held-out bubble against the same regrid
-cadence without the network and the plain regrid every 10 steps.
+cell halved) and states its error against the FP64 run after the same steps,
+beside the FP64 run from a set-up perturbed by 1e-7 (the flow's own
+sensitivity). fp32_learned is the metric's pair: FP32 with the learned regrid
+(the committed network, amrlbm/regrid_cnn_weights.npy; full regrid every 50
+steps with its predictions, the plain criterion judging every step on the
+device) on the same bubble, timed the same way, against plain FP64 and
+against the same cadence without the network. fp32.learned_regrid_cfg1
+deploys the network on the 128x256 bubble (the learned configuration).
 
 large_working_set repeats the measurement on 16 independent copies of the
 same domain in one mesh (about 4,600 leaf blocks, 0.5 GB of FP64 leaf state,
@@ -232,36 +236,77 @@ def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64, radius=No
     }
 
 
-def learned_regrid(amrlbm, w, steps=2000, variant=None):
-    """The learned regrid (committed network) on a held-out bubble (of the
-    same geometry by default), FP32, against the same cadence without the
-    network and the plain regrid every 10 steps; missed refinements are the
-    judge's checks that found the interface on an under-refined block."""
+def learned_regrid(amrlbm, w, steps=2000):
+    """Config 4: the committed network deployed on workload w (trained on the
+    generator of amrlbm.learned), FP32, full regrid every HORIZON steps with
+    the plain criterion judging every step on the device, against the same
+    cadence without the network and the plain regrid every 10 steps."""
     import numpy as np
     from amrlbm import learned
     if not os.path.exists(learned.WEIGHTS):
         return None
-    var = variant or learned.EVAL_VARIANT
-    ev = learned.evaluate(w, np.load(learned.WEIGHTS), steps, variant_=var)
-    out = {"workload": f"{w.name} geometry, bubble R={var[0]} at {tuple(var[1:])}, {steps} steps, "
-                       f"full regrid every {learned.HORIZON} steps, plain judge every 10",
-           "precision": "f32"}
+    ev = learned.evaluate(w, np.load(learned.WEIGHTS), steps)
+    out = {"workload": f"{w.name} bubble (D={int(w.D)}), {steps} steps, full regrid every {learned.HORIZON} steps, "
+                       f"plain criterion judging every step on the device", "precision": "f32",
+           "threshold": learned._threshold()}
     for k, v in ev.items():
         out[k] = {f: (round(x, 4) if isinstance(x, float) else x) for f, x in v.items() if f != "diag"}
     base = ev["plain_every_10"]["diag"]
     out["learned"]["mass_rel_vs_plain"] = abs(ev["learned"]["diag"][3] - base[3]) / abs(base[3])
     out["learned"]["rise_velocity_rel_vs_plain"] = abs(ev["learned"]["diag"][7] - base[7]) / abs(base[7])
-    out["learned"]["missed_refinement_rate"] = ev["learned"]["misses"] / max(ev["learned"]["checks"], 1)
     out["gain_mlups_vs_no_network"] = round(ev["learned"]["mlups"] / ev["no_network"]["mlups"], 3)
     out["gain_mlups_vs_plain_every_10"] = round(ev["learned"]["mlups"] / ev["plain_every_10"]["mlups"], 3)
+    return out
+
+
+def fp32_learned(amrlbm, w, steps, warmup, fp64_value, fp64_diag):
+    """The north star's pair: FP32 with the learned regrid on the headline
+    bubble, timed as measure() times (windows of exactly `steps` steps under
+    the learned cadence), with the same cadence without the network beside
+    it, and its error against the plain FP64 run after the same steps."""
+    import numpy as np
+    from amrlbm import learned
+    if not os.path.exists(learned.WEIGHTS):
+        return None
+    wts = np.load(learned.WEIGHTS)
+    out = {"workload": describe(w, learned.HORIZON, 32).replace(f"regrid every {learned.HORIZON}",
+                                                                  f"full regrid every {learned.HORIZON} with the "
+                                                                  f"network's predictions, plain criterion judging "
+                                                                  f"every step on the device"),
+           "unit": "MLUPS (finest-equivalent)"}
+    for name, net in (("learned", wts), ("no_network", None)):
+        with amrlbm.Mesh(w, precision=32) as m:
+            m.setup_rising_bubble()
+            learned.run(m, warmup, net)
+            m.synchronize()
+            walls, st = [], None
+            for _ in range(windows_for(steps)):
+                t0 = time.perf_counter()
+                s1 = learned.run(m, steps, net)
+                m.synchronize()
+                walls.append(time.perf_counter() - t0)
+                st = s1 if st is None else {f: st[f] + s1[f] for f in st}
+            walls.sort()
+            wall = walls[len(walls) // 2]
+            e = {"value": round(w.fine_cells * steps / wall / 1e6, 1), "ms_per_step": round(wall * 1e3 / steps, 4),
+                 "windows": len(walls), "leaf_blocks": m.counts()["nleaf"]}
+            e.update(st)
+            e["missed_refinement_rate"] = st["misses"] / max(st["checks"], 1)
+            if name == "learned":
+                d = m.output()
+                e["error_vs_fp64"] = {"mass_rel": abs(d[3] - fp64_diag[3]) / abs(fp64_diag[3]),
+                                      "rise_velocity_rel": abs(d[7] - fp64_diag[7]) / abs(fp64_diag[7])}
+            out[name] = e
+    out["speedup_vs_plain_fp64"] = round(out["learned"]["value"] / fp64_value, 3)
+    out["gain_vs_no_network"] = round(out["learned"]["value"] / out["no_network"]["value"], 3)
     return out
 
 
 def multi_bubble(amrlbm, steps, k):
     """The 32-bubble case (amrlbm.multibubble): FP64, FP32 and FP32 with the
     learned regrid (full regrid every 50 steps, plain criterion judging every
-    10) from the same device set-up, and the FP32 runs' total mass and
-    per-bubble centroids against FP64's."""
+    step on the device) from the same device set-up, and the FP32 runs' total
+    mass and per-bubble centroids against FP64's."""
     import numpy as np
     from amrlbm import learned
     from amrlbm import multibubble as mb
@@ -454,11 +499,10 @@ def main():
                     "rise_velocity_rel": abs(dp[7] - diag[7]) / abs(diag[7]),
                     "centroid_abs_cells": max(abs(dp[5] - diag[5]), abs(dp[6] - diag[6]))},
                 "targets": {"mass_rel": 1e-6, "rise_velocity_rel": 5e-3}},
-            "learned_regrid": learned_regrid(amrlbm, w),
-            # the network trained on the 1024x2048 geometry, deployed on the 128x256 bubble (D=32 at (64, 64))
-            "learned_regrid_cfg1": learned_regrid(amrlbm, amrlbm.WORKLOADS["cfg1"],
-                                                  variant=(16.0, 64.0, 64.0)),
+            # config 4: the network trained on the generator, deployed on the 128x256 bubble (D=32 at (64, 64))
+            "learned_regrid_cfg1": learned_regrid(amrlbm, amrlbm.WORKLOADS["cfg1"]),
         }
+        out["fp32_learned"] = fp32_learned(amrlbm, w, args.steps, args.warmup, r["value"], diag)
     if args.large_workload:
         wl = amrlbm.WORKLOADS[args.large_workload]
         warm = max(args.warmup // 5, 10)

@@ -243,8 +243,10 @@ int amrlbm_regrid_stats(amrlbm_mesh *m, double out[8]);
 
 /* ---- learned regrid predictor (no counterpart in the reference) ----------
  * A 3-layer 3x3 convolutional network (16 channels, BF16 weights on MFMA,
- * FP32 accumulate) over one leaf block's C (18 x 18, first ghost ring) and
- * Ux, Uy (interior, times 10, zero ring): input rows of 3 x 18 x 18 floats;
+ * FP32 accumulate) over one leaf block's phase field as a signed distance
+ * to the interface, (W/4) ln(C/(1-C)) in cells clamped to +-20 and divided by
+ * 16 (18 x 18, first ghost ring), and Ux, Uy (interior, times 10, zero
+ * ring): input rows of 3 x 18 x 18 floats;
  * output two logits per block: its region refined / derefined k steps ahead.
  * weights: AMRLBM_CNN_NPARAM floats (conv1 16x27, b1, conv2 16x144, b2,
  * conv3 2x144, b3; row k of a 3x3 filter is tap * cin_count + cin, tap =
@@ -262,6 +264,20 @@ int amrlbm_cnn_train(const float *x, const float *y, int n, float *weights, int 
                      float pos_weight, unsigned seed, float *epoch_loss);
 /* the network input of every leaf of the mesh: x (nLeaf rows), ids (block ids) */
 int amrlbm_cnn_features(amrlbm_mesh *m, float *x, int *ids);
+/* The mesh's learned regrid predictor for amrlbm_step_learned: weights as
+ * above (copied to the device), a block flagged when logit 0 > threshold;
+ * weights NULL removes it. */
+int amrlbm_set_regrid_network(amrlbm_mesh *m, const float *weights, double threshold);
+/* nsteps steps under the learned regrid's cadence, all decisions on the
+ * device: the plain criterion (Refine_Derefine.f90:18-54) judges every step's
+ * state, and a state with the interface on an under-refined block is
+ * regridded at once (Refine_Derefine_Mesh, a miss); at every multiple of
+ * predict_every steps (T counted as amrlbm_step counts it) Refine_Derefine_Mesh
+ * runs with the blocks the network flags refined too (amrlbm_set_regrid_network;
+ * without one, the plain criterion alone). The host works only when the mesh
+ * changes. stats: [0] full regrids, [1] blocks refined on the network's flag
+ * alone, [2] misses (judge regrids), [3] steps judged (accumulated). */
+int amrlbm_step_learned(amrlbm_mesh *m, int nsteps, int predict_every, long long stats[4]);
 
 /* ---- per-block kernels on caller (host) arrays, reference layouts --------- */
 int amrlbm_collision_block(const amrlbm_params *p, int dx, const double *C, const double *P,
