@@ -125,7 +125,7 @@ def lib():
             "amrlbm_cnn_train": [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_float), ip, ip, ctypes.c_float, ctypes.c_float, ctypes.c_uint, ctypes.POINTER(ctypes.c_float)],
             "amrlbm_cnn_features": [vp, ctypes.POINTER(ctypes.c_float), _IP],
             "amrlbm_regrid_check": [vp, _IP],
-            "amrlbm_set_regrid_network": [vp, ctypes.POINTER(ctypes.c_float), dp],
+            "amrlbm_set_regrid_network": [vp, ctypes.POINTER(ctypes.c_float), dp, dp],
             "amrlbm_step_learned": [vp, ip, ip, ctypes.POINTER(ctypes.c_longlong)],
             "amrlbm_refine_derefine_predicted": [vp, _IP, ip],
             "amrlbm_load_snapshot": [vp, ctypes.c_char_p],
@@ -399,20 +399,20 @@ class Mesh:
         ids = np.ascontiguousarray(ids, dtype=np.int32)
         _check(lib().amrlbm_refine_derefine_predicted(self.h, _i(ids), int(ids.size)))
 
-    def set_regrid_network(self, weights, threshold=0.0):
+    def set_regrid_network(self, weights, threshold=0.0, gate=0.25):
         """The learned predictor amrlbm_step_learned uses (None: none)."""
         if weights is None:
-            _check(lib().amrlbm_set_regrid_network(self.h, None, 0.0))
+            _check(lib().amrlbm_set_regrid_network(self.h, None, 0.0, 0.0))
             return
         w = np.ascontiguousarray(weights, dtype=np.float32)
         assert w.size == CNN_NPARAM
-        _check(lib().amrlbm_set_regrid_network(self.h, _f(w), float(threshold)))
+        _check(lib().amrlbm_set_regrid_network(self.h, _f(w), float(threshold), float(gate)))
 
-    def step_learned(self, nsteps, predict_every=50):
+    def step_learned(self, nsteps, full_every=50):
         """nsteps steps under the learned regrid's cadence (amrlbm_step_learned):
         {full_regrids, predicted_blocks, misses, judged}."""
         st = (ctypes.c_longlong * 4)()
-        _check(lib().amrlbm_step_learned(self.h, int(nsteps), int(predict_every), st))
+        _check(lib().amrlbm_step_learned(self.h, int(nsteps), int(full_every), st))
         return {"full_regrids": st[0], "predicted_blocks": st[1], "misses": st[2], "judged": st[3]}
 
     def cnn_features(self):

@@ -5,8 +5,11 @@ enter are refined before it arrives and the full regrid can run every
 HORIZON steps. The plain gradient criterion (Refine_Derefine.f90:18-54,
 :327-352) stays the judge: the library tests it on the device at every
 step's state, and the interface on an under-refined block triggers an
-immediate plain regrid (a miss). amrlbm_step_learned runs that cadence with
-every decision on the device; the host works only when the mesh changes.
+immediate plain regrid (a miss). The network never changes the mesh on its
+own: its flags join every regrid that runs (full or miss), refining ahead
+the blocks the interface is about to reach, which removes the misses they
+would have caused. amrlbm_step_learned runs that cadence with every
+decision on the device; the host works only when the mesh changes.
 
 The network, its training and inference are the library's HIP kernels
 (amrlbm_cnn_*, amrlbm_set_regrid_network); this module builds the training
@@ -109,18 +112,18 @@ def generate(w, variants, steps, every=HORIZON, horizon=HORIZON, regrid_every=10
     return np.concatenate(X), np.concatenate(Y)
 
 
-def run(mesh, nsteps, weights=None, predict_every=HORIZON, threshold=None):
+def run(mesh, nsteps, weights=None, full_every=HORIZON, threshold=None):
     """Advance the mesh nsteps under the learned regrid's cadence
-    (amrlbm_step_learned): the full regrid every predict_every steps with the
-    blocks the network flags (weights; none: the plain criterion alone) and
-    the plain criterion judging every step on the device."""
+    (amrlbm_step_learned): the full regrid every full_every steps, the plain
+    criterion judging every step on the device, and every regrid refining the
+    blocks the network flags too (weights; none: the plain criterion alone)."""
     mesh.set_regrid_network(weights, _threshold() if threshold is None else threshold)
-    st = mesh.step_learned(nsteps, predict_every)
+    st = mesh.step_learned(nsteps, full_every)
     return {"predicted_regrids": st["full_regrids"], "predicted_blocks": st["predicted_blocks"],
             "misses": st["misses"], "checks": st["judged"]}
 
 
-def evaluate(w, weights, steps, precision=32, variant_=None, predict_every=HORIZON):
+def evaluate(w, weights, steps, precision=32, variant_=None, full_every=HORIZON):
     """The learned regrid against the same cadence without the network, and
     against the plain regrid every 10 steps, on workload w (a bubble variant
     of it with variant_ = (radius, x0, y0))."""
@@ -132,7 +135,7 @@ def evaluate(w, weights, steps, precision=32, variant_=None, predict_every=HORIZ
             m.step(1, 1 << 30)   # C's ghost ring and U, P of a stepped state, as every regrid sees them
             m.synchronize()
             t0 = time.perf_counter()
-            st = run(m, steps, wts, predict_every)
+            st = run(m, steps, wts, full_every)
             m.synchronize()
             st["wall_s"] = time.perf_counter() - t0
             st["mlups"] = wv.fine_cells * steps / st["wall_s"] / 1e6

@@ -204,7 +204,8 @@ void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream
 void launch_err(const DevMesh &m, const int *list, int n, double *err, double *err_slot, hipStream_t s);
 void launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, int min_levels, int max_levels,
                          int levelmin, int levelmax, double refin_tol, double deref_tol, int *changed,
-                         hipStream_t s, const int *pred = nullptr);
+                         hipStream_t s, const int *judge = nullptr, int jstop = 0);
+void launch_judge_check(const int *judge, int jstop, int *changed, hipStream_t s);
 // pred[list[k]] = logits[2 k] > threshold and err_slot[list[k]] > gate (the
 // learned predictor's refinement flags by block slot)
 void launch_pred_flags(const float *logits, const int *list, int n, float threshold, const double *err_slot,

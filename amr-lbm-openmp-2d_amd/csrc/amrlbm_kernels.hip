@@ -724,19 +724,32 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMeshT<T> m, int cur, const in
             const T x = __shfl_xor(dc, o);
             if (x > dc) dc = x;
         }
-        __shared__ T wmax[4];
-        if ((tid & 63) == 0) wmax[tid >> 6] = dc;
-        __syncthreads();
-        if (tid == 0) {
-            T e = wmax[0];
-            for (int w = 1; w < 4; w++)
-                if (wmax[w] > e) e = wmax[w];
-            const double err = (double)(sqrt(e) / (T(2) * (T)me.dx));
-            if (err_slot) err_slot[slot] = err;
-            if (ctl.judge_on) {
-                // the judge: Flag_for_refinement's criterion (Refine_Derefine.f90:18-54) on this leaf
+        if (!err_slot) {
+            // the judge alone: sqrt (correctly rounded), the division by 2 dx (a
+            // power of two) and the widening are monotone, so the block's
+            // Err_Interface passes the tolerance iff one wavefront's maximum
+            // does, and each wavefront tests its own with no barrier
+            if ((tid & 63) == 0) {
+                const double err = (double)(sqrt(dc) / (T(2) * (T)me.dx));
                 const int lo = ENS ? m.lv[2 * me.inst] : ctl.min_level, hi = ENS ? m.lv[2 * me.inst + 1] : ctl.max_level;
                 if (me.level < lo || (me.level < hi && err > ctl.refin_tol)) atomicMin(ctl.judge, ctl.ord);
+            }
+        } else {
+            __shared__ T wmax[4];
+            if ((tid & 63) == 0) wmax[tid >> 6] = dc;
+            __syncthreads();
+            if (tid == 0) {
+                T e = wmax[0];
+                for (int w = 1; w < 4; w++)
+                    if (wmax[w] > e) e = wmax[w];
+                const double err = (double)(sqrt(e) / (T(2) * (T)me.dx));
+                err_slot[slot] = err;
+                if (ctl.judge_on) {
+                    // the judge: Flag_for_refinement's criterion (Refine_Derefine.f90:18-54) on this leaf
+                    const int lo = ENS ? m.lv[2 * me.inst] : ctl.min_level,
+                              hi = ENS ? m.lv[2 * me.inst + 1] : ctl.max_level;
+                    if (me.level < lo || (me.level < hi && err > ctl.refin_tol)) atomicMin(ctl.judge, ctl.ord);
+                }
             }
         }
     }
@@ -916,16 +929,27 @@ __global__ __launch_bounds__(256) void k_err(DevMeshT<T> m, const int *__restric
 // its non-leaf watch blocks once, and the passes only test LDS bits.
 // One workgroup. More than RC_CAND candidates: answers "changed" and leaves
 // the decision to the host. changed[1] = 1 iff any refinement flag is set.
+// With judge (the stop ordinal of a judged segment): changed[2] = *judge, and
+// a stop before jstop (a state inside the segment needed refinement, and the
+// segment's last state was never reached) answers "changed, refinement".
 constexpr int RC_CAND = 512;
 __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int nblks, const double *__restrict__ err_slot,
                                                       int min_levels, int max_levels, int levelmin, int levelmax,
                                                       double refin_tol, double deref_tol, int *changed,
-                                                      const int *__restrict__ pred)
+                                                      const int *__restrict__ judge, int jstop)
 {
     extern __shared__ unsigned dmask[];
     __shared__ int cand[RC_CAND], clev[RC_CAND], watch[RC_CAND * 16];
     __shared__ int ncand;
     const int tid = threadIdx.x, nw = (nblks + 31) >> 5;
+    if (judge) {
+        const int J = *judge;
+        if (tid == 0) changed[2] = J;
+        if (J < jstop) {
+            if (tid == 0) changed[0] = changed[1] = 1;
+            return;
+        }
+    }
     for (int w = tid; w < nw; w += 1024) dmask[w] = 0u;
     if (tid == 0) ncand = 0;
     __syncthreads();
@@ -934,9 +958,7 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
         const BlockDev b = m.blk[n];
         if (b.level < 0) continue;
         const int lo = m.lv ? m.lv[2 * b.inst] : min_levels, hi = m.lv ? m.lv[2 * b.inst + 1] : max_levels;
-        // the plain criterion, and with pred the blocks the learned predictor
-        // flags (Refine_Derefine_Mesh with extra flags: the same closure follows)
-        if (b.isleaf && (b.level < lo || (b.level < hi && (err_slot[n] > refin_tol || (pred && pred[n]))))) ref = 1;
+        if (b.isleaf && (b.level < lo || (b.level < hi && err_slot[n] > refin_tol))) ref = 1;
         if (!b.hasleaf || b.level < lo || b.child0 < 0) continue;
         bool d = true;
         for (int c = b.child0; c < b.child0 + 4; c++) {
@@ -1706,11 +1728,25 @@ void launch_err(const DevMesh &m, const int *list, int n, double *err, double *e
 
 void launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, int min_levels, int max_levels,
                          int levelmin, int levelmax, double refin_tol, double deref_tol, int *changed,
-                         hipStream_t s, const int *pred)
+                         hipStream_t s, const int *judge, int jstop)
 {
     const size_t lds = sizeof(unsigned) * (size_t)((nblks + 31) / 32 + 1);
     hipLaunchKernelGGL(k_regrid_check, dim3(1), dim3(1024), lds, s, typed<double>(m), nblks, err_slot, min_levels,
-                       max_levels, levelmin, levelmax, refin_tol, deref_tol, changed, pred);
+                       max_levels, levelmin, levelmax, refin_tol, deref_tol, changed, judge, jstop);
+}
+
+// the decision after a judged segment that ends between full regrids:
+// changed[2] = *judge, and "changed, refinement" iff the judge stopped it
+__global__ void k_judge_check(const int *judge, int jstop, int *changed)
+{
+    const int J = *judge;
+    changed[2] = J;
+    if (J < jstop) changed[0] = changed[1] = 1;
+}
+
+void launch_judge_check(const int *judge, int jstop, int *changed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_judge_check, dim3(1), dim3(1), 0, s, judge, jstop, changed);
 }
 
 // the learned predictor's refinement flag per block slot: logit 0 of leaf k

@@ -120,7 +120,6 @@ struct amrlbm_mesh {
     int *d_err_flags = nullptr;      // [0] a fill met a level jump
     unsigned *d_sync = nullptr;      // the segment kernel's grid barrier words (fault word at 15)
     int *d_judge = nullptr;          // the device judge's stop ordinal (enqueue_segment with judge)
-    int *h_judge = nullptr;          // pinned copy of it
     // the learned regrid predictor of amrlbm_step_learned: its weights, the
     // network inputs and logits of every leaf, its refinement flags by slot
     float *d_net_w = nullptr, *d_net_x = nullptr, *d_net_logit = nullptr;
@@ -128,6 +127,7 @@ struct amrlbm_mesh {
     int net_cap = 0;
     bool has_net = false;
     float net_threshold = 0.0f;
+    double net_gate = 0.25;          // Err_Interface a predicted block must reach, in refinement tolerances
     int *d_lists = nullptr;          // leaves grouped by level, then all leaves (ascending)
     int lists_cap = 0;
     int *d_pairs = nullptr;          // restrict / prolong work lists
@@ -135,7 +135,7 @@ struct amrlbm_mesh {
     double *d_err = nullptr;         // Err_Interface per leaf
     double *d_err_slot = nullptr;    // Err_Interface per block slot (leaves)
     int *d_changed = nullptr;        // [0] the device regrid decision (the step kernels' skip word),
-                                     // [1] any refinement flag
+                                     // [1] any refinement flag, [2] a judged segment's stop ordinal
     int *h_changed = nullptr;        // pinned copy of it
     double *d_part = nullptr;        // diagnostics partials
     double *d_diag = nullptr;
@@ -1166,8 +1166,10 @@ int phase_macro_g(amrlbm_mesh *m)
 // ---------------------------------------------------------------------------
 // extra: blocks flagged for refinement besides the criterion; frozen: the
 // instances of an ensemble mesh left as they are (no flags)
+// d_pred: the learned predictor's flags by block slot on the device (read with
+// Err_Interface); *pred_count grows by the leaves refined on them alone
 int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector<char> *extra = nullptr,
-           const std::vector<char> *frozen = nullptr)
+           const std::vector<char> *frozen = nullptr, const int *d_pred = nullptr, long long *pred_count = nullptr)
 {
     int rc = upload_tables(m);
     if (rc) return rc;
@@ -1177,6 +1179,8 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
     launch_err(m->dm, m->d_lists + m->all_off, nleaf, m->d_err, nullptr, m->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(m->h_pinned, m->d_err, sizeof(double) * nleaf, hipMemcpyDeviceToHost, m->stream));
+    std::vector<int> pred(d_pred ? m->nblks : 0);
+    if (d_pred) HIPCHK(hipMemcpyAsync(pred.data(), d_pred, sizeof(int) * m->nblks, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
     std::vector<double> err(m->nblks + 1, 0.0);
     for (int k = 0; k < nleaf; k++) err[m->disp[k]] = m->h_pinned[k];
@@ -1190,7 +1194,10 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
             const int lo = min_level_of(m, b), hi = max_level_of(m, b);
             if (b.level < lo) b.refin = true;
             else if (b.level < hi && err[n] > refin_tol) b.refin = true;
-            else if (extra && b.level < hi && (*extra)[n]) b.refin = true;   // predicted
+            else if (b.level < hi && ((extra && (*extra)[n]) || (d_pred && pred[n - 1]))) {   // predicted
+                b.refin = true;
+                if (pred_count) ++*pred_count;
+            }
         }
     }
     // Enforce_1_Level_Jump / Check_Level_Jump (:58-93): least fixed point
@@ -1382,13 +1389,12 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipMemsetAsync(m->d_err_flags, 0, sizeof(int), m->stream));
     HIPCHK(hipMalloc(&m->d_sync, sizeof(unsigned) * kSegSyncWords));
     HIPCHK(hipMalloc(&m->d_judge, sizeof(int)));
-    HIPCHK(hipHostMalloc(&m->h_judge, sizeof(int), hipHostMallocDefault));
     HIPCHK(hipMemsetAsync(m->d_sync, 0, sizeof(unsigned) * kSegSyncWords, m->stream));
     HIPCHK(hipMalloc(&m->d_err, sizeof(double) * S));
     HIPCHK(hipMalloc(&m->d_err_slot, sizeof(double) * S));
-    HIPCHK(hipMalloc(&m->d_changed, sizeof(int) * 2));
-    HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int) * 2, m->stream));
-    HIPCHK(hipHostMalloc(&m->h_changed, sizeof(int) * 2, hipHostMallocDefault));
+    HIPCHK(hipMalloc(&m->d_changed, sizeof(int) * 4));
+    HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int) * 4, m->stream));
+    HIPCHK(hipHostMalloc(&m->h_changed, sizeof(int) * 4, hipHostMallocDefault));
     HIPCHK(hipEventCreateWithFlags(&m->ev_check, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&m->ev_tab, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&m->ev_pairs, hipEventDisableTiming));
@@ -1729,7 +1735,6 @@ int amrlbm_destroy(amrlbm_mesh *m)
     (void)hipFree(m->d_err_flags);
     (void)hipFree(m->d_sync);
     (void)hipFree(m->d_judge);
-    if (m->h_judge) (void)hipHostFree(m->h_judge);
     (void)hipFree(m->d_net_w);
     (void)hipFree(m->d_net_x);
     (void)hipFree(m->d_net_logit);
@@ -2695,9 +2700,11 @@ static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahea
     return check_err_flags(m);
 }
 
-// The learned predictor's refinement flags of every leaf at the current state
-// (network inputs, forward pass and flags by slot, on the device).
-static int predict_refinement(amrlbm_mesh *m)
+// The learned predictor's refinement flags of every leaf at the current state,
+// by block slot in d_pred (network inputs, forward pass and flags, on the
+// device), for the regrid that follows. fresh_err: d_err_slot already holds
+// Err_Interface of this state (a segment's end kernel ran on it).
+static int enqueue_prediction(amrlbm_mesh *m, bool fresh_err)
 {
     const int n = nleaves(m);
     if (n > m->net_cap) {
@@ -2709,81 +2716,101 @@ static int predict_refinement(amrlbm_mesh *m)
         HIPCHK(hipMalloc(&m->d_net_logit, sizeof(float) * 2 * (size_t)cap));
         m->net_cap = cap;
     }
+    if (!fresh_err) launch_err(m->dm, all_leaves(m), n, m->d_err, m->d_err_slot, m->stream);
     launch_cnn_features(m->dm, all_leaves(m), n, m->d_net_x, m->ph.W, m->stream);
     launch_cnn(m->d_net_x, nullptr, n, m->d_net_w, m->d_net_logit, nullptr, 0.f, nullptr, nullptr, m->stream);
     // a prediction counts where the interface's profile already reaches the
-    // block (Err_Interface above a quarter of the refinement tolerance), so
-    // that refined children of a predicted block, which no training state
-    // held, do not call for the next level in turn
-    launch_pred_flags(m->d_net_logit, all_leaves(m), n, m->net_threshold, m->d_err_slot, 0.25 * 0.002, m->d_pred,
-                      m->stream);
+    // block (Err_Interface above net_gate times the refinement tolerance)
+    launch_pred_flags(m->d_net_logit, all_leaves(m), n, m->net_threshold, m->d_err_slot, m->net_gate * 0.002,
+                      m->d_pred, m->stream);
     HIPCHK(hipGetLastError());
     return AMRLBM_OK;
 }
 
-// The learned regrid's cadence (no counterpart in the reference): segments up
-// to the next multiple of predict_every with the plain criterion judging every
-// state on the device; a state it finds needing refinement is regridded at
-// once (a miss). At each multiple of predict_every, the regrid decision takes
-// the plain criterion plus the network's flags (when a network is set) and
-// runs Refine_Derefine_Mesh with them when the mesh changes.
-static int step_learned(amrlbm_mesh *m, int nsteps, int predict_every, long long st[4])
+// The decision after a judged segment of len steps that ends at state T: at a
+// full regrid (T a multiple of the full-regrid interval) the plain criterion
+// over every leaf, unless the judge stopped the segment before its last
+// state; between full regrids, the judge alone. Its copy (decision, any
+// refinement, stop ordinal) reaches h_changed when ev_check fires.
+static int enqueue_learned_check(amrlbm_mesh *m, int len, bool full)
 {
-    int rc = AMRLBM_OK, done = 0;
-    while (done < nsteps) {
-        determine_min_max_levels(m);
-        if ((rc = upload_tables(m))) return rc;
-        const int len = std::min(nsteps - done, predict_every - m->T % predict_every);
-        if ((rc = launch_steps(m, len, true))) return rc;
-        HIPCHK(hipMemcpyAsync(m->h_judge, m->d_judge, sizeof(int), hipMemcpyDeviceToHost, m->stream));
-        HIPCHK(hipStreamSynchronize(m->stream));
-        const int J = *m->h_judge;
-        const int ran = J <= len ? J : len;
-        m->T += ran;
-        done += ran;
-        st[3] += ran;
-        if (J <= len) st[2]++;   // the judge found the interface on an under-refined block
-        if (m->T % predict_every != 0) {
-            if (J <= len) {
+    if (full)
+        launch_regrid_check(m->dm, m->nblks, m->d_err_slot, m->prm.min_levels, m->prm.max_levels, m->levelmin,
+                            m->levelmax, 0.002, 0.001, m->d_changed, m->stream, m->d_judge, len);
+    else
+        launch_judge_check(m->d_judge, len + 1, m->d_changed, m->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(m->h_changed, m->d_changed, sizeof(int) * 3, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipEventRecord(m->ev_check, m->stream));
+    m->regrid_checks++;
+    return AMRLBM_OK;
+}
+
+// The learned regrid's cadence (no counterpart in the reference). Full
+// regrids every full_every steps take the plain criterion; between them the
+// plain criterion judges every state on the device, and the first state it
+// finds needing refinement is regridded at once (a miss). The network never
+// changes the mesh on its own: its flags join every regrid that does happen,
+// so that blocks the interface is about to reach are refined ahead and the
+// misses they would cause do not come. Stream order as in amrlbm_step:
+// segment r, decision r, segment r+1, ...; the host reads decision r while
+// segment r+1 runs, and a segment behind a "changed" decision skips all its
+// work and is taken back. *ahead_len as in step_segments.
+static int step_learned(amrlbm_mesh *m, int nsteps, int full_every, long long st[4], int *ahead_len)
+{
+    int rc = AMRLBM_OK, done = 0, plen = 0;
+    bool pending = false, pfull = false;
+    while (done < nsteps || pending) {
+        bool launched = false;
+        int len = 0;
+        if (done < nsteps) {
+            len = std::min(nsteps - done, full_every - m->T % full_every);
+            if (!pending) {
+                determine_min_max_levels(m);
+                if ((rc = upload_tables(m))) return rc;
+            }
+            if ((rc = launch_steps(m, len, true))) return rc;
+            m->T += len;
+            done += len;
+            launched = true;
+            if (pending) *ahead_len = len;
+        }
+        if (pending) {
+            pending = false;
+            HIPCHK(hipEventSynchronize(m->ev_check));
+            const int J = m->h_changed[2];
+            const int ran = J < plen ? J : plen;   // steps the decided segment ran
+            st[3] += ran;
+            if (J <= plen) st[2]++;                  // the judge found a leaf needing refinement
+            const bool at_full = pfull && J >= plen;
+            if (at_full) st[0]++;
+            if (m->h_changed[0]) {
+                HIPCHK(hipStreamSynchronize(m->stream));
+                if (launched) {
+                    m->T -= len;
+                    done -= len;
+                    m->segments_skipped++;
+                }
+                *ahead_len = 0;
+                m->T -= plen - ran;
+                done -= plen - ran;
                 determine_min_max_levels(m);
                 m->regrid_changes++;
-                if ((rc = regrid(m, 0.002, 0.001))) return rc;
+                if (m->has_net && (rc = enqueue_prediction(m, at_full))) return rc;
+                if ((rc = regrid(m, 0.002, 0.001, nullptr, nullptr, m->has_net ? m->d_pred : nullptr, &st[1])))
+                    return rc;
+                HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int), m->stream));
+                continue;
             }
-            continue;
-        }
-        // the full regrid: the plain criterion and the predictions
-        determine_min_max_levels(m);
-        st[0]++;
-        const int n = nleaves(m);
-        if (m->has_net && (rc = predict_refinement(m))) return rc;
-        launch_regrid_check(m->dm, m->nblks, m->d_err_slot, m->prm.min_levels, m->prm.max_levels, m->levelmin,
-                            m->levelmax, 0.002, 0.001, m->d_changed, m->stream, m->has_net ? m->d_pred : nullptr);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(m->h_changed, m->d_changed, sizeof(int), hipMemcpyDeviceToHost, m->stream));
-        HIPCHK(hipStreamSynchronize(m->stream));
-        const int changed = *m->h_changed;
-        HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int) * 2, m->stream));
-        if (!changed) {
+            *ahead_len = 0;
             clear_regrid_flags(m);
-            continue;
         }
-        std::vector<char> extra;
-        if (m->has_net) {   // the device's flags, as the decision used them
-            std::vector<int> pred((size_t)m->nblks);
-            HIPCHK(hipMemcpyAsync(pred.data(), m->d_pred, sizeof(int) * m->nblks, hipMemcpyDeviceToHost, m->stream));
-            HIPCHK(hipStreamSynchronize(m->stream));
-            extra.assign(m->nblks + 1, 0);
-            for (int k = 0; k < n; k++) {
-                const int id = m->disp[k];
-                const Block &b = m->blk[id];
-                if (pred[id - 1] && b.level < max_level_of(m, b)) {
-                    extra[id] = 1;
-                    st[1]++;
-                }
-            }
+        if (launched) {
+            plen = len;
+            pfull = m->T % full_every == 0;
+            if ((rc = enqueue_learned_check(m, len, pfull))) return rc;
+            pending = true;
         }
-        m->regrid_changes++;
-        if ((rc = regrid(m, 0.002, 0.001, m->has_net ? &extra : nullptr))) return rc;
     }
     determine_min_max_levels(m);
     return check_err_flags(m);
@@ -2791,9 +2818,10 @@ static int step_learned(amrlbm_mesh *m, int nsteps, int predict_every, long long
 
 extern "C" {
 
-int amrlbm_set_regrid_network(amrlbm_mesh *m, const float *weights, double threshold)
+int amrlbm_set_regrid_network(amrlbm_mesh *m, const float *weights, double threshold, double gate)
 {
     if (!m) return fail(AMRLBM_ERR_ARG, "null mesh");
+    if (!(gate >= 0.0)) return fail(AMRLBM_ERR_ARG, "bad prediction gate");
     if (!weights) {
         m->has_net = false;
         return AMRLBM_OK;
@@ -2807,17 +2835,21 @@ int amrlbm_set_regrid_network(amrlbm_mesh *m, const float *weights, double thres
     HIPCHK(hipStreamSynchronize(m->stream));
     m->has_net = true;
     m->net_threshold = (float)threshold;
+    m->net_gate = gate;
     return AMRLBM_OK;
 }
 
-int amrlbm_step_learned(amrlbm_mesh *m, int nsteps, int predict_every, long long stats[4])
+int amrlbm_step_learned(amrlbm_mesh *m, int nsteps, int full_every, long long stats[4])
 {
     NEED_READY(m);
-    if (nsteps < 0 || predict_every < 1 || !stats) return fail(AMRLBM_ERR_ARG, "bad learned-step arguments");
+    if (nsteps < 0 || full_every < 1 || !stats) return fail(AMRLBM_ERR_ARG, "bad learned-step arguments");
     HIPCHK(hipEventRecord(m->ev0, m->stream));
-    const int rc = step_learned(m, nsteps, predict_every, stats);
+    int ahead_len = 0;
+    const int rc = step_learned(m, nsteps, full_every, stats, &ahead_len);
     if (rc) {
+        // as in amrlbm_step: no decision left behind, a skipped segment taken back
         (void)hipStreamSynchronize(m->stream);
+        if (ahead_len && *m->h_changed) m->T -= ahead_len;
         (void)hipMemsetAsync(m->d_changed, 0, sizeof(int) * 2, m->stream);
         (void)hipStreamSynchronize(m->stream);
         (void)hipGetLastError();

@@ -265,19 +265,23 @@ int amrlbm_cnn_train(const float *x, const float *y, int n, float *weights, int 
 /* the network input of every leaf of the mesh: x (nLeaf rows), ids (block ids) */
 int amrlbm_cnn_features(amrlbm_mesh *m, float *x, int *ids);
 /* The mesh's learned regrid predictor for amrlbm_step_learned: weights as
- * above (copied to the device), a block flagged when logit 0 > threshold;
- * weights NULL removes it. */
-int amrlbm_set_regrid_network(amrlbm_mesh *m, const float *weights, double threshold);
+ * above (copied to the device), a leaf flagged when logit 0 > threshold and
+ * its Err_Interface exceeds gate times the refinement tolerance (gate >= 0;
+ * the interface's profile already reaches the block); weights NULL removes it. */
+int amrlbm_set_regrid_network(amrlbm_mesh *m, const float *weights, double threshold, double gate);
 /* nsteps steps under the learned regrid's cadence, all decisions on the
- * device: the plain criterion (Refine_Derefine.f90:18-54) judges every step's
- * state, and a state with the interface on an under-refined block is
- * regridded at once (Refine_Derefine_Mesh, a miss); at every multiple of
- * predict_every steps (T counted as amrlbm_step counts it) Refine_Derefine_Mesh
- * runs with the blocks the network flags refined too (amrlbm_set_regrid_network;
- * without one, the plain criterion alone). The host works only when the mesh
- * changes. stats: [0] full regrids, [1] blocks refined on the network's flag
- * alone, [2] misses (judge regrids), [3] steps judged (accumulated). */
-int amrlbm_step_learned(amrlbm_mesh *m, int nsteps, int predict_every, long long stats[4]);
+ * device: at every multiple of full_every steps (T counted as amrlbm_step
+ * counts it) the plain criterion (Refine_Derefine.f90:18-54, :97-194) decides
+ * whether Refine_Derefine_Mesh changes the mesh; between them it judges every
+ * step's state, and a state with the interface on an under-refined block is
+ * regridded at once (a miss). The network (amrlbm_set_regrid_network) never
+ * changes the mesh on its own: every regrid that does run also refines the
+ * leaves it flags, so that the blocks the interface is about to reach are
+ * refined ahead of it. Segments run back to back as in amrlbm_step; the host
+ * works only when the mesh changes. stats (accumulated): [0] full regrids
+ * reached, [1] leaves refined on the network's flag alone, [2] misses,
+ * [3] steps judged. */
+int amrlbm_step_learned(amrlbm_mesh *m, int nsteps, int full_every, long long stats[4]);
 
 /* ---- per-block kernels on caller (host) arrays, reference layouts --------- */
 int amrlbm_collision_block(const amrlbm_params *p, int dx, const double *C, const double *P,
