@@ -434,12 +434,12 @@ class Mesh:
         """Device regrid decisions taken by step(), how many changed the mesh,
         how many segments launched ahead of a change were taken back, and the
         host time spent on regrids, tables, ghost programs and graphs."""
-        out = (ctypes.c_double * 8)()
+        out = (ctypes.c_double * 9)()
         _check(lib().amrlbm_regrid_stats(self.h, out))
         return {"checks": int(out[0]), "changes": int(out[1]), "segments_taken_back": int(out[2]),
                 "host_ms": {"regrid": round(out[3], 3), "tables": round(out[4], 3),
                             "programs": round(out[5], 3), "graphs": round(out[6], 3)},
-                "graphs_built": int(out[7])}
+                "graphs_built": int(out[7]), "device_flag_changes": int(out[8])}
 
     def set_tuning(self, order, depth, execution=0):
         """Launch variant (amrlbm_set_tuning): leaf order 0-3, hg operand depth

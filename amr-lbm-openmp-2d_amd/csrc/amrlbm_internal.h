@@ -202,9 +202,15 @@ void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Pro
                     bool macro_h, bool copy_ghosts, hipStream_t s, const int *judge = nullptr, int ord = 0);
 void launch_macro_h(const DevMesh &m, int cur, const int *list, int n, hipStream_t s);
 void launch_err(const DevMesh &m, const int *list, int n, double *err, double *err_slot, hipStream_t s);
-void launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, int min_levels, int max_levels,
-                         int levelmin, int levelmax, double refin_tol, double deref_tol, int *changed,
-                         hipStream_t s, const int *judge = nullptr, int jstop = 0);
+// the device regrid decision and flags (k_regrid_check): at most RC_REF
+// refinement flags and RC_CAND derefinement candidates in its lists
+constexpr int RC_CAND = 512, RC_REF = 1024;
+constexpr int kRegridDerefAt = 3 + RC_REF, kRegridFlagWords = 3 + RC_REF + RC_CAND;
+constexpr size_t kRegridStaticLds = sizeof(int) * (18 * RC_CAND + RC_REF + 3);
+int regrid_check_max_blocks();
+hipError_t launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, int min_levels, int max_levels,
+                               int levelmin, int levelmax, double refin_tol, double deref_tol, int *changed,
+                               hipStream_t s, const int *judge = nullptr, int jstop = 0, int *flags = nullptr);
 void launch_judge_check(const int *judge, int jstop, int *changed, hipStream_t s);
 // pred[list[k]] = logits[2 k] > threshold and err_slot[list[k]] > gate (the
 // learned predictor's refinement flags by block slot)

@@ -918,52 +918,107 @@ __global__ __launch_bounds__(256) void k_err(DevMeshT<T> m, const int *__restric
 }
 
 // Whether Refine_Derefine_Mesh (Refine_Derefine.f90:2-14) would change the
-// mesh, decided on the device from Err_Interface of every leaf: true iff
-// Flag_for_refinement (:18-54) flags any leaf (the 2:1 closure only adds
-// flags), or a derefinement survives Flag_for_derefinement (:97-145) and its
-// level-by-level neighbour check (:149-194). With no refinement flag, the
-// check's Refin terms are false, and a candidate at level L is cancelled iff
-// one of the two children next to it of a non-leaf neighbour (pairs
-// :155-163) is itself a non-leaf without a surviving Deref flag; those flags
-// are final once the pass over level L+1 is done. So each candidate gathers
-// its non-leaf watch blocks once, and the passes only test LDS bits.
-// One workgroup. More than RC_CAND candidates: answers "changed" and leaves
-// the decision to the host. changed[1] = 1 iff any refinement flag is set.
+// mesh, and the flags it would set, decided on the device from Err_Interface
+// of every leaf:
+//  - Flag_for_refinement (:18-54) and Enforce_1_Level_Jump (:58-93): the
+//    criterion's flags, then the least fixed point under "a leaf one level
+//    coarser next to a flagged leaf is flagged", reached breadth first from
+//    the criterion's flags (any order reaches the same set);
+//  - Flag_for_derefinement (:97-145): a parent of leaves none of which is
+//    flagged for refinement and all of whose children pass the tolerance;
+//  - its level-by-level check (:149-194): a candidate at level L is cancelled
+//    iff one of the two children next to it of a non-leaf neighbour (pairs
+//    :155-163) is flagged for refinement, or is a non-leaf without a
+//    surviving derefinement flag; those flags are final once the pass over
+//    level L+1 is done, so each candidate gathers its watch blocks once and
+//    the passes only test LDS bits.
+// One workgroup. changed[0] = 1 iff the mesh changes, changed[1] = 1 iff any
+// refinement flag is set. With flags (kRegridFlagWords ints): flags[0] = 1
+// when the block slots of the flags[1] refinement flags follow from flags[3]
+// and those of the flags[2] derefinement flags from flags[kRegridDerefAt];
+// 0 when there were more than the lists hold (RC_REF flagged leaves, RC_CAND
+// candidates; the host then flags itself).
 // With judge (the stop ordinal of a judged segment): changed[2] = *judge, and
 // a stop before jstop (a state inside the segment needed refinement, and the
-// segment's last state was never reached) answers "changed, refinement".
-constexpr int RC_CAND = 512;
+// segment's last state was never reached) answers "changed, refinement" with
+// no flags (they belong to that earlier state).
 __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int nblks, const double *__restrict__ err_slot,
                                                       int min_levels, int max_levels, int levelmin, int levelmax,
                                                       double refin_tol, double deref_tol, int *changed,
-                                                      const int *__restrict__ judge, int jstop)
+                                                      const int *__restrict__ judge, int jstop, int *flags)
 {
-    extern __shared__ unsigned dmask[];
-    __shared__ int cand[RC_CAND], clev[RC_CAND], watch[RC_CAND * 16];
-    __shared__ int ncand;
+    extern __shared__ unsigned masks[];
+    __shared__ int cand[RC_CAND], clev[RC_CAND], watch[RC_CAND * 16], rlist[RC_REF];
+    __shared__ int ncand, nref, nder;
     const int tid = threadIdx.x, nw = (nblks + 31) >> 5;
+    unsigned *dmask = masks, *rmask = masks + nw;
+    auto rbit = [&](int c) { return (rmask[c >> 5] >> (c & 31)) & 1u; };
     if (judge) {
         const int J = *judge;
         if (tid == 0) changed[2] = J;
         if (J < jstop) {
-            if (tid == 0) changed[0] = changed[1] = 1;
+            if (tid == 0) {
+                changed[0] = changed[1] = 1;
+                if (flags) flags[0] = 0;
+            }
             return;
         }
     }
-    for (int w = tid; w < nw; w += 1024) dmask[w] = 0u;
-    if (tid == 0) ncand = 0;
+    for (int w = tid; w < 2 * nw; w += 1024) masks[w] = 0u;
+    if (tid == 0) ncand = nref = nder = 0;
     __syncthreads();
-    int ref = 0;
+    // the criterion's refinement flags
     for (int n = tid; n < nblks; n += 1024) {
         const BlockDev b = m.blk[n];
-        if (b.level < 0) continue;
+        if (b.level < 0 || !b.isleaf) continue;
         const int lo = m.lv ? m.lv[2 * b.inst] : min_levels, hi = m.lv ? m.lv[2 * b.inst + 1] : max_levels;
-        if (b.isleaf && (b.level < lo || (b.level < hi && err_slot[n] > refin_tol))) ref = 1;
-        if (!b.hasleaf || b.level < lo || b.child0 < 0) continue;
+        if (b.level < lo || (b.level < hi && err_slot[n] > refin_tol)) {
+            atomicOr(&rmask[n >> 5], 1u << (n & 31));
+            const int ix = atomicAdd(&nref, 1);
+            if (ix < RC_REF) rlist[ix] = n;
+        }
+    }
+    __syncthreads();
+    // the 2:1 closure, one breadth-first round per pass
+    int head = 0;
+    bool lists_ok = true;
+    while (true) {
+        const int tail = nref;
+        if (tail > RC_REF) {
+            lists_ok = false;
+            break;
+        }
+        for (int t = head + tid; t < tail; t += 1024) {
+            const int n = rlist[t];
+            const int L = m.blk[n].level;
+            for (int k = 0; k < 8; k++) {
+                const NbDesc e = m.nbd[(size_t)n * 8 + k];
+                if (e.id <= 0 || !e.isleaf || e.level != L - 1) continue;
+                const int q = e.id - 1;
+                const unsigned bit = 1u << (q & 31);
+                if (!(atomicOr(&rmask[q >> 5], bit) & bit)) {
+                    const int ix = atomicAdd(&nref, 1);
+                    if (ix < RC_REF) rlist[ix] = q;
+                }
+            }
+        }
+        __syncthreads();
+        const int now = nref;
+        __syncthreads();   // every thread has read it before the next round adds to it
+        if (now == tail) break;
+        head = tail;
+    }
+    const int nr = nref;
+    // derefinement candidates
+    for (int n = tid; n < nblks; n += 1024) {
+        const BlockDev b = m.blk[n];
+        if (b.level < 0 || !b.hasleaf || b.child0 < 0) continue;
+        const int lo = m.lv ? m.lv[2 * b.inst] : min_levels;
+        if (b.level < lo) continue;
         bool d = true;
         for (int c = b.child0; c < b.child0 + 4; c++) {
             const double e = m.blk[c].isleaf ? err_slot[c] : 0.0;
-            if (e > deref_tol) d = false;
+            if (rbit(c) || e > deref_tol) d = false;
         }
         if (d) {
             atomicOr(&dmask[n >> 5], 1u << (n & 31));
@@ -974,33 +1029,29 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
             }
         }
     }
-    if (__syncthreads_or(ref)) {
+    __syncthreads();
+    const int nc = ncand;
+    if (nc > RC_CAND) {   // the host decides and flags
         if (tid == 0) {
             changed[0] = 1;
-            changed[1] = 1;
+            changed[1] = nr > 0;
+            if (flags) flags[0] = 0;
         }
         return;
     }
-    if (tid == 0) changed[1] = 0;
-    const int nc = ncand;
-    if (nc > RC_CAND) {
-        if (tid == 0) changed[0] = 1;
-        return;
-    }
-    // children adjacent to direction k (1-based, Refine_Derefine.f90:155-163)
+    // children adjacent to direction k (1-based, Refine_Derefine.f90:155-163):
+    // a flagged one cancels the candidate at once, a non-leaf one is watched
     const int pair_c1[8] = {2, 1, 3, 1, 4, 2, 3, 1};
     const int pair_c2[8] = {4, 3, 4, 2, 4, 2, 3, 1};
     for (int t = tid; t < 8 * nc; t += 1024) {
         const int ci = t >> 3, k = t & 7, n = cand[ci];
         int w0 = -1, w1 = -1;
-        const int v = m.nbd[(size_t)n * 8 + k].id;
-        if (v > 0) {
-            const BlockDev q = m.blk[v - 1];
-            if (!q.isleaf && q.child0 >= 0) {
-                const int c1 = q.child0 + pair_c1[k] - 1, c2 = q.child0 + pair_c2[k] - 1;
-                if (!m.blk[c1].isleaf) w0 = c1;
-                if (!m.blk[c2].isleaf) w1 = c2;
-            }
+        const NbDesc e = m.nbd[(size_t)n * 8 + k];
+        if (e.id > 0 && !e.isleaf && e.child0 >= 0) {
+            const int c1 = e.child0 + pair_c1[k] - 1, c2 = e.child0 + pair_c2[k] - 1;
+            if (rbit(c1) || rbit(c2)) atomicAnd(&dmask[n >> 5], ~(1u << (n & 31)));
+            if (!m.blk[c1].isleaf) w0 = c1;
+            if (!m.blk[c2].isleaf) w1 = c2;
         }
         watch[ci * 16 + 2 * k] = w0;
         watch[ci * 16 + 2 * k + 1] = w1;
@@ -1019,10 +1070,26 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
         }
         __syncthreads();
     }
-    int any = 0;
-    for (int w = tid; w < nw; w += 1024) any |= dmask[w] != 0u;
-    any = __syncthreads_or(any);
-    if (tid == 0) changed[0] = any;
+    // the surviving derefinement flags (candidates only carry dmask bits)
+    for (int ci = tid; ci < nc; ci += 1024) {
+        const int n = cand[ci];
+        if ((dmask[n >> 5] >> (n & 31)) & 1u) {
+            const int ix = atomicAdd(&nder, 1);
+            if (flags && lists_ok) flags[kRegridDerefAt + ix] = n;
+        }
+    }
+    if (flags && lists_ok)
+        for (int t = tid; t < nr; t += 1024) flags[3 + t] = rlist[t];
+    __syncthreads();
+    if (tid == 0) {
+        changed[0] = (nr > 0 || nder > 0) ? 1 : 0;
+        changed[1] = nr > 0 ? 1 : 0;
+        if (flags) {
+            flags[0] = lists_ok ? 1 : 0;
+            flags[1] = nr;
+            flags[2] = nder;
+        }
+    }
 }
 
 // Restrict_Bilinear (Restrict.f90:14-65) with Bilinear_from_Fine(_f)
@@ -1726,13 +1793,30 @@ void launch_err(const DevMesh &m, const int *list, int n, double *err, double *e
         hipLaunchKernelGGL(k_err<double>, dim3(n), dim3(256), 0, s, typed<double>(m), list, err, err_slot);
 }
 
-void launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, int min_levels, int max_levels,
-                         int levelmin, int levelmax, double refin_tol, double deref_tol, int *changed,
-                         hipStream_t s, const int *judge, int jstop)
+int regrid_check_max_blocks()
 {
-    const size_t lds = sizeof(unsigned) * (size_t)((nblks + 31) / 32 + 1);
+    // dynamic LDS: two bit masks over the block slots, beside the kernel's
+    // static lists, within the 160 KB a gfx950 workgroup may hold
+    return (int)(((160 * 1024 - kRegridStaticLds - 2 * sizeof(unsigned)) / (2 * sizeof(unsigned))) * 32);
+}
+
+hipError_t launch_regrid_check(const DevMesh &m, int nblks, const double *err_slot, int min_levels, int max_levels,
+                               int levelmin, int levelmax, double refin_tol, double deref_tol, int *changed,
+                               hipStream_t s, const int *judge, int jstop, int *flags)
+{
+    if (nblks > regrid_check_max_blocks()) return hipErrorInvalidValue;
+    const size_t lds = sizeof(unsigned) * 2 * (size_t)((nblks + 31) / 32 + 1);
+    // static and dynamic LDS beyond the default 64 KB per workgroup is asked for explicitly
+    static size_t granted = 64 * 1024 - kRegridStaticLds - 256;   // 256: the compiler's own alignment
+    if (lds > granted) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_regrid_check),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        granted = lds;
+    }
     hipLaunchKernelGGL(k_regrid_check, dim3(1), dim3(1024), lds, s, typed<double>(m), nblks, err_slot, min_levels,
-                       max_levels, levelmin, levelmax, refin_tol, deref_tol, changed, judge, jstop);
+                       max_levels, levelmin, levelmax, refin_tol, deref_tol, changed, judge, jstop, flags);
+    return hipGetLastError();
 }
 
 // the decision after a judged segment that ends between full regrids:

@@ -135,7 +135,8 @@ struct amrlbm_mesh {
     double *d_err = nullptr;         // Err_Interface per leaf
     double *d_err_slot = nullptr;    // Err_Interface per block slot (leaves)
     int *d_changed = nullptr;        // [0] the device regrid decision (the step kernels' skip word),
-                                     // [1] any refinement flag, [2] a judged segment's stop ordinal
+                                     // [1] any refinement flag, [2] a judged segment's stop ordinal,
+                                     // [4 ..] the decision's flags (k_regrid_check's flag lists)
     int *h_changed = nullptr;        // pinned copy of it
     double *d_part = nullptr;        // diagnostics partials
     double *d_diag = nullptr;
@@ -153,6 +154,7 @@ struct amrlbm_mesh {
     bool pairs_recorded = false;
     double last_step_ms = 0.0;
     int regrid_checks = 0, regrid_changes = 0, segments_skipped = 0, graphs_built = 0;
+    int device_flag_regrids = 0;     // regrids whose flags came from the device decision
     // host wall time (ms): Refine_Derefine_Mesh, table upload, ghost-program
     // build (part of the upload), graph capture and instantiation
     double host_ms_regrid = 0.0, host_ms_tables = 0.0, host_ms_programs = 0.0, host_ms_graphs = 0.0;
@@ -268,15 +270,18 @@ struct DeviceBuffer {
     char *bytes() const { return static_cast<char *>(p); }
 };
 
-// adds the wall time of its scope to acc (ms)
+// adds the wall time of its scope (or up to stop()) to acc (ms)
 struct HostTimer {
     double &acc;
+    bool running = true;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     explicit HostTimer(double &a) : acc(a) {}
-    ~HostTimer()
+    void stop()
     {
-        acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (running) acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        running = false;
     }
+    ~HostTimer() { stop(); }
 };
 
 // instance of a block in an ensemble mesh (its roots' pair), else 0
@@ -1166,14 +1171,11 @@ int phase_macro_g(amrlbm_mesh *m)
 // ---------------------------------------------------------------------------
 // extra: blocks flagged for refinement besides the criterion; frozen: the
 // instances of an ensemble mesh left as they are (no flags)
-// d_pred: the learned predictor's flags by block slot on the device (read with
-// Err_Interface); *pred_count grows by the leaves refined on them alone
-int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector<char> *extra = nullptr,
-           const std::vector<char> *frozen = nullptr, const int *d_pred = nullptr, long long *pred_count = nullptr)
+// Refine and Deref flags of Refine_Derefine_Mesh (Refine_Derefine.f90:18-194)
+// computed on the host from Err_Interface of every leaf (downloaded here)
+int host_flags(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector<char> *extra,
+               const std::vector<char> *frozen, const int *d_pred, long long *pred_count)
 {
-    int rc = upload_tables(m);
-    if (rc) return rc;
-    HostTimer timer(m->host_ms_regrid);
     const int nleaf = (int)m->leaf.size();
     // Err_Interface of every leaf on the device (Refine_Derefine.f90:36, :122)
     launch_err(m->dm, m->d_lists + m->all_off, nleaf, m->d_err, nullptr, m->stream);
@@ -1249,6 +1251,40 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
                 }
             }
         }
+    return AMRLBM_OK;
+}
+
+// d_pred: the learned predictor's flags by block slot on the device (read with
+// Err_Interface); *pred_count grows by the leaves refined on them alone;
+// dev_flags: the Refine and Deref flags already decided on the device (the
+// flag lists of k_regrid_check), used instead of the host's
+int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector<char> *extra = nullptr,
+           const std::vector<char> *frozen = nullptr, const int *d_pred = nullptr, long long *pred_count = nullptr,
+           const int *dev_flags = nullptr)
+{
+    int rc = upload_tables(m);
+    if (rc) return rc;
+    HostTimer timer(m->host_ms_regrid);
+    if (dev_flags) {
+        // the device decision's flags (k_regrid_check), as host_flags sets them
+        for (int n = 1; n <= m->nblks; n++) m->blk[n].refin = m->blk[n].deref = false;
+        for (int i = 0; i < dev_flags[1]; i++) m->blk[dev_flags[3 + i] + 1].refin = true;
+        for (int i = 0; i < dev_flags[2]; i++) m->blk[dev_flags[kRegridDerefAt + i] + 1].deref = true;
+        if (std::getenv("AMRLBM_VERIFY_FLAGS")) {
+            std::vector<char> dev(2 * (m->nblks + 1));
+            for (int n = 1; n <= m->nblks; n++) {
+                dev[2 * n] = m->blk[n].refin;
+                dev[2 * n + 1] = m->blk[n].deref;
+            }
+            if ((rc = host_flags(m, refin_tol, deref_tol, nullptr, nullptr, nullptr, nullptr))) return rc;
+            for (int n = 1; n <= m->nblks; n++)
+                if (dev[2 * n] != m->blk[n].refin || dev[2 * n + 1] != m->blk[n].deref)
+                    return fail(AMRLBM_ERR_STATE, "device regrid flags of block " + std::to_string(n) +
+                                                      " differ from the host's");
+        }
+    } else if ((rc = host_flags(m, refin_tol, deref_tol, extra, frozen, d_pred, pred_count))) {
+        return rc;
+    }
     // Derefine (:198-211): restrict on the device, then free the children
     std::vector<int> rpairs;
     for (int n = 1; n <= m->nblks; n++)
@@ -1308,6 +1344,7 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
     }
     if (rpairs.empty() && ppairs.empty()) return AMRLBM_OK;   // flags only: tables and graphs stay valid
     array_of_leaf_blocks(m);
+    timer.stop();
     return upload_tables(m);
 }
 
@@ -1392,9 +1429,9 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipMemsetAsync(m->d_sync, 0, sizeof(unsigned) * kSegSyncWords, m->stream));
     HIPCHK(hipMalloc(&m->d_err, sizeof(double) * S));
     HIPCHK(hipMalloc(&m->d_err_slot, sizeof(double) * S));
-    HIPCHK(hipMalloc(&m->d_changed, sizeof(int) * 4));
-    HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int) * 4, m->stream));
-    HIPCHK(hipHostMalloc(&m->h_changed, sizeof(int) * 4, hipHostMallocDefault));
+    HIPCHK(hipMalloc(&m->d_changed, sizeof(int) * (4 + kRegridFlagWords)));
+    HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int) * (4 + kRegridFlagWords), m->stream));
+    HIPCHK(hipHostMalloc(&m->h_changed, sizeof(int) * (4 + kRegridFlagWords), hipHostMallocDefault));
     HIPCHK(hipEventCreateWithFlags(&m->ev_check, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&m->ev_tab, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&m->ev_pairs, hipEventDisableTiming));
@@ -1604,15 +1641,17 @@ int launch_steps(amrlbm_mesh *m, int len, bool judge = false)
 // Enqueue the device regrid decision at the current step, behind a segment
 // whose last kernel left Err_Interface of every leaf in d_err_slot:
 // k_regrid_check writes 1 to d_changed iff the host regrid would change the
-// mesh (or it cannot tell), and its copy reaches h_changed when ev_check fires.
+// mesh (or it cannot tell) and the flags it would set after it, and their
+// copy reaches h_changed when ev_check fires.
 // Until the next check, a nonzero d_changed makes the step kernels return at
 // once, so a segment launched behind the check leaves the state as it is.
 int enqueue_regrid_check(amrlbm_mesh *m)
 {
-    launch_regrid_check(m->dm, m->nblks, m->d_err_slot, m->prm.min_levels, m->prm.max_levels, m->levelmin,
-                        m->levelmax, 0.002, 0.001, m->d_changed, m->stream);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(m->h_changed, m->d_changed, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(launch_regrid_check(m->dm, m->nblks, m->d_err_slot, m->prm.min_levels, m->prm.max_levels, m->levelmin,
+                               m->levelmax, 0.002, 0.001, m->d_changed, m->stream, nullptr, 0, m->d_changed + 4));
+    // the decision and its flag lists in one copy
+    HIPCHK(hipMemcpyAsync(m->h_changed, m->d_changed, sizeof(int) * (4 + kRegridFlagWords), hipMemcpyDeviceToHost,
+                          m->stream));
     HIPCHK(hipEventRecord(m->ev_check, m->stream));
     m->regrid_checks++;
     return AMRLBM_OK;
@@ -2613,9 +2652,8 @@ int amrlbm_regrid_check(amrlbm_mesh *m, int out[2])
     determine_min_max_levels(m);
     // C, its first ghost ring and U, P are current between steps
     launch_err(m->dm, all_leaves(m), nleaves(m), m->d_err, m->d_err_slot, m->stream);
-    launch_regrid_check(m->dm, m->nblks, m->d_err_slot, m->prm.min_levels, m->prm.max_levels, m->levelmin,
-                        m->levelmax, 0.002, 0.001, m->d_changed, m->stream);
-    HIPCHK(hipGetLastError());
+    HIPCHK(launch_regrid_check(m->dm, m->nblks, m->d_err_slot, m->prm.min_levels, m->prm.max_levels, m->levelmin,
+                               m->levelmax, 0.002, 0.001, m->d_changed, m->stream));
     HIPCHK(hipMemcpyAsync(m->h_changed, m->d_changed, sizeof(int) * 2, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
     out[0] = m->h_changed[0];
@@ -2668,7 +2706,14 @@ static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahea
             pending = false;
             HIPCHK(hipEventSynchronize(m->ev_check));
             if (*m->h_changed) {
-                HIPCHK(hipStreamSynchronize(m->stream));
+                // with the device's flags the host regrids at once: the
+                // segment launched behind the decision drains meanwhile (its
+                // kernels skip), and every change to the device state is
+                // queued behind it; without them the host flags from a fresh
+                // Err_Interface once the stream is idle
+                const bool dev_flags = m->h_changed[4] == 1;
+                if (!dev_flags) HIPCHK(hipStreamSynchronize(m->stream));
+                m->device_flag_regrids += dev_flags;
                 if (launched) {
                     m->T -= len;
                     done -= len;
@@ -2679,7 +2724,8 @@ static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahea
                 // the skip word stays set while the host regrids (no step
                 // kernel runs meanwhile); it is cleared below, or by the
                 // caller on an error return
-                if ((rc = regrid(m, 0.002, 0.001))) return rc;
+                if ((rc = regrid(m, 0.002, 0.001, nullptr, nullptr, nullptr, nullptr, dev_flags ? m->h_changed + 4 : nullptr)))
+                    return rc;
                 HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int), m->stream));
                 continue;
             }
@@ -2735,8 +2781,8 @@ static int enqueue_prediction(amrlbm_mesh *m, bool fresh_err)
 static int enqueue_learned_check(amrlbm_mesh *m, int len, bool full)
 {
     if (full)
-        launch_regrid_check(m->dm, m->nblks, m->d_err_slot, m->prm.min_levels, m->prm.max_levels, m->levelmin,
-                            m->levelmax, 0.002, 0.001, m->d_changed, m->stream, m->d_judge, len);
+        HIPCHK(launch_regrid_check(m->dm, m->nblks, m->d_err_slot, m->prm.min_levels, m->prm.max_levels, m->levelmin,
+                                   m->levelmax, 0.002, 0.001, m->d_changed, m->stream, m->d_judge, len));
     else
         launch_judge_check(m->d_judge, len + 1, m->d_changed, m->stream);
     HIPCHK(hipGetLastError());
@@ -3289,7 +3335,7 @@ int amrlbm_last_step_ms(amrlbm_mesh *m, double *ms)
     return AMRLBM_OK;
 }
 
-int amrlbm_regrid_stats(amrlbm_mesh *m, double out[8])
+int amrlbm_regrid_stats(amrlbm_mesh *m, double out[9])
 {
     if (!m || !out) return fail(AMRLBM_ERR_ARG, "null argument");
     out[0] = m->regrid_checks;
@@ -3300,6 +3346,7 @@ int amrlbm_regrid_stats(amrlbm_mesh *m, double out[8])
     out[5] = m->host_ms_programs;
     out[6] = m->host_ms_graphs;
     out[7] = m->graphs_built;
+    out[8] = m->device_flag_regrids;
     return AMRLBM_OK;
 }
 

@@ -233,13 +233,15 @@ int amrlbm_output_noalloc(amrlbm_mesh *m, double out[9]);
 int amrlbm_synchronize(amrlbm_mesh *m);
 /* device time (ms) of the last amrlbm_step call, measured with HIP events */
 int amrlbm_last_step_ms(amrlbm_mesh *m, double *ms);
-/* regrid bookkeeping since amrlbm_create (no reference counterpart; instrumentation):
- * out[0] device regrid decisions taken by amrlbm_step, out[1] of them that changed the mesh,
- * out[2] segments launched ahead of such a change and taken back, host wall ms spent in
- * out[3] Refine_Derefine_Mesh (excluding table upload), out[4] table upload (including
- * out[5] ghost-program build), out[6] graph capture and instantiation or update; out[7] graphs
- * captured (built, or updated after a topology change) */
-int amrlbm_regrid_stats(amrlbm_mesh *m, double out[8]);
+/* regrid bookkeeping since amrlbm_create, set-up included (no reference counterpart;
+ * instrumentation): out[0] device regrid decisions taken by amrlbm_step, out[1] of them that
+ * changed the mesh, out[2] segments launched ahead of such a change and taken back, host wall
+ * ms spent in out[3] Refine_Derefine_Mesh (excluding table upload), out[4] table upload
+ * (including out[5] ghost-program build and upload), out[6] graph capture and instantiation or
+ * update; out[7] graphs captured (built, or updated after a topology change); out[8] changes
+ * whose Refine and Deref flags came with the device decision (the others were flagged on the
+ * host from a fresh Err_Interface) */
+int amrlbm_regrid_stats(amrlbm_mesh *m, double out[9]);
 
 /* ---- learned regrid predictor (no counterpart in the reference) ----------
  * A 3-layer 3x3 convolutional network (16 channels, BF16 weights on MFMA,

@@ -65,10 +65,10 @@ def step_by_phases(m, t, k, g=None):
 @pytest.mark.parametrize("execution", [0, 1])
 @pytest.mark.parametrize("name", ["tiny_rise", "cfg1_rise", "cfg2_regrid"])
 def test_fused_steps_across_mesh_changes_equal_reference(amrlbm, workloads, name, execution):
-    """amrlbm_step (fused kernels, regrid decided on the device, segments taken
-    back when the mesh changes; the step graph or the persistent segment
-    kernel) over the whole run, then its last step phase by phase: the final
-    state's SHA-256 equals the reference build's."""
+    """amrlbm_step (fused kernels, regrid decided and flagged on the device,
+    segments taken back when the mesh changes; the step graph or the
+    persistent segment kernel) over the whole run, then its last step phase by
+    phase: the final state's SHA-256 equals the reference build's."""
     g = RISE[name]
     w = workloads[g["workload"]]
     n, k = g["steps"], g["regrid_every"]
@@ -82,6 +82,7 @@ def test_fused_steps_across_mesh_changes_equal_reference(amrlbm, workloads, name
         st = m.regrid_stats()
         print(f"[rise] {name}: {st}", flush=True)
         assert st["changes"] >= len(g["change_steps"]), st
+        assert st["device_flag_changes"] == st["changes"], st   # every change flagged on the device
         step_by_phases(m, n - 1, k)
         m.determine_min_max_levels()
         assert snap_hash(m) == g["final"], "final state differs from the reference build"
@@ -128,3 +129,27 @@ def test_fused_steps_across_mesh_changes_equal_oracle(amrlbm, oracle_mod, worklo
         rel = np.abs(d_gpu - d_orc) / np.maximum(np.abs(d_orc), 1e-300)
         print("[rise] diagnostics rel err", rel)
         assert np.all(rel[:8] <= 1e-12), rel
+
+
+@pytest.mark.parametrize("case", ["cfg2-64", "cfg2-32", "ens64-32"])
+def test_device_flags_equal_host_flags(amrlbm, workloads, monkeypatch, case):
+    """Every mesh change of amrlbm_step takes its Refine and Deref flags from
+    the device decision (k_regrid_check: the criterion, the 2:1 closure, the
+    derefinement flags and their level-by-level check). With
+    AMRLBM_VERIFY_FLAGS set the library also flags each such state on the host
+    (Err_Interface downloaded afresh) and fails on any difference: cfg2 over
+    4,000 steps in FP64 and FP32, and the 64-instance ensemble (per-instance
+    level ranges, about 45,000 block slots) over 1,000 steps."""
+    from amrlbm.ensemble import ensemble
+    name, prec = case.rsplit("-", 1)
+    w = ensemble(64, seed=2024) if name == "ens64" else workloads[name]
+    steps = 1000 if name == "ens64" else 4000
+    monkeypatch.setenv("AMRLBM_VERIFY_FLAGS", "1")
+    print(f"[flags] {case}: {steps} steps, regrid every 10", flush=True)
+    with amrlbm.Mesh(w, precision=int(prec)) as m:
+        m.setup_rising_bubble()
+        m.step(steps, 10)
+        st = m.regrid_stats()
+    print(f"[flags] {case}: {st}", flush=True)
+    assert st["changes"] >= 2, st
+    assert st["device_flag_changes"] == st["changes"], st
