@@ -408,7 +408,7 @@ class Mesh:
         assert w.size == CNN_NPARAM
         _check(lib().amrlbm_set_regrid_network(self.h, _f(w), float(threshold), float(gate)))
 
-    def step_learned(self, nsteps, full_every=50):
+    def step_learned(self, nsteps, full_every=20):
         """nsteps steps under the learned regrid's cadence (amrlbm_step_learned):
         {full_regrids, predicted_blocks, misses, judged}."""
         st = (ctypes.c_longlong * 4)()

@@ -2,7 +2,7 @@
 and velocity predicts whether the block's region will be refined (or
 derefined) HORIZON steps ahead, so that blocks the interface is about to
 enter are refined before it arrives and the full regrid can run every
-HORIZON steps. The plain gradient criterion (Refine_Derefine.f90:18-54,
+FULL_EVERY steps. The plain gradient criterion (Refine_Derefine.f90:18-54,
 :327-352) stays the judge: the library tests it on the device at every
 step's state, and the interface on an under-refined block triggers an
 immediate plain regrid (a miss). The network never changes the mesh on its
@@ -41,7 +41,8 @@ from . import Mesh, cnn_forward, cnn_init, cnn_train
 from .ensemble import Ensemble, Stream
 from .workloads import WORKLOADS, Workload
 
-HORIZON = 50
+HORIZON = 50       # labels: the topology this many steps after the input state
+FULL_EVERY = 20    # the cadence's full-regrid interval (amrlbm_step_learned)
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "regrid_cnn_weights.npy")
 # the decision threshold on logit 0 chosen on held-out trajectories, with their scores
 THRESHOLD_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "regrid_cnn_threshold.json")
@@ -112,7 +113,7 @@ def generate(w, variants, steps, every=HORIZON, horizon=HORIZON, regrid_every=10
     return np.concatenate(X), np.concatenate(Y)
 
 
-def run(mesh, nsteps, weights=None, full_every=HORIZON, threshold=None):
+def run(mesh, nsteps, weights=None, full_every=FULL_EVERY, threshold=None):
     """Advance the mesh nsteps under the learned regrid's cadence
     (amrlbm_step_learned): the full regrid every full_every steps, the plain
     criterion judging every step on the device, and every regrid refining the
@@ -123,7 +124,7 @@ def run(mesh, nsteps, weights=None, full_every=HORIZON, threshold=None):
             "misses": st["misses"], "checks": st["judged"]}
 
 
-def evaluate(w, weights, steps, precision=32, variant_=None, full_every=HORIZON):
+def evaluate(w, weights, steps, precision=32, variant_=None, full_every=FULL_EVERY):
     """The learned regrid against the same cadence without the network, and
     against the plain regrid every 10 steps, on workload w (a bubble variant
     of it with variant_ = (radius, x0, y0))."""

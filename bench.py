@@ -26,11 +26,14 @@ FP32 as deviations from the liquid at rest, FP32 collision; bytes per leaf
 cell halved) and states its error against the FP64 run after the same steps,
 beside the FP64 run from a set-up perturbed by 1e-7 (the flow's own
 sensitivity). fp32_learned is the metric's pair: FP32 with the learned regrid
-(the committed network, amrlbm/regrid_cnn_weights.npy; full regrid every 50
-steps with its predictions, the plain criterion judging every step on the
-device) on the same bubble, timed the same way, against plain FP64 and
-against the same cadence without the network. fp32.learned_regrid_cfg1
-deploys the network on the 128x256 bubble (the learned configuration).
+(the committed network, amrlbm/regrid_cnn_weights.npy; full regrid every 20
+steps, the plain criterion judging every step on the device, the network's
+flags joining every regrid that runs) on the same bubble, timed the same way,
+against plain FP64 and against the same cadence without the network.
+fp32.learned_regrid_cfg1 deploys the network on the 128x256 bubble (the
+learned configuration). regrid_interval sweeps the regrid interval of the
+FP64 run over the configuration's 20,000 steps against the reference's
+every-step cadence (amrlbm.tuning).
 
 large_working_set repeats the measurement on 16 independent copies of the
 same domain in one mesh (about 4,600 leaf blocks, 0.5 GB of FP64 leaf state,
@@ -166,6 +169,36 @@ def kernels_only(amrlbm, w, calibrate_mb, precisions, variant=None):
                           "roofline": {kk: roofline(kk, kt, nleaf, w, prec) for kk in kt}}), flush=True)
 
 
+def profile_steps(amrlbm, w, steps, k, precisions, variant=None):
+    """The whole-step profiling run: steps steps of workload w with the regrid
+    every k (its mesh changes included), every launch of the step as in the
+    step graph but issued one by one with HIP events between them
+    (amrlbm_profile_steps; no graph); per kernel class the device time per
+    launch and the achieved rate over the leaf cells it covered, one JSON
+    line per precision."""
+    for prec in precisions:
+        with amrlbm.Mesh(w, precision=prec) as m:
+            m.setup_rising_bubble()
+            if variant:
+                m.set_tuning(*variant)
+            st = m.profile_steps(steps, k)
+            nleaf = m.counts()["nleaf"]
+        scale = 0.5 if prec == 32 else 1.0
+        cls = {}
+        for key, v in st.items():
+            if key == "regrid":
+                continue
+            n = max(v["launches"], 1)
+            gbs = ALG_BYTES[key] * scale * v["cells"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0
+            cls[key] = {"kernel": KERNEL_NAMES[key], "launches": v["launches"],
+                        "avg_launch_us": round(v["ms"] * 1e3 / n, 3), "achieved": round(gbs, 2),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        print(json.dumps({"mode": "profile-steps", "workload": w.name, "dtype": f"f{prec}", "steps": steps,
+                          "regrid_every": k, "leaf_blocks_end": nleaf, "regrid_calls": st["regrid"]["count"],
+                          "regrid_ms_per_call": round(st["regrid"]["ms"] / max(st["regrid"]["count"], 1), 4),
+                          "variant": list(variant) if variant else None, "kernels": cls}), flush=True)
+
+
 TUNE = {"on": True}
 
 
@@ -238,16 +271,18 @@ def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64, radius=No
 
 def learned_regrid(amrlbm, w, steps=2000):
     """Config 4: the committed network deployed on workload w (trained on the
-    generator of amrlbm.learned), FP32, full regrid every HORIZON steps with
-    the plain criterion judging every step on the device, against the same
-    cadence without the network and the plain regrid every 10 steps."""
+    generator of amrlbm.learned), FP32, full regrid every FULL_EVERY steps
+    with the plain criterion judging every step on the device and the
+    network's flags joining every regrid, against the same cadence without
+    the network and the plain regrid every 10 steps."""
     import numpy as np
     from amrlbm import learned
     if not os.path.exists(learned.WEIGHTS):
         return None
     ev = learned.evaluate(w, np.load(learned.WEIGHTS), steps)
-    out = {"workload": f"{w.name} bubble (D={int(w.D)}), {steps} steps, full regrid every {learned.HORIZON} steps, "
-                       f"plain criterion judging every step on the device", "precision": "f32",
+    out = {"workload": f"{w.name} bubble (D={int(w.D)}), {steps} steps, full regrid every {learned.FULL_EVERY} "
+                       f"steps, plain criterion judging every step on the device, the network's flags joining "
+                       f"every regrid", "precision": "f32",
            "threshold": learned._threshold()}
     for k, v in ev.items():
         out[k] = {f: (round(x, 4) if isinstance(x, float) else x) for f, x in v.items() if f != "diag"}
@@ -269,22 +304,25 @@ def fp32_learned(amrlbm, w, steps, warmup, fp64_value, fp64_diag):
     if not os.path.exists(learned.WEIGHTS):
         return None
     wts = np.load(learned.WEIGHTS)
-    out = {"workload": describe(w, learned.HORIZON, 32).replace(f"regrid every {learned.HORIZON}",
-                                                                  f"full regrid every {learned.HORIZON} with the "
-                                                                  f"network's predictions, plain criterion judging "
-                                                                  f"every step on the device"),
+    out = {"workload": describe(w, learned.FULL_EVERY, 32).replace(
+               f"regrid every {learned.FULL_EVERY}",
+               f"full regrid every {learned.FULL_EVERY}, plain criterion judging every step on the device, "
+               f"the network's flags joining every regrid"),
            "unit": "MLUPS (finest-equivalent)"}
     for name, net in (("learned", wts), ("no_network", None)):
         with amrlbm.Mesh(w, precision=32) as m:
             m.setup_rising_bubble()
-            learned.run(m, warmup, net)
+            m.set_regrid_network(net, learned._threshold())   # once: the windows only step
+            m.step_learned(warmup, learned.FULL_EVERY)
             m.synchronize()
             walls, st = [], None
             for _ in range(windows_for(steps)):
                 t0 = time.perf_counter()
-                s1 = learned.run(m, steps, net)
+                s1 = m.step_learned(steps, learned.FULL_EVERY)
                 m.synchronize()
                 walls.append(time.perf_counter() - t0)
+                s1 = {"predicted_regrids": s1["full_regrids"], "predicted_blocks": s1["predicted_blocks"],
+                      "misses": s1["misses"], "checks": s1["judged"]}
                 st = s1 if st is None else {f: st[f] + s1[f] for f in st}
             walls.sort()
             wall = walls[len(walls) // 2]
@@ -296,16 +334,42 @@ def fp32_learned(amrlbm, w, steps, warmup, fp64_value, fp64_diag):
                 d = m.output()
                 e["error_vs_fp64"] = {"mass_rel": abs(d[3] - fp64_diag[3]) / abs(fp64_diag[3]),
                                       "rise_velocity_rel": abs(d[7] - fp64_diag[7]) / abs(fp64_diag[7])}
+                out["diag"] = [float(v) for v in d]   # for the every-step comparison; not printed
             out[name] = e
     out["speedup_vs_plain_fp64"] = round(out["learned"]["value"] / fp64_value, 3)
     out["gain_vs_no_network"] = round(out["learned"]["value"] / out["no_network"]["value"], 3)
     return out
 
 
+def regrid_interval(amrlbm, w, steps):
+    """The regrid-interval knob (amrlbm.tuning): FP64 runs of workload w over
+    `steps` steps from the set-up at each interval, their speed and their
+    Output diagnostics against the reference's every-step cadence, and the
+    fastest interval within the stated bounds."""
+    from amrlbm import tuning
+    rows = tuning.sweep(w, steps=steps)
+    best = tuning.choose(rows)
+    keep = ("regrid_every", "us_per_step", "changes", "leaves", "rise_velocity_rel", "mass_rel", "centroid_abs")
+    entry = {"workload": describe(w, 1, 64).replace("regrid every 1", "regrid interval swept"), "steps": steps,
+             "baseline": "regrid every step (Main_Loop, AMR_Rising_2D_OMP.f90:360)",
+             "bounds": {"rise_velocity_rel": tuning.RISE_REL, "centroid_abs": tuning.CENTROID_ABS,
+                        "mass_rel": tuning.MASS_REL},
+             "rows": [{f: (round(r[f], 9) if isinstance(r[f], float) else r[f]) for f in keep} for r in rows],
+             "chosen": best["regrid_every"]}
+    return entry, rows[0]["diag"]
+
+
+def versus(d, ref):
+    """Output diagnostics d against ref: relative mass and rise velocity, centroid distance (cells)."""
+    return {"mass_rel": abs(d[3] - ref[3]) / abs(ref[3]), "rise_velocity_rel": abs(d[7] - ref[7]) / abs(ref[7]),
+            "centroid_abs_cells": max(abs(d[5] - ref[5]), abs(d[6] - ref[6]))}
+
+
 def multi_bubble(amrlbm, steps, k):
     """The 32-bubble case (amrlbm.multibubble): FP64, FP32 and FP32 with the
-    learned regrid (full regrid every 50 steps, plain criterion judging every
-    step on the device) from the same device set-up, and the FP32 runs' total
+    learned regrid (full regrid every learned.FULL_EVERY steps, plain
+    criterion judging every step on the device, the network's flags joining
+    every regrid) from the same device set-up, and the FP32 runs' total
     mass and per-bubble centroids against FP64's."""
     import numpy as np
     from amrlbm import learned
@@ -334,7 +398,7 @@ def multi_bubble(amrlbm, steps, k):
             wall = time.perf_counter() - t0
             runs[name] = {"diag": m.output(), "regions": mb.gas_regions(m)}
             e = {"value": round(w.fine_cells * steps / wall / 1e6, 1), "ms_per_step": round(wall * 1e3 / steps, 4),
-                 "leaf_blocks": m.counts()["nleaf"], "regrid_every": 50 if use_net else k}
+                 "leaf_blocks": m.counts()["nleaf"], "regrid_every": learned.FULL_EVERY if use_net else k}
             if st:
                 e.update({f: st[f] for f in ("predicted_regrids", "predicted_blocks", "misses", "checks")})
             out[name] = e
@@ -384,7 +448,8 @@ def ensemble_run(amrlbm, n, seed, steps, k):
                 out[f"fp{prec}"] = {"error": str(exc)}
                 continue
             changes = s1["changes"] - s0["changes"]
-            host = sum(s1["host_ms"][f] - s0["host_ms"][f] for f in ("regrid", "graphs"))
+            # Refine_Derefine_Mesh, the table and ghost-program upload (programs included) and graph capture
+            host = sum(s1["host_ms"][f] - s0["host_ms"][f] for f in ("regrid", "tables", "graphs"))
             out[f"fp{prec}"] = {
                 "value": round(e.fine_cells * steps / wall / 1e6, 1), "ms_per_step": round(wall * 1e3 / steps, 4),
                 "leaf_blocks": [c0["nleaf"], m.counts()["nleaf"]], "mesh_changes": changes,
@@ -434,6 +499,9 @@ def main():
                     help="steps of the 32-bubble case (0 to skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10000,
                     help="steps of the 64-instance ensemble (0 to skip)")
+    ap.add_argument("--interval-steps", type=int, default=-1,
+                    help="steps of each run of the regrid-interval sweep (default: as many as each headline run "
+                         "takes, warmup included; 0 to skip)")
     ap.add_argument("--ensemble-size", type=int, default=64)
     ap.add_argument("--ensemble-seed", type=int, default=2024)
     ap.add_argument("--no-autotune", action="store_true",
@@ -442,6 +510,8 @@ def main():
                     help="launch variant 'order,depth' for --kernels-only (amrlbm_set_tuning)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: time the step kernels on the set-up mesh only (no whole steps)")
+    ap.add_argument("--profile-steps", action="store_true",
+                    help="profiling run: --steps whole steps with regrids, launches issued one by one (no graph)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -455,6 +525,10 @@ def main():
     if args.kernels_only:
         variant = tuple(int(v) for v in args.variant.split(",")) if args.variant else None
         kernels_only(amrlbm, w, args.calibrate_copy_mb, (64,) if args.no_fp32 else (64, 32), variant)
+        return
+    if args.profile_steps:
+        variant = tuple(int(v) for v in args.variant.split(",")) if args.variant else None
+        profile_steps(amrlbm, w, args.steps, k, (64,) if args.no_fp32 else (64, 32), variant)
         return
     r = measure(amrlbm, w, args.steps, args.warmup, k, args.calibrate_copy_mb)
     c0, c1 = r["c0"], r["c1"]
@@ -477,6 +551,7 @@ def main():
         "tuning": r["tuning"],
         "diagnostics": {"mass": diag[3], "rise_velocity": diag[7], "centroid_y": diag[6]},
     }
+    d32 = None
     if not args.no_fp32:
         r32 = measure(amrlbm, w, args.steps, args.warmup, k, precision=32)
         d32 = r32["diag"]
@@ -503,6 +578,18 @@ def main():
             "learned_regrid_cfg1": learned_regrid(amrlbm, amrlbm.WORKLOADS["cfg1"]),
         }
         out["fp32_learned"] = fp32_learned(amrlbm, w, args.steps, args.warmup, r["value"], diag)
+    learned_diag = out["fp32_learned"].pop("diag", None) if out.get("fp32_learned") else None
+    total = args.warmup + args.steps * windows_for(args.steps)   # the steps every headline run took
+    n_int = args.interval_steps if args.interval_steps >= 0 else total
+    if n_int > 0:
+        out["regrid_interval"], every = regrid_interval(amrlbm, w, n_int)
+        if n_int == total:   # the same steps: each run against the every-step FP64 run
+            cmp = {"fp64_regrid_every_10": versus(diag, every)}
+            if d32 is not None:
+                cmp["fp32_regrid_every_10"] = versus(d32, every)
+            if learned_diag is not None:
+                cmp["fp32_learned"] = versus(learned_diag, every)
+            out["regrid_interval"]["vs_fp64_every_step"] = cmp
     if args.large_workload:
         wl = amrlbm.WORKLOADS[args.large_workload]
         warm = max(args.warmup // 5, 10)

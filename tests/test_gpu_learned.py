@@ -83,7 +83,7 @@ def test_learned_cadence_judges_every_step(amrlbm, workloads):
         st = learned.run(m, 2200, wts)
         p.step(2200, 10)
         print(f"[learned] cfg1_rise 2200 steps: {st}", flush=True)
-        assert st["checks"] == 2200 and st["predicted_regrids"] == 44
+        assert st["checks"] == 2200 and st["predicted_regrids"] == 2200 // learned.FULL_EVERY
         assert m.regrid_check()[1] is False
         dm, dp = m.output(), p.output()
         assert dm[8] == 0 and abs(dm[3] - dp[3]) <= 1e-5 * abs(dp[3])
@@ -110,8 +110,9 @@ def test_learned_cadence_split_calls(amrlbm, workloads):
 
 
 def test_deployed_on_cfg1_bubble(amrlbm, workloads):
-    """Deployed on the 128x256 bubble (D = 32), regrid every 50 steps with the
-    plain criterion judging every step: the missed-refinement rate (judge
+    """Deployed on the 128x256 bubble (D = 32), full regrid every
+    learned.FULL_EVERY steps with the plain criterion judging every step and
+    the network's flags joining every regrid: the missed-refinement rate (judge
     regrids per judged step) and the throughput against the same cadence
     without the network and the plain regrid every 10 steps are reported;
     the misses stay under 1 % of the judged steps."""

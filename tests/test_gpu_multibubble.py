@@ -64,8 +64,8 @@ def test_multibubble_setup_and_fp32_vs_fp64(amrlbm):
 
 
 def test_multibubble_learned_cadence_fp32(amrlbm):
-    """The learned regrid's cadence (full regrid every 50 steps, the plain
-    criterion judging every step on the device, no network) on the 32 bubbles
+    """The learned regrid's cadence (full regrid every learned.FULL_EVERY
+    steps, the plain criterion judging every step on the device, no network) on the 32 bubbles
     in FP32 over 3000 steps: the state stays finite and the mesh the same
     size as under the plain regrid every 10 steps. Regrids issued back to
     back (no step between them) refined every just-derefined block again:
@@ -81,7 +81,7 @@ def test_multibubble_learned_cadence_fp32(amrlbm):
             mb.setup(m, bub)
             if name == "learned":
                 st = learned.run(m, 3000, None)
-                assert st["checks"] == 3000 and st["predicted_regrids"] == 60
+                assert st["checks"] == 3000 and st["predicted_regrids"] == 3000 // learned.FULL_EVERY
             else:
                 m.step(3000, 10)
             d = m.output(allow_nonfinite=True)

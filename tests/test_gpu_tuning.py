@@ -76,3 +76,20 @@ def test_autotune_keeps_the_fastest_and_the_state(amrlbm, workloads):
         a.step(200, 10)
         b.step(200, 10)
         _same(_state(a), _state(b))
+
+
+def test_regrid_interval_sweep(amrlbm, workloads):
+    """The regrid-interval knob (amrlbm.tuning) on the faster-rising 128x256
+    bubble over 2,200 steps: the every-step run is the baseline, the
+    interval-10 run states its distance from it, and the choice is one of
+    the swept intervals within the bounds."""
+    from amrlbm import tuning
+    rows = tuning.sweep(workloads["cfg1_rise"], intervals=(1, 10), steps=2200)
+    for r in rows:
+        print(f"[interval] every {r['regrid_every']}: {r['us_per_step']} us/step, {r['changes']} changes, "
+              f"rise velocity {r['rise_velocity_rel']:.2e}, centroid {r['centroid_abs']:.3f}, mass {r['mass_rel']:.2e}",
+              flush=True)
+    assert rows[0]["rise_velocity_rel"] == 0.0 and rows[0]["changes"] >= 3
+    assert rows[1]["changes"] >= 1 and rows[1]["diag"][8] == 0
+    best = tuning.choose(rows)
+    assert best["regrid_every"] in (1, 10)

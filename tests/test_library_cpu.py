@@ -51,3 +51,20 @@ def test_workload_physics_matches_reference_derivation(workloads):
     assert w.max_levels == 2 and w.max_blocks == 96
     c2 = workloads["cfg2"]
     assert c2.fine_shape == (1024, 2048) and c2.max_levels == 6 and c2.min_levels == 3
+
+
+def test_regrid_interval_choice():
+    """amrlbm.tuning.choose keeps the fastest interval whose diagnostics stay
+    within every bound against the baseline row, and the baseline itself when
+    no other row qualifies."""
+    from amrlbm import tuning
+
+    def row(k, us, rise=0.0, cen=0.0, mass=0.0):
+        return {"regrid_every": k, "us_per_step": us, "rise_velocity_rel": rise, "centroid_abs": cen,
+                "mass_rel": mass}
+    rows = [row(1, 40.0), row(10, 26.0, 1e-3, 0.02), row(20, 25.0, 4e-3, 0.05), row(50, 24.0, 1e-2, 0.3)]
+    assert tuning.choose(rows)["regrid_every"] == 20
+    assert tuning.choose(rows, rise_rel=2e-3)["regrid_every"] == 10
+    rows[2]["mass_rel"] = 2e-6
+    assert tuning.choose(rows)["regrid_every"] == 10
+    assert tuning.choose([row(1, 40.0), row(10, 26.0, 1.0)])["regrid_every"] == 1
