@@ -13,14 +13,21 @@
 ! Macroscopic_Values_g_AMR.
 !
 ! Work placement. The driver calls the three whole-mesh ghost fills once per
-! step, each followed by per-block loops (Main_Loop, AMR_Rising_2D_OMP.f90:
-! 318-354, with Phase_Field_LBM_h_g_AMR :381-430 inlined into it). Each fill
-! here also does the per-block work that follows it, for every leaf on the
-! GPU, so the per-block calls return at once:
-!   Fill_Ghost_Cells_C2 : C2 fill, then Collision_h_g_AMR of every leaf
-!   Fill_Ghost_Cells_hg : hg fill, then the stream and C = SUM(h) of every leaf
-!   Fill_Ghost_Cells_C  : C fill, then Macroscopic_Values_g_AMR of every leaf
-! which is the order the driver runs them in, with nothing between.
+! step, each followed by per-block loops under its own timer (Main_Loop,
+! AMR_Rising_2D_OMP.f90:318-357, and Phase_Field_LBM_h_g_AMR :381-430). Each
+! phase runs once for every leaf on the GPU, inside the call the driver times
+! it with, and is complete on the device when that call returns:
+!   Fill_Ghost_Cells_C2                  : the C2 fill            (t_Ghost_C2)
+!   first Collision_h_g_AMR of the loop  : every leaf's collision (t_collision)
+!   Fill_Ghost_Cells_hg                  : the hg fill            (t_Ghost_hg)
+!   first Perfect_Shift_AMR or
+!   Streaming_LaxWendroff_AMR            : every leaf's stream    (t_advection)
+!   first Macroscopic_Values_h_AMR       : C = SUM(h)             (t_Macroscopic_h)
+!   Fill_Ghost_Cells_C                   : the C fill             (t_Ghost_C)
+!   first Macroscopic_Values_g_AMR       : U and P                (t_Macroscopic_g)
+! The loops call the per-block routines from OpenMP threads: the first call
+! of a loop (any thread) runs the phase in a critical section, the others
+! find it done.
 !
 ! State. The GPU holds the authoritative mesh from the first call of
 ! Fill_Ghost_Cells_C2 on (the first time step). Before that, during the
@@ -117,6 +124,10 @@ MODULE amrlbm_capi
 			TYPE(C_PTR), VALUE :: mesh
 			REAL(C_DOUBLE), VALUE :: rtol, dtol
 		END FUNCTION
+		INTEGER(C_INT) FUNCTION amrlbm_synchronize(mesh) BIND(C, NAME='amrlbm_synchronize')
+			IMPORT :: C_INT, C_PTR
+			TYPE(C_PTR), VALUE :: mesh
+		END FUNCTION
 		TYPE(C_PTR) FUNCTION amrlbm_last_error() BIND(C, NAME='amrlbm_last_error')
 			IMPORT :: C_PTR
 		END FUNCTION
@@ -134,8 +145,36 @@ MODULE amrlbm_dropin
 	LOGICAL, SAVE :: device_current = .False.	! the GPU holds the mesh state
 	LOGICAL, SAVE :: running = .False.			! the time loop has started
 	INTEGER, TARGET, ALLOCATABLE, SAVE :: nb_store(:,:)	! targets of Blk(n)%nghbr(k)%n
+	! the per-block loops' phases still to run this step: collision, stream,
+	! C = SUM(h), U and P (set by the whole-mesh call before each loop)
+	INTEGER, PARAMETER :: PH_COLLIDE = 1, PH_STREAM = 2, PH_MACRO_H = 3, PH_MACRO_G = 4
+	LOGICAL, SAVE :: due(4) = .False.
 
 CONTAINS
+
+	! the whole-mesh work of a per-block loop, run by the loop's first call
+	! (whichever OpenMP thread makes it) and complete on the device before
+	! that call returns
+	SUBROUTINE Run_Phase( ph )
+		INTEGER, INTENT(IN) :: ph
+		!$OMP CRITICAL (amrlbm_phase)
+		IF( due(ph) ) THEN
+			SELECT CASE( ph )
+			CASE( PH_COLLIDE )
+				CALL Check( amrlbm_collide(mesh), 'Collision_h_g_AMR' )
+			CASE( PH_STREAM )
+				CALL Check( amrlbm_stream(mesh), 'Streaming' )
+				due(PH_MACRO_H) = .True.
+			CASE( PH_MACRO_H )
+				CALL Check( amrlbm_macroscopic_values_h(mesh), 'Macroscopic_Values_h_AMR' )
+			CASE( PH_MACRO_G )
+				CALL Check( amrlbm_macroscopic_values_g(mesh), 'Macroscopic_Values_g_AMR' )
+			END SELECT
+			CALL Check( amrlbm_synchronize(mesh), 'synchronize' )
+			due(ph) = .False.
+		END IF
+		!$OMP END CRITICAL (amrlbm_phase)
+	END SUBROUTINE Run_Phase
 
 	SUBROUTINE Check( rc, what )
 		INTEGER(C_INT), INTENT(IN) :: rc
@@ -344,25 +383,27 @@ SUBROUTINE Fill_Ghost_Cells_C2		! Fill_Ghost_Cells_C2.f90:2-40
 	running = .True.
 	CALL Check_Levels
 	CALL Check( amrlbm_fill_ghost_cells_c2(mesh), 'Fill_Ghost_Cells_C2' )
-	! the driver collides every leaf next (AMR_Rising_2D_OMP.f90:391-399)
-	CALL Check( amrlbm_collide(mesh), 'Collision_h_g_AMR' )
+	CALL Check( amrlbm_synchronize(mesh), 'synchronize' )
+	! the driver's collision loop follows (AMR_Rising_2D_OMP.f90:391-399)
+	due(PH_COLLIDE) = .True.
 END SUBROUTINE Fill_Ghost_Cells_C2
 
 SUBROUTINE Fill_Ghost_Cells_hg		! Fill_Ghost_Cells_hg.f90:2-49
 	USE amrlbm_dropin
 	IMPLICIT NONE
 	CALL Check( amrlbm_fill_ghost_cells_hg(mesh), 'Fill_Ghost_Cells_hg' )
-	! then the stream (AMR_Rising_2D_OMP.f90:410-425) and C = SUM(h) (:328-335)
-	CALL Check( amrlbm_stream(mesh), 'Streaming' )
-	CALL Check( amrlbm_macroscopic_values_h(mesh), 'Macroscopic_Values_h_AMR' )
+	CALL Check( amrlbm_synchronize(mesh), 'synchronize' )
+	! the stream loop follows (AMR_Rising_2D_OMP.f90:410-425), then C = SUM(h) (:328-335)
+	due(PH_STREAM) = .True.
 END SUBROUTINE Fill_Ghost_Cells_hg
 
 SUBROUTINE Fill_Ghost_Cells_C		! Fill_Ghost_Cells_C.f90:2-40
 	USE amrlbm_dropin
 	IMPLICIT NONE
 	CALL Check( amrlbm_fill_ghost_cells_c(mesh), 'Fill_Ghost_Cells_C' )
-	! then U and P (AMR_Rising_2D_OMP.f90:346-354)
-	CALL Check( amrlbm_macroscopic_values_g(mesh), 'Macroscopic_Values_g_AMR' )
+	CALL Check( amrlbm_synchronize(mesh), 'synchronize' )
+	! the U and P loop follows (AMR_Rising_2D_OMP.f90:346-354)
+	due(PH_MACRO_G) = .True.
 END SUBROUTINE Fill_Ghost_Cells_C
 
 SUBROUTINE Refine_Derefine_Mesh( refinTol, derefTol )	! Refine_Derefine.f90:2-14
@@ -382,8 +423,8 @@ SUBROUTINE Refine_Derefine_Mesh( refinTol, derefTol )	! Refine_Derefine.f90:2-14
 	END IF
 END SUBROUTINE Refine_Derefine_Mesh
 
-! per-block routines whose work the fills above have done for every leaf; the
-! driver calls them from its OpenMP loops, so they only read module state
+! per-block routines: the first call of each loop runs the phase for every
+! leaf (Run_Phase), the others return
 
 SUBROUTINE Collision_h_g_AMR( nx, ny, dx, C, P, Ux, Uy, h, g )	! LBM_Subroutines.f90:116-189
 	USE amrlbm_dropin
@@ -392,6 +433,7 @@ SUBROUTINE Collision_h_g_AMR( nx, ny, dx, C, P, Ux, Uy, h, g )	! LBM_Subroutines
 	REAL(8), INTENT(IN) :: C(-1:nx+2,-1:ny+2), P(nx,ny), Ux(nx,ny), Uy(nx,ny)
 	REAL(8), INTENT(INOUT) :: h(9, 0:nx+1,0:ny+1), g(9, 0:nx+1,0:ny+1)
 	IF( .NOT. device_current ) CALL Host_Call_Error( 'Collision_h_g_AMR' )
+	CALL Run_Phase( PH_COLLIDE )
 END SUBROUTINE Collision_h_g_AMR
 
 SUBROUTINE Perfect_Shift_AMR( nx, ny, f )	! LBM_Subroutines.f90:41-62
@@ -400,6 +442,7 @@ SUBROUTINE Perfect_Shift_AMR( nx, ny, f )	! LBM_Subroutines.f90:41-62
 	INTEGER, INTENT(IN) :: nx, ny
 	REAL(8), INTENT(INOUT) :: f(9, 0:nx+1,0:ny+1)
 	IF( .NOT. device_current ) CALL Host_Call_Error( 'Perfect_Shift_AMR' )
+	CALL Run_Phase( PH_STREAM )
 END SUBROUTINE Perfect_Shift_AMR
 
 SUBROUTINE Streaming_LaxWendroff_AMR( nx, ny, cfl, f )	! LBM_Subroutines.f90:66-90
@@ -409,6 +452,7 @@ SUBROUTINE Streaming_LaxWendroff_AMR( nx, ny, cfl, f )	! LBM_Subroutines.f90:66-
 	REAL(8), INTENT(IN) :: cfl
 	REAL(8), INTENT(INOUT) :: f(9, 0:nx+1,0:ny+1)
 	IF( .NOT. device_current ) CALL Host_Call_Error( 'Streaming_LaxWendroff_AMR' )
+	CALL Run_Phase( PH_STREAM )
 END SUBROUTINE Streaming_LaxWendroff_AMR
 
 SUBROUTINE Macroscopic_Values_h_AMR( nx, ny, h, C )	! LBM_Subroutines.f90:193-201
@@ -418,6 +462,7 @@ SUBROUTINE Macroscopic_Values_h_AMR( nx, ny, h, C )	! LBM_Subroutines.f90:193-20
 	REAL(8), INTENT(IN) :: h(9, 0:nx+1, 0:ny+1)
 	REAL(8), INTENT(INOUT) :: C(-1:nx+2,-1:ny+2)
 	IF( .NOT. device_current ) CALL Host_Call_Error( 'Macroscopic_Values_h_AMR' )
+	CALL Run_Phase( PH_MACRO_H )
 END SUBROUTINE Macroscopic_Values_h_AMR
 
 SUBROUTINE Macroscopic_Values_g_AMR( nx, ny, dx, g, C, P, Ux, Uy )	! LBM_Subroutines.f90:205-240
@@ -427,6 +472,7 @@ SUBROUTINE Macroscopic_Values_g_AMR( nx, ny, dx, g, C, P, Ux, Uy )	! LBM_Subrout
 	REAL(8), INTENT(IN) :: g(9, 0:nx+1,0:ny+1), C(-1:nx+2,-1:ny+2)
 	REAL(8), INTENT(OUT) :: P(nx,ny), Ux(nx,ny), Uy(nx,ny)
 	IF( .NOT. device_current ) CALL Host_Call_Error( 'Macroscopic_Values_g_AMR' )
+	CALL Run_Phase( PH_MACRO_G )
 END SUBROUTINE Macroscopic_Values_g_AMR
 
 SUBROUTINE Host_Call_Error( what )

@@ -3,7 +3,8 @@ Tecplot writer (AMR_Rising_2D_OMP.f90, TecPlot_ASCII.f90) linked with the
 Fortran shim and libamrlbm.so in place of the hot-path files
 (oracle/build_dropin.sh). It must print the reference program's Output table
 and write its Tecplot files byte for byte (16x16 blocks, L0=64, three roots,
-12800 steps, regrid every step; mesh grows from 31 to 63 blocks)."""
+12800 steps, regrid every step; mesh grows from 31 to 63 blocks), with each
+of the reference's phase timers timing its own phase."""
 import hashlib
 import json
 import os
@@ -48,3 +49,13 @@ def test_reference_program_on_gpu_library(tmp_path):
            for f in sorted(os.listdir(tmp_path)) if f.startswith("Rising_Bubble_2D_")}
     assert tec == golden["tecplot"]
     print(f"[dropin] {len(got)} Output rows and {len(tec)} Tecplot files identical to the reference program")
+    # the reference's phase timers (AMR_Rising_2D_OMP.f90:366-375) each time their own phase on the GPU
+    timers = {}
+    for line in res.stdout.splitlines():
+        parts = line.replace(":", " ").split()
+        if len(parts) == 2 and parts[0].startswith("t_"):
+            timers[parts[0]] = float(parts[1])
+    print(f"[dropin] phase timers (s): {timers}")
+    for t in ("t_collision", "t_advection", "t_Ghost_C2", "t_Ghost_hg", "t_Ghost_C", "t_Macroscopic_h",
+              "t_Macroscopic_g", "t_refine"):
+        assert timers.get(t, 0.0) > 0.0, (t, timers)
