@@ -941,11 +941,12 @@ int ensure_int_buffer(int **buf, int *cap, int need)
     return AMRLBM_OK;
 }
 
-// Launch order of the leaves. A launch hands workgroup k to XCD k % 8, and each
-// XCD has its own L2, so the leaves go along a Morton curve (finest-cell
-// origin of the block), the curve is cut into 8 runs of neighbouring blocks,
-// and run c takes the positions k = c mod 8: a block and most of its
-// neighbours then run on one XCD in every launch. In an ensemble mesh the
+// Launch order of the leaves. A launch hands workgroup k to XCD (k + s) % 8,
+// s the XCD its first workgroup lands on (it varies from launch to launch),
+// and each XCD has its own L2, so the leaves go along a Morton curve
+// (finest-cell origin of the block), the curve is cut into 8 runs of
+// neighbouring blocks, and run c takes the positions k = c mod 8: a block and
+// most of its neighbours then run on one XCD in every launch. In an ensemble mesh the
 // curve runs instance by instance, each in its own frame (instances differ
 // in cell size, so their absolute origins overlap).
 void dispatch_order(amrlbm_mesh *m)
