@@ -196,6 +196,9 @@ void launch_prog_c(const DevMesh &m, int cur, const int *list, int n, int kind, 
                    double *err_slot = nullptr, const StepCtl &ctl = StepCtl{});
 // one int to v (a fill kernel, so that a captured graph holds no memset node)
 void launch_fill_int(int *p, int v, hipStream_t s);
+// out[kind][k] = slot[kind][list[k]] for k < n: the three kinds' ghost-program
+// headers in launch order from those kept by block slot
+void launch_gather_headers(PHdr *const slot[3], PHdr *const out[3], const int *list, int n, hipStream_t s);
 // hg fill from a ghost program (or ghosts from memory with prog.hdr == null),
 // then optionally the stream (+ C = sum h, + ghost ring carried over)
 void launch_prog_hg(const DevMesh &m, int cur, const int *list, int n, const Prog &prog, bool stream_after,

@@ -1745,6 +1745,27 @@ __global__ void k_fill_int(int *p, int v) { *p = v; }
 
 void launch_fill_int(int *p, int v, hipStream_t s) { hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(1), 0, s, p, v); }
 
+// the ghost-program headers of the leaves in launch order, from the headers
+// kept by block slot: out[kind][k] = slot[kind][list[k]]
+__global__ __launch_bounds__(256) void k_gather_headers(const PHdr *s0, const PHdr *s1, const PHdr *s2, PHdr *o0,
+                                                        PHdr *o1, PHdr *o2, const int *__restrict__ list, int n)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) {
+        const int slot = list[k];
+        o0[k] = s0[slot];
+        o1[k] = s1[slot];
+        o2[k] = s2[slot];
+    }
+}
+
+void launch_gather_headers(PHdr *const slot[3], PHdr *const out[3], const int *list, int n, hipStream_t s)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_gather_headers, dim3((n + 255) / 256), dim3(256), 0, s, slot[0], slot[1], slot[2], out[0],
+                           out[1], out[2], list, n);
+}
+
 // the hg kernel of operand depth d: 2/1, 4/2 or 8/4 recipes in flight per thread
 template <class T>
 static void hg_launch(int d, dim3 grid, size_t lds, hipStream_t s, const DevMeshT<T> &m, int cur, const int *list, int n,

@@ -147,6 +147,14 @@ struct amrlbm_mesh {
     std::vector<int> lvl_cnt;
     int all_off = 0;
     bool tables_dirty = true;
+    // the host's copy of the device block and neighbour tables; between two
+    // uploads, the blocks whose rows a regrid changed (rows_all: every row,
+    // after a set-up, a restart or a caller-built topology)
+    std::vector<BlockDev> h_bd;
+    std::vector<NbDesc> h_nbd;
+    std::vector<int> dirty_rows;
+    std::vector<char> dirty_mark;
+    bool rows_all = true;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_check = nullptr;
     hipEvent_t ev_tab = nullptr;   // after the last batch scatter (and the work before it)
     bool tab_recorded = false;     // ev_tab has been recorded (an unrecorded event must not be waited on)
@@ -164,7 +172,8 @@ struct amrlbm_mesh {
     // (n - 1) * kProgSlot[kind] of d_pnode (an arena of prog_slots slots),
     // so a mesh change uploads only the leaves whose programs were rebuilt;
     // per leaf (launch order) a header pointing there
-    PHdr *d_phdr[3] = {nullptr, nullptr, nullptr};
+    PHdr *d_phdr[3] = {nullptr, nullptr, nullptr};        // per leaf, launch order (gathered on the device)
+    PHdr *d_phdr_slot[3] = {nullptr, nullptr, nullptr};   // per block slot, as built
     PNode *d_pnode[3] = {nullptr, nullptr, nullptr};
     size_t phdr_cap[3] = {0, 0, 0};
     int prog_slots = 0;
@@ -182,6 +191,20 @@ struct amrlbm_mesh {
     long long prog_nodes_uploaded = 0;
     Prog prog[3] = {};
     std::vector<ProgCache> pcache;   // by block id
+    // by block id: the ids whose cached programs read the block (ProgCache
+    // deps); the blocks whose table rows the last upload changed, whose
+    // dependants are the only programs that can be stale (check_all: every
+    // leaf's, after a full upload)
+    std::vector<std::vector<int>> rdeps;
+    std::vector<int> changed_blocks;
+    bool check_all = true;
+    // the ids whose Refin or Deref flag the last flag pass set
+    std::vector<int> flagged;
+    // the leaves' Morton keys, sorted, kept across mesh changes (orders 0, 1,
+    // 3): the blocks whose rows changed since are re-keyed and merged in
+    std::vector<std::pair<unsigned long long, int>> morton;
+    std::vector<int> morton_changed;
+    bool morton_all = true;
     // per kind and block id: the header of its cached program (offset into its
     // slot), its aux cells and recipes, so a mesh change gathers the headers of
     // every leaf from compact arrays
@@ -327,14 +350,56 @@ void unplace(amrlbm_mesh *m, int n)
     if (b.level >= 0 && m->cell_at(b.level, b.I, b.J) == n) m->cell_at(b.level, b.I, b.J) = 0;
 }
 
-// Array_of_Leaf_Blocks (AMR_Rising_2D_OMP.f90:588-610)
-void array_of_leaf_blocks(amrlbm_mesh *m)
+// Array_of_Leaf_Blocks (AMR_Rising_2D_OMP.f90:588-610). rows_marked: the
+// caller has marked every block whose table row changed (mark_changed);
+// otherwise every row goes up again.
+void array_of_leaf_blocks(amrlbm_mesh *m, bool rows_marked = false)
 {
     m->leaf.clear();
     for (int n = 1; n <= m->nblks; n++)
         if (m->blk[n].isleaf) m->leaf.push_back(n);
     m->tables_dirty = true;
+    if (!rows_marked) {   // a topology from elsewhere: its flags as given
+        m->rows_all = true;
+        m->flagged.clear();
+        for (int n = 1; n <= m->nblks; n++)
+            if (m->blk[n].refin || m->blk[n].deref) m->flagged.push_back(n);
+    }
     m->topo_version++;
+}
+
+// block id's table row to go up with the next upload
+void mark_row(amrlbm_mesh *m, int id)
+{
+    if (id < 1) return;
+    if ((int)m->dirty_mark.size() <= id) m->dirty_mark.resize((size_t)id + 1025, 0);
+    if (!m->dirty_mark[id]) {
+        m->dirty_mark[id] = 1;
+        m->dirty_rows.push_back(id);
+    }
+}
+
+// The table rows a change of block t's leaf status or children can touch: its
+// own, its parent's (hasLeaf) and those of every block whose neighbour entries
+// may name t or its children: the blocks nb_value gives t (its level or
+// coarser), their parents and their children. Under 2:1 balance no finer block
+// touches t. Called on the final topology of a regrid, for every parent refined
+// or derefined and every child created; a freed child's row is marked alone.
+void mark_changed(amrlbm_mesh *m, int t)
+{
+    mark_row(m, t);
+    const Block &b = m->blk[t];
+    if (b.level < 0) return;
+    if (b.parent > 0) mark_row(m, b.parent);
+    for (int k = 1; k <= 8; k++) {
+        const int v = nb_value(m, t, k);
+        if (v <= 0) continue;
+        mark_row(m, v);
+        const Block &q = m->blk[v];
+        if (q.parent > 0) mark_row(m, q.parent);
+        if (!q.isleaf && q.has_child)
+            for (int c = 0; c < 4; c++) mark_row(m, q.child[c]);
+    }
 }
 
 // Determine_Min_Max_Levels (AMR_Rising_2D_OMP.f90:434-452)
@@ -721,18 +786,39 @@ int rebuild_programs(amrlbm_mesh *m)
     // (the idle team spins between regrids and takes host time from the step
     // loop) and 568 ms with 1.
     const int team = nl >= 128 ? std::min(omp_get_max_threads(), 8) : 1;
-    std::vector<char> stale(nl);
+    if (m->rdeps.size() < m->blk.size()) m->rdeps.resize(m->blk.size());
+    // the leaves to check: every one after a full upload, else those that read
+    // a block whose row changed, and the changed blocks that are leaves now
+    std::vector<int> cand;
+    if (m->check_all) {
+        cand = m->leaf;
+    } else {
+        for (int b : m->changed_blocks) {
+            if (m->blk[b].isleaf) cand.push_back(b);
+            for (int id : m->rdeps[b])
+                if (m->blk[id].isleaf) cand.push_back(id);
+        }
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    }
+    m->changed_blocks.clear();
+    m->check_all = false;
+    const int nc = (int)cand.size();
+    std::vector<char> stale(nc);
     std::vector<int> dirty;
+    std::vector<std::vector<std::pair<int, BlockSig>>> old_deps;
     int nd = 0;
 #pragma omp parallel num_threads(team)
     {
 #pragma omp for schedule(static)
-        for (int k = 0; k < nl; k++) stale[k] = !cache_current(m, m->leaf[k]);
+        for (int k = 0; k < nc; k++) stale[k] = !cache_current(m, cand[k]);
 #pragma omp single
         {
-            for (int k = 0; k < nl; k++)
-                if (stale[k]) dirty.push_back(m->leaf[k]);
+            for (int k = 0; k < nc; k++)
+                if (stale[k]) dirty.push_back(cand[k]);
             nd = (int)dirty.size();
+            old_deps.resize(nd);
+            for (int i = 0; i < nd; i++) old_deps[i] = std::move(m->pcache[dirty[i]].deps);
         }
         std::vector<int> d;
 #pragma omp for schedule(dynamic, 1)
@@ -773,6 +859,19 @@ int rebuild_programs(amrlbm_mesh *m)
         }
         m->cost_id[id] = program_operands(pc.p[KIND_C2]) + program_operands(pc.p[KIND_HG]);
         pc.valid = true;
+    }
+    // the reverse index follows each rebuilt program's dependencies
+    for (int i = 0; i < nd; i++) {
+        const int id = dirty[i];
+        for (const auto &dep : old_deps[i]) {
+            std::vector<int> &r = m->rdeps[dep.first];
+            auto it = std::find(r.begin(), r.end(), id);
+            if (it != r.end()) {
+                *it = r.back();
+                r.pop_back();
+            }
+        }
+        for (const auto &dep : m->pcache[id].deps) m->rdeps[dep.first].push_back(id);
     }
     m->progs_built += nd;
     if (std::getenv("AMRLBM_VERIFY_PROGRAMS")) {
@@ -863,7 +962,6 @@ int upload_programs(amrlbm_mesh *m)
 {
     HostTimer timer(m->host_ms_programs);
     const int nl = (int)m->leaf.size();
-    const int nthr = prog_threads(nl);
     // the arenas hold a slot per block id up to nblks; grown (and refilled) as ids pass it
     if (m->nblks > m->prog_slots) {
         const int cap = std::min(m->prm.max_blocks, std::max(m->nblks + m->nblks / 2, 1024));
@@ -875,17 +973,18 @@ int upload_programs(amrlbm_mesh *m)
         m->prog_slots = cap;
         for (ProgCache &pc : m->pcache) pc.on_device = false;
     }
-    std::vector<int> up;   // leaves whose programs are not in their slots yet
-    for (int k = 0; k < nl; k++)
-        if (!m->pcache[m->disp[k]].on_device) up.push_back(m->disp[k]);
+    std::vector<int> up;   // leaves whose programs are not in their slots yet, ascending id
+    for (int n : m->leaf)
+        if (!m->pcache[n].on_device) up.push_back(n);
     const int nup = (int)up.size();
     for (int kind = 0; kind < 3; kind++) {
+        // the LDS the kernels size by: serial (a team of another size costs more
+        // to wake than these integer reads take)
         int naux = 0, nmax = 0;
         const int *na = m->naux_id[kind].data(), *nn = m->nn_id[kind].data();
-#pragma omp parallel for schedule(static) num_threads(nthr) reduction(max : naux, nmax)
-        for (int k = 0; k < nl; k++) {
-            naux = std::max(naux, na[m->disp[k]]);
-            nmax = std::max(nmax, nn[m->disp[k]]);
+        for (int id : m->leaf) {
+            naux = std::max(naux, na[id]);
+            nmax = std::max(nmax, nn[id]);
         }
         if ((size_t)nl > m->phdr_cap[kind]) {
             if (m->d_phdr[kind]) HIPCHK(hipFree(m->d_phdr[kind]));
@@ -910,14 +1009,19 @@ int upload_programs(amrlbm_mesh *m)
         for (int i = 0; i < nup; i++)
             batch_seg(m, m->d_pnode[kind] + (size_t)(up[i] - 1) * kProgSlot[kind], base + sizeof(PNode) * at[i],
                       sizeof(PNode) * (at[i + 1] - at[i]));
-        // the headers of every leaf in launch order
-        rc = batch_alloc(m, sizeof(PHdr) * nl, &base);
+        // the headers of the leaves in up, each to its block slot; the kernels'
+        // launch-order headers are gathered from them on the device
+        // (launch_gather_headers, after the batch)
+        rc = batch_alloc(m, sizeof(PHdr) * nup, &base);
         if (rc) return rc;
         PHdr *hdr = reinterpret_cast<PHdr *>(m->h_up + base);
-        const PHdr *hid = m->hdr_id[kind].data();
-#pragma omp parallel for schedule(static) num_threads(nthr)
-        for (int k = 0; k < nl; k++) hdr[k] = hid[m->disp[k]];
-        batch_seg(m, m->d_phdr[kind], base, sizeof(PHdr) * nl);
+        for (int i = 0; i < nup; i++) hdr[i] = m->hdr_id[kind][up[i]];
+        for (int i = 0; i < nup;) {   // runs of consecutive ids, one segment each
+            int j = i + 1;
+            while (j < nup && up[j] == up[j - 1] + 1) j++;
+            batch_seg(m, m->d_phdr_slot[kind] + (up[i] - 1), base + sizeof(PHdr) * i, sizeof(PHdr) * (j - i));
+            i = j;
+        }
         m->prog[kind].hdr = m->d_phdr[kind];
         m->prog[kind].nodes = m->d_pnode[kind];
         m->prog[kind].naux = naux;   // the hg kernel sizes its LDS by these
@@ -949,49 +1053,126 @@ int ensure_int_buffer(int **buf, int *cap, int need)
 // most of its neighbours then run on one XCD in every launch. In an ensemble mesh the
 // curve runs instance by instance, each in its own frame (instances differ
 // in cell size, so their absolute origins overlap).
+// a leaf's Morton key: its instance above bit 40, then the finest-cell origin
+// of the block in the instance's frame, x and y bits interleaved
+unsigned long long morton_key(const amrlbm_mesh *m, int id)
+{
+    const Block &b = m->blk[id];
+    const int q = inst_of(m, b), J = b.J - q * (m->tile_roots << b.level);
+    const unsigned x = (unsigned)(b.I * b.dx * NX), y = (unsigned)(J * b.dx * NY);
+    // an ensemble's frames lie below 2^19 (L0 <= nx 2^14, two roots), leaving bits 40.. for the instance
+    const int nbits = m->ens ? 20 : 31;
+    unsigned long long z = 0;
+    for (int bit = 0; bit < nbits; bit++)
+        z |= (unsigned long long)((x >> bit) & 1u) << (2 * bit) | (unsigned long long)((y >> bit) & 1u) << (2 * bit + 1);
+    return (unsigned long long)q << 40 | z;
+}
+
 void dispatch_order(amrlbm_mesh *m)
 {
     const int n = (int)m->leaf.size();
-    std::vector<std::pair<unsigned long long, int>> key(n);
-#pragma omp parallel for schedule(static) if (n > 4096)
-    for (int k = 0; k < n; k++) {
-        const Block &b = m->blk[m->leaf[k]];
-        const int q = inst_of(m, b), J = b.J - q * (m->tile_roots << b.level);
-        const unsigned x = (unsigned)(b.I * b.dx * NX), y = (unsigned)(J * b.dx * NY);
-        // an ensemble's frames lie below 2^19 (L0 <= nx 2^14, two roots), leaving bits 40.. for the instance
-        const int nbits = m->ens ? 20 : 31;
-        unsigned long long z = 0;
-        for (int bit = 0; bit < nbits; bit++)
-            z |= (unsigned long long)((x >> bit) & 1u) << (2 * bit) | (unsigned long long)((y >> bit) & 1u) << (2 * bit + 1);
-        key[k] = {(unsigned long long)q << 40 | z, m->leaf[k]};
-    }
-    if (m->order == 2) {   // ascending id (Array_of_Leaf_Blocks)
+    if (m->order == 2) {   // ascending id (Array_of_Leaf_Blocks); the sorted keys are not kept meanwhile
         m->disp = m->leaf;
+        m->morton_all = true;
+        m->morton_changed.clear();
         return;
     }
-    if (n > 4096)
-        __gnu_parallel::sort(key.begin(), key.end());
-    else
-        std::sort(key.begin(), key.end());
+    // the sorted keys: after a mesh change, the entries of the blocks whose rows
+    // changed are dropped, and those that are leaves now come back re-keyed, in
+    // one merge pass
+    std::vector<std::pair<unsigned long long, int>> &sorted = m->morton;
+    if (!m->morton_all) {
+        std::vector<char> drop((size_t)m->nblks + 1, 0);
+        std::vector<std::pair<unsigned long long, int>> add;
+        for (int id : m->morton_changed) {
+            if (id > m->nblks || drop[id]) continue;
+            drop[id] = 1;
+            if (m->blk[id].isleaf) add.push_back({morton_key(m, id), id});
+        }
+        std::sort(add.begin(), add.end());
+        std::vector<std::pair<unsigned long long, int>> out;
+        out.reserve(n);
+        size_t a = 0;
+        for (const auto &e : sorted) {
+            if (drop[e.second]) continue;
+            while (a < add.size() && add[a] < e) out.push_back(add[a++]);
+            out.push_back(e);
+        }
+        while (a < add.size()) out.push_back(add[a++]);
+        sorted.swap(out);
+        if ((int)sorted.size() != n) m->morton_all = true;   // the leaf set disagrees: start over
+    }
+    if (m->morton_all) {
+        sorted.resize(n);
+#pragma omp parallel for schedule(static) num_threads(std::min(omp_get_max_threads(), 8)) if (n > 4096)
+        for (int k = 0; k < n; k++) sorted[k] = {morton_key(m, m->leaf[k]), m->leaf[k]};
+        if (n > 4096)
+            __gnu_parallel::sort(sorted.begin(), sorted.end());
+        else
+            std::sort(sorted.begin(), sorted.end());
+        m->morton_all = false;
+    }
+    m->morton_changed.clear();
     if (m->order == 1) {   // the Morton curve as it runs
         m->disp.resize(n);
-        for (int k = 0; k < n; k++) m->disp[k] = key[k].second;
+        for (int k = 0; k < n; k++) m->disp[k] = sorted[k].second;
         return;
+    }
+    std::vector<std::pair<unsigned long long, int>> runs;
+    const std::vector<std::pair<unsigned long long, int>> *key = &sorted;
+    if (m->order == 3) {   // each run's blocks with the most ghost operand loads first (on a copy)
+        runs = sorted;
+        key = &runs;
     }
     const int q = n / 8, r = n % 8;   // runs 0..r-1 hold q + 1 blocks, the others q
     m->disp.assign(n, 0);
     int start = 0;
     for (int c = 0; c < 8; c++) {
         const int len = q + (c < r ? 1 : 0);
-        if (m->order == 3)   // the run's blocks with the most ghost operand loads first
-            std::stable_sort(key.begin() + start, key.begin() + start + len,
-                             [m](const std::pair<unsigned long long, int> &a, const std::pair<unsigned long long, int> &b) {
-                                 return m->cost_id[a.second] > m->cost_id[b.second];
+        if (m->order == 3)
+            std::stable_sort(runs.begin() + start, runs.begin() + start + len,
+                             [m](const std::pair<unsigned long long, int> &x, const std::pair<unsigned long long, int> &y) {
+                                 return m->cost_id[x.second] > m->cost_id[y.second];
                              });
-        for (int i = 0; i < len; i++) m->disp[8 * i + c] = key[start + i].second;
+        for (int i = 0; i < len; i++) m->disp[8 * i + c] = (*key)[start + i].second;
         start += len;
     }
 }
+
+// the device table row of block n (BlockDev and its 8 NbDesc) into bd, nbt;
+// a missing neighbour (2:1 balance violated) sets *bad_k
+void table_row(amrlbm_mesh *m, int n, BlockDev &d, NbDesc *nbt, int *bad_k)
+{
+    const Block &b = m->blk[n];
+    d = BlockDev{};
+    for (int k = 0; k < 8; k++) nbt[k] = NbDesc{0, -1, 0, -1};
+    d.level = b.level;
+    d.dx = b.dx;
+    d.I = b.I;
+    d.J = b.J;
+    d.isleaf = b.isleaf ? 1 : 0;
+    d.child0 = (b.level >= 0 && !b.isleaf && b.has_child) ? b.child[0] - 1 : -1;
+    d.hasleaf = (b.level >= 0 && b.hasleaf) ? 1 : 0;
+    d.inst = b.level >= 0 ? inst_of(m, b) : 0;
+    if (b.level < 0) return;
+    for (int k = 1; k <= 8; k++) {
+        const int v = nb_value(m, n, k);
+        if (v == 0) {
+            if (!*bad_k) *bad_k = k;
+            continue;
+        }
+        NbDesc &e = nbt[k - 1];
+        e.id = v;
+        if (v > 0) {
+            const Block &q = m->blk[v];
+            e.level = q.level;
+            e.isleaf = q.isleaf ? 1 : 0;
+            e.child0 = (!q.isleaf && q.has_child) ? q.child[0] - 1 : -1;
+        }
+    }
+}
+
+bool same_row(const BlockDev &a, const BlockDev &b) { return std::memcmp(&a, &b, sizeof(BlockDev)) == 0; }
 
 int upload_tables(amrlbm_mesh *m)
 {
@@ -1001,53 +1182,87 @@ int upload_tables(amrlbm_mesh *m)
     // tables, lists and programs go out in one upload batch
     int rcb = batch_begin(m);
     if (rcb) return rcb;
-    size_t tab_at = 0;
-    rcb = batch_alloc(m, sizeof(BlockDev) * nb + sizeof(NbDesc) * 8 * (size_t)nb, &tab_at);
-    if (rcb) return rcb;
-    BlockDev *bd = reinterpret_cast<BlockDev *>(m->h_up + tab_at);
-    NbDesc *nbt = reinterpret_cast<NbDesc *>(bd + nb);
+    if ((int)m->h_bd.size() < nb) {
+        m->h_bd.resize(nb);
+        m->h_nbd.resize(8 * (size_t)nb);
+    }
     int bad_block = 0, bad_k = 0;   // a block without neighbour k (2:1 balance violated)
-    // the same team size as the program rebuild that follows, so that it finds the threads awake
+    if (m->rows_all) {
+        // every row, on the same team size as the program rebuild that follows (it finds the threads awake)
 #pragma omp parallel for schedule(static) num_threads(nb >= 256 ? std::min(omp_get_max_threads(), 8) : 1)
-    for (int n = 1; n <= nb; n++) {
-        const Block &b = m->blk[n];
-        BlockDev &d = bd[n - 1];
-        d = BlockDev{};
-        for (int k = 0; k < 8; k++) nbt[(size_t)(n - 1) * 8 + k] = NbDesc{0, -1, 0, -1};
-        d.level = b.level;
-        d.dx = b.dx;
-        d.I = b.I;
-        d.J = b.J;
-        d.isleaf = b.isleaf ? 1 : 0;
-        d.child0 = (b.level >= 0 && !b.isleaf && b.has_child) ? b.child[0] - 1 : -1;
-        d.hasleaf = (b.level >= 0 && b.hasleaf) ? 1 : 0;
-        d.inst = b.level >= 0 ? inst_of(m, b) : 0;
-        if (b.level >= 0)
-            for (int k = 1; k <= 8; k++) {
-                int v = nb_value(m, n, k);
-                if (v == 0) {
+        for (int n = 1; n <= nb; n++) {
+            int k = 0;
+            table_row(m, n, m->h_bd[n - 1], &m->h_nbd[(size_t)(n - 1) * 8], &k);
+            if (k) {
 #pragma omp critical(amrlbm_bad_nb)
-                    if (!bad_block || n < bad_block) {
-                        bad_block = n;
-                        bad_k = k;
-                    }
-                    continue;
-                }
-                NbDesc &e = nbt[(size_t)(n - 1) * 8 + k - 1];
-                e.id = v;
-                if (v > 0) {
-                    const Block &q = m->blk[v];
-                    e.level = q.level;
-                    e.isleaf = q.isleaf ? 1 : 0;
-                    e.child0 = (!q.isleaf && q.has_child) ? q.child[0] - 1 : -1;
+                if (!bad_block || n < bad_block) {
+                    bad_block = n;
+                    bad_k = k;
                 }
             }
+        }
+        size_t tab_at = 0;
+        rcb = batch_alloc(m, sizeof(BlockDev) * nb + sizeof(NbDesc) * 8 * (size_t)nb, &tab_at);
+        if (rcb) return rcb;
+        std::memcpy(m->h_up + tab_at, m->h_bd.data(), sizeof(BlockDev) * nb);
+        std::memcpy(m->h_up + tab_at + sizeof(BlockDev) * nb, m->h_nbd.data(), sizeof(NbDesc) * 8 * (size_t)nb);
+        batch_seg(m, m->d_blk, tab_at, sizeof(BlockDev) * nb);
+        batch_seg(m, m->d_nbd, tab_at + sizeof(BlockDev) * nb, sizeof(NbDesc) * 8 * (size_t)nb);
+    } else {
+        // the rows the regrid marked, in runs of consecutive ids
+        std::sort(m->dirty_rows.begin(), m->dirty_rows.end());
+        std::vector<int> &d = m->dirty_rows;
+        for (int n : d) {
+            if (n > nb) continue;
+            int k = 0;
+            table_row(m, n, m->h_bd[n - 1], &m->h_nbd[(size_t)(n - 1) * 8], &k);
+            if (k && (!bad_block || n < bad_block)) {
+                bad_block = n;
+                bad_k = k;
+            }
+        }
+        for (size_t i = 0; i < d.size();) {
+            if (d[i] > nb) break;
+            size_t j = i + 1;
+            while (j < d.size() && d[j] == d[j - 1] + 1 && d[j] <= nb) j++;
+            const int first = d[i] - 1, cnt = (int)(j - i);
+            size_t at = 0;
+            rcb = batch_alloc(m, sizeof(BlockDev) * cnt + sizeof(NbDesc) * 8 * (size_t)cnt, &at);
+            if (rcb) return rcb;
+            std::memcpy(m->h_up + at, &m->h_bd[first], sizeof(BlockDev) * cnt);
+            std::memcpy(m->h_up + at + sizeof(BlockDev) * cnt, &m->h_nbd[(size_t)first * 8], sizeof(NbDesc) * 8 * cnt);
+            batch_seg(m, m->d_blk + first, at, sizeof(BlockDev) * cnt);
+            batch_seg(m, m->d_nbd + (size_t)first * 8, at + sizeof(BlockDev) * cnt, sizeof(NbDesc) * 8 * (size_t)cnt);
+            i = j;
+        }
+        if (std::getenv("AMRLBM_VERIFY_TABLES")) {   // every row against a fresh build
+            for (int n = 1; n <= nb; n++) {
+                BlockDev fb;
+                NbDesc fn[8];
+                int k = 0;
+                table_row(m, n, fb, fn, &k);
+                if (!same_row(fb, m->h_bd[n - 1]) ||
+                    std::memcmp(fn, &m->h_nbd[(size_t)(n - 1) * 8], sizeof(fn)) != 0)
+                    return fail(AMRLBM_ERR_TOPOLOGY, "table row of block " + std::to_string(n) +
+                                                         " was not marked by the regrid that changed it");
+            }
+        }
     }
+    for (int n : m->dirty_rows) m->dirty_mark[n] = 0;
+    if (m->rows_all) {
+        m->check_all = true;
+        m->changed_blocks.clear();
+        m->morton_all = true;
+        m->morton_changed.clear();
+    } else {
+        m->changed_blocks.insert(m->changed_blocks.end(), m->dirty_rows.begin(), m->dirty_rows.end());
+        m->morton_changed.insert(m->morton_changed.end(), m->dirty_rows.begin(), m->dirty_rows.end());
+    }
+    m->dirty_rows.clear();
+    m->rows_all = false;
     if (bad_block)
         return fail(AMRLBM_ERR_TOPOLOGY, "block " + std::to_string(bad_block) + " has no neighbour " +
                                              std::to_string(bad_k) + " (2:1 balance violated)");
-    batch_seg(m, m->d_blk, tab_at, sizeof(BlockDev) * nb);
-    batch_seg(m, m->d_nbd, tab_at + sizeof(BlockDev) * nb, sizeof(NbDesc) * 8 * (size_t)nb);
     if (!m->leaf.empty()) {   // the programs first: launch order 3 weighs them
         int rc0 = rebuild_programs(m);
         if (rc0) return rc0;
@@ -1079,6 +1294,10 @@ int upload_tables(amrlbm_mesh *m)
     }
     rc = batch_flush(m);
     if (rc) return rc;
+    if (!m->leaf.empty()) {
+        launch_gather_headers(m->d_phdr_slot, m->d_phdr, m->d_lists + m->all_off, (int)m->leaf.size(), m->stream);
+        HIPCHK(hipGetLastError());
+    }
     m->tables_dirty = false;
     return AMRLBM_OK;
 }
@@ -1172,6 +1391,8 @@ int phase_macro_g(amrlbm_mesh *m)
 // ---------------------------------------------------------------------------
 // extra: blocks flagged for refinement besides the criterion; frozen: the
 // instances of an ensemble mesh left as they are (no flags)
+void clear_regrid_flags(amrlbm_mesh *m);
+
 // Refine and Deref flags of Refine_Derefine_Mesh (Refine_Derefine.f90:18-194)
 // computed on the host from Err_Interface of every leaf (downloaded here)
 int host_flags(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector<char> *extra,
@@ -1268,9 +1489,11 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
     HostTimer timer(m->host_ms_regrid);
     if (dev_flags) {
         // the device decision's flags (k_regrid_check), as host_flags sets them
-        for (int n = 1; n <= m->nblks; n++) m->blk[n].refin = m->blk[n].deref = false;
+        clear_regrid_flags(m);
         for (int i = 0; i < dev_flags[1]; i++) m->blk[dev_flags[3 + i] + 1].refin = true;
         for (int i = 0; i < dev_flags[2]; i++) m->blk[dev_flags[kRegridDerefAt + i] + 1].deref = true;
+        for (int i = 0; i < dev_flags[1]; i++) m->flagged.push_back(dev_flags[3 + i] + 1);
+        for (int i = 0; i < dev_flags[2]; i++) m->flagged.push_back(dev_flags[kRegridDerefAt + i] + 1);
         if (std::getenv("AMRLBM_VERIFY_FLAGS")) {
             std::vector<char> dev(2 * (m->nblks + 1));
             for (int n = 1; n <= m->nblks; n++) {
@@ -1283,24 +1506,39 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
                     return fail(AMRLBM_ERR_STATE, "device regrid flags of block " + std::to_string(n) +
                                                       " differ from the host's");
         }
-    } else if ((rc = host_flags(m, refin_tol, deref_tol, extra, frozen, d_pred, pred_count))) {
-        return rc;
+    } else {
+        if ((rc = host_flags(m, refin_tol, deref_tol, extra, frozen, d_pred, pred_count))) return rc;
+        m->flagged.clear();
+        for (int n = 1; n <= m->nblks; n++)
+            if (m->blk[n].refin || m->blk[n].deref) m->flagged.push_back(n);
     }
+    // the flagged blocks in the reference's loop orders: Derefine by ascending
+    // id (the freed children are unflagged leaves, so freeing never flags or
+    // unflags a later block), Refine level by level, each level by ascending
+    // id (the children it creates are unflagged, so the passes over every id
+    // would find the same blocks in the same order)
+    std::vector<int> dlist, rlist;
+    for (int n : m->flagged) {
+        if (m->blk[n].deref) dlist.push_back(n);
+        if (m->blk[n].refin && m->blk[n].level >= m->levelmin && m->blk[n].level <= m->levelmax) rlist.push_back(n);
+    }
+    std::sort(dlist.begin(), dlist.end());
+    std::sort(rlist.begin(), rlist.end(), [m](int a, int b) {
+        return m->blk[a].level != m->blk[b].level ? m->blk[a].level < m->blk[b].level : a < b;
+    });
     // Derefine (:198-211): restrict on the device, then free the children
     std::vector<int> rpairs;
-    for (int n = 1; n <= m->nblks; n++)
-        if (m->blk[n].deref) {
-            rpairs.push_back(n - 1);
-            rpairs.push_back(m->blk[n].child[0] - 1);
-            m->blk[n].alloc = true;
-            remove_children(m, n);
-        }
+    for (int n : dlist) {
+        rpairs.push_back(n - 1);
+        rpairs.push_back(m->blk[n].child[0] - 1);
+        m->blk[n].alloc = true;
+        remove_children(m, n);
+    }
     // Refine (:262-294): ids from the free-list head or Nblks+1
     std::vector<int> ppairs;
-    for (int L = m->levelmin; L <= m->levelmax; L++) {
-        int nend = m->nblks;
-        for (int n = 1; n <= nend; n++) {
-            if (m->blk[n].level != L || !m->blk[n].refin) continue;
+    {
+        for (int n : rlist) {
+            if (!m->blk[n].refin) continue;
             int c1;
             if (!m->freeq.empty()) {
                 c1 = m->freeq.front();
@@ -1344,7 +1582,15 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
         // no wait here: the table upload below is ordered after these kernels on the stream
     }
     if (rpairs.empty() && ppairs.empty()) return AMRLBM_OK;   // flags only: tables and graphs stay valid
-    array_of_leaf_blocks(m);
+    for (size_t i = 0; i < rpairs.size(); i += 2) {
+        mark_changed(m, rpairs[i] + 1);
+        for (int c = rpairs[i + 1] + 1; c <= rpairs[i + 1] + 4; c++) mark_row(m, c);   // freed (or reused below)
+    }
+    for (size_t i = 0; i < ppairs.size(); i += 2) {
+        mark_changed(m, ppairs[i] + 1);
+        for (int c = ppairs[i + 1] + 1; c <= ppairs[i + 1] + 4; c++) mark_changed(m, c);
+    }
+    array_of_leaf_blocks(m, true);
     timer.stop();
     return upload_tables(m);
 }
@@ -1423,6 +1669,7 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipMemsetAsync(m->dm.PU, 0, e * PUB * S, m->stream));
     HIPCHK(hipMalloc(&m->d_blk, sizeof(BlockDev) * S));
     HIPCHK(hipMalloc(&m->d_nbd, sizeof(NbDesc) * 8 * S));
+    for (int kind = 0; kind < 3; kind++) HIPCHK(hipMalloc(&m->d_phdr_slot[kind], sizeof(PHdr) * S));
     HIPCHK(hipMalloc(&m->d_err_flags, sizeof(int)));
     HIPCHK(hipMemsetAsync(m->d_err_flags, 0, sizeof(int), m->stream));
     HIPCHK(hipMalloc(&m->d_sync, sizeof(unsigned) * kSegSyncWords));
@@ -1460,6 +1707,7 @@ void reset_topology(amrlbm_mesh *m)
     m->freeq.clear();
     m->leaf.clear();
     m->tables_dirty = true;
+    m->rows_all = true;
     m->topo_version++;
 }
 
@@ -1659,9 +1907,11 @@ int enqueue_regrid_check(amrlbm_mesh *m)
 }
 
 // the flags a regrid that changes nothing leaves: Refin and Deref all false
+// (those the last flag pass set)
 void clear_regrid_flags(amrlbm_mesh *m)
 {
-    for (int n = 1; n <= m->nblks; n++) m->blk[n].refin = m->blk[n].deref = false;
+    for (int n : m->flagged) m->blk[n].refin = m->blk[n].deref = false;
+    m->flagged.clear();
 }
 
 // steps to the next regrid (or to the end of the call)
@@ -1767,6 +2017,7 @@ int amrlbm_destroy(amrlbm_mesh *m)
     (void)hipFree(m->d_nbd);
     for (int k = 0; k < 3; k++) {
         (void)hipFree(m->d_phdr[k]);
+        (void)hipFree(m->d_phdr_slot[k]);
         (void)hipFree(m->d_pnode[k]);
     }
     if (m->h_up) (void)hipHostFree(m->h_up);

@@ -135,16 +135,22 @@ def test_fused_steps_across_mesh_changes_equal_oracle(amrlbm, oracle_mod, worklo
 def test_device_flags_equal_host_flags(amrlbm, workloads, monkeypatch, case):
     """Every mesh change of amrlbm_step takes its Refine and Deref flags from
     the device decision (k_regrid_check: the criterion, the 2:1 closure, the
-    derefinement flags and their level-by-level check). With
-    AMRLBM_VERIFY_FLAGS set the library also flags each such state on the host
-    (Err_Interface downloaded afresh) and fails on any difference: cfg2 over
-    4,000 steps in FP64 and FP32, and the 64-instance ensemble (per-instance
-    level ranges, about 45,000 block slots) over 1,000 steps."""
+    derefinement flags and their level-by-level check), and uploads only the
+    table rows the regrid marked, and rebuilds only the ghost programs that
+    read a block whose row changed. With AMRLBM_VERIFY_FLAGS set the library
+    also flags each such state on the host (Err_Interface downloaded afresh),
+    with AMRLBM_VERIFY_TABLES it rebuilds every table row after each upload,
+    with AMRLBM_VERIFY_PROGRAMS every program, and it fails on any
+    difference: cfg2 over 4,000 steps in FP64 and FP32,
+    and the 64-instance ensemble (per-instance level ranges, about 45,000
+    block slots) over 1,000 steps."""
     from amrlbm.ensemble import ensemble
     name, prec = case.rsplit("-", 1)
     w = ensemble(64, seed=2024) if name == "ens64" else workloads[name]
     steps = 1000 if name == "ens64" else 4000
     monkeypatch.setenv("AMRLBM_VERIFY_FLAGS", "1")
+    monkeypatch.setenv("AMRLBM_VERIFY_TABLES", "1")
+    monkeypatch.setenv("AMRLBM_VERIFY_PROGRAMS", "1")
     print(f"[flags] {case}: {steps} steps, regrid every 10", flush=True)
     with amrlbm.Mesh(w, precision=int(prec)) as m:
         m.setup_rising_bubble()
@@ -153,3 +159,21 @@ def test_device_flags_equal_host_flags(amrlbm, workloads, monkeypatch, case):
     print(f"[flags] {case}: {st}", flush=True)
     assert st["changes"] >= 2, st
     assert st["device_flag_changes"] == st["changes"], st
+
+
+@pytest.mark.parametrize("name", ["tiny_rise", "cfg1_rise"])
+def test_table_rows_marked_on_every_regrid(amrlbm, workloads, monkeypatch, name):
+    """Regrid after every step (the reference's cadence) on the fast-rising
+    bubbles, with AMRLBM_VERIFY_TABLES and AMRLBM_VERIFY_PROGRAMS set: after
+    each upload every table row and every ghost program equals a fresh build,
+    so every row a refinement, derefinement or freed-id reuse changed was
+    marked and uploaded, and every program reading it rebuilt."""
+    g = RISE[name]
+    monkeypatch.setenv("AMRLBM_VERIFY_TABLES", "1")
+    monkeypatch.setenv("AMRLBM_VERIFY_PROGRAMS", "1")
+    with amrlbm.Mesh(workloads[g["workload"]]) as m:
+        m.setup_rising_bubble()
+        m.step(g["steps"], 1)
+        st = m.regrid_stats()
+    print(f"[tables] {name}: {st}", flush=True)
+    assert st["changes"] >= len(g["change_steps"]), st
