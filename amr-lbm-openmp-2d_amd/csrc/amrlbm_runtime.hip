@@ -128,7 +128,7 @@ struct amrlbm_mesh {
     bool has_net = false;
     float net_threshold = 0.0f;
     double net_gate = 0.25;          // Err_Interface a predicted block must reach, in refinement tolerances
-    int *d_lists = nullptr;          // leaves grouped by level, then all leaves (ascending)
+    int *d_lists = nullptr;          // the leaves in launch order (slot indices, id - 1)
     int lists_cap = 0;
     int *d_pairs = nullptr;          // restrict / prolong work lists
     int pairs_cap = 0;
@@ -143,9 +143,6 @@ struct amrlbm_mesh {
     double *h_pinned = nullptr;      // staging (err, diagnostics)
     double *scratch = nullptr;       // this device's copy of the file-scope scratch array
     int cur = 0;
-    std::vector<int> lvl_off;        // offsets into d_lists per level (levels 0..max_levels), + all-leaves
-    std::vector<int> lvl_cnt;
-    int all_off = 0;
     bool tables_dirty = true;
     // the host's copy of the device block and neighbour tables; between two
     // uploads, the blocks whose rows a regrid changed (rows_all: every row,
@@ -155,6 +152,11 @@ struct amrlbm_mesh {
     std::vector<int> dirty_rows;
     std::vector<char> dirty_mark;
     bool rows_all = true;
+    // the ids whose rows the last upload rebuilt, ascending (every id whose
+    // leaf status changed is among them); leaf_delta_all: every id
+    std::vector<int> leaf_delta;
+    bool leaf_delta_all = true;
+    bool leaf_list_bad = false;      // AMRLBM_VERIFY_TABLES: a merged leaf list differed from a scan
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_check = nullptr;
     hipEvent_t ev_tab = nullptr;   // after the last batch scatter (and the work before it)
     bool tab_recorded = false;     // ev_tab has been recorded (an unrecorded event must not be waited on)
@@ -202,8 +204,9 @@ struct amrlbm_mesh {
     std::vector<int> flagged;
     // the leaves' Morton keys, sorted, kept across mesh changes (orders 0, 1,
     // 3): the blocks whose rows changed since are re-keyed and merged in
-    std::vector<std::pair<unsigned long long, int>> morton;
+    std::vector<std::pair<unsigned long long, int>> morton, morton_next;
     std::vector<int> morton_changed;
+    std::vector<char> morton_drop;
     bool morton_all = true;
     // per kind and block id: the header of its cached program (offset into its
     // slot), its aux cells and recipes, so a mesh change gathers the headers of
@@ -211,6 +214,13 @@ struct amrlbm_mesh {
     std::vector<PHdr> hdr_id[3];
     std::vector<int> naux_id[3], nn_id[3];
     std::vector<int> cost_id;   // operand loads of the C2 and hg programs (launch order 3)
+    // ids whose programs were rebuilt since the last program upload
+    std::vector<int> prog_pending;
+    // per kind, over the leaves: how many programs have each aux-cell count
+    // and each recipe count (the kernels size their LDS by the largest);
+    // counted: the id's naux_id / nn_id are in the counts
+    std::vector<int> hist_aux[3], hist_nn[3];
+    std::vector<char> counted;
     long long progs_built = 0;       // leaves whose programs were (re)built
 
     // one HIP graph per (segment length, population-buffer parity), updated
@@ -355,9 +365,35 @@ void unplace(amrlbm_mesh *m, int n)
 // otherwise every row goes up again.
 void array_of_leaf_blocks(amrlbm_mesh *m, bool rows_marked = false)
 {
-    m->leaf.clear();
-    for (int n = 1; n <= m->nblks; n++)
-        if (m->blk[n].isleaf) m->leaf.push_back(n);
+    if (rows_marked && !m->rows_all) {
+        // every id whose leaf status changed has its row marked: the list
+        // without the marked ids, merged with those that are leaves now
+        std::vector<int> &d = m->dirty_rows;
+        std::sort(d.begin(), d.end());
+        const std::vector<int> &old = m->leaf;
+        std::vector<int> out;
+        out.reserve(old.size() + d.size());
+        auto it = old.begin();
+        for (int id : d) {
+            auto lo = std::lower_bound(it, old.end(), id);
+            out.insert(out.end(), it, lo);
+            it = lo;
+            if (it != old.end() && *it == id) ++it;
+            if (id <= m->nblks && m->blk[id].isleaf) out.push_back(id);
+        }
+        out.insert(out.end(), it, old.end());
+        m->leaf.swap(out);
+        if (std::getenv("AMRLBM_VERIFY_TABLES")) {
+            std::vector<int> full;
+            for (int n = 1; n <= m->nblks; n++)
+                if (m->blk[n].isleaf) full.push_back(n);
+            if (full != m->leaf) m->leaf_list_bad = true;
+        }
+    } else {
+        m->leaf.clear();
+        for (int n = 1; n <= m->nblks; n++)
+            if (m->blk[n].isleaf) m->leaf.push_back(n);
+    }
     m->tables_dirty = true;
     if (!rows_marked) {   // a topology from elsewhere: its flags as given
         m->rows_all = true;
@@ -523,9 +559,40 @@ int nb_value(amrlbm_mesh *m, int n, int k);
 struct ProgBuild {
     amrlbm_mesh *m;
     int kind, R, n;
-    std::vector<PNode> nodes;
-    std::vector<int> nstage;                 // stage of each node
-    std::vector<int> cell_stage;             // stage of the node that writes LDS cell c
+    // the node list, each node's stage and the stage of the node that writes
+    // LDS cell c: the host thread's, reused by every build on that thread
+    struct Lists {
+        std::vector<PNode> nodes;
+        std::vector<int> nstage;
+        std::vector<int> cell_stage = std::vector<int>((FS > CS ? FS : CS) + NAUX_MAX, 0);
+    };
+    static Lists &fresh_lists()
+    {
+        thread_local Lists tl;
+        tl.nodes.clear();
+        tl.nstage.clear();
+        return tl;
+    }
+    Lists &lists = fresh_lists();
+    std::vector<PNode> &nodes = lists.nodes;
+    std::vector<int> &nstage = lists.nstage;
+    std::vector<int> &cell_stage = lists.cell_stage;   // sized for every dest; read only once written
+    // the neighbours (nb_value) of the blocks this build reads, asked for up to
+    // three times per node: direct-mapped by id, each k filled when first asked
+    static constexpr int NBC = 32, NB_UNSET = 1 << 30;
+    int nbc_id[NBC] = {};
+    int nbc_v[NBC][8];
+    int nbv(int b, int k)
+    {
+        const int h = b & (NBC - 1);
+        if (nbc_id[h] != b) {
+            nbc_id[h] = b;
+            for (int j = 0; j < 8; j++) nbc_v[h][j] = NB_UNSET;
+        }
+        int &v = nbc_v[h][k - 1];
+        if (v == NB_UNSET) v = nb_value(m, b, k);
+        return v;
+    }
     // node memo: open addressing on (block, x, y); -2 marks a node being built.
     // The table is the host thread's, reused by every build on that thread: a
     // slot belongs to this build iff its stamp is the build's.
@@ -548,10 +615,17 @@ struct ProgBuild {
     }
     int naux = 0;
     std::string err;
-    std::vector<int> *deps = nullptr;   // every block the build reads (see ProgCache)
-    void dep(int b)   // runs of one block (most refs of a node) are kept once; the caller sorts and dedupes
+    // every block the build reads (see ProgCache), each once: a block goes
+    // into deps when its mark is not the caller's stamp yet
+    std::vector<int> *deps = nullptr;
+    unsigned *dep_mark = nullptr;
+    unsigned dep_stamp = 0;
+    void dep(int b)
     {
-        if (deps && b > 0 && (deps->empty() || deps->back() != b)) deps->push_back(b);
+        if (deps && b > 0 && dep_mark[b] != dep_stamp) {
+            dep_mark[b] = dep_stamp;
+            deps->push_back(b);
+        }
     }
 
     int own_cells() const { return kind == KIND_HG ? FS : CS; }
@@ -596,10 +670,10 @@ struct ProgBuild {
         int st = 0, s2 = 0;
         dep(b);
         const Block &B = m->blk[b];
-        if (y <= 0) dep(nb_value(m, b, 3));
-        if (y >= NY + 1) dep(nb_value(m, b, 4));
-        const bool bot = y <= 0 && nb_value(m, b, 3) < 0;
-        const bool top = y >= NY + 1 && nb_value(m, b, 4) < 0;
+        if (y <= 0) dep(nbv(b, 3));
+        if (y >= NY + 1) dep(nbv(b, 4));
+        const bool bot = y <= 0 && nbv(b, 3) < 0;
+        const bool top = y >= NY + 1 && nbv(b, 4) < 0;
         if (bot || top) {
             if (kind == KIND_HG) {   // Bounce_Back_3/4 (BC_Cell.f90:2-42)
                 const int yi = bot ? 1 : NY;
@@ -618,7 +692,7 @@ struct ProgBuild {
             static const int dtab[3][3] = {{4, 0, 5}, {2, -1, 3}, {6, 1, 7}};   // [sx+1][sy+1]
             const int sx = x < 1 ? -1 : (x > NX ? 1 : 0), sy = y < 1 ? -1 : (y > NY ? 1 : 0);
             const int d = dtab[sx + 1][sy + 1];
-            const int N1 = nb_value(m, b, d + 1);
+            const int N1 = d < 0 ? 0 : nbv(b, d + 1);
             dep(N1);
             if (d < 0 || N1 <= 0) {
                 err = "ghost cell (" + std::to_string(x) + "," + std::to_string(y) + ") of block " + std::to_string(b) +
@@ -668,7 +742,6 @@ struct ProgBuild {
         }
         nd.dest = dest;
         nd.own = (kind == KIND_HG && b == n) ? pcell(x, y) : -1;
-        if ((int)cell_stage.size() <= dest) cell_stage.resize(dest + 1, 0);
         cell_stage[dest] = st + 1;
         nodes.push_back(nd);
         nstage.push_back(st + 1);
@@ -685,14 +758,23 @@ struct ProgBuild {
 };
 
 
-void build_leaf_program(amrlbm_mesh *m, int kind, int n, LeafProg &out, std::vector<int> *deps = nullptr)
+// deps (optional): the blocks the program reads are appended, each once per
+// value of stamp (dep_mark holds a stamp per block id)
+void build_leaf_program(amrlbm_mesh *m, int kind, int n, LeafProg &out, std::vector<int> *deps = nullptr,
+                        unsigned *dep_mark = nullptr, unsigned stamp = 0)
 {
+    out.nodes.clear();
+    out.err.clear();
+    out.naux = out.nst = 0;
+    std::fill(out.end, out.end + MAX_STAGES, 0);
     ProgBuild pb;
     pb.m = m;
     pb.kind = kind;
     pb.R = kind == KIND_C2 ? 2 : 1;
     pb.n = n;
     pb.deps = deps;
+    pb.dep_mark = dep_mark;
+    pb.dep_stamp = stamp;
     pb.dep(n);
     if (pb.build()) {
         out.err = "ghost program of block " + std::to_string(n) + ": " + pb.err;
@@ -709,7 +791,8 @@ void build_leaf_program(amrlbm_mesh *m, int kind, int n, LeafProg &out, std::vec
             if (v < 0) {
                 ok = -1 - v < own + pb.naux;
             } else {
-                int b = v / slotsz + 1, c = v % slotsz;
+                const int q = kind == KIND_HG ? v / POPB : v / CS;   // constant divisors
+                int b = q + 1, c = v - q * slotsz;
                 bool in = kind == KIND_HG ? c < NX * NY : ProgBuild::interior(c % CW - 1, c / CW - 1);
                 ok = b <= m->nblks && m->blk[b].level >= 0 && m->blk[b].isleaf && in;
             }
@@ -732,12 +815,16 @@ void build_leaf_program(amrlbm_mesh *m, int kind, int n, LeafProg &out, std::vec
     }
     out.nst = nst;
     out.naux = pb.naux;
-    out.nodes.reserve(pb.nodes.size());
+    // stage by stage, each in build order
+    int cnt[MAX_STAGES + 1] = {};
+    for (int v : pb.nstage) cnt[v]++;
+    int pos[MAX_STAGES + 1] = {};
     for (int st = 1; st <= nst; st++) {
-        for (size_t i = 0; i < pb.nodes.size(); i++)
-            if (pb.nstage[i] == st) out.nodes.push_back(pb.nodes[i]);
-        out.end[st - 1] = (int)out.nodes.size();
+        pos[st] = st == 1 ? 0 : out.end[st - 2];
+        out.end[st - 1] = pos[st] + cnt[st];
     }
+    out.nodes.resize(pb.nodes.size());
+    for (size_t i = 0; i < pb.nodes.size(); i++) out.nodes[pos[pb.nstage[i]]++] = pb.nodes[i];
 }
 
 // nodes per block slot in the device arenas, by kind (C1, C2, HG): the most
@@ -771,6 +858,17 @@ int program_operands(const LeafProg &lp)
         ops += type == NT_COPY ? 1 : (type >= NT_BB_BOT ? 3 : 9);
     }
     return ops;
+}
+
+// one id's aux-cell and recipe counts into (sign 1) or out of (-1) the
+// per-kind histograms over the leaves
+void count_programs(amrlbm_mesh *m, int id, int sign)
+{
+    for (int kind = 0; kind < 3; kind++) {
+        m->hist_aux[kind][m->naux_id[kind][id]] += sign;
+        m->hist_nn[kind][m->nn_id[kind][id]] += sign;
+    }
+    m->counted[id] = sign > 0;
 }
 
 int rebuild_programs(amrlbm_mesh *m)
@@ -821,18 +919,22 @@ int rebuild_programs(amrlbm_mesh *m)
             for (int i = 0; i < nd; i++) old_deps[i] = std::move(m->pcache[dirty[i]].deps);
         }
         std::vector<int> d;
+        // per block id, the stamp of the last leaf whose build listed it
+        thread_local std::vector<unsigned> mark;
+        thread_local unsigned stamp = 0;
+        if (mark.size() < m->blk.size()) mark.resize(m->blk.size(), 0u);
 #pragma omp for schedule(dynamic, 1)
         for (int i = 0; i < nd; i++) {   // one leaf's three programs, then the blocks they read
             ProgCache &pc = m->pcache[dirty[i]];
             pc.valid = false;
             pc.on_device = false;
             d.clear();
-            for (int kind = 0; kind < 3; kind++) {
-                pc.p[kind] = LeafProg{};
-                build_leaf_program(m, kind, dirty[i], pc.p[kind], &d);
+            if (++stamp == 0) {   // wrap: clear once
+                std::fill(mark.begin(), mark.end(), 0u);
+                stamp = 1;
             }
+            for (int kind = 0; kind < 3; kind++) build_leaf_program(m, kind, dirty[i], pc.p[kind], &d, mark.data(), stamp);
             std::sort(d.begin(), d.end());
-            d.erase(std::unique(d.begin(), d.end()), d.end());
             pc.deps.clear();
             for (int b : d) pc.deps.push_back({b, block_sig(m->blk[b])});
         }
@@ -844,9 +946,11 @@ int rebuild_programs(amrlbm_mesh *m)
             m->nn_id[kind].resize(m->pcache.size(), 0);
         }
     if (m->cost_id.size() < m->pcache.size()) m->cost_id.resize(m->pcache.size(), 0);
+    if (m->counted.size() < m->pcache.size()) m->counted.resize(m->pcache.size(), 0);
     for (int i = 0; i < nd; i++) {
         const int id = dirty[i];
         ProgCache &pc = m->pcache[id];
+        if (m->counted[id]) count_programs(m, id, -1);   // its counts change below
         for (int kind = 0; kind < 3; kind++) {
             const LeafProg &lp = pc.p[kind];
             if (!lp.err.empty()) return fail(AMRLBM_ERR_TOPOLOGY, lp.err);
@@ -860,6 +964,7 @@ int rebuild_programs(amrlbm_mesh *m)
         m->cost_id[id] = program_operands(pc.p[KIND_C2]) + program_operands(pc.p[KIND_HG]);
         pc.valid = true;
     }
+    m->prog_pending.insert(m->prog_pending.end(), dirty.begin(), dirty.end());
     // the reverse index follows each rebuilt program's dependencies
     for (int i = 0; i < nd; i++) {
         const int id = dirty[i];
@@ -972,20 +1077,60 @@ int upload_programs(amrlbm_mesh *m)
         }
         m->prog_slots = cap;
         for (ProgCache &pc : m->pcache) pc.on_device = false;
+        m->leaf_delta_all = true;
     }
-    std::vector<int> up;   // leaves whose programs are not in their slots yet, ascending id
-    for (int n : m->leaf)
-        if (!m->pcache[n].on_device) up.push_back(n);
+    // the leaves whose programs are not in their slots yet, ascending id: after
+    // a full upload or a regrown arena any leaf, else a rebuilt one or one that
+    // became a leaf (its row changed) with its cached programs; the aux-cell and
+    // recipe histograms follow the same ids
+    const bool all = m->leaf_delta_all || m->hist_aux[0].empty();
+    std::vector<int> up;
+    if (all) {
+        for (int n : m->leaf)
+            if (!m->pcache[n].on_device) up.push_back(n);
+        for (int kind = 0; kind < 3; kind++) {
+            m->hist_aux[kind].assign(NAUX_MAX + 1, 0);
+            m->hist_nn[kind].assign(kProgSlot[kind] + 1, 0);
+        }
+        std::fill(m->counted.begin(), m->counted.end(), 0);
+        for (int n : m->leaf) count_programs(m, n, 1);
+    } else {
+        std::vector<int> &c = m->prog_pending;
+        c.insert(c.end(), m->leaf_delta.begin(), m->leaf_delta.end());
+        std::sort(c.begin(), c.end());
+        c.erase(std::unique(c.begin(), c.end()), c.end());
+        for (int n : c) {
+            const bool lf = n <= m->nblks && m->blk[n].isleaf;
+            if (lf && !m->pcache[n].on_device) up.push_back(n);
+            if (m->counted[n] && !lf) count_programs(m, n, -1);
+            else if (!m->counted[n] && lf) count_programs(m, n, 1);
+        }
+    }
+    m->prog_pending.clear();
+    m->leaf_delta_all = false;
+    if (std::getenv("AMRLBM_VERIFY_PROGRAMS")) {   // the kept lists against passes over every leaf
+        for (int n : m->leaf)
+            if (!m->pcache[n].on_device && !std::binary_search(up.begin(), up.end(), n))
+                return fail(AMRLBM_ERR_TOPOLOGY, "leaf " + std::to_string(n) + " has no programs on the device");
+        for (int kind = 0; kind < 3; kind++) {
+            int naux = 0, nmax = 0;
+            for (int id : m->leaf) {
+                naux = std::max(naux, m->naux_id[kind][id]);
+                nmax = std::max(nmax, m->nn_id[kind][id]);
+            }
+            int ha = NAUX_MAX, hn = kProgSlot[kind];
+            while (ha > 0 && !m->hist_aux[kind][ha]) ha--;
+            while (hn > 0 && !m->hist_nn[kind][hn]) hn--;
+            if (ha != naux || hn != nmax)
+                return fail(AMRLBM_ERR_TOPOLOGY, "kept program counts differ from a pass over the leaves");
+        }
+    }
     const int nup = (int)up.size();
     for (int kind = 0; kind < 3; kind++) {
-        // the LDS the kernels size by: serial (a team of another size costs more
-        // to wake than these integer reads take)
-        int naux = 0, nmax = 0;
-        const int *na = m->naux_id[kind].data(), *nn = m->nn_id[kind].data();
-        for (int id : m->leaf) {
-            naux = std::max(naux, na[id]);
-            nmax = std::max(nmax, nn[id]);
-        }
+        // the LDS the kernels size by: the largest counts present
+        int naux = NAUX_MAX, nmax = kProgSlot[kind];
+        while (naux > 0 && !m->hist_aux[kind][naux]) naux--;
+        while (nmax > 0 && !m->hist_nn[kind][nmax]) nmax--;
         if ((size_t)nl > m->phdr_cap[kind]) {
             if (m->d_phdr[kind]) HIPCHK(hipFree(m->d_phdr[kind]));
             m->d_phdr[kind] = nullptr;
@@ -1082,24 +1227,30 @@ void dispatch_order(amrlbm_mesh *m)
     // one merge pass
     std::vector<std::pair<unsigned long long, int>> &sorted = m->morton;
     if (!m->morton_all) {
-        std::vector<char> drop((size_t)m->nblks + 1, 0);
+        std::vector<char> &drop = m->morton_drop;   // all zero between calls
+        if (drop.size() < (size_t)m->nblks + 1) drop.resize((size_t)m->nblks + 1025, 0);
         std::vector<std::pair<unsigned long long, int>> add;
+        std::vector<int> dropped;
         for (int id : m->morton_changed) {
             if (id > m->nblks || drop[id]) continue;
             drop[id] = 1;
+            dropped.push_back(id);
             if (m->blk[id].isleaf) add.push_back({morton_key(m, id), id});
         }
         std::sort(add.begin(), add.end());
-        std::vector<std::pair<unsigned long long, int>> out;
-        out.reserve(n);
-        size_t a = 0;
+        std::vector<std::pair<unsigned long long, int>> &out = m->morton_next;
+        out.resize(sorted.size() + add.size());
+        auto *o = out.data();
+        const auto *ad = add.data(), *ae = ad + add.size();
         for (const auto &e : sorted) {
             if (drop[e.second]) continue;
-            while (a < add.size() && add[a] < e) out.push_back(add[a++]);
-            out.push_back(e);
+            while (ad != ae && *ad < e) *o++ = *ad++;
+            *o++ = e;
         }
-        while (a < add.size()) out.push_back(add[a++]);
+        while (ad != ae) *o++ = *ad++;
+        out.resize(o - out.data());
         sorted.swap(out);
+        for (int id : dropped) drop[id] = 0;
         if ((int)sorted.size() != n) m->morton_all = true;   // the leaf set disagrees: start over
     }
     if (m->morton_all) {
@@ -1236,6 +1387,10 @@ int upload_tables(amrlbm_mesh *m)
             i = j;
         }
         if (std::getenv("AMRLBM_VERIFY_TABLES")) {   // every row against a fresh build
+            if (m->leaf_list_bad) {
+                m->leaf_list_bad = false;
+                return fail(AMRLBM_ERR_TOPOLOGY, "the leaf list kept across the regrid differs from a scan");
+            }
             for (int n = 1; n <= nb; n++) {
                 BlockDev fb;
                 NbDesc fn[8];
@@ -1249,6 +1404,8 @@ int upload_tables(amrlbm_mesh *m)
         }
     }
     for (int n : m->dirty_rows) m->dirty_mark[n] = 0;
+    m->leaf_delta_all = m->rows_all;
+    m->leaf_delta.assign(m->dirty_rows.begin(), m->dirty_rows.end());
     if (m->rows_all) {
         m->check_all = true;
         m->changed_blocks.clear();
@@ -1267,22 +1424,14 @@ int upload_tables(amrlbm_mesh *m)
         int rc0 = rebuild_programs(m);
         if (rc0) return rc0;
     }
-    // leaves grouped by level, each group ascending; then all leaves
-    const int NL = m->nlev();
-    m->lvl_off.assign(NL + 1, 0);
-    m->lvl_cnt.assign(NL, 0);
-    for (int n : m->leaf) m->lvl_cnt[m->blk[n].level]++;
-    for (int L = 0; L < NL; L++) m->lvl_off[L + 1] = m->lvl_off[L] + m->lvl_cnt[L];
-    const size_t nlist = (size_t)m->lvl_off[NL] + m->leaf.size();
+    // the leaves in launch order
+    const size_t nlist = m->leaf.size();
     size_t lists_at = 0;
     rcb = batch_alloc(m, sizeof(int) * nlist, &lists_at);
     if (rcb) return rcb;
     int *lists = reinterpret_cast<int *>(m->h_up + lists_at);
-    std::vector<int> fillpos(m->lvl_off.begin(), m->lvl_off.end() - 1);
-    for (int n : m->leaf) lists[fillpos[m->blk[n].level]++] = n - 1;
-    m->all_off = m->lvl_off[NL];
     dispatch_order(m);
-    for (size_t k = 0; k < m->disp.size(); k++) lists[m->all_off + k] = m->disp[k] - 1;
+    for (size_t k = 0; k < nlist; k++) lists[k] = m->disp[k] - 1;
     int rc = ensure_int_buffer(&m->d_lists, &m->lists_cap, (int)nlist);
     if (rc) return rc;
     batch_seg(m, m->d_lists, lists_at, sizeof(int) * nlist);
@@ -1295,14 +1444,14 @@ int upload_tables(amrlbm_mesh *m)
     rc = batch_flush(m);
     if (rc) return rc;
     if (!m->leaf.empty()) {
-        launch_gather_headers(m->d_phdr_slot, m->d_phdr, m->d_lists + m->all_off, (int)m->leaf.size(), m->stream);
+        launch_gather_headers(m->d_phdr_slot, m->d_phdr, m->d_lists, (int)m->leaf.size(), m->stream);
         HIPCHK(hipGetLastError());
     }
     m->tables_dirty = false;
     return AMRLBM_OK;
 }
 
-const int *all_leaves(amrlbm_mesh *m) { return m->d_lists + m->all_off; }
+const int *all_leaves(amrlbm_mesh *m) { return m->d_lists; }
 int nleaves(const amrlbm_mesh *m) { return (int)m->leaf.size(); }
 
 int check_err_flags(amrlbm_mesh *m)
@@ -1400,7 +1549,7 @@ int host_flags(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::ve
 {
     const int nleaf = (int)m->leaf.size();
     // Err_Interface of every leaf on the device (Refine_Derefine.f90:36, :122)
-    launch_err(m->dm, m->d_lists + m->all_off, nleaf, m->d_err, nullptr, m->stream);
+    launch_err(m->dm, m->d_lists, nleaf, m->d_err, nullptr, m->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(m->h_pinned, m->d_err, sizeof(double) * nleaf, hipMemcpyDeviceToHost, m->stream));
     std::vector<int> pred(d_pred ? m->nblks : 0);
@@ -1737,7 +1886,7 @@ int init_bubble(amrlbm_mesh *m, double R, double X0, double Y0)
         int rc2 = upload(m, m->dm.C, (size_t)(m->leaf[k] - 1) * CS, CS, &cbuf[k * CS], [cs](size_t) { return cs; });
         if (rc2) return rc2;
     }
-    launch_init_fields(m->dm, m->cur, m->d_lists + m->all_off, (int)nl, m->ph, R, m->stream);
+    launch_init_fields(m->dm, m->cur, m->d_lists, (int)nl, m->ph, R, m->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(m->stream));
     return AMRLBM_OK;
@@ -2144,7 +2293,7 @@ int amrlbm_setup_bubbles(amrlbm_mesh *m, int n, const double *R, const double *x
     return setup_mesh(m, 1, [&](amrlbm_mesh *mm) {
         int r2 = upload_tables(mm);
         if (r2) return r2;
-        launch_init_fields(mm->dm, mm->cur, mm->d_lists + mm->all_off, nleaves(mm), mm->ph, 0.0, mm->stream, d_bub, n);
+        launch_init_fields(mm->dm, mm->cur, mm->d_lists, nleaves(mm), mm->ph, 0.0, mm->stream, d_bub, n);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(mm->stream));
         return AMRLBM_OK;
@@ -2184,7 +2333,7 @@ static int init_ensemble(amrlbm_mesh *m)
         rc = upload(m, m->dm.C, (size_t)(m->leaf[k] - 1) * CS, CS, &cbuf[k * CS], [cs](size_t) { return cs; });
         if (rc) return rc;
     }
-    launch_init_fields(m->dm, m->cur, m->d_lists + m->all_off, (int)nl, m->ph, 0.0, m->stream, nullptr, 0, m->d_iR);
+    launch_init_fields(m->dm, m->cur, m->d_lists, (int)nl, m->ph, 0.0, m->stream, nullptr, 0, m->d_iR);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(m->stream));
     return AMRLBM_OK;
@@ -2326,7 +2475,7 @@ int amrlbm_output_instances(amrlbm_mesh *m, int n, double *out)
     if (!out || !m->ens || n != m->ens) return fail(AMRLBM_ERR_ARG, "not an ensemble mesh of n instances");
     int rc = upload_tables(m);
     if (rc) return rc;
-    launch_diag(m->dm, m->d_lists + m->all_off, (int)m->leaf.size(), m->ph, m->d_part, m->d_diag, m->stream, n,
+    launch_diag(m->dm, m->d_lists, (int)m->leaf.size(), m->ph, m->d_part, m->d_diag, m->stream, n,
                 m->d_diag_inst);
     HIPCHK(hipGetLastError());
     std::vector<double> d(9 * (size_t)n);
@@ -3537,7 +3686,7 @@ int amrlbm_output(amrlbm_mesh *m, double out[9])
     NEED_READY(m);
     int rc = upload_tables(m);
     if (rc) return rc;
-    launch_diag(m->dm, m->d_lists + m->all_off, (int)m->leaf.size(), m->ph, m->d_part, m->d_diag, m->stream);
+    launch_diag(m->dm, m->d_lists, (int)m->leaf.size(), m->ph, m->d_part, m->d_diag, m->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(m->h_pinned, m->d_diag, sizeof(double) * 9, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
@@ -3560,7 +3709,7 @@ int amrlbm_output_noalloc(amrlbm_mesh *m, double out[9])
         return fail(AMRLBM_ERR_CAPACITY, std::to_string(n) + " leaves: amrlbm_output_noalloc takes at most " +
                                              std::to_string(kScratchLeaves));
     double *part = m->scratch, *tot = m->scratch + kScratchTotals;
-    launch_diag(m->dm, m->d_lists + m->all_off, n, m->ph, part, tot, m->stream);
+    launch_diag(m->dm, m->d_lists, n, m->ph, part, tot, m->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(m->h_pinned, tot, sizeof(double) * 9, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
