@@ -117,7 +117,7 @@ struct amrlbm_mesh {
     DevMesh dm{};
     BlockDev *d_blk = nullptr;
     NbDesc *d_nbd = nullptr;
-    int *d_err_flags = nullptr;      // [0] a fill met a level jump
+    int *d_err_flags = nullptr;      // [0] a fill met a level jump (d_changed[3]: it travels with each decision)
     unsigned *d_sync = nullptr;      // the segment kernel's grid barrier words (fault word at 15)
     int *d_judge = nullptr;          // the device judge's stop ordinal (enqueue_segment with judge)
     // the learned regrid predictor of amrlbm_step_learned: its weights, the
@@ -136,6 +136,7 @@ struct amrlbm_mesh {
     double *d_err_slot = nullptr;    // Err_Interface per block slot (leaves)
     int *d_changed = nullptr;        // [0] the device regrid decision (the step kernels' skip word),
                                      // [1] any refinement flag, [2] a judged segment's stop ordinal,
+                                     // [3] the error word d_err_flags,
                                      // [4 ..] the decision's flags (k_regrid_check's flag lists)
     int *h_changed = nullptr;        // pinned copy of it
     double *d_part = nullptr;        // diagnostics partials
@@ -164,6 +165,7 @@ struct amrlbm_mesh {
     bool pairs_recorded = false;
     double last_step_ms = 0.0;
     int regrid_checks = 0, regrid_changes = 0, segments_skipped = 0, graphs_built = 0;
+    bool seg_kernel_ran = false;     // the persistent segment kernel ran in the current stepping call
     int device_flag_regrids = 0;     // regrids whose flags came from the device decision
     // host wall time (ms): Refine_Derefine_Mesh, table upload, ghost-program
     // build (part of the upload), graph capture and instantiation
@@ -1819,8 +1821,6 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipMalloc(&m->d_blk, sizeof(BlockDev) * S));
     HIPCHK(hipMalloc(&m->d_nbd, sizeof(NbDesc) * 8 * S));
     for (int kind = 0; kind < 3; kind++) HIPCHK(hipMalloc(&m->d_phdr_slot[kind], sizeof(PHdr) * S));
-    HIPCHK(hipMalloc(&m->d_err_flags, sizeof(int)));
-    HIPCHK(hipMemsetAsync(m->d_err_flags, 0, sizeof(int), m->stream));
     HIPCHK(hipMalloc(&m->d_sync, sizeof(unsigned) * kSegSyncWords));
     HIPCHK(hipMalloc(&m->d_judge, sizeof(int)));
     HIPCHK(hipMemsetAsync(m->d_sync, 0, sizeof(unsigned) * kSegSyncWords, m->stream));
@@ -1828,8 +1828,9 @@ int alloc_device(amrlbm_mesh *m)
     HIPCHK(hipMalloc(&m->d_err_slot, sizeof(double) * S));
     HIPCHK(hipMalloc(&m->d_changed, sizeof(int) * (4 + kRegridFlagWords)));
     HIPCHK(hipMemsetAsync(m->d_changed, 0, sizeof(int) * (4 + kRegridFlagWords), m->stream));
+    m->d_err_flags = m->d_changed + 3;
     HIPCHK(hipHostMalloc(&m->h_changed, sizeof(int) * (4 + kRegridFlagWords), hipHostMallocDefault));
-    HIPCHK(hipEventCreateWithFlags(&m->ev_check, hipEventDisableTiming));
+    HIPCHK(hipEventCreate(&m->ev_check));   // timed: a stepping call that ends on a decision ends its timing there
     HIPCHK(hipEventCreateWithFlags(&m->ev_tab, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&m->ev_pairs, hipEventDisableTiming));
     HIPCHK(hipMalloc(&m->d_part, sizeof(double) * 9 * S));
@@ -2014,6 +2015,7 @@ bool use_segment_kernel(amrlbm_mesh *m)
 // (the same state afterwards as the segment's step graph).
 int launch_segment_kernel(amrlbm_mesh *m, int len)
 {
+    m->seg_kernel_ran = true;
     hipError_t e = launch_segment(m->dm, m->cur, all_leaves(m), nleaves(m), m->prog[KIND_C2], m->prog[KIND_HG],
                                   m->prog[KIND_C1], m->ph, len, m->d_err_slot, m->d_sync, m->stream);
     if (e != hipSuccess) {
@@ -2172,7 +2174,6 @@ int amrlbm_destroy(amrlbm_mesh *m)
     if (m->h_up) (void)hipHostFree(m->h_up);
     (void)hipFree(m->d_up);
     if (m->h_pairs) (void)hipHostFree(m->h_pairs);
-    (void)hipFree(m->d_err_flags);
     (void)hipFree(m->d_sync);
     (void)hipFree(m->d_judge);
     (void)hipFree(m->d_net_w);
@@ -3076,6 +3077,40 @@ int amrlbm_refine_derefine_predicted(amrlbm_mesh *m, const int *ids, int n)
 
 }  // extern "C"
 
+// The end of a stepping call (ev0 recorded at its start). tail_checked: the
+// call's last stream work is a decision copy the host has waited for
+// (ev_check); that copy carried the error word, and its event ends the
+// call's timing, so no further round trip is needed. Otherwise the error
+// word is copied behind the call's work and one wait (ev1) covers both.
+// The segment kernel's barrier word is read (check_err_flags) only after a
+// call that ran it.
+static int end_of_steps(amrlbm_mesh *m, bool tail_checked)
+{
+    if (m->seg_kernel_ran) {
+        HIPCHK(hipEventRecord(m->ev1, m->stream));
+        HIPCHK(hipEventSynchronize(m->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+        m->last_step_ms = ms;
+        return check_err_flags(m);
+    }
+    hipEvent_t end = m->ev_check;
+    if (!tail_checked) {
+        HIPCHK(hipMemcpyAsync(m->h_changed + 3, m->d_err_flags, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+        HIPCHK(hipEventRecord(m->ev1, m->stream));
+        HIPCHK(hipEventSynchronize(m->ev1));
+        end = m->ev1;
+    }
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ev0, end));
+    m->last_step_ms = ms;
+    if (m->h_changed[3]) {
+        HIPCHK(hipMemsetAsync(m->d_err_flags, 0, sizeof(int), m->stream));
+        return fail(AMRLBM_ERR_TOPOLOGY, "ghost fill met a neighbour more than one level apart");
+    }
+    return AMRLBM_OK;
+}
+
 // The steps of amrlbm_step. Stream order: segment r, regrid check r, segment
 // r+1, ... The host reads check r while segment r+1 runs. When check r says the
 // mesh changes, segment r+1 has skipped all its work: the host takes its steps
@@ -3086,11 +3121,13 @@ static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahea
 {
     int rc = AMRLBM_OK;
     HIPCHK(hipEventRecord(m->ev0, m->stream));
+    m->seg_kernel_ran = false;
     int done = 0;
-    bool pending = false;
+    bool pending = false, tail_checked = false;
     while (done < nsteps || pending) {
         bool launched = false;
         int len = 0;
+        tail_checked = false;
         if (done < nsteps) {
             len = segment_length(m, nsteps - done, regrid_every);
             if (!pending) {
@@ -3132,6 +3169,7 @@ static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahea
             }
             *ahead_len = 0;
             clear_regrid_flags(m);
+            tail_checked = !launched;
         }
         if (launched && m->T % regrid_every == 0) {
             if ((rc = enqueue_regrid_check(m))) return rc;
@@ -3139,12 +3177,7 @@ static int step_segments(amrlbm_mesh *m, int nsteps, int regrid_every, int *ahea
         }
     }
     determine_min_max_levels(m);
-    HIPCHK(hipEventRecord(m->ev1, m->stream));
-    HIPCHK(hipEventSynchronize(m->ev1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
-    m->last_step_ms = ms;
-    return check_err_flags(m);
+    return end_of_steps(m, tail_checked);
 }
 
 // The learned predictor's refinement flags of every leaf at the current state,
@@ -3187,7 +3220,7 @@ static int enqueue_learned_check(amrlbm_mesh *m, int len, bool full)
     else
         launch_judge_check(m->d_judge, len + 1, m->d_changed, m->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(m->h_changed, m->d_changed, sizeof(int) * 3, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipMemcpyAsync(m->h_changed, m->d_changed, sizeof(int) * 4, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipEventRecord(m->ev_check, m->stream));
     m->regrid_checks++;
     return AMRLBM_OK;
@@ -3206,10 +3239,12 @@ static int enqueue_learned_check(amrlbm_mesh *m, int len, bool full)
 static int step_learned(amrlbm_mesh *m, int nsteps, int full_every, long long st[4], int *ahead_len)
 {
     int rc = AMRLBM_OK, done = 0, plen = 0;
-    bool pending = false, pfull = false;
+    bool pending = false, pfull = false, tail_checked = false;
+    m->seg_kernel_ran = false;
     while (done < nsteps || pending) {
         bool launched = false;
         int len = 0;
+        tail_checked = false;
         if (done < nsteps) {
             len = std::min(nsteps - done, full_every - m->T % full_every);
             if (!pending) {
@@ -3251,6 +3286,7 @@ static int step_learned(amrlbm_mesh *m, int nsteps, int full_every, long long st
             }
             *ahead_len = 0;
             clear_regrid_flags(m);
+            tail_checked = !launched;
         }
         if (launched) {
             plen = len;
@@ -3260,7 +3296,7 @@ static int step_learned(amrlbm_mesh *m, int nsteps, int full_every, long long st
         }
     }
     determine_min_max_levels(m);
-    return check_err_flags(m);
+    return end_of_steps(m, tail_checked);
 }
 
 extern "C" {
@@ -3302,11 +3338,6 @@ int amrlbm_step_learned(amrlbm_mesh *m, int nsteps, int full_every, long long st
         (void)hipGetLastError();
         return rc;
     }
-    HIPCHK(hipEventRecord(m->ev1, m->stream));
-    HIPCHK(hipEventSynchronize(m->ev1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
-    m->last_step_ms = ms;
     return AMRLBM_OK;
 }
 
