@@ -66,6 +66,12 @@ KERNEL_NAMES = {"c2_collide": "k_prog_c<OP_MACRO_G_COLLIDE>", "hg_stream": "k_pr
 LAUNCHES_PER_STEP = {"c2_collide": 1.0, "hg_stream": 1.0}   # c_macro_g: once per segment
 
 
+def progress(what, t0=[time.perf_counter()]):
+    """One line on stderr as each entry of the line finishes (the JSON line
+    itself goes to stdout at the end)."""
+    print(f"bench: {what} done, {time.perf_counter() - t0[0]:.1f} s", file=sys.stderr, flush=True)
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -531,6 +537,7 @@ def main():
         profile_steps(amrlbm, w, args.steps, k, (64,) if args.no_fp32 else (64, 32), variant)
         return
     r = measure(amrlbm, w, args.steps, args.warmup, k, args.calibrate_copy_mb)
+    progress("fp64 headline")
     c0, c1 = r["c0"], r["c1"]
     nleaf_mean = 0.5 * (c0["nleaf"] + c1["nleaf"])
     diag = r["diag"]
@@ -554,10 +561,12 @@ def main():
     d32 = None
     if not args.no_fp32:
         r32 = measure(amrlbm, w, args.steps, args.warmup, k, precision=32)
+        progress("fp32")
         d32 = r32["diag"]
         # the flow's own sensitivity: FP64 from a set-up whose bubble radius is 1e-7 larger
         rp = measure(amrlbm, w, args.steps, args.warmup, k, precision=64, radius=w.radius * (1.0 + 1e-7))
         dp = rp["diag"]
+        progress("fp64 perturbed set-up")
         out["fp32"] = {
             "workload": describe(w, k, 32), "value": round(r32["value"], 1), "unit": "MLUPS (finest-equivalent)",
             "speedup_vs_fp64": round(r32["value"] / r["value"], 3),
@@ -577,12 +586,15 @@ def main():
             # config 4: the network trained on the generator, deployed on the 128x256 bubble (D=32 at (64, 64))
             "learned_regrid_cfg1": learned_regrid(amrlbm, amrlbm.WORKLOADS["cfg1"]),
         }
+        progress("learned regrid on the 128x256 bubble")
         out["fp32_learned"] = fp32_learned(amrlbm, w, args.steps, args.warmup, r["value"], diag)
+        progress("fp32 learned")
     learned_diag = out["fp32_learned"].pop("diag", None) if out.get("fp32_learned") else None
     total = args.warmup + args.steps * windows_for(args.steps)   # the steps every headline run took
     n_int = args.interval_steps if args.interval_steps >= 0 else total
     if n_int > 0:
         out["regrid_interval"], every = regrid_interval(amrlbm, w, n_int)
+        progress("regrid interval sweep")
         if n_int == total:   # the same steps: each run against the every-step FP64 run
             cmp = {"fp64_regrid_every_10": versus(diag, every)}
             if d32 is not None:
@@ -609,10 +621,13 @@ def main():
                 "tuning": rl["tuning"],
             }
         out["large_working_set"] = entries
+        progress("large working set")
     if args.multi_steps > 0:
         out["multi_bubble"] = multi_bubble(amrlbm, args.multi_steps, k)
+        progress("multi bubble")
     if args.ensemble_steps > 0:
         out["ensemble"] = ensemble_run(amrlbm, args.ensemble_size, args.ensemble_seed, args.ensemble_steps, k)
+        progress("ensemble")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, k)
     print(json.dumps(out), flush=True)
