@@ -119,6 +119,8 @@ def lib():
             "amrlbm_set_tuning": [vp, _IP],
             "amrlbm_get_tuning": [vp, _IP],
             "amrlbm_autotune": [vp, ip, ip, _DP],
+            "amrlbm_set_mass_correction": [vp, ip],
+            "amrlbm_get_mass_budget": [vp, ip, _DP],
             "amrlbm_save_snapshot": [vp, ctypes.c_char_p],
             "amrlbm_cnn_forward": [ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
             "amrlbm_cnn_gradient": [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ip, ctypes.POINTER(ctypes.c_float), ctypes.c_float, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
@@ -458,6 +460,20 @@ class Mesh:
         t = np.zeros(96)
         _check(lib().amrlbm_autotune(self.h, int(reps), int(regrid_every), _d(t)))
         return [(int(t[4 * v]), int(t[4 * v + 1]), int(t[4 * v + 2]), float(t[4 * v + 3])) for v in range(24)]
+
+    def set_mass_correction(self, on=True):
+        """Hold each instance's FP64 phase-field mass at its budget (its value
+        now plus each later mesh change's restrict/prolong change), corrected
+        at every segment end (amrlbm_set_mass_correction)."""
+        _check(lib().amrlbm_set_mass_correction(self.h, 1 if on else 0))
+
+    def mass_budget(self):
+        """[instances, 4]: target, mass before the last correction, interface
+        weight, lambda (amrlbm_get_mass_budget)."""
+        n = self.w.instances if hasattr(self.w, "members") else 1
+        out = np.zeros((n, 4))
+        _check(lib().amrlbm_get_mass_budget(self.h, n, _d(out)))
+        return out
 
     def output(self, allow_nonfinite=False):
         """rho_min, rho_max, max|Uy|, mass, bubble area, centroid x, y, rise velocity, non-finite."""

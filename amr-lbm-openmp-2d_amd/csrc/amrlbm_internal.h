@@ -227,6 +227,20 @@ void launch_init_fields(const DevMesh &m, int cur, const int *list, int n, const
 // mesh) also per instance into out_inst[ninst][9]
 void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, double *partial, double *out,
                  hipStream_t s, int ninst = 0, double *out_inst = nullptr);
+// Phase-field mass correction. launch_mass_sum: the FP64 sums of C dx^2 and
+// of the interface weight max(0, 4 C (1 - C)) dx^2 over the leaves in list
+// (part: 2 doubles per leaf), folded per instance (ninst workgroups) into
+// mass[4 q .. 4 q + 3] = target, S, W, lambda as mode says; launch_mass_apply:
+// dC = lambda max(0, 4 C (1 - C)) on every leaf cell, C += dC, h_q +=
+// Gamma_q(U) dC, and g_q += (Gamma_q(U) - w_q) (rhoh - rhol) dC / 3, which
+// keeps U and P.
+// gate (sum only; apply always): the launch honours the skip word and the
+// judge's stop ordinal ord, as the step kernels of a segment do.
+enum MassMode { MASS_SAVE = 0, MASS_TARGET = 1, MASS_BUDGET = 2, MASS_CORRECT = 3 };
+void launch_mass_sum(const DevMesh &m, const int *list, int n, int ninst, double *part, double *mass, int mode,
+                     int gate, const int *judge, int ord, hipStream_t s);
+void launch_mass_apply(const DevMesh &m, int cur, const int *list, int n, const double *mass, const Phys &ph,
+                       const int *judge, int ord, hipStream_t s);
 // n copies of 16-byte words: copy k takes desc[3k+2] words from src + desc[3k]
 // to dst + desc[3k+1] (offsets in words)
 // a segment of an upload batch: nwords 4-byte words from word src_word of the

@@ -1372,6 +1372,132 @@ __global__ __launch_bounds__(256) void k_diag_final_inst(DevMeshT<double> m, con
     if (tid < 9) out[(size_t)q * 9 + tid] = red[tid][0];
 }
 
+// ---- phase-field mass correction (FP64 on the device) ----------------------
+// The launches of a stepping segment return at once when the skip word is set
+// or the judge stopped the segment before state `ord` (as the step kernels
+// do, so a segment launched ahead of a decision leaves the state whole);
+// those of a mesh change (gate 0) always run.
+__device__ __forceinline__ bool mass_skipped(const int *skip, int gate, const int *judge, int ord)
+{
+    return gate && (*skip || (judge && *judge < ord));
+}
+
+// stage 1: per leaf, in FP64, part[2k] = sum of C dx^2 over the block's
+// cells (the phase-field mass), part[2k+1] = sum of max(0, 4 C (1 - C)) dx^2
+// (the interface weight a correction is spread with); a fixed reduction tree
+template <class T>
+__global__ __launch_bounds__(256) void k_mass_partial(DevMeshT<T> m, const int *__restrict__ list, double *part,
+                                                      int gate, const int *judge, int ord)
+{
+    if (mass_skipped(m.skip, gate, judge, ord)) return;
+    const int slot = list[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int i = tid % NX + 1, j = tid / NX + 1;
+    const double dx = (double)m.blk[slot].dx, dx2 = dx * dx;
+    const double c = (double)m.C[(size_t)slot * CS + cidx(i, j)] + c_shift<T>();
+    double a = c * dx2, w = fmax(0.0, 4.0 * c * (1.0 - c)) * dx2;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {   // a butterfly: every lane ends with the same sums
+        a += __shfl_xor(a, o);
+        w += __shfl_xor(w, o);
+    }
+    __shared__ double red[2][4];
+    if ((tid & 63) == 0) {
+        red[0][tid >> 6] = a;
+        red[1][tid >> 6] = w;
+    }
+    __syncthreads();
+    if (tid < 2) part[2 * (size_t)blockIdx.x + tid] = ((red[tid][0] + red[tid][1]) + red[tid][2]) + red[tid][3];
+}
+
+// stage 2, workgroup q = instance q: S, W = the sums over the instance's
+// leaves in leaf-list order, then by mode (mass[4q .. 4q+3] = target, S at
+// the last fold, W, lambda):
+//   MASS_SAVE    mass[4q+1] = S (before a mesh change)
+//   MASS_TARGET  target = S (the budget starts here)
+//   MASS_BUDGET  target += S - mass[4q+1] (the change a mesh change made)
+//   MASS_CORRECT lambda = (target - S) / W (0 without interface cells)
+__global__ __launch_bounds__(256) void k_mass_final(DevMeshT<double> m, const int *__restrict__ list,
+                                                    const double *__restrict__ part, int n, int ninst, double *mass,
+                                                    int mode, int gate, const int *judge, int ord)
+{
+    if (mass_skipped(m.skip, gate, judge, ord)) return;
+    const int q = blockIdx.x, tid = threadIdx.x;
+    double a = 0.0, w = 0.0;
+    for (int k = tid; k < n; k += 256) {
+        if (ninst > 1 && m.blk[list[k]].inst != q) continue;
+        a += part[2 * (size_t)k];
+        w += part[2 * (size_t)k + 1];
+    }
+    __shared__ double red[2][256];
+    red[0][tid] = a;
+    red[1][tid] = w;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+            red[0][tid] += red[0][tid + s];
+            red[1][tid] += red[1][tid + s];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double *b = mass + 4 * q;
+        const double S = red[0][0], W = red[1][0];
+        if (mode == MASS_SAVE) {
+            b[1] = S;
+        } else if (mode == MASS_TARGET) {
+            b[0] = S;
+            b[1] = S;
+            b[2] = W;
+            b[3] = 0.0;
+        } else if (mode == MASS_BUDGET) {
+            b[0] += S - b[1];
+        } else {
+            b[1] = S;
+            b[2] = W;
+            b[3] = W > 0.0 ? (b[0] - S) / W : 0.0;
+        }
+    }
+}
+
+// stage 3: with dC = lambda max(0, 4 C (1 - C)), evaluated in FP64 on every
+// leaf cell of the state in buffer cur: C += dC and h_q += Gamma_q(u) dC
+// (h's equilibrium share C Gamma_q, Equilibrium_h_g_AMR, LBM_Subroutines.f90:
+// 2-21: sum_q h_q moves by dC, the phase field's flux by u dC); the density
+// moves by drho = (rhoh - rhol) dC, and g_q += (Gamma_q(u) - w_q) drho / 3
+// keeps the cell's velocity and pressure: g's first moment is rho u / 3, its
+// zeroth P (Macroscopic_Values_g_AMR, :205-240). u: the cell's last U.
+template <class T>
+__global__ __launch_bounds__(256) void k_mass_apply(DevMeshT<T> m, int cur, const int *__restrict__ list,
+                                                    const double *__restrict__ mass, Phys ph_arg, const int *judge,
+                                                    int ord)
+{
+    if (mass_skipped(m.skip, 1, judge, ord)) return;
+    const int slot = list[blockIdx.x];
+    const int inst = m.blk[slot].inst;
+    const double lam = mass[4 * inst + 3];
+    if (lam == 0.0) return;
+    const Phys &ph = m.phs ? m.phs[inst] : ph_arg;
+    const int tid = threadIdx.x;
+    const int i = tid % NX + 1, j = tid / NX + 1;
+    T *C = m.C + (size_t)slot * CS + cidx(i, j);
+    const double c = (double)*C + c_shift<T>();
+    const double dc = lam * fmax(0.0, 4.0 * c * (1.0 - c));
+    const double drho3 = (ph.rhoh - ph.rhol) * dc / 3.0;
+    const T *PU = m.PU + (size_t)slot * PUB;
+    const double ux = (double)PU[PS + pidx(i, j)], uy = (double)PU[2 * PS + pidx(i, j)];
+    const double uu = ux * ux + uy * uy;
+    T *F = m.pop[cur] + (size_t)slot * POPB;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        const double eu = (double)c_ex[q] * ux + (double)c_ey[q] * uy;
+        const double gw = wa(q) * (3.0 * eu + 4.5 * eu * eu - 1.5 * uu);   // Gamma_q - w_q
+        F[fidx(0, q, i, j)] = (T)((double)F[fidx(0, q, i, j)] + (wa(q) + gw) * dc);
+        F[fidx(1, q, i, j)] = (T)((double)F[fidx(1, q, i, j)] + gw * drho3);
+    }
+    *C = (T)((double)*C + dc);
+}
+
 // Ghost-program upload: descriptor k copies desc[3k+2] 16-byte words from
 // src + desc[3k] to dst + desc[3k+1] (one workgroup each).
 // One upload batch: segment k (one workgroup) copies desc[k].nwords 4-byte
@@ -1918,6 +2044,32 @@ void launch_diag(const DevMesh &m, const int *list, int n, const Phys &ph, doubl
     if (ninst > 0 && out_inst)
         hipLaunchKernelGGL(k_diag_final_inst, dim3(ninst), dim3(256), 0, s, typed<double>(m), list, partial, n,
                            out_inst);
+}
+
+void launch_mass_sum(const DevMesh &m, const int *list, int n, int ninst, double *part, double *mass, int mode,
+                     int gate, const int *judge, int ord, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (m.f32)
+        hipLaunchKernelGGL(k_mass_partial<float>, dim3(n), dim3(256), 0, s, typed<float>(m), list, part, gate, judge,
+                           ord);
+    else
+        hipLaunchKernelGGL(k_mass_partial<double>, dim3(n), dim3(256), 0, s, typed<double>(m), list, part, gate, judge,
+                           ord);
+    hipLaunchKernelGGL(k_mass_final, dim3(ninst), dim3(256), 0, s, typed<double>(m), list, part, n, ninst, mass, mode,
+                       gate, judge, ord);
+}
+
+void launch_mass_apply(const DevMesh &m, int cur, const int *list, int n, const double *mass, const Phys &ph,
+                       const int *judge, int ord, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (m.f32)
+        hipLaunchKernelGGL(k_mass_apply<float>, dim3(n), dim3(256), 0, s, typed<float>(m), cur, list, mass, ph, judge,
+                           ord);
+    else
+        hipLaunchKernelGGL(k_mass_apply<double>, dim3(n), dim3(256), 0, s, typed<double>(m), cur, list, mass, ph,
+                           judge, ord);
 }
 
 void launch_scatter_segments(const void *src, const ScatterSeg *desc, int n, hipStream_t s)

@@ -255,6 +255,15 @@ struct amrlbm_mesh {
     double *d_diag_inst = nullptr;   // [ens][9]
     int ens_cap = 0;
 
+    // the phase-field mass correction (amrlbm_set_mass_correction): at every
+    // segment end the FP64 mass of each instance is put back to its budget
+    // (its value when the correction was switched on plus what each mesh
+    // change's restrict and prolong moved it by)
+    int mass_on = 0;
+    double *d_mass = nullptr;        // [instance][4]: target, S at the last fold, W, lambda
+    double *d_mass_part = nullptr;   // [leaf][2] partial sums (max_blocks leaves)
+    int mass_ninst = 0;
+
     int nlev() const { return prm.max_levels + 1; }
     int width(int L) const { return 1 << L; }
     int height(int L) const { return prm.nroots << L; }
@@ -1456,6 +1465,27 @@ int upload_tables(amrlbm_mesh *m)
 const int *all_leaves(amrlbm_mesh *m) { return m->d_lists; }
 int nleaves(const amrlbm_mesh *m) { return (int)m->leaf.size(); }
 
+// The mass correction at a segment end, on the state in buffer cur: each
+// instance's FP64 phase-field mass is folded and put back to its target by a
+// change of C spread over the interface cells. judge / ord: the segment's
+// judge and end ordinal (the launches return with the segment's end kernel).
+void enqueue_mass_correction(amrlbm_mesh *m, const int *judge, int ord)
+{
+    const int n = nleaves(m);
+    launch_mass_sum(m->dm, all_leaves(m), n, m->mass_ninst, m->d_mass_part, m->d_mass, MASS_CORRECT, 1, judge, ord,
+                    m->stream);
+    launch_mass_apply(m->dm, m->cur, all_leaves(m), n, m->d_mass, m->ph, judge, ord, m->stream);
+}
+
+// The budget's share of a mesh change: before restrict and prolong, the mass
+// over the old leaves (MASS_SAVE); after the new tables, target += the
+// difference (MASS_BUDGET). These run whatever the skip word says.
+void enqueue_mass_budget(amrlbm_mesh *m, int mode)
+{
+    launch_mass_sum(m->dm, all_leaves(m), nleaves(m), m->mass_ninst, m->d_mass_part, m->d_mass, mode, 0, nullptr, 0,
+                    m->stream);
+}
+
 int check_err_flags(amrlbm_mesh *m)
 {
     int f[2] = {0, 0};
@@ -1726,7 +1756,9 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
         HIPCHK(hipEventRecord(m->ev_pairs, m->stream));
         m->pairs_recorded = true;
         // the kernels read the pre-regrid block table (parent dx, children);
-        // it is replaced only after them
+        // it is replaced only after them (the old leaf list too, which the
+        // mass budget sums over first)
+        if (m->mass_on) enqueue_mass_budget(m, MASS_SAVE);
         launch_restrict(m->dm, m->cur, m->d_pairs, (int)rpairs.size() / 2, m->stream);
         launch_prolong(m->dm, m->cur, m->d_pairs + rpairs.size(), (int)ppairs.size() / 2, m->ph, m->stream);
         HIPCHK(hipGetLastError());
@@ -1743,7 +1775,11 @@ int regrid(amrlbm_mesh *m, double refin_tol, double deref_tol, const std::vector
     }
     array_of_leaf_blocks(m, true);
     timer.stop();
-    return upload_tables(m);
+    rc = upload_tables(m);
+    if (rc || !m->mass_on) return rc;
+    enqueue_mass_budget(m, MASS_BUDGET);
+    HIPCHK(hipGetLastError());
+    return AMRLBM_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1859,6 +1895,7 @@ void reset_topology(amrlbm_mesh *m)
     m->tables_dirty = true;
     m->rows_all = true;
     m->topo_version++;
+    m->mass_on = 0;   // a new state has no budget: the caller switches the correction on again
 }
 
 // analytic phase field of Initialize_AMR_Rising_Bubble (AMR_Rising_2D_OMP.f90:174-185)
@@ -1940,6 +1977,9 @@ int enqueue_segment(amrlbm_mesh *m, int len, bool judge, After after)
     end.out = -1;
     end.ord = len;
     end.judge_on = judge;
+    // the mass correction before the end kernel, so that U, P and
+    // Err_Interface follow the corrected C
+    if (m->mass_on) enqueue_mass_correction(m, ctl.judge, len);
     launch_prog_c(m->dm, cur, lst, n, KIND_C1, OP_MACRO_G, m->prog[KIND_C1], m->ph, m->stream, m->d_err_slot, end);
     return after(2, n * PS);
 }
@@ -2002,7 +2042,7 @@ int segment_graph(amrlbm_mesh *m, int len, hipGraphExec_t *out, bool judge = fal
 // ghost programs need.
 bool use_segment_kernel(amrlbm_mesh *m)
 {
-    if (m->exec != 1) return false;
+    if (m->exec != 1 || m->mass_on) return false;   // the segment kernel carries no mass correction
     const size_t lds = segment_lds_bytes(m->dm, m->prog[KIND_HG]);
     if (lds != m->seg_lds) {
         m->seg_lds = lds;
@@ -2196,6 +2236,8 @@ int amrlbm_destroy(amrlbm_mesh *m)
     (void)hipFree(m->d_lv);
     (void)hipFree(m->d_iR);
     (void)hipFree(m->d_diag_inst);
+    (void)hipFree(m->d_mass);
+    (void)hipFree(m->d_mass_part);
     if (m->h_pinned) (void)hipHostFree(m->h_pinned);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
     if (m->ev1) (void)hipEventDestroy(m->ev1);
@@ -3750,6 +3792,42 @@ int amrlbm_output_noalloc(amrlbm_mesh *m, double out[9])
     out[6] = d[6] / d[4];
     out[7] = d[7] / d[4];
     if (d[8] != 0.0) return fail(AMRLBM_ERR_NONFINITE, "non-finite phase field in " + std::to_string((long)d[8]) + " cells");
+    return AMRLBM_OK;
+}
+
+int amrlbm_set_mass_correction(amrlbm_mesh *m, int on)
+{
+    NEED_READY(m);
+    if (on != 0 && on != 1) return fail(AMRLBM_ERR_ARG, "the mass correction is 0 (off) or 1 (on)");
+    drop_graphs(m);   // the segments' graphs are captured again with or without it
+    m->mass_on = 0;
+    if (!on) return AMRLBM_OK;
+    const int ninst = m->ens ? (int)m->inst.size() : 1;
+    if (!m->d_mass_part)
+        HIPCHK(hipMalloc(&m->d_mass_part, sizeof(double) * 2 * (size_t)m->prm.max_blocks));
+    if (ninst > m->mass_ninst || !m->d_mass) {
+        HIPCHK(hipStreamSynchronize(m->stream));
+        (void)hipFree(m->d_mass);
+        m->d_mass = nullptr;
+        HIPCHK(hipMalloc(&m->d_mass, sizeof(double) * 4 * (size_t)ninst));
+    }
+    m->mass_ninst = ninst;
+    int rc = upload_tables(m);   // the device leaf list is current
+    if (rc) return rc;
+    enqueue_mass_budget(m, MASS_TARGET);
+    HIPCHK(hipGetLastError());
+    m->mass_on = 1;
+    return AMRLBM_OK;
+}
+
+int amrlbm_get_mass_budget(amrlbm_mesh *m, int n, double *out)
+{
+    NEED_READY(m);
+    if (!m->mass_on) return fail(AMRLBM_ERR_STATE, "the mass correction is off");
+    if (!out || n != m->mass_ninst)
+        return fail(AMRLBM_ERR_ARG, "the budget has " + std::to_string(m->mass_ninst) + " instances");
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipMemcpy(out, m->d_mass, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost));
     return AMRLBM_OK;
 }
 

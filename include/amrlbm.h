@@ -213,6 +213,29 @@ int amrlbm_get_tuning(amrlbm_mesh *m, int knobs[3]);
  * execution, microseconds per step (-1: the segment kernel does not fit). */
 int amrlbm_autotune(amrlbm_mesh *m, int reps, int regrid_every, double table[96]);
 
+/* Phase-field mass correction (no counterpart in the reference, whose path
+ * is not mass-conserving: the coarse/fine ghost interpolation of its steps
+ * and its Prolong move the mass of Output, AMR_Rising_2D_OMP.f90:478).
+ * on = 1: from the current state, each instance's phase-field mass
+ * S = sum over leaf cells of C dx^2 (Output's mass is rhol * area +
+ * (rhoh - rhol) S) is kept in FP64 on the device at its budget: S now, plus
+ * the change each later mesh change's restrict and prolong make (summed in
+ * FP64 before and after them). At every segment end (every regrid interval,
+ * and the end of every stepping call) S is folded in FP64 and put back to
+ * the budget by dC = lambda max(0, 4 C (1 - C)) on the interface cells
+ * (lambda from the fold): C += dC, h_q += Gamma_q(U) dC and g_q +=
+ * (Gamma_q(U) - w_q) (rhoh - rhol) dC / 3, which keep the cell's velocity and
+ * pressure, before U, P and Err_Interface of that state are computed. It removes the steps' mass change (FP32
+ * rounding and the interpolation's own) and keeps the regrids'. Off (0) by
+ * default and after any set-up, restart or amrlbm_set_topology; with it on
+ * the persistent segment kernel is not used. Works on FP64 and FP32 meshes
+ * (an FP64 mesh then no longer follows the reference bit for bit). */
+int amrlbm_set_mass_correction(amrlbm_mesh *m, int on);
+/* The budget of each instance (n = 1, or the ensemble's instance count):
+ * out[4 q .. 4 q + 3] = target, S at the last correction before it was
+ * applied, the interface weight W, lambda. AMRLBM_ERR_STATE when off. */
+int amrlbm_get_mass_budget(amrlbm_mesh *m, int n, double *out);
+
 /* FP64 diagnostics over the leaf interiors (Output, AMR_Rising_2D_OMP.f90:456-489):
  * out[0..8] = rho_min, rho_max, max|Uy|, mass (sum rho dx^2), bubble area
  * (sum (1-C) dx^2), bubble centroid x, y, bubble rise velocity, non-finite count.
