@@ -7,9 +7,9 @@ with the leaf at the same (level, I, J) of a one-domain FP64 run of that
 instance (which matches the oracle bit for bit, test_gpu_parity.py), across
 regrids that change one instance's mesh and not another's: the FP64
 ensemble bitwise, the FP32 ensemble within the FP32 path's bounds against
-FP64 (those of test_gpu_fp32.py; the FP32 cell functions let the compiler
-fuse multiply-adds, so an ensemble's FP32 kernels, another instantiation,
-agree with a one-domain FP32 run to rounding only). The 64-instance ensemble
+FP64 (those of test_gpu_fp32.py) and bitwise against the instance run alone
+in FP32 (the FP32 cell functions fuse multiply-adds per source expression,
+so the ensemble's kernels compute the same bits). The 64-instance ensemble
 drawn with seed 2024 runs as drawn, and two of its instances whose fluids
 are not the reference's run against the oracle with their own constants."""
 import numpy as np
@@ -108,6 +108,19 @@ def test_instances_evolve_as_alone(amrlbm, precision):
                 for f, v in err.items():
                     assert v <= FIELD_ABS[f], (q, f, v)
                 assert mass <= MASS_REL and rise <= RISE_REL and cen <= CENTROID_ABS
+                with amrlbm.Mesh(w, precision=32) as a32:   # the same instance alone, FP32: the same bits
+                    a32.setup_rising_bubble()
+                    a32.step(steps, k)
+                    l32 = {key[1:]: n for key, n in _leaves(a32, False).items()}
+                    assert mine == set(l32), f"instance {q}: the FP32 leaf sets differ"
+                    for (L, I, J), n in l32.items():
+                        ref = a32.get_block(n)
+                        got = e.get_block(le[(q, L, I, J)])
+                        for f in ("C", "P", "Ux", "Uy"):
+                            assert np.array_equal(got[f], ref[f]), (q, L, I, J, f)
+                        for f in ("h", "g"):
+                            assert np.array_equal(got[f][:, 1:-1, 1:-1], ref[f][:, 1:-1, 1:-1]), (q, L, I, J, f)
+                    print(f"[ensemble] FP32 instance {w.name}: {len(l32)} leaves bitwise equal to its FP32 run alone")
         if precision == 64:
             print(f"[ensemble] FP64: {nbits} leaves bitwise equal to the instances run alone")
 
