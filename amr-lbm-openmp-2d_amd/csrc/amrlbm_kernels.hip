@@ -36,6 +36,20 @@ constexpr int c_opp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
 __constant__ int c_dsx[8] = {-1, 1, 0, 0, -1, -1, 1, 1};
 __constant__ int c_dsy[8] = {0, 0, -1, 1, -1, 1, -1, 1};
 
+// 1 / dx^p for the power-of-two cell size dx: 2^(-p log2 dx), one exponent
+// adjustment with exactly the value of the IEEE division (cells are never
+// finer than 1 nor coarser than 2^14, so no exponent range is left)
+__device__ __forceinline__ double inv_pow2(int dx, int p) { return ldexp(1.0, -p * (__ffs(dx) - 1)); }
+__device__ __forceinline__ float inv_pow2f(int dx, int p) { return ldexpf(1.0f, -p * (__ffs(dx) - 1)); }
+template <class T>
+__device__ __forceinline__ T inv_dx_of(int dx)
+{
+    if constexpr (std::is_same<T, float>::value)
+        return inv_pow2f(dx, 1);
+    else
+        return inv_pow2(dx, 1);
+}
+
 // lattice weights Wa = [16,4,4,4,4,1,1,1,1]/36 (D2Q9_Mod.f90:8)
 __device__ __forceinline__ double wa(int q) { return q == 0 ? 16.0 / 36.0 : (q < 5 ? 4.0 / 36.0 : 1.0 / 36.0); }
 
@@ -78,7 +92,7 @@ __device__ __forceinline__ double chem_pot(const double *sC, int i, int j, int d
     double c = sC[cidx(i, j)];
     // x / dx^2 as x * (1 / dx^2): dx is a power of two, so both are the correctly
     // rounded x / dx^2 (the same bits, subnormals included)
-    return 4.0 * ph.beta * c * (c - 1.0) * (c - 0.5) - ph.kappa * d2c * (1.0 / (double)(dx * dx));
+    return 4.0 * ph.beta * c * (c - 1.0) * (c - 0.5) - ph.kappa * d2c * inv_pow2(dx, 2);
 }
 
 // Gradient_ED_AMR (LBM_Subroutines.f90:281-289)
@@ -92,7 +106,7 @@ __device__ __forceinline__ void gradient_ed(const double e[9], double &gx, doubl
 // power-of-two dx
 __device__ __forceinline__ void ed_c(const double *sC, int i, int j, int dx, double e[9])
 {
-    const double inv_dx = 1.0 / (double)dx;
+    const double inv_dx = inv_pow2(dx, 1);
 #pragma unroll
     for (int k = 0; k < 9; k++)
         e[k] = (sC[cidx(i + c_ex[k], j + c_ey[k])] - sC[cidx(i - c_ex[k], j - c_ey[k])]) * 0.5 * inv_dx;
@@ -101,7 +115,7 @@ __device__ __forceinline__ void ed_c(const double *sC, int i, int j, int dx, dou
 // ED_M_AMR (LBM_Subroutines.f90:262-277)
 __device__ __forceinline__ void ed_m(const double *sC, int i, int j, int dx, double e[9])
 {
-    const double inv_dx = 1.0 / (double)dx;
+    const double inv_dx = inv_pow2(dx, 1);
 #pragma unroll
     for (int k = 0; k < 9; k++)
         e[k] = (-sC[cidx(i - c_ex[k], j - c_ey[k])] - 3.0 * sC[cidx(i, j)] + 5.0 * sC[cidx(i + c_ex[k], j + c_ey[k])] -
@@ -210,7 +224,7 @@ __device__ __forceinline__ void macro_g_cell(const double *sC, int i, int j, int
                                              double &P, double &Ux, double &Uy)
 {
     const double dRho6 = ph.drho6;
-    const double inv_dx = 1.0 / (double)dx;   // exact: dx is a power of two
+    const double inv_dx = inv_pow2(dx, 1);   // exact: dx is a power of two
     double gx = (div_c<3>(sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)]) +
                  div_c<12>(sC[cidx(i + 1, j + 1)] + sC[cidx(i + 1, j - 1)] - sC[cidx(i - 1, j + 1)] -
                            sC[cidx(i - 1, j - 1)])) *
@@ -361,7 +375,7 @@ __device__ __forceinline__ void collide_cell(const float *sC, int i, int j, int 
                                              const PhysF &ph, float h[9], float g[9])
 {
 #pragma clang fp contract(on)
-    const float inv_dx = 1.0f / (float)dx;
+    const float inv_dx = inv_pow2f(dx, 1);
     float e[9], hlp[9], gmw[9], heqd[9], geq[9], gx, gy;
     const float mu = chem_pot(sC, i, j, inv_dx * inv_dx, ph);
     const float ct = sC[cidx(i, j)];
@@ -397,7 +411,7 @@ __device__ __forceinline__ void macro_g_cell(const float *sC, int i, int j, int 
                                              float &P, float &Ux, float &Uy)
 {
 #pragma clang fp contract(on)
-    const float inv_dx = 1.0f / (float)dx;
+    const float inv_dx = inv_pow2f(dx, 1);
     const float gx = ((sC[cidx(i + 1, j)] - sC[cidx(i - 1, j)]) * (1.0f / 3.0f) +
                       (sC[cidx(i + 1, j + 1)] + sC[cidx(i + 1, j - 1)] - sC[cidx(i - 1, j + 1)] - sC[cidx(i - 1, j - 1)]) *
                           (1.0f / 12.0f)) *
@@ -785,7 +799,7 @@ __device__ __forceinline__ int lidx(int c, int x, int y, int lcf) { return c * l
 template <class T>
 __device__ __forceinline__ T stream_cell(const T *sF, int lcf, T *__restrict__ G, int set, int i, int j, int dx)
 {
-    const T cfl = T(1) / (T)dx;
+    const T cfl = inv_dx_of<T>(dx);   // 1 / dx, exact
     const T a = cfl / T(2) * (T(1) + cfl), b = T(1) - cfl * cfl, c = cfl / T(2) * (T(1) - cfl);
     T s = T(0);
 #pragma unroll
@@ -1626,7 +1640,7 @@ __device__ __forceinline__ void hg_stage_both(const PNode *sN, int b, int e, con
 template <class T>
 __device__ __forceinline__ void stream_cell_regs(const T *sF, int lcf, int i, int j, int dx, T f[9])
 {
-    const T cfl = T(1) / (T)dx;
+    const T cfl = inv_dx_of<T>(dx);   // 1 / dx, exact
     const T a = cfl / T(2) * (T(1) + cfl), b = T(1) - cfl * cfl, c = cfl / T(2) * (T(1) - cfl);
 #pragma unroll
     for (int q = 0; q < 9; q++) {
