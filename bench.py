@@ -280,18 +280,18 @@ def measure(amrlbm, w, steps, warmup, k, calibrate_mb=0, precision=64, radius=No
     }
 
 
-def learned_regrid(amrlbm, w, steps=2000):
-    """Config 4: the committed network deployed on workload w (trained on the
-    generator of amrlbm.learned), FP32, full regrid every FULL_EVERY steps
-    with the plain criterion judging every step on the device and the
-    network's flags joining every regrid, against the same cadence without
-    the network and the plain regrid every 10 steps."""
+def learned_regrid(amrlbm, w, steps=10000, full_every=50):
+    """The learned regrid as its configuration states it: the committed
+    network (trained on the generator of amrlbm.learned) deployed on workload
+    w, FP32, full regrid every 50 steps with the plain criterion judging every
+    step on the device and the network's flags joining every regrid, against
+    the same cadence without the network and the plain regrid every 10 steps."""
     import numpy as np
     from amrlbm import learned
     if not os.path.exists(learned.WEIGHTS):
         return None
-    ev = learned.evaluate(w, np.load(learned.WEIGHTS), steps)
-    out = {"workload": f"{w.name} bubble (D={int(w.D)}), {steps} steps, full regrid every {learned.FULL_EVERY} "
+    ev = learned.evaluate(w, np.load(learned.WEIGHTS), steps, full_every=full_every)
+    out = {"workload": f"{w.name} bubble (D={int(w.D)}), {steps} steps, full regrid every {full_every} "
                        f"steps, plain criterion judging every step on the device, the network's flags joining "
                        f"every regrid", "precision": "f32",
            "threshold": learned._threshold()}
