@@ -613,6 +613,15 @@ __device__ __forceinline__ T eval_hg(const PNode &nd, int set, int q, const T *_
     return (GLOBAL_ONLY || r >= 0) ? ldg<COH>(Fc + r + (set * 9 + qo) * GS) : sF[-1 - r + qo * lcf];
 }
 
+// t / ns for 0 <= t < 18 ns and 0 < ns <= NODES_HG, as a multiply and a
+// shift: m = 2^22 / ns + 1, taken once per stage. Exact over that range: m /
+// 2^22 exceeds 1 / ns by less than 2^-22, so t m / 2^22 exceeds t / ns by less
+// than 18 NODES_HG / 2^22 < 1 / NODES_HG <= 1 / ns, which keeps it below the
+// next multiple of 1 / ns; the products stay below 2^31.
+static_assert(18 * NODES_HG * NODES_HG < (1 << 22), "the quotient shift needs a wider margin");
+__device__ __forceinline__ unsigned quot_mul(int ns) { return (1u << 22) / (unsigned)ns + 1u; }
+__device__ __forceinline__ int quot(int t, unsigned m) { return (int)(((unsigned)t * m) >> 22); }
+
 // One stage of an hg ghost program for one population set: recipes [b, e)
 // of the block (staged in sN) applied to the set's 9 populations. Items run
 // recipe-fastest, so the lanes of a wavefront read neighbouring cells of one
@@ -622,13 +631,15 @@ __device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, int set,
                                          int lcf, T *F, bool store, int tid)
 {
     const int ns = e - b;
+    if (ns <= 0) return;
+    const unsigned mq = quot_mul(ns);
     for (int base = 0; base < 9 * ns; base += 256 * U) {
         T v[U];
 #pragma unroll
         for (int r = 0; r < U; r++) {
             const int t = base + tid + 256 * r;
             if (t < 9 * ns) {
-                const int q = t / ns;
+                const int q = quot(t, mq);
                 v[r] = eval_hg<GLOBAL_ONLY>(sN[b + t - q * ns], set, q, Fc, sF, lcf);
             }
         }
@@ -636,7 +647,7 @@ __device__ __forceinline__ void hg_stage(const PNode *sN, int b, int e, int set,
         for (int r = 0; r < U; r++) {
             const int t = base + tid + 256 * r;
             if (t < 9 * ns) {
-                const int q = t / ns;
+                const int q = quot(t, mq);
                 const PNode &nd = sN[b + t - q * ns];
                 sF[q * lcf + nd.dest] = v[r];
                 if (store && nd.own >= 0) F[(set * 9 + q) * GS + nd.own] = v[r];
@@ -1613,13 +1624,15 @@ __device__ __forceinline__ void hg_stage_both(const PNode *sN, int b, int e, con
                                               int tid)
 {
     const int ns = e - b, nitem = 18 * ns;
+    if (ns <= 0) return;
+    const unsigned mq = quot_mul(ns);
     for (int base = 0; base < nitem; base += 256 * U) {
         T v[U];
 #pragma unroll
         for (int r = 0; r < U; r++) {
             const int t = base + tid + 256 * r;
             if (t < nitem) {
-                const int c = t / ns, set = c >= 9 ? 1 : 0;
+                const int c = quot(t, mq), set = c >= 9 ? 1 : 0;
                 v[r] = eval_hg<GLOBAL_ONLY, true>(sN[b + t - c * ns], set, c - 9 * set, X, sF + 9 * set * lcf, lcf);
             }
         }
@@ -1627,7 +1640,7 @@ __device__ __forceinline__ void hg_stage_both(const PNode *sN, int b, int e, con
         for (int r = 0; r < U; r++) {
             const int t = base + tid + 256 * r;
             if (t < nitem) {
-                const int c = t / ns;
+                const int c = quot(t, mq);
                 sF[c * lcf + sN[b + t - c * ns].dest] = v[r];
             }
         }
