@@ -541,12 +541,13 @@ __device__ __forceinline__ void prefetch_node(const PNode &nd, const T *__restri
     for (int r = 0; r < 9; r++) gv[r] = (r < nref && nd.ref[r] >= 0) ? ldg<COH>(G + nd.ref[r] + gs) : T(0);
 }
 
-// one recipe from prefetched global operands gv and LDS L (+ shift ls)
-template <class T>
+// one recipe from prefetched global operands gv and LDS L (+ shift ls);
+// GLOBAL_ONLY: a first-stage recipe, whose operands are all in gv
+template <bool GLOBAL_ONLY = false, class T>
 __device__ __forceinline__ T eval_pref(const PNode &nd, const T gv[9], const T *L, int ls)
 {
     const int type = nd.info & 15;
-#define PV(r) (nd.ref[r] >= 0 ? gv[r] : L[-1 - nd.ref[r] + ls])
+#define PV(r) (GLOBAL_ONLY || nd.ref[r] >= 0 ? gv[r] : L[-1 - nd.ref[r] + ls])
     if (type == NT_COPY) return PV(0);
     T wx[3], wy[3];
     wpat((nd.info >> 4) & 3, wx);
@@ -717,7 +718,7 @@ __global__ __launch_bounds__(256) void k_prog_c(DevMeshT<T> m, int cur, const in
     if (fill) {
         const T cin = C[cidx(i, j)];
         if (tid < n1) {
-            const T v = eval_pref(nd0, gv, (const T *)sC, 0);
+            const T v = eval_pref<true>(nd0, gv, (const T *)sC, 0);
             sC[nd0.dest] = v;
             if (nd0.dest < CS) C[nd0.dest] = v;
         }
@@ -1680,7 +1681,7 @@ __device__ __forceinline__ void fill_c_block(const PHdr &hd, const PNode &nd0, c
     T gv[9];
     if (tid < total) prefetch_node<true>(nd0, Cg, 0, gv);
     if (tid < n1) {
-        const T v = eval_pref(nd0, gv, (const T *)sC, 0);
+        const T v = eval_pref<true>(nd0, gv, (const T *)sC, 0);
         sC[nd0.dest] = v;
         if (nd0.dest < CS) C[nd0.dest] = v;
     }
