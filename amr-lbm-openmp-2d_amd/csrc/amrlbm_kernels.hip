@@ -975,6 +975,30 @@ __global__ __launch_bounds__(256) void k_err(DevMeshT<T> m, const int *__restric
 // a stop before jstop (a state inside the segment needed refinement, and the
 // segment's last state was never reached) answers "changed, refinement" with
 // no flags (they belong to that earlier state).
+// Ordered stream compaction in a 1024-thread workgroup: the threads whose
+// pred holds append value to list after *count, in thread order (wavefront
+// ballots, a prefix sum over the 16 waves' counts, each lane's rank among the
+// flagged lanes of its wave); entries past cap are counted, not stored.
+// Every thread of the workgroup calls it (it holds barriers).
+__device__ __forceinline__ void append_ordered(bool pred, int value, int *list, int cap, int *count, int *wave_cnt)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long bal = __ballot(pred);
+    if (lane == 0) wave_cnt[wave] = __popcll(bal);
+    __syncthreads();
+    int ix = *count;
+    for (int w = 0; w < wave; w++) ix += wave_cnt[w];
+    ix += __popcll(bal & ((1ull << lane) - 1ull));
+    if (pred && ix < cap) list[ix] = value;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = *count;
+        for (int w = 0; w < 16; w++) t += wave_cnt[w];
+        *count = t;
+    }
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int nblks, const double *__restrict__ err_slot,
                                                       int min_levels, int max_levels, int levelmin, int levelmax,
                                                       double refin_tol, double deref_tol, int *changed,
@@ -982,7 +1006,7 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
 {
     extern __shared__ unsigned masks[];
     __shared__ int cand[RC_CAND], clev[RC_CAND], watch[RC_CAND * 16], rlist[RC_REF];
-    __shared__ int ncand, nref, nder;
+    __shared__ int ncand, nref, nder, wave_cnt[16];
     const int tid = threadIdx.x, nw = (nblks + 31) >> 5;
     unsigned *dmask = masks, *rmask = masks + nw;
     auto rbit = [&](int c) { return (rmask[c >> 5] >> (c & 31)) & 1u; };
@@ -1000,18 +1024,20 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
     for (int w = tid; w < 2 * nw; w += 1024) masks[w] = 0u;
     if (tid == 0) ncand = nref = nder = 0;
     __syncthreads();
-    // the criterion's refinement flags
-    for (int n = tid; n < nblks; n += 1024) {
-        const BlockDev b = m.blk[n];
-        if (b.level < 0 || !b.isleaf) continue;
-        const int lo = m.lv ? m.lv[2 * b.inst] : min_levels, hi = m.lv ? m.lv[2 * b.inst + 1] : max_levels;
-        if (b.level < lo || (b.level < hi && err_slot[n] > refin_tol)) {
-            atomicOr(&rmask[n >> 5], 1u << (n & 31));
-            const int ix = atomicAdd(&nref, 1);
-            if (ix < RC_REF) rlist[ix] = n;
+    // the criterion's refinement flags, listed in ascending slot order
+    for (int base = 0; base < nblks; base += 1024) {
+        const int n = base + tid;
+        bool f = false;
+        if (n < nblks) {
+            const BlockDev b = m.blk[n];
+            if (b.level >= 0 && b.isleaf) {
+                const int lo = m.lv ? m.lv[2 * b.inst] : min_levels, hi = m.lv ? m.lv[2 * b.inst + 1] : max_levels;
+                f = b.level < lo || (b.level < hi && err_slot[n] > refin_tol);
+            }
         }
+        if (f) atomicOr(&rmask[n >> 5], 1u << (n & 31));
+        append_ordered(f, n, rlist, RC_REF, &nref, wave_cnt);
     }
-    __syncthreads();
     // the 2:1 closure, one breadth-first round per pass
     int head = 0;
     bool lists_ok = true;
@@ -1042,28 +1068,27 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
         head = tail;
     }
     const int nr = nref;
-    // derefinement candidates
-    for (int n = tid; n < nblks; n += 1024) {
-        const BlockDev b = m.blk[n];
-        if (b.level < 0 || !b.hasleaf || b.child0 < 0) continue;
-        const int lo = m.lv ? m.lv[2 * b.inst] : min_levels;
-        if (b.level < lo) continue;
-        bool d = true;
-        for (int c = b.child0; c < b.child0 + 4; c++) {
-            const double e = m.blk[c].isleaf ? err_slot[c] : 0.0;
-            if (rbit(c) || e > deref_tol) d = false;
-        }
-        if (d) {
-            atomicOr(&dmask[n >> 5], 1u << (n & 31));
-            const int ix = atomicAdd(&ncand, 1);
-            if (ix < RC_CAND) {
-                cand[ix] = n;
-                clev[ix] = b.level;
+    // derefinement candidates, listed in ascending slot order
+    for (int base = 0; base < nblks; base += 1024) {
+        const int n = base + tid;
+        bool d = false;
+        if (n < nblks) {
+            const BlockDev b = m.blk[n];
+            const int lo = m.lv ? m.lv[2 * b.inst] : min_levels;
+            if (b.level >= lo && b.hasleaf && b.child0 >= 0) {
+                d = true;
+                for (int c = b.child0; c < b.child0 + 4; c++) {
+                    const double e = m.blk[c].isleaf ? err_slot[c] : 0.0;
+                    if (rbit(c) || e > deref_tol) d = false;
+                }
             }
         }
+        if (d) atomicOr(&dmask[n >> 5], 1u << (n & 31));
+        append_ordered(d, n, cand, RC_CAND, &ncand, wave_cnt);
     }
-    __syncthreads();
     const int nc = ncand;
+    for (int ci = tid; ci < nc && ci < RC_CAND; ci += 1024) clev[ci] = m.blk[cand[ci]].level;
+    __syncthreads();
     if (nc > RC_CAND) {   // the host decides and flags
         if (tid == 0) {
             changed[0] = 1;
@@ -1103,13 +1128,14 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
         }
         __syncthreads();
     }
-    // the surviving derefinement flags (candidates only carry dmask bits)
-    for (int ci = tid; ci < nc; ci += 1024) {
-        const int n = cand[ci];
-        if ((dmask[n >> 5] >> (n & 31)) & 1u) {
-            const int ix = atomicAdd(&nder, 1);
-            if (flags && lists_ok) flags[kRegridDerefAt + ix] = n;
-        }
+    // the surviving derefinement flags (candidates only carry dmask bits), in
+    // ascending slot order
+    for (int base = 0; base < nc; base += 1024) {
+        const int ci = base + tid;
+        const int n = ci < nc ? cand[ci] : 0;
+        const bool keep = ci < nc && ((dmask[n >> 5] >> (n & 31)) & 1u);
+        append_ordered(keep, n, (flags && lists_ok) ? flags + kRegridDerefAt : wave_cnt, (flags && lists_ok) ? RC_CAND : 0,
+                       &nder, wave_cnt);
     }
     if (flags && lists_ok)
         for (int t = tid; t < nr; t += 1024) flags[3 + t] = rlist[t];
