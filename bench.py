@@ -46,6 +46,11 @@ instances of uneven size, level range and fluid drawn from a seed, in one
 mesh (every launch over the leaves of all instances), FP32 and FP64, with
 each instance's FP32 error against FP64. Both report finest-equivalent MLUPS
 summed over the whole mesh.
+
+--sensitivity is an accuracy run, not a throughput one: every 2,000 steps the
+FP32 run's Output against the FP64 run's, beside FP64 runs whose h and g got
+relative noise of 1e-7 (FP32's size) once at set-up or again before every
+--noise-every steps.
 """
 import argparse
 import json
@@ -376,6 +381,59 @@ def versus(d, ref):
             "centroid_abs_cells": max(abs(d[5] - ref[5]), abs(d[6] - ref[6]))}
 
 
+def perturb_leaves(mesh, rng, noise):
+    """Every leaf's interior h and g times (1 + noise * U(-1, 1)), drawn per
+    value; C = sum of h again (FP64 mesh)."""
+    for b in mesh.leaves():
+        d = mesh.get_block(b)
+        h, g, C = d["h"], d["g"], d["C"]
+        h[:, 1:-1, 1:-1] *= 1.0 + noise * rng.uniform(-1.0, 1.0, h[:, 1:-1, 1:-1].shape)
+        g[:, 1:-1, 1:-1] *= 1.0 + noise * rng.uniform(-1.0, 1.0, g[:, 1:-1, 1:-1].shape)
+        C[2:-2, 2:-2] = h[:, 1:-1, 1:-1].sum(axis=0)
+        mesh.put_block(b, h=h, g=g, C=C)
+
+
+def sensitivity(amrlbm, w, steps, k, noise=1e-7, every=10, seed=1, check_every=2000):
+    """The FP32 path's error against FP64 beside the flow's own sensitivity
+    to noise of FP32's size: FP64 runs whose h and g get relative noise of
+    `noise` once at set-up, or again before every `every` steps (host-side,
+    through the per-block entry points). Every check_every steps, each run's
+    Output against the plain FP64 run's."""
+    import numpy as np
+
+    def run(precision, inject=0):
+        rng = np.random.default_rng(seed)
+        rows = []
+        with amrlbm.Mesh(w, precision=precision) as m:
+            m.setup_rising_bubble()
+            if inject < 0:
+                perturb_leaves(m, rng, noise)
+            t = 0
+            while t < steps:
+                if inject > 0:
+                    perturb_leaves(m, rng, noise)
+                n = min(inject if inject > 0 else check_every, check_every - t % check_every, steps - t)
+                m.step(n, k)
+                t += n
+                if t % check_every == 0 or t == steps:
+                    rows.append(m.output().tolist())
+                    if inject > 0:
+                        progress(f"sensitivity: noise every {inject} steps, step {t}")
+        return rows
+
+    base = run(64)
+    progress("sensitivity: fp64")
+    out = {"steps": steps, "regrid_every": k, "noise": noise, "seed": seed,
+           "at_steps": [min((i + 1) * check_every, steps) for i in range(len(base))]}
+    for name, precision, inject in (("fp32", 32, 0), ("fp64_noise_at_setup", 64, -1),
+                                    (f"fp64_noise_every_{every}_steps", 64, every)):
+        rows = run(precision, inject)
+        out[name] = {f: [versus(d, r)[f] for d, r in zip(rows, base)]
+                     for f in ("mass_rel", "rise_velocity_rel", "centroid_abs_cells")}
+        progress(f"sensitivity: {name}")
+    return out
+
+
 def multi_bubble(amrlbm, steps, k):
     """The 32-bubble case (amrlbm.multibubble): FP64, FP32 and FP32 with the
     learned regrid (full regrid every learned.FULL_EVERY steps, plain
@@ -523,6 +581,10 @@ def main():
                     help="profiling run: time the step kernels on the set-up mesh only (no whole steps)")
     ap.add_argument("--profile-steps", action="store_true",
                     help="profiling run: --steps whole steps with regrids, launches issued one by one (no graph)")
+    ap.add_argument("--sensitivity", action="store_true",
+                    help="accuracy run: the FP32 error against FP64 beside FP64 runs perturbed by FP32-sized "
+                         "noise at set-up and every --noise-every steps, over --steps steps (several minutes)")
+    ap.add_argument("--noise-every", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -540,6 +602,9 @@ def main():
     if args.profile_steps:
         variant = tuple(int(v) for v in args.variant.split(",")) if args.variant else None
         profile_steps(amrlbm, w, args.steps, k, (64,) if args.no_fp32 else (64, 32), variant)
+        return
+    if args.sensitivity:
+        print(json.dumps({"sensitivity": sensitivity(amrlbm, w, args.steps, k, every=args.noise_every)}), flush=True)
         return
     r = measure(amrlbm, w, args.steps, args.warmup, k, args.calibrate_copy_mb)
     progress("fp64 headline")
