@@ -999,6 +999,17 @@ __device__ __forceinline__ void append_ordered(bool pred, int value, int *list, 
     __syncthreads();
 }
 
+// With INFO, the block table and each leaf's Err_Interface are read from
+// memory once, into a packed word per block slot in LDS (bits 0-4: level + 1,
+// 0 for a free slot; 5: leaf; 6: parent of leaves; 7: level at or above the
+// block's minimum; 8: a leaf's Err_Interface above deref_tol; 9-31: first
+// child + 1), so that the later passes, which follow children and candidates,
+// wait on LDS rather than on memory; without it (more slots than LDS holds)
+// they read the table.
+constexpr int kInfoLevel = 0x1f, kInfoLeaf = 1 << 5, kInfoHasLeaf = 1 << 6, kInfoLoOk = 1 << 7, kInfoErrD = 1 << 8,
+              kInfoChildShift = 9;
+
+template <bool INFO>
 __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int nblks, const double *__restrict__ err_slot,
                                                       int min_levels, int max_levels, int levelmin, int levelmax,
                                                       double refin_tol, double deref_tol, int *changed,
@@ -1008,8 +1019,11 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
     __shared__ int cand[RC_CAND], clev[RC_CAND], watch[RC_CAND * 16], rlist[RC_REF];
     __shared__ int ncand, nref, nder, wave_cnt[16];
     const int tid = threadIdx.x, nw = (nblks + 31) >> 5;
-    unsigned *dmask = masks, *rmask = masks + nw;
+    unsigned *dmask = masks, *rmask = masks + nw, *info = masks + 2 * nw;
     auto rbit = [&](int c) { return (rmask[c >> 5] >> (c & 31)) & 1u; };
+    // what the passes after the first read of a block slot
+    auto level_of = [&](int n) { return INFO ? (int)(info[n] & kInfoLevel) - 1 : m.blk[n].level; };
+    auto leaf_of = [&](int n) { return INFO ? (info[n] & kInfoLeaf) != 0 : m.blk[n].isleaf != 0; };
     if (judge) {
         const int J = *judge;
         if (tid == 0) changed[2] = J;
@@ -1030,10 +1044,14 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
         bool f = false;
         if (n < nblks) {
             const BlockDev b = m.blk[n];
-            if (b.level >= 0 && b.isleaf) {
-                const int lo = m.lv ? m.lv[2 * b.inst] : min_levels, hi = m.lv ? m.lv[2 * b.inst + 1] : max_levels;
-                f = b.level < lo || (b.level < hi && err_slot[n] > refin_tol);
-            }
+            const int lo = m.lv ? m.lv[2 * b.inst] : min_levels, hi = m.lv ? m.lv[2 * b.inst + 1] : max_levels;
+            const bool leaf = b.level >= 0 && b.isleaf;
+            const double e = leaf ? err_slot[n] : 0.0;
+            if (leaf) f = b.level < lo || (b.level < hi && e > refin_tol);
+            if (INFO)
+                info[n] = (unsigned)(b.level + 1) | (leaf ? kInfoLeaf : 0) | (b.hasleaf ? kInfoHasLeaf : 0) |
+                          (b.level >= lo ? kInfoLoOk : 0) | (leaf && e > deref_tol ? kInfoErrD : 0) |
+                          ((unsigned)(b.child0 + 1) << kInfoChildShift);
         }
         if (f) atomicOr(&rmask[n >> 5], 1u << (n & 31));
         append_ordered(f, n, rlist, RC_REF, &nref, wave_cnt);
@@ -1049,7 +1067,7 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
         }
         for (int t = head + tid; t < tail; t += 1024) {
             const int n = rlist[t];
-            const int L = m.blk[n].level;
+            const int L = level_of(n);
             for (int k = 0; k < 8; k++) {
                 const NbDesc e = m.nbd[(size_t)n * 8 + k];
                 if (e.id <= 0 || !e.isleaf || e.level != L - 1) continue;
@@ -1073,13 +1091,23 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
         const int n = base + tid;
         bool d = false;
         if (n < nblks) {
-            const BlockDev b = m.blk[n];
-            const int lo = m.lv ? m.lv[2 * b.inst] : min_levels;
-            if (b.level >= lo && b.hasleaf && b.child0 >= 0) {
-                d = true;
-                for (int c = b.child0; c < b.child0 + 4; c++) {
-                    const double e = m.blk[c].isleaf ? err_slot[c] : 0.0;
-                    if (rbit(c) || e > deref_tol) d = false;
+            if (INFO) {
+                const unsigned w = info[n];
+                const int c0 = (int)(w >> kInfoChildShift) - 1;
+                if ((w & kInfoLoOk) && (w & kInfoHasLeaf) && c0 >= 0) {
+                    d = true;
+                    for (int c = c0; c < c0 + 4; c++)
+                        if (rbit(c) || (info[c] & kInfoErrD)) d = false;
+                }
+            } else {
+                const BlockDev b = m.blk[n];
+                const int lo = m.lv ? m.lv[2 * b.inst] : min_levels;
+                if (b.level >= lo && b.hasleaf && b.child0 >= 0) {
+                    d = true;
+                    for (int c = b.child0; c < b.child0 + 4; c++) {
+                        const double e = m.blk[c].isleaf ? err_slot[c] : 0.0;
+                        if (rbit(c) || e > deref_tol) d = false;
+                    }
                 }
             }
         }
@@ -1087,7 +1115,7 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
         append_ordered(d, n, cand, RC_CAND, &ncand, wave_cnt);
     }
     const int nc = ncand;
-    for (int ci = tid; ci < nc && ci < RC_CAND; ci += 1024) clev[ci] = m.blk[cand[ci]].level;
+    for (int ci = tid; ci < nc && ci < RC_CAND; ci += 1024) clev[ci] = level_of(cand[ci]);
     __syncthreads();
     if (nc > RC_CAND) {   // the host decides and flags
         if (tid == 0) {
@@ -1108,8 +1136,8 @@ __global__ __launch_bounds__(1024) void k_regrid_check(DevMeshT<double> m, int n
         if (e.id > 0 && !e.isleaf && e.child0 >= 0) {
             const int c1 = e.child0 + pair_c1[k] - 1, c2 = e.child0 + pair_c2[k] - 1;
             if (rbit(c1) || rbit(c2)) atomicAnd(&dmask[n >> 5], ~(1u << (n & 31)));
-            if (!m.blk[c1].isleaf) w0 = c1;
-            if (!m.blk[c2].isleaf) w1 = c2;
+            if (!leaf_of(c1)) w0 = c1;
+            if (!leaf_of(c2)) w1 = c2;
         }
         watch[ci * 16 + 2 * k] = w0;
         watch[ci * 16 + 2 * k + 1] = w1;
@@ -2013,17 +2041,26 @@ hipError_t launch_regrid_check(const DevMesh &m, int nblks, const double *err_sl
                                hipStream_t s, const int *judge, int jstop, int *flags)
 {
     if (nblks > regrid_check_max_blocks()) return hipErrorInvalidValue;
-    const size_t lds = sizeof(unsigned) * 2 * (size_t)((nblks + 31) / 32 + 1);
+    const size_t masks = sizeof(unsigned) * 2 * (size_t)((nblks + 31) / 32 + 1);
+    // the packed block words beside the masks where the LDS holds them
+    const size_t with_info = masks + sizeof(unsigned) * (size_t)nblks;
+    const bool info = with_info <= 160 * 1024 - kRegridStaticLds - 256;
+    const size_t lds = info ? with_info : masks;
+    const void *fn = info ? reinterpret_cast<const void *>(&k_regrid_check<true>)
+                          : reinterpret_cast<const void *>(&k_regrid_check<false>);
     // static and dynamic LDS beyond the default 64 KB per workgroup is asked for explicitly
-    static size_t granted = 64 * 1024 - kRegridStaticLds - 256;   // 256: the compiler's own alignment
-    if (lds > granted) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_regrid_check),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static size_t granted[2] = {64 * 1024 - kRegridStaticLds - 256, 64 * 1024 - kRegridStaticLds - 256};   // 256: the compiler's own alignment
+    if (lds > granted[info]) {
+        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
-        granted = lds;
+        granted[info] = lds;
     }
-    hipLaunchKernelGGL(k_regrid_check, dim3(1), dim3(1024), lds, s, typed<double>(m), nblks, err_slot, min_levels,
-                       max_levels, levelmin, levelmax, refin_tol, deref_tol, changed, judge, jstop, flags);
+    if (info)
+        hipLaunchKernelGGL(k_regrid_check<true>, dim3(1), dim3(1024), lds, s, typed<double>(m), nblks, err_slot,
+                           min_levels, max_levels, levelmin, levelmax, refin_tol, deref_tol, changed, judge, jstop, flags);
+    else
+        hipLaunchKernelGGL(k_regrid_check<false>, dim3(1), dim3(1024), lds, s, typed<double>(m), nblks, err_slot,
+                           min_levels, max_levels, levelmin, levelmax, refin_tol, deref_tol, changed, judge, jstop, flags);
     return hipGetLastError();
 }
 
