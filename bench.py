@@ -159,19 +159,22 @@ def roofline(key, kt, nleaf, w, precision):
             "bytes_per_launch": round(b)}
 
 
-def kernels_only(amrlbm, w, calibrate_mb, precisions, variant=None):
+def kernels_only(amrlbm, w, calibrate_mb, precisions, variant=None, regrid_checks=0):
     """The profiling run: the set-up mesh of workload w in launch variant
     (order, depth) if given (the autotuner's choice in the default run; the
     tuner itself launches step graphs, which this run avoids), then the three
     step kernels launched back to back (amrlbm_time_kernels, the launches the
-    bench line's roofline times), with no whole steps; one JSON line per
-    precision."""
+    bench line's roofline times), with no whole steps, then regrid_checks
+    regrid decisions on the same state (amrlbm_regrid_check: Err_Interface and
+    the decision kernel); one JSON line per precision."""
     for prec in precisions:
         with amrlbm.Mesh(w, precision=prec) as m:
             m.setup_rising_bubble()
             if variant:
                 m.set_tuning(*variant)
             kt = m.time_kernels(200)
+            for _ in range(regrid_checks):
+                m.regrid_check()
             nleaf = m.counts()["nleaf"]
         if calibrate_mb:
             calibration_copy(calibrate_mb << 20)
@@ -579,6 +582,8 @@ def main():
                     help="launch variant 'order,depth' for --kernels-only (amrlbm_set_tuning)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: time the step kernels on the set-up mesh only (no whole steps)")
+    ap.add_argument("--regrid-checks", type=int, default=0,
+                    help="with --kernels-only: that many regrid decisions on the set-up state after the kernels")
     ap.add_argument("--profile-steps", action="store_true",
                     help="profiling run: --steps whole steps with regrids, launches issued one by one (no graph)")
     ap.add_argument("--sensitivity", action="store_true",
@@ -597,7 +602,7 @@ def main():
     k = args.regrid_every
     if args.kernels_only:
         variant = tuple(int(v) for v in args.variant.split(",")) if args.variant else None
-        kernels_only(amrlbm, w, args.calibrate_copy_mb, (64,) if args.no_fp32 else (64, 32), variant)
+        kernels_only(amrlbm, w, args.calibrate_copy_mb, (64,) if args.no_fp32 else (64, 32), variant, args.regrid_checks)
         return
     if args.profile_steps:
         variant = tuple(int(v) for v in args.variant.split(",")) if args.variant else None
