@@ -89,6 +89,29 @@ def test_learned_cadence_judges_every_step(amrlbm, workloads):
         assert dm[8] == 0 and abs(dm[3] - dp[3]) <= 1e-5 * abs(dp[3])
 
 
+def test_network_flags_join_the_regrid(amrlbm, workloads):
+    """The network's flags reach the regrid: with a threshold every logit
+    passes and no Err_Interface gate, every leaf below the finest level that
+    the interface's profile touches is flagged at each mesh change, so the
+    run refines blocks the plain criterion leaves alone and ends with more
+    leaves than the same cadence without the network."""
+    from amrlbm import learned
+    wts = np.load(learned.WEIGHTS)
+    w = workloads["cfg1_rise"]
+    with amrlbm.Mesh(w, precision=32) as m, amrlbm.Mesh(w, precision=32) as p:
+        m.setup_rising_bubble()
+        p.setup_rising_bubble()
+        m.set_regrid_network(wts, threshold=-1e30, gate=0.0)
+        p.set_regrid_network(None)
+        st = m.step_learned(1000, learned.FULL_EVERY)
+        sp = p.step_learned(1000, learned.FULL_EVERY)
+        print(f"[learned] every logit passing: {st}, {m.counts()['nleaf']} leaves; "
+              f"no network: {sp}, {p.counts()['nleaf']} leaves", flush=True)
+        assert sp["predicted_blocks"] == 0 and st["predicted_blocks"] > 0
+        assert m.counts()["nleaf"] > p.counts()["nleaf"]
+        assert m.output()[8] == 0
+
+
 def test_learned_cadence_split_calls(amrlbm, workloads):
     """The cadence counts steps as amrlbm_step does (T), so calls split
     anywhere give the same run, bit for bit."""
